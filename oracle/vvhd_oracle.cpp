@@ -1,0 +1,1579 @@
+// vvhd_oracle.cpp -- FP64 CPU restatement of libvvhd's per-step particle hot
+// path (tree, epsilon + merge, convective, diffusive, flow-move, SLAE,
+// integrals).
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation in
+// vvflow_amd/.  Only tests/, the smoke entry point and the CPU-baseline leg of
+// bench.py may load it.  It is written from the reference's semantics (each
+// function cites the vvflow/vvflow file:line it follows) and keeps the
+// reference's operation order; it is compiled with -ffp-contract=off so that
+// every +,-,*,/ rounds exactly as the reference's SSE2 build does.
+//
+// Pinned by: the reference's own translation units (TSortedTree.cpp,
+// MEpsilonFast.cpp, MDiffusiveFast.cpp) built under oracle/_ref and compared
+// bit-for-bit (tests/test_oracle_ref.py, tests/golden/); TBody known answers
+// (libvvhd/test/test_TBody.hpp:78-160); closed forms and brute force
+// (tests/test_oracle.py).  The LAPACK-backed SLAE solve is "parity unpinned"
+// against LAPACK (not in the image); it is pinned by its residual instead.
+
+#include "vvhd_oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double k2Pi = 2. * kPi;
+constexpr double k1_2Pi = 1. / (2. * kPi);
+constexpr double k1_Pi = 1. / kPi;
+constexpr int kLeafMaxList = 16;          // TSortedTree.cpp:10
+constexpr double kExpArgLimit = -8.;      // MDiffusiveFast.cpp:6
+constexpr double kRemoveEps = 1e-10;      // MFlowmove.hpp:8 default
+const double kInf = std::numeric_limits<double>::infinity();
+const double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+inline double sgn(double x) { return (x > 0) ? 1 : ((x < 0) ? -1 : 0); }
+inline double sq(double x) { return x * x; }
+
+struct V2 {
+    double x, y;
+};
+inline V2 operator+(V2 a, V2 b) { return {a.x + b.x, a.y + b.y}; }
+inline V2 operator-(V2 a, V2 b) { return {a.x - b.x, a.y - b.y}; }
+inline V2 operator*(V2 a, double c) { return {a.x * c, a.y * c}; }
+inline V2 operator*(double c, V2 a) { return {a.x * c, a.y * c}; }
+inline V2 operator/(V2 a, double c) { return a * (1. / c); }   // TVec.hpp:23
+inline double dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+inline double abs2(V2 a) { return a.x * a.x + a.y * a.y; }
+inline double vabs(V2 a) { return std::sqrt(a.x * a.x + a.y * a.y); }
+inline V2 rotl(V2 a) { return {-a.y, a.x}; }
+inline bool iszero(V2 a) { return std::fabs(a.x) + std::fabs(a.y) < 1E-10; }
+inline void addto(double* v, V2 a) { v[0] += a.x; v[1] += a.y; }
+
+struct Obj {          // TObj layout (TObj.hpp:10-16)
+    double x, y, g, vx, vy, inv_eps;
+};
+static_assert(sizeof(Obj) == 48, "TObj layout");
+
+struct Seg {          // TAtt fields (TBody.hpp:17-55)
+    double x, y, g, inv_eps, cx, cy, dlx, dly, gsum, hsum, fric, Cp, Fr, Nu;
+    uint32_t slip, eq_no;
+    V2 r() const { return {x, y}; }
+    V2 corner() const { return {cx, cy}; }
+    V2 dl() const { return {dlx, dly}; }
+};
+static_assert(sizeof(Seg) == sizeof(vvo_seg_state), "seg layout");
+
+struct Body {         // TBody (TBody.hpp:67-158), rigid and static
+    int first = 0, n = 0;
+    double holder[3] = {0, 0, 0}, dpos[3] = {0, 0, 0};
+    double kspring[3] = {kInf, kInf, kInf}, damping[3] = {0, 0, 0};
+    double speed_slae[3] = {0, 0, 0}, speed_slae_prev[3] = {0, 0, 0};
+    double force_hydro[3] = {0, 0, 0}, force_holder[3] = {0, 0, 0};
+    double friction[3] = {0, 0, 0}, friction_prev[3] = {0, 0, 0};
+    double fdt_dead[3] = {0, 0, 0};
+    double g_dead = 0, nusselt = 0, density = 1.0;
+    double speed_c[3] = {0, 0, 0};   // speed_x/y/o evaluated (constants)
+    double force_o = 0;
+    int special = 0;
+    bool kutta = false;
+    int eq_forces_no = 0;
+    bool slip = false;
+    double slen = 0, area = 0, moi_cofm = 0, min_disc_r2 = 0;
+    V2 cofm{0, 0}, rect_bl{0, 0}, rect_tr{0, 0};
+    V2 axis() const { return V2{holder[0], holder[1]} + V2{dpos[0], dpos[1]}; }
+    double moi_axis() const { return moi_cofm + abs2(axis() - cofm) * area; }
+    static bool isrigid(double k) { return !(k >= 0 && std::isfinite(k)); }
+};
+
+struct CM {
+    double x, y, g, inv_eps;
+};
+
+struct Node {
+    double x = 0, y = 0, h = 0, w = 0;
+    int depth = 0;
+    int v0 = 0, v1 = 0;
+    std::vector<int> segs;
+    CM cmp{0, 0, 0, kInf}, cmm{0, 0, 0, kInf};   // snode ctor, TSortedTree.cpp:25
+    int ch1 = -1, parent = -1;
+    int leaf_index = -1;
+};
+
+}  // namespace
+
+struct vvo_ctx {
+    std::string err;
+    double re = kInf, dt = 1, inf_vx = 0, inf_vy = 0, inf_g = 0, time = 0;
+    std::vector<Obj> vort;
+    std::vector<Seg> segs;
+    std::vector<Body> bodies;
+    // tree
+    bool built = false;
+    int far_criteria = 8;
+    double min_node = 0, max_node = DBL_MAX;
+    std::vector<Node> nodes;
+    std::vector<int> leaves;                      // DFS order
+    std::vector<std::vector<int>> near, far;      // per leaf
+    // SLAE cache (TMatrix.cpp:225-259: inverse reused while the matrix is unchanged)
+    std::vector<double> inv_cache_src, inv_cache;
+    bool inv_cache_valid = false;
+};
+
+namespace {
+
+// ---------------------------------------------------------------- bodies --
+
+// TBody::doUpdateSegments (TBody.cpp:200-215)
+void update_segments(vvo_ctx* c, Body& b) {
+    for (int k = 0; k < b.n; k++) {
+        Seg& s = c->segs[b.first + k];
+        const Seg& nx = c->segs[b.first + (k + 1) % b.n];
+        s.dlx = nx.cx - s.cx;
+        s.dly = nx.cy - s.cy;
+        s.inv_eps = 3.0 / vabs(s.dl());
+        V2 r = 0.5 * (nx.corner() + s.corner());
+        s.x = r.x;
+        s.y = r.y;
+    }
+}
+
+// TBody::doFillProperties (TBody.cpp:283-343)
+void fill_properties(vvo_ctx* c, Body& b) {
+    b.slip = false;
+    b.slen = 0;
+    b.area = 0;
+    if (!b.n) return;
+    for (int k = 0; k < b.n; k++) {
+        const Seg& s = c->segs[b.first + k];
+        b.slip = b.slip || s.slip;
+        b.slen += vabs(s.dl());
+        b.area += s.y * s.dlx;
+    }
+    if (b.area == 0) {
+        V2 L{0., 0.};
+        for (int k = 0; k < b.n; k++) {
+            const Seg& s = c->segs[b.first + k];
+            V2 t = s.r() * vabs(s.dl());
+            L.x += t.x;
+            L.y += t.y;
+        }
+        b.cofm = L / b.slen;
+        b.moi_cofm = 0.;
+    } else {
+        V2 S3{0., 0.};
+        double moi12 = 0.;
+        for (int k = 0; k < b.n; k++) {
+            const Seg& s = c->segs[b.first + k];
+            V2 c1 = s.corner(), c2 = c->segs[b.first + (k + 1) % b.n].corner();
+            double cr = dot(rotl(c1), c2);
+            V2 t = s.r() * cr;
+            S3.x -= t.x;
+            S3.y -= t.y;
+            moi12 -= (abs2(c1) + dot(c1, c2) + abs2(c2)) * cr;
+        }
+        b.cofm = S3 / (3 * b.area);
+        b.moi_cofm = moi12 / 12. - b.area * abs2(b.cofm);
+    }
+    b.min_disc_r2 = 0;
+    b.rect_bl = {kInf, kInf};
+    b.rect_tr = {-kInf, -kInf};
+    for (int k = 0; k < b.n; k++) {
+        V2 cc = c->segs[b.first + k].corner();
+        double r2 = abs2(cc - b.cofm);
+        if (r2 > b.min_disc_r2) b.min_disc_r2 = r2;
+        if (cc.x > b.rect_tr.x) b.rect_tr.x = cc.x;
+        if (cc.y > b.rect_tr.y) b.rect_tr.y = cc.y;
+        if (cc.x < b.rect_bl.x) b.rect_bl.x = cc.x;
+        if (cc.y < b.rect_bl.y) b.rect_bl.y = cc.y;
+    }
+}
+
+// TBody::isPointInContour over the corner list (TBody.cpp:238-281)
+int point_invalid(vvo_ctx* c, const Body& b, V2 p) {
+    bool in = b.area <= 0;   // isInsideValid
+    if (!in && (p.x < b.rect_bl.x || p.y < b.rect_bl.y || p.x > b.rect_tr.x ||
+                p.y > b.rect_tr.y || abs2(p - b.cofm) > b.min_disc_r2))
+        return -1;
+    for (int i = 0, j = b.n - 1; i < b.n; j = i++) {
+        V2 vi = c->segs[b.first + i].corner(), vj = c->segs[b.first + j].corner();
+        if (((vi.y < vj.y) && (vi.y < p.y) && (p.y <= vj.y) &&
+             ((vj.y - vi.y) * (p.x - vi.x) > (vj.x - vi.x) * (p.y - vi.y))) ||
+            ((vi.y > vj.y) && (vi.y > p.y) && (p.y >= vj.y) &&
+             ((vj.y - vi.y) * (p.x - vi.x) < (vj.x - vi.x) * (p.y - vi.y))))
+            in = !in;
+    }
+    if (!in) return -1;
+    int best = -1;
+    double bd = std::numeric_limits<double>::max();
+    for (int k = 0; k < b.n; k++) {
+        double d2 = abs2(c->segs[b.first + k].r() - p);
+        if (d2 < bd) {
+            best = b.first + k;
+            bd = d2;
+        }
+    }
+    return best;
+}
+
+// ------------------------------------------------------------------ tree --
+
+inline bool goes_left(bool by_x, double px, double py, double sx, double sy) {
+    return by_x ? (px < sx) : (py < sy);
+}
+
+// snode::DistributeContent(range&...) (TSortedTree.cpp:81-99): Hoare partition
+// of [first, last) around the node centre; returns the split position.
+int hoare_split(Obj* a, int first, int last, bool by_x, double sx, double sy) {
+    int i = first, j = last;
+    while (i < j) {
+        while (i < last && (by_x ? (a[i].x < sx) : (a[i].y < sy))) i++;
+        do {
+            j--;
+        } while (j >= first && (by_x ? (a[j].x >= sx) : (a[j].y >= sy)));
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    return i;
+}
+
+// snode::Stretch (TSortedTree.cpp:101-137): tight box over the node's lists.
+void stretch(vvo_ctx* c, Node& nd) {
+    V2 tr{-std::numeric_limits<double>::max(), -std::numeric_limits<double>::max()};
+    V2 bl{std::numeric_limits<double>::max(), std::numeric_limits<double>::max()};
+    for (int s : nd.segs) {
+        const Seg& q = c->segs[s];
+        tr.x = std::max(tr.x, q.x);
+        tr.y = std::max(tr.y, q.y);
+        bl.x = std::min(bl.x, q.x);
+        bl.y = std::min(bl.y, q.y);
+    }
+    for (int k = nd.v0; k < nd.v1; k++) {
+        const Obj& o = c->vort[k];
+        tr.x = std::max(tr.x, o.x);
+        tr.y = std::max(tr.y, o.y);
+        bl.x = std::min(bl.x, o.x);
+        bl.y = std::min(bl.y, o.y);
+    }
+    nd.x = (bl + tr).x * 0.5;
+    nd.y = (bl + tr).y * 0.5;
+    nd.h = (tr - bl).y;
+    nd.w = (tr - bl).x;
+}
+
+// snode::DivideNode (TSortedTree.cpp:36-79), recursive, children created in
+// pairs; leaves appended in DFS order.
+void divide(vvo_ctx* c, int id) {
+    {
+        Node& nd = c->nodes[id];
+        if (std::max(nd.h, nd.w) < c->max_node && std::min(nd.h, nd.w) <= c->min_node) {
+            nd.leaf_index = (int)c->leaves.size();
+            c->leaves.push_back(id);
+            return;
+        }
+        long maxlist = std::max<long>((long)nd.segs.size(), (long)(nd.v1 - nd.v0));
+        if (std::max(nd.h, nd.w) < c->max_node && maxlist < kLeafMaxList) {
+            nd.leaf_index = (int)c->leaves.size();
+            c->leaves.push_back(id);
+            return;
+        }
+    }
+    int c1 = (int)c->nodes.size();
+    c->nodes.emplace_back();
+    c->nodes.emplace_back();
+    Node& nd = c->nodes[id];
+    Node& a = c->nodes[c1];
+    Node& b = c->nodes[c1 + 1];
+    nd.ch1 = c1;
+    a.parent = b.parent = id;
+    a.depth = b.depth = nd.depth + 1;
+    bool by_x = nd.h < nd.w;
+    int split = hoare_split(c->vort.data(), nd.v0, nd.v1, by_x, nd.x, nd.y);
+    a.v0 = nd.v0;
+    a.v1 = split;
+    b.v0 = split;
+    b.v1 = nd.v1;
+    for (int s : nd.segs) {   // TSortedTree.cpp:139-148 (stable)
+        if (goes_left(by_x, c->segs[s].x, c->segs[s].y, nd.x, nd.y))
+            a.segs.push_back(s);
+        else
+            b.segs.push_back(s);
+    }
+    nd.segs.clear();
+    stretch(c, a);
+    stretch(c, b);
+    divide(c, c1);
+    divide(c, c1 + 1);
+}
+
+// snode::CalculateCMass / FromScratch (TSortedTree.cpp:150-197)
+void calc_cmass(vvo_ctx* c, int id) {
+    Node& nd = c->nodes[id];
+    if (nd.ch1 < 0) {
+        double px = 0, py = 0, pg = 0, mx = 0, my = 0, mg = 0;
+        for (int k = nd.v0; k < nd.v1; k++) {
+            const Obj& o = c->vort[k];
+            if (o.g > 0) {
+                px += o.x * o.g;
+                py += o.y * o.g;
+                pg += o.g;
+            } else {
+                mx += o.x * o.g;
+                my += o.y * o.g;
+                mg += o.g;
+            }
+        }
+        if (pg) {
+            px *= (1. / pg);
+            py *= (1. / pg);
+        } else {
+            px = nd.x;
+            py = nd.y;
+        }
+        if (mg) {
+            mx *= (1. / mg);
+            my *= (1. / mg);
+        } else {
+            mx = nd.x;
+            my = nd.y;
+        }
+        nd.cmp = {px, py, pg, kInf};
+        nd.cmm = {mx, my, mg, kInf};
+        return;
+    }
+    calc_cmass(c, nd.ch1);
+    calc_cmass(c, nd.ch1 + 1);
+    Node& n2 = c->nodes[id];
+    const Node& a = c->nodes[n2.ch1];
+    const Node& b = c->nodes[n2.ch1 + 1];
+    auto combine = [&](CM& cm, const CM& ca, const CM& cb) {
+        double sumg = ca.g + cb.g;
+        if (sumg) {
+            double inv = 1. / sumg;
+            cm.x = (ca.x * ca.g + cb.x * cb.g) * inv;
+            cm.y = (ca.y * ca.g + cb.y * cb.g) * inv;
+            cm.g = sumg;
+        } else {
+            cm = {n2.x, n2.y, 0, 0};   // TObj(x, y, 0): _1_eps = 0
+        }
+    };
+    combine(n2.cmp, a.cmp, b.cmp);
+    combine(n2.cmm, a.cmm, b.cmm);
+}
+
+// snode::FindNearNodes (TSortedTree.cpp:199-217)
+void find_near(vvo_ctx* c, int leaf, int top, std::vector<int>& nearv, std::vector<int>& farv) {
+    const Node& L = c->nodes[leaf];
+    const Node& T = c->nodes[top];
+    V2 dr = V2{T.x, T.y} - V2{L.x, L.y};
+    double half_perim = T.h + T.w + L.h + L.w;
+    if (abs2(dr) > c->far_criteria * half_perim * half_perim) {
+        farv.push_back(top);
+        return;
+    }
+    if (T.ch1 >= 0) {
+        find_near(c, leaf, T.ch1, nearv, farv);
+        find_near(c, leaf, T.ch1 + 1, nearv, farv);
+        return;
+    }
+    nearv.push_back(top);
+}
+
+int find_node(vvo_ctx* c, V2 p) {   // stree::findNode (TSortedTree.cpp:284-303)
+    if (!c->built || c->nodes.empty()) return -1;
+    int id = 0;
+    while (c->nodes[id].ch1 >= 0) {
+        const Node& n = c->nodes[id];
+        if (n.h < n.w)
+            id = (p.x < n.x) ? n.ch1 : n.ch1 + 1;
+        else
+            id = (p.y < n.y) ? n.ch1 : n.ch1 + 1;
+    }
+    return id;
+}
+
+// --------------------------------------------------------------- epsilon --
+
+struct EpsCtx {
+    vvo_ctx* c;
+    int merged = 0;
+    int32_t* merges;
+    int32_t max_merges;
+    int last_n1 = -1, last_n2 = -1;
+};
+
+// MEpsilonFast::MergeVortexes (MEpsilonFast.cpp:111-126)
+void merge_pair(EpsCtx& e, int i, int j) {
+    Obj* a = &e.c->vort[i];
+    Obj* b = &e.c->vort[j];
+    if (e.merges && e.merged < e.max_merges) {
+        e.merges[2 * e.merged] = i;
+        e.merges[2 * e.merged + 1] = j;
+    }
+    e.merged++;
+    if (sgn(a->g) == sgn(b->g)) {
+        double inv = 1. / (a->g + b->g);
+        double nx = (a->x * a->g + b->x * b->g) * inv;
+        double ny = (a->y * a->g + b->y * b->g) * inv;
+        a->x = nx;
+        a->y = ny;
+    } else if (std::fabs(a->g) < std::fabs(b->g)) {
+        a->x = b->x;
+        a->y = b->y;
+    }
+    a->g += b->g;
+    b->g = 0;
+}
+
+// MEpsilonFast::epsv (MEpsilonFast.cpp:128-173)
+double epsv(EpsCtx& e, int leaf_node, int i, double crit_sq) {
+    vvo_ctx* c = e.c;
+    double r1 = std::numeric_limits<double>::max(), r2 = r1;
+    int n1 = -1, n2 = -1;
+    const Obj& t = c->vort[i];
+    const Node& L = c->nodes[leaf_node];
+    for (int m : c->near[L.leaf_index]) {
+        const Node& M = c->nodes[m];
+        for (int j = M.v0; j < M.v1; j++) {
+            const Obj& s = c->vort[j];
+            if (!s.g || j == i) continue;
+            double d2 = abs2(V2{t.x, t.y} - V2{s.x, s.y});
+            if (r1 > d2) {
+                r2 = r1;
+                n2 = n1;
+                r1 = d2;
+                n1 = j;
+            } else if (r2 > d2) {
+                r2 = d2;
+                n2 = j;
+            }
+        }
+    }
+    e.last_n1 = n1;
+    e.last_n2 = n2;
+    if (n1 < 0) return std::numeric_limits<double>::min();
+    if (n2 < 0) return std::sqrt(r1);
+    if (std::isnan(crit_sq)) return std::sqrt(r2);
+    int s1 = (int)sgn(c->vort[n1].g), s2 = (int)sgn(c->vort[n2].g), st = (int)sgn(t.g);
+    if ((r1 < crit_sq) || ((s1 == s2) && (s1 != st))) {
+        merge_pair(e, i, n1);
+        return epsv(e, leaf_node, i, kNaN);
+    }
+    return std::sqrt(r2);
+}
+
+// MEpsilonFast::nearestBodySegment (MEpsilonFast.cpp:214-253)
+int nearest_segment(vvo_ctx* c, int leaf_node, V2 p) {
+    int att = -1;
+    double res = std::numeric_limits<double>::max();
+    const Node& L = c->nodes[leaf_node];
+    for (int m : c->near[L.leaf_index]) {
+        for (int s : c->nodes[m].segs) {
+            double d2 = abs2(p - c->segs[s].r());
+            if (d2 < res) {
+                res = d2;
+                att = s;
+            }
+        }
+    }
+    if (att >= 0) return att;
+    for (const Body& b : c->bodies) {
+        for (int k = 0; k < b.n; k++) {
+            double d2 = abs2(p - c->segs[b.first + k].r());
+            if (d2 < res) {
+                res = d2;
+                att = b.first + k;
+            } else {
+                k += 9;   // "speed up" skip of the reference
+            }
+        }
+    }
+    return att;
+}
+
+// ------------------------------------------------------------ convective --
+
+// MConvectiveFast::biot_savart (MConvectiveFast.cpp:116-120)
+inline V2 biot_savart(double ox, double oy, double og, double oinv_eps, V2 p) {
+    V2 dr = p - V2{ox, oy};
+    return rotl(dr) * (og / (abs2(dr) + sq(1. / oinv_eps)));
+}
+
+// MConvectiveFast::near_nodes_influence (MConvectiveFast.cpp:122-137)
+V2 near_influence(vvo_ctx* c, const Node& L, V2 p) {
+    V2 res{0, 0};
+    for (int m : c->near[L.leaf_index]) {
+        const Node& M = c->nodes[m];
+        for (int j = M.v0; j < M.v1; j++) {
+            const Obj& s = c->vort[j];
+            if (!s.g) continue;
+            V2 b = biot_savart(s.x, s.y, s.g, s.inv_eps, p);
+            res.x += b.x;
+            res.y += b.y;
+        }
+    }
+    res = res * k1_2Pi;
+    return res;
+}
+
+// MConvectiveFast::far_nodes_influence (MConvectiveFast.cpp:139-151)
+V2 far_influence(vvo_ctx* c, const Node& L, V2 p) {
+    V2 res{0, 0};
+    for (int f : c->far[L.leaf_index]) {
+        const Node& F = c->nodes[f];
+        V2 b = biot_savart(F.cmp.x, F.cmp.y, F.cmp.g, F.cmp.inv_eps, p);
+        res.x += b.x;
+        res.y += b.y;
+        b = biot_savart(F.cmm.x, F.cmm.y, F.cmm.g, F.cmm.inv_eps, p);
+        res.x += b.x;
+        res.y += b.y;
+    }
+    res = res * k1_2Pi;
+    return res;
+}
+
+// MConvectiveFast::SegmentInfluence_linear_source (MConvectiveFast.cpp:440-457)
+V2 seg_linear_source(V2 p, const Seg& s, double q1, double q2) {
+    typedef std::complex<double> cd;
+    cd z(p.x, p.y), zc(s.x, s.y), dz(s.dlx, s.dly);
+    cd zs1 = zc - dz * 0.5, zs2 = zc + dz * 0.5;
+    cd z1 = z - zs1, z2 = z - zs2;
+    cd zV = ((q2 - q1) - std::conj(q2 * z1 - q1 * z2) / std::conj(dz) *
+                             std::log(std::conj(z2) / std::conj(z1))) /
+            std::conj(dz);
+    return {zV.real(), zV.imag()};
+}
+
+// MConvectiveFast::body_list_influence (MConvectiveFast.cpp:172-215)
+V2 body_influence(vvo_ctx* c, V2 p) {
+    V2 res{0, 0};
+    for (const Body& b : c->bodies) {
+        if (b.slip) {
+            for (int k = 0; k < b.n; k++) {
+                const Seg& s = c->segs[b.first + k];
+                if (s.slip) {
+                    V2 t = biot_savart(s.x, s.y, s.g, s.inv_eps, p);
+                    res.x += t.x;
+                    res.y += t.y;
+                }
+            }
+        }
+        V2 sr{b.speed_slae[0], b.speed_slae[1]};
+        double so = b.speed_slae[2];
+        if (!(std::fabs(sr.x) + std::fabs(sr.y) + std::fabs(so) < 1E-10)) {
+            for (int k = 0; k < b.n; k++) {
+                const Seg& s = c->segs[b.first + k];
+                double d2 = abs2(p - s.r());
+                if (d2 < abs2(s.dl())) {
+                    V2 vs1 = sr + so * rotl(s.corner() - b.axis());
+                    double g1 = -dot(vs1, s.dl());
+                    double q1 = -dot(rotl(vs1), s.dl());
+                    V2 vs2 = sr + so * rotl(s.corner() + s.dl() - b.axis());
+                    double g2 = -dot(vs2, s.dl());
+                    double q2 = -dot(rotl(vs2), s.dl());
+                    V2 t = rotl(seg_linear_source(p, s, g1, g2));
+                    res.x += t.x;
+                    res.y += t.y;
+                    t = seg_linear_source(p, s, q1, q2);
+                    res.x += t.x;
+                    res.y += t.y;
+                } else {
+                    V2 vs = sr + so * rotl(s.r() - b.axis());
+                    double g = -dot(vs, s.dl());
+                    double q = -dot(rotl(vs), s.dl());
+                    V2 dr = p - s.r();
+                    V2 t = (dr * q + rotl(dr) * g) / d2;
+                    res.x += t.x;
+                    res.y += t.y;
+                }
+            }
+        }
+    }
+    return res * k1_2Pi;
+}
+
+// Per-leaf body of MConvectiveFast::process_all_lists (MConvectiveFast.cpp:41-113)
+void convective_leaf(vvo_ctx* c, int leaf_idx) {
+    const Node& L = c->nodes[c->leaves[leaf_idx]];
+    double T1 = 0, T2 = 0, T3 = 0, T4 = 0;
+    for (int f : c->far[leaf_idx]) {
+        const Node& F = c->nodes[f];
+        V2 dP = V2{L.x, L.y} - V2{F.cmp.x, F.cmp.y};
+        V2 dM = V2{L.x, L.y} - V2{F.cmm.x, F.cmm.y};
+        double fP1 = F.cmp.g / abs2(dP);
+        double fM1 = F.cmm.g / abs2(dM);
+        double fP2 = F.cmp.g / sq(abs2(dP));
+        double fM2 = F.cmm.g / sq(abs2(dM));
+        T1 -= (fP1 * dP.y + fM1 * dM.y);
+        T2 += (fP1 * dP.x + fM1 * dM.x);
+        T3 += (fP2 * dP.y * dP.x + fM2 * dM.y * dM.x);
+        T4 += (fP2 * (sq(dP.y) - sq(dP.x)) + fM2 * (sq(dM.y) - sq(dM.x)));
+    }
+    T1 *= k1_2Pi;
+    T2 *= k1_2Pi;
+    T3 *= k1_Pi;
+    T4 *= k1_2Pi;
+    V2 centre{L.x, L.y};
+    V2 inf{c->inf_vx, c->inf_vy};
+    for (int i = L.v0; i < L.v1; i++) {
+        Obj& o = c->vort[i];
+        if (!o.g) continue;
+        V2 p{o.x, o.y};
+        V2 drl = p - centre;
+        double* v = &o.vx;
+        addto(v, inf);
+        addto(v, near_influence(c, L, p));
+        addto(v, V2{0, 0} * k1_2Pi);   // sink_list_influence: no sinks
+        addto(v, body_influence(c, p));
+        addto(v, V2{T1, T2});
+        addto(v, V2{dot(V2{T3, T4}, drl), dot(V2{T4, -T3}, drl)});
+    }
+}
+
+// ------------------------------------------------------------- diffusive --
+
+// MDiffusiveFast::vortex_influence (MDiffusiveFast.cpp:93-105)
+inline void vortex_influence(const Obj& v, const Obj& vj, V2& i2, double& i1) {
+    if (sgn(v.g) != sgn(vj.g)) return;
+    V2 dr = V2{v.x, v.y} - V2{vj.x, vj.y};
+    if (iszero(dr)) return;
+    double drabs = vabs(dr);
+    double exparg = -drabs * v.inv_eps;
+    if (exparg < kExpArgLimit) return;
+    double i1tmp = vj.g * std::exp(exparg);
+    V2 t = dr * (i1tmp / drabs);
+    i2.x += t.x;
+    i2.y += t.y;
+    i1 += i1tmp;
+}
+
+// MDiffusiveFast::segment_influence (MDiffusiveFast.cpp:107-123)
+inline void segment_influence(const Obj& v, Seg& rk, V2& i3, double& i0, double* fric) {
+    V2 dr = V2{v.x, v.y} - rk.r();
+    double drabs2 = abs2(dr);
+    double drabs = std::sqrt(drabs2);
+    double exparg = -drabs * v.inv_eps;
+    if (exparg < kExpArgLimit) return;
+    double expres = std::exp(exparg);
+    V2 dS = rotl(rk.dl());
+    i3.x += dS.x * expres;
+    i3.y += dS.y * expres;
+    i0 += (drabs * v.inv_eps + 1) / drabs2 * dot(dr, dS) * expres;
+    *fric += sq(v.inv_eps) * v.g * expres * vabs(dS);
+}
+
+// Per-leaf body of MDiffusiveFast::process_vort_list (MDiffusiveFast.cpp:15-47);
+// fric contributions go to fric[] (the segments' own field when serial).
+void diffusive_leaf(vvo_ctx* c, int leaf_idx, double* fric) {
+    const Node& L = c->nodes[c->leaves[leaf_idx]];
+    const double re = c->re;
+    for (int i = L.v0; i < L.v1; i++) {
+        Obj& o = c->vort[i];
+        if (!o.g) continue;
+        V2 S2{0, 0}, S3{0, 0};
+        double S1 = 0, S0 = 0;
+        for (int m : c->near[leaf_idx]) {
+            const Node& M = c->nodes[m];
+            for (int j = M.v0; j < M.v1; j++) {
+                const Obj& s = c->vort[j];
+                if (!s.g) continue;
+                vortex_influence(o, s, S2, S1);
+            }
+            for (int sidx : M.segs) segment_influence(o, c->segs[sidx], S3, S0, &fric[sidx]);
+        }
+        if ((sgn(S1) != sgn(o.g)) || (std::fabs(S1) < std::fabs(0.1 * o.g))) S1 = 0.1 * o.g;
+        V2 t = S2 * (o.inv_eps / (re * S1));
+        o.vx += t.x;
+        o.vy += t.y;
+        if (S0 > kPi) S0 = kPi;
+        t = S3 * (sq(o.inv_eps) / (re * (k2Pi - S0)));
+        o.vx += t.x;
+        o.vy += t.y;
+    }
+}
+
+// ------------------------------------------------------------------ SLAE --
+
+// MConvectiveFast::_2PI_Xi_g_near / _dist / _2PI_Xi_g (MConvectiveFast.cpp:275-310)
+inline double xi_g_near(V2 p, V2 pc, V2 dl, double rd) { return dot(pc - p, dl) / sq(rd); }
+inline double xi_g_dist(V2 p, V2 p1, V2 p2) { return 0.5 * std::log(abs2(p - p2) / abs2(p - p1)); }
+inline double xi_q_near(V2 p, V2 pc, V2 dl, double rd) { return dot(pc - p, rotl(dl)) / sq(rd); }
+inline double xi_q_dist(V2 p, V2 p1, V2 p2) {
+    V2 d1 = p - p1, d2 = p - p2;
+    return std::atan2(dot(d1, rotl(d2)), dot(d1, d2));
+}
+
+double xi_g(V2 p, const Seg& s, double rd) {
+    double rd_sqr = sq(rd);
+    double dr1 = abs2(p - s.corner());
+    double dr2 = abs2(p - s.corner() - s.dl());
+    if (dr1 >= rd_sqr && dr2 >= rd_sqr) return 0.5 * std::log(dr2 / dr1);
+    if (dr1 <= rd_sqr && dr2 <= rd_sqr) return xi_g_near(p, s.r(), s.dl(), rd);
+    double a0 = abs2(s.dl());
+    double b0 = dot(p - s.r(), s.dl());
+    double d = std::sqrt(b0 * b0 - a0 * (abs2(p - s.r()) - rd * rd));
+    double k = (b0 + d) / a0;
+    if ((k <= -0.5) || (k >= 0.5)) k = (b0 - d) / a0;
+    V2 p3 = s.r() + k * s.dl();
+    if (dr1 < rd_sqr)
+        return xi_g_near(p, 0.5 * (p3 + s.corner()), (p3 - s.corner()), rd) +
+               xi_g_dist(p, p3, s.corner() + s.dl());
+    return xi_g_dist(p, s.corner(), p3) +
+           xi_g_near(p, 0.5 * (s.corner() + s.dl() + p3), (s.corner() + s.dl() - p3), rd);
+}
+
+// MConvectiveFast::_2PI_Xi (MConvectiveFast.cpp:324-373); the &p == &seg.r
+// shift applies when p is the segment's own centre.
+V2 xi_vec(V2 p, const Seg& s, double rd, bool p_is_seg_centre) {
+    if (p_is_seg_centre) return xi_vec(p + rotl(s.dl()) * 0.001, s, rd, false);
+    double rd_sqr = sq(rd);
+    V2 p1 = s.corner(), p2 = s.corner() + s.dl();
+    double dr1 = abs2(p - p1), dr2 = abs2(p - p2);
+    if (dr1 >= rd_sqr && dr2 >= rd_sqr) {
+        V2 dp1 = p - p1, dp2 = p - p2;
+        return {0.5 * std::log(dr2 / dr1), std::atan2(dot(dp1, rotl(dp2)), dot(dp1, dp2))};
+    }
+    if (dr1 <= rd_sqr && dr2 <= rd_sqr) {
+        V2 dp = p - s.r();
+        return V2{dot(dp, s.dl()), dot(dp, rotl(s.dl()))} / sq(rd);
+    }
+    double a0 = abs2(s.dl());
+    double b0 = dot(p - s.r(), s.dl());
+    double d = std::sqrt(b0 * b0 - a0 * (abs2(p - s.r()) - rd * rd));
+    double k = (b0 + d) / a0;
+    if ((k <= -0.5) || (k >= 0.5)) k = (b0 - d) / a0;
+    V2 p3 = s.r() + k * s.dl();
+    if (dr1 < rd_sqr)
+        return V2{xi_g_near(p, 0.5 * (p1 + p3), p3 - p1, rd), xi_q_near(p, 0.5 * (p1 + p3), p3 - p1, rd)} +
+               V2{xi_g_dist(p, p3, p2), xi_q_dist(p, p3, p2)};
+    return V2{xi_g_dist(p, p1, p3), xi_q_dist(p, p1, p3)} +
+           V2{xi_g_near(p, 0.5 * (p3 + p2), p2 - p3, rd), xi_q_near(p, 0.5 * (p3 + p2), p2 - p3, rd)};
+}
+
+// MConvectiveFast::NodeInfluence (MConvectiveFast.cpp:398-418)
+double node_influence(vvo_ctx* c, int node, const Seg& s) {
+    double res = 0;
+    const Node& L = c->nodes[node];
+    for (int m : c->near[L.leaf_index]) {
+        const Node& M = c->nodes[m];
+        for (int j = M.v0; j < M.v1; j++) {
+            const Obj& o = c->vort[j];
+            if (!o.g) continue;
+            res += xi_g(V2{o.x, o.y}, s, 1. / o.inv_eps) * o.g;
+        }
+    }
+    for (int f : c->far[L.leaf_index]) {
+        const Node& F = c->nodes[f];
+        res += xi_g_dist(V2{F.cmp.x, F.cmp.y}, s.corner(), s.corner() + s.dl()) * F.cmp.g;
+        res += xi_g_dist(V2{F.cmm.x, F.cmm.y}, s.corner(), s.corner() + s.dl()) * F.cmm.g;
+    }
+    return res * k1_2Pi;
+}
+
+// Partial-pivot LU with row interchanges (the algorithm of LAPACK dgetrf/dgesv,
+// unblocked).  a is row-major n x n; returns 0 or the 1-based singular column.
+int lu_factor(std::vector<double>& a, int n, std::vector<int>& piv) {
+    piv.assign(n, 0);
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = std::fabs(a[(size_t)k * n + k]);
+        for (int i = k + 1; i < n; i++) {
+            double v = std::fabs(a[(size_t)i * n + k]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        piv[k] = p;
+        if (a[(size_t)p * n + k] == 0) return k + 1;
+        if (p != k)
+            for (int j = 0; j < n; j++) std::swap(a[(size_t)k * n + j], a[(size_t)p * n + j]);
+        double inv = 1. / a[(size_t)k * n + k];
+        for (int i = k + 1; i < n; i++) {
+            double l = a[(size_t)i * n + k] * inv;
+            a[(size_t)i * n + k] = l;
+            if (l != 0)
+                for (int j = k + 1; j < n; j++) a[(size_t)i * n + j] -= l * a[(size_t)k * n + j];
+        }
+    }
+    return 0;
+}
+
+void lu_solve(const std::vector<double>& a, int n, const std::vector<int>& piv, double* b) {
+    for (int k = 0; k < n; k++)
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+    for (int i = 0; i < n; i++) {
+        double s = b[i];
+        for (int j = 0; j < i; j++) s -= a[(size_t)i * n + j] * b[j];
+        b[i] = s;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++) s -= a[(size_t)i * n + j] * b[j];
+        b[i] = s / a[(size_t)i * n + i];
+    }
+}
+
+struct Slae {
+    vvo_ctx* c;
+    int n;
+    std::vector<double> A, rhs;
+    std::vector<int> base;   // first column of each body
+    double& cell(int row, int col) { return A[(size_t)row * n + col]; }
+    int col_g(int b, int k) const { return base[b] + k; }
+    int col_speed(int b, int comp) const { return base[b] + c->bodies[b].n + comp; }
+    int col_hydro(int b, int comp) const { return base[b] + c->bodies[b].n + 3 + comp; }
+    int col_holder(int b, int comp) const { return base[b] + c->bodies[b].n + 6 + comp; }
+};
+
+// MConvectiveFast::_2PI_A123 (MConvectiveFast.cpp:375-396)
+void a123(Slae& S, const Seg& seg, int ibody, int b, double* A1, double* A2, double* A3) {
+    vvo_ctx* c = S.c;
+    const Body& B = c->bodies[b];
+    *A1 = (ibody == b) ? k2Pi * seg.dly : 0;
+    *A2 = (ibody == b) ? -k2Pi * seg.dlx : 0;
+    *A3 = 0;
+    if (Body::isrigid(B.kspring[2]) && (B.speed_c[2] == 0)) return;   // no root bodies here
+    for (int k = 0; k < B.n; k++) {
+        const Seg& l = c->segs[B.first + k];
+        bool self = (&l == &seg);
+        V2 Xi = xi_vec(l.r(), seg, vabs(l.dl()) * 0.25, self);
+        V2 r0 = l.r() - B.axis();
+        *A3 -= dot(Xi, V2{dot(rotl(r0), l.dl()), -dot(l.dl(), r0)});
+    }
+}
+
+void fill_matrix(Slae& S) {
+    vvo_ctx* c = S.c;
+    std::fill(S.A.begin(), S.A.end(), 0.);
+    std::fill(S.rhs.begin(), S.rhs.end(), 0.);
+    const double t = c->time;
+    const double dt = c->dt;
+    int eq = 0;
+    int once = 0;
+    double gsum_all = 0;
+    for (const Obj& o : c->vort) gsum_all += o.g;   // Space::gsum (TSpace.hpp:93-99)
+    for (int b = 0; b < (int)c->bodies.size(); b++) {
+        Body& B = c->bodies[b];
+        for (int k = 0; k < B.n; k++) {
+            const Seg& seg = c->segs[B.first + k];
+            if (k != B.special) {
+                // fillSlipEquationForSegment (MConvectiveFast.cpp:459-482)
+                S.rhs[eq] = dot(rotl(V2{c->inf_vx, c->inf_vy}), seg.dl());
+                S.rhs[eq] += dot(rotl(V2{0, 0} * k1_2Pi), seg.dl());
+                int node = find_node(c, seg.r());
+                S.rhs[eq] -= node_influence(c, node, seg);
+                for (int jb = 0; jb < (int)c->bodies.size(); jb++) {
+                    const Body& J = c->bodies[jb];
+                    for (int m = 0; m < J.n; m++) {
+                        const Seg& l = c->segs[J.first + m];
+                        S.cell(eq, S.col_g(jb, m)) = xi_g(l.corner(), seg, vabs(l.dl()) * 0.25) * k1_2Pi;
+                    }
+                    double A1, A2, A3;
+                    a123(S, seg, b, jb, &A1, &A2, &A3);
+                    S.cell(eq, S.col_speed(jb, 0)) = A1 * k1_2Pi;
+                    S.cell(eq, S.col_speed(jb, 1)) = A2 * k1_2Pi;
+                    S.cell(eq, S.col_speed(jb, 2)) = A3 * k1_2Pi;
+                }
+            } else if (B.kutta) {
+                S.rhs[eq] = 0;   // fillZeroEquationForSegment (:484-492)
+                S.cell(eq, S.col_g(b, k)) = 1;
+            } else if (!once++) {
+                // fillInfSteadyEquationForSegment (:508-522)
+                S.rhs[eq] = -gsum_all - c->inf_g;
+                for (int jb = 0; jb < (int)c->bodies.size(); jb++) {
+                    const Body& J = c->bodies[jb];
+                    for (int m = 0; m < J.n; m++) S.cell(eq, S.col_g(jb, m)) = 1;
+                    S.cell(eq, S.col_speed(jb, 2)) = 2 * J.area;
+                }
+            } else {
+                // fillSteadyEquationForSegment (:494-506)
+                S.rhs[eq] = B.g_dead + 2 * B.area * B.speed_slae_prev[2];
+                for (int m = 0; m < B.n; m++) S.cell(eq, S.col_g(b, m)) = 1;
+                S.cell(eq, S.col_speed(b, 2)) = 2 * B.area;
+            }
+            eq++;
+        }
+        // Speed / Hooke rows (:916-966); collisions are not modelled.
+        for (int comp = 0; comp < 3; comp++) {
+            if (Body::isrigid(B.kspring[comp]) || !(t > 0)) {
+                S.rhs[eq] = B.speed_c[comp];   // fillSpeed?Equation (:803-862), no root body
+                S.cell(eq, S.col_speed(b, comp)) = 1;
+            } else {
+                // fillHooke?Equation (:750-792)
+                S.rhs[eq] = B.kspring[comp] * B.dpos[comp] + (comp == 2 ? B.force_o : 0.);
+                S.cell(eq, S.col_speed(b, comp)) = -B.damping[comp];
+                S.cell(eq, S.col_holder(b, comp)) = 1;
+            }
+            eq++;
+        }
+        const double _1_dt = 1 / dt;
+        const double _1_2dt = 0.5 / dt;
+        const V2 rcc = B.cofm - B.axis();
+        const double area = B.area;
+        const double* sp = B.speed_slae_prev;
+        // fillHydroXEquation (:533-562)
+        S.rhs[eq] = +area * _1_dt * sp[0] + area * _1_dt * rotl(2 * B.cofm + rcc).x * sp[2] +
+                    sq(sp[2]) * area * rcc.x - B.friction_prev[0] + B.fdt_dead[0] * _1_dt;
+        for (int m = 0; m < B.n; m++) S.cell(eq, S.col_g(b, m)) = _1_dt * (-c->segs[B.first + m].cy);
+        S.cell(eq, S.col_speed(b, 0)) = area * _1_dt;
+        S.cell(eq, S.col_speed(b, 1)) = 0;
+        S.cell(eq, S.col_speed(b, 2)) = area * _1_dt * (-2 * B.cofm.y - rcc.y);
+        S.cell(eq, S.col_hydro(b, 0)) = -1;
+        eq++;
+        // fillHydroYEquation (:564-593)
+        S.rhs[eq] = +area * _1_dt * sp[1] + area * _1_dt * rotl(2 * B.cofm + rcc).y * sp[2] +
+                    sq(sp[2]) * area * rcc.y - B.friction_prev[1] + B.fdt_dead[1] * _1_dt;
+        for (int m = 0; m < B.n; m++) S.cell(eq, S.col_g(b, m)) = _1_dt * (c->segs[B.first + m].cx);
+        S.cell(eq, S.col_speed(b, 0)) = 0;
+        S.cell(eq, S.col_speed(b, 1)) = area * _1_dt;
+        S.cell(eq, S.col_speed(b, 2)) = area * _1_dt * (2 * B.cofm.x + rcc.x);
+        S.cell(eq, S.col_hydro(b, 1)) = -1;
+        eq++;
+        // fillHydroOEquation (:595-623)
+        S.rhs[eq] = +dot(rotl(rcc) * (area * _1_dt), V2{sp[0], sp[1]}) + 2 * B.moi_axis() * _1_dt * sp[2] -
+                    B.friction_prev[2] + B.fdt_dead[2] * _1_2dt;
+        for (int m = 0; m < B.n; m++)
+            S.cell(eq, S.col_g(b, m)) = _1_2dt * abs2(c->segs[B.first + m].corner() - B.axis());
+        S.cell(eq, S.col_speed(b, 0)) = area * _1_dt * (-rcc.y);
+        S.cell(eq, S.col_speed(b, 1)) = area * _1_dt * (rcc.x);
+        S.cell(eq, S.col_speed(b, 2)) = 2 * B.moi_axis() * _1_dt;
+        S.cell(eq, S.col_hydro(b, 2)) = -1;
+        eq++;
+        const double rho = B.density;
+        // fillNewtonXEquation (:634-667); gravity = 0, no child bodies
+        S.rhs[eq] = -area * _1_dt * rho * sp[0] + area * _1_dt * rho * rcc.y * sp[2] -
+                    area * rho * rcc.x * sq(sp[2]) - (rho - 1.0) * 0. * area;
+        S.cell(eq, S.col_speed(b, 0)) = -area * _1_dt * rho;
+        S.cell(eq, S.col_speed(b, 1)) = 0;
+        S.cell(eq, S.col_speed(b, 2)) = area * _1_dt * rho * rcc.y;
+        S.cell(eq, S.col_hydro(b, 0)) = 1;
+        S.cell(eq, S.col_holder(b, 0)) = -1;
+        eq++;
+        // fillNewtonYEquation (:669-702)
+        S.rhs[eq] = -area * _1_dt * rho * sp[1] - area * _1_dt * rho * rcc.x * sp[2] -
+                    area * rho * rcc.y * sq(sp[2]) - (rho - 1.0) * 0. * area;
+        S.cell(eq, S.col_speed(b, 0)) = 0;
+        S.cell(eq, S.col_speed(b, 1)) = -area * _1_dt * rho;
+        S.cell(eq, S.col_speed(b, 2)) = -area * _1_dt * rho * rcc.x;
+        S.cell(eq, S.col_hydro(b, 1)) = 1;
+        S.cell(eq, S.col_holder(b, 1)) = -1;
+        eq++;
+        // fillNewtonOEquation (:704-739)
+        S.rhs[eq] = -area * _1_dt * rho * dot(rotl(rcc), V2{sp[0], sp[1]}) - B.moi_axis() * _1_dt * rho * sp[2] -
+                    (rho - 1.0) * dot(rotl(rcc), V2{0, 0}) * area;
+        S.cell(eq, S.col_speed(b, 0)) = +area * _1_dt * rho * rcc.y;
+        S.cell(eq, S.col_speed(b, 1)) = -area * _1_dt * rho * rcc.x;
+        S.cell(eq, S.col_speed(b, 2)) = -B.moi_axis() * _1_dt * rho;
+        S.cell(eq, S.col_hydro(b, 2)) = 1;
+        S.cell(eq, S.col_holder(b, 2)) = -1;
+        eq++;
+    }
+}
+
+}  // namespace
+
+// =================================================================== C API
+
+extern "C" {
+
+vvo_ctx* vvo_create(void) { return new vvo_ctx(); }
+void vvo_destroy(vvo_ctx* c) { delete c; }
+const char* vvo_last_error(vvo_ctx* c) { return c->err.c_str(); }
+
+void vvo_set_params(vvo_ctx* c, double re, double dt, double inf_vx, double inf_vy, double inf_g, double time) {
+    c->re = re;
+    c->dt = dt;
+    c->inf_vx = inf_vx;
+    c->inf_vy = inf_vy;
+    c->inf_g = inf_g;
+    c->time = time;
+}
+
+void vvo_set_vortices(vvo_ctx* c, const double* objs6, int64_t n) {
+    c->built = false;
+    c->vort.resize((size_t)n);
+    if (n) std::memcpy(c->vort.data(), objs6, (size_t)n * sizeof(Obj));
+}
+int64_t vvo_num_vortices(vvo_ctx* c) { return (int64_t)c->vort.size(); }
+void vvo_get_vortices(vvo_ctx* c, double* objs6) {
+    if (!c->vort.empty()) std::memcpy(objs6, c->vort.data(), c->vort.size() * sizeof(Obj));
+}
+
+int vvo_add_body(vvo_ctx* c, const vvo_corner* corners, int32_t n) {
+    Body b;
+    b.first = (int)c->segs.size();
+    b.n = n;
+    for (int k = 0; k < n; k++) {
+        Seg s;
+        std::memset(&s, 0, sizeof(s));
+        s.cx = corners[k].x;
+        s.cy = corners[k].y;
+        s.slip = corners[k].slip;
+        c->segs.push_back(s);
+    }
+    c->bodies.push_back(b);
+    Body& bb = c->bodies.back();
+    update_segments(c, bb);
+    fill_properties(c, bb);
+    // Space::EnumerateBodies (TSpace.cpp:189-203)
+    int eq = 0;
+    for (Body& x : c->bodies) {
+        for (int k = 0; k < x.n; k++) c->segs[x.first + k].eq_no = (uint32_t)eq++;
+        x.eq_forces_no = eq;
+        eq += 9;
+    }
+    c->built = false;
+    return 0;
+}
+int32_t vvo_num_bodies(vvo_ctx* c) { return (int32_t)c->bodies.size(); }
+int32_t vvo_num_segments(vvo_ctx* c) { return (int32_t)c->segs.size(); }
+void vvo_get_segments(vvo_ctx* c, vvo_seg_state* out) {
+    if (!c->segs.empty()) std::memcpy(out, c->segs.data(), c->segs.size() * sizeof(Seg));
+}
+void vvo_set_segments(vvo_ctx* c, const vvo_seg_state* in) {
+    if (!c->segs.empty()) std::memcpy(c->segs.data(), in, c->segs.size() * sizeof(Seg));
+}
+
+void vvo_get_body(vvo_ctx* c, int32_t bi, vvo_body_state* o) {
+    const Body& b = c->bodies[bi];
+    std::memset(o, 0, sizeof(*o));
+    std::memcpy(o->holder, b.holder, sizeof(o->holder));
+    std::memcpy(o->dpos, b.dpos, sizeof(o->dpos));
+    std::memcpy(o->speed_slae, b.speed_slae, sizeof(o->speed_slae));
+    std::memcpy(o->speed_slae_prev, b.speed_slae_prev, sizeof(o->speed_slae_prev));
+    std::memcpy(o->force_hydro, b.force_hydro, sizeof(o->force_hydro));
+    std::memcpy(o->force_holder, b.force_holder, sizeof(o->force_holder));
+    std::memcpy(o->friction, b.friction, sizeof(o->friction));
+    std::memcpy(o->friction_prev, b.friction_prev, sizeof(o->friction_prev));
+    std::memcpy(o->fdt_dead, b.fdt_dead, sizeof(o->fdt_dead));
+    o->g_dead = b.g_dead;
+    o->nusselt = b.nusselt;
+    o->slen = b.slen;
+    o->area = b.area;
+    o->cofm[0] = b.cofm.x;
+    o->cofm[1] = b.cofm.y;
+    o->moi_cofm = b.moi_cofm;
+    o->first_seg = b.first;
+    o->nseg = b.n;
+    o->special_segment_no = b.special;
+    o->kutta = b.kutta;
+}
+
+void vvo_set_body(vvo_ctx* c, int32_t bi, const vvo_body_state* o) {
+    Body& b = c->bodies[bi];
+    std::memcpy(b.holder, o->holder, sizeof(b.holder));
+    std::memcpy(b.dpos, o->dpos, sizeof(b.dpos));
+    std::memcpy(b.speed_slae, o->speed_slae, sizeof(b.speed_slae));
+    std::memcpy(b.speed_slae_prev, o->speed_slae_prev, sizeof(b.speed_slae_prev));
+    std::memcpy(b.force_hydro, o->force_hydro, sizeof(b.force_hydro));
+    std::memcpy(b.force_holder, o->force_holder, sizeof(b.force_holder));
+    std::memcpy(b.friction, o->friction, sizeof(b.friction));
+    std::memcpy(b.friction_prev, o->friction_prev, sizeof(b.friction_prev));
+    std::memcpy(b.fdt_dead, o->fdt_dead, sizeof(b.fdt_dead));
+    b.g_dead = o->g_dead;
+    b.nusselt = o->nusselt;
+    b.special = o->special_segment_no;
+    b.kutta = o->kutta != 0;
+}
+
+int32_t vvo_point_invalid(vvo_ctx* c, int32_t body, double x, double y) {
+    return point_invalid(c, c->bodies[body], V2{x, y});
+}
+
+// stree::build (TSortedTree.cpp:232-265)
+int vvo_tree_build(vvo_ctx* c, int far_criteria, double min_node, double max_node) {
+    c->far_criteria = far_criteria;
+    c->min_node = min_node;
+    c->max_node = max_node;
+    c->nodes.clear();
+    c->leaves.clear();
+    c->near.clear();
+    c->far.clear();
+    c->nodes.emplace_back();
+    Node& root = c->nodes[0];
+    for (const Body& b : c->bodies)
+        for (int k = 0; k < b.n; k++) root.segs.push_back(b.first + k);
+    root.v0 = 0;
+    root.v1 = (int)c->vort.size();
+    stretch(c, c->nodes[0]);
+    divide(c, 0);
+    calc_cmass(c, 0);
+    c->near.resize(c->leaves.size());
+    c->far.resize(c->leaves.size());
+    for (size_t l = 0; l < c->leaves.size(); l++) find_near(c, c->leaves[l], 0, c->near[l], c->far[l]);
+    c->built = true;
+    return 0;
+}
+
+void vvo_tree_destroy(vvo_ctx* c) {
+    c->built = false;
+    c->nodes.clear();
+    c->leaves.clear();
+    c->near.clear();
+    c->far.clear();
+}
+
+int32_t vvo_tree_num_nodes(vvo_ctx* c) { return (int32_t)c->nodes.size(); }
+int32_t vvo_tree_num_leaves(vvo_ctx* c) { return (int32_t)c->leaves.size(); }
+
+void vvo_tree_nodes(vvo_ctx* c, double* geom, int32_t* ints, double* cm, double* cm_inv_eps) {
+    for (size_t i = 0; i < c->nodes.size(); i++) {
+        const Node& n = c->nodes[i];
+        if (geom) {
+            geom[4 * i + 0] = n.x;
+            geom[4 * i + 1] = n.y;
+            geom[4 * i + 2] = n.h;
+            geom[4 * i + 3] = n.w;
+        }
+        if (ints) {
+            ints[6 * i + 0] = n.v0;
+            ints[6 * i + 1] = n.v1;
+            ints[6 * i + 2] = n.ch1;
+            ints[6 * i + 3] = n.parent;
+            ints[6 * i + 4] = n.depth;
+            ints[6 * i + 5] = (int32_t)n.segs.size();
+        }
+        if (cm) {
+            cm[6 * i + 0] = n.cmp.x;
+            cm[6 * i + 1] = n.cmp.y;
+            cm[6 * i + 2] = n.cmp.g;
+            cm[6 * i + 3] = n.cmm.x;
+            cm[6 * i + 4] = n.cmm.y;
+            cm[6 * i + 5] = n.cmm.g;
+        }
+        if (cm_inv_eps) {
+            cm_inv_eps[2 * i + 0] = n.cmp.inv_eps;
+            cm_inv_eps[2 * i + 1] = n.cmm.inv_eps;
+        }
+    }
+}
+
+void vvo_tree_leaves(vvo_ctx* c, int32_t* out) {
+    for (size_t i = 0; i < c->leaves.size(); i++) out[i] = c->leaves[i];
+}
+
+int64_t vvo_tree_list_size(vvo_ctx* c, int far) {
+    int64_t s = 0;
+    for (auto& v : (far ? c->far : c->near)) s += (int64_t)v.size();
+    return s;
+}
+
+void vvo_tree_lists(vvo_ctx* c, int far, int64_t* off, int32_t* nodes) {
+    auto& L = far ? c->far : c->near;
+    int64_t p = 0;
+    for (size_t i = 0; i < L.size(); i++) {
+        off[i] = p;
+        for (int x : L[i]) nodes[p++] = x;
+    }
+    off[L.size()] = p;
+}
+
+void vvo_tree_leaf_segs(vvo_ctx* c, int64_t* off, int32_t* segs) {
+    int64_t p = 0;
+    for (size_t i = 0; i < c->leaves.size(); i++) {
+        off[i] = p;
+        for (int s : c->nodes[c->leaves[i]].segs) segs[p++] = s;
+    }
+    off[c->leaves.size()] = p;
+}
+
+int32_t vvo_tree_find(vvo_ctx* c, double x, double y) { return find_node(c, V2{x, y}); }
+
+// MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:11-63)
+int vvo_epsilon(vvo_ctx* c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merges, int32_t max_merges,
+                int32_t* merged) {
+    if (!c->built) {
+        c->err = "epsilon: tree is not built";
+        return 1;
+    }
+    EpsCtx e{c, 0, merges, max_merges};
+    const int64_t N = (int64_t)c->vort.size();
+    if (nn1)
+        for (int64_t i = 0; i < N; i++) nn1[i] = -1;
+    if (nn2)
+        for (int64_t i = 0; i < N; i++) nn2[i] = -1;
+    for (size_t l = 0; l < c->leaves.size(); l++) {
+        const int leaf = c->leaves[l];
+        const Node& L = c->nodes[leaf];
+        V2 centre{L.x, L.y};
+        int att = nearest_segment(c, leaf, centre);
+        double crit;
+        if (merge) {
+            crit = (att >= 0) ? 0.16 * abs2(c->segs[att].dl()) * (1. + vabs(centre - c->segs[att].r())) : 0;
+        } else {
+            crit = kNaN;
+        }
+        double restriction = (att >= 0) ? vabs(c->segs[att].dl()) * (1.0 / 3.0) : 0;
+        for (int i = L.v0; i < L.v1; i++) {
+            if (!c->vort[i].g) continue;
+            double eps = epsv(e, leaf, i, crit);
+            c->vort[i].inv_eps = 1.0 / std::max(eps, restriction);
+            if (nn1) nn1[i] = e.last_n1;
+            if (nn2) nn2[i] = e.last_n2;
+        }
+    }
+    if (merged) *merged = e.merged;
+    return 0;
+}
+
+// MConvectiveFast::process_all_lists (MConvectiveFast.cpp:36-114)
+int vvo_convective(vvo_ctx* c, int threads, int64_t leaf_begin, int64_t leaf_end) {
+    if (!c->built) {
+        c->err = "convective: tree is not built";
+        return 1;
+    }
+    int64_t nl = (int64_t)c->leaves.size();
+    if (leaf_end < 0 || leaf_end > nl) leaf_end = nl;
+    if (leaf_begin < 0) leaf_begin = 0;
+    if (threads <= 0) {
+        for (int64_t l = leaf_begin; l < leaf_end; l++) convective_leaf(c, (int)l);
+    } else {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+        for (int64_t l = leaf_begin; l < leaf_end; l++) convective_leaf(c, (int)l);
+    }
+    return 0;
+}
+
+// MConvectiveFast::velocity (MConvectiveFast.cpp:20-34)
+int vvo_velocity(vvo_ctx* c, const double* xy, int64_t n, double* vout) {
+    if (!c->built) {
+        c->err = "velocity: tree is not built";
+        return 1;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        V2 p{xy[2 * i], xy[2 * i + 1]};
+        V2 res{0, 0};
+        int node = find_node(c, p);
+        if (node >= 0) {
+            const Node& L = c->nodes[node];
+            V2 t = near_influence(c, L, p);
+            res.x += t.x;
+            res.y += t.y;
+            t = far_influence(c, L, p);
+            res.x += t.x;
+            res.y += t.y;
+            t = V2{0, 0} * k1_2Pi;
+            res.x += t.x;
+            res.y += t.y;
+            t = body_influence(c, p);
+            res.x += t.x;
+            res.y += t.y;
+            res.x += c->inf_vx;
+            res.y += c->inf_vy;
+        }
+        vout[2 * i] = res.x;
+        vout[2 * i + 1] = res.y;
+    }
+    return 0;
+}
+
+// MDiffusiveFast::process_vort_list (MDiffusiveFast.cpp:8-48)
+int vvo_diffusive(vvo_ctx* c, int threads, int64_t leaf_begin, int64_t leaf_end) {
+    if (!c->built) {
+        c->err = "diffusive: tree is not built";
+        return 1;
+    }
+    int64_t nl = (int64_t)c->leaves.size();
+    if (leaf_end < 0 || leaf_end > nl) leaf_end = nl;
+    if (leaf_begin < 0) leaf_begin = 0;
+    const size_t ns = c->segs.size();
+    if (threads <= 0) {
+        std::vector<double> fric(ns);
+        for (size_t s = 0; s < ns; s++) fric[s] = c->segs[s].fric;
+        for (int64_t l = leaf_begin; l < leaf_end; l++) diffusive_leaf(c, (int)l, fric.data());
+        for (size_t s = 0; s < ns; s++) c->segs[s].fric = fric[s];
+    } else {
+        // Leaves over threads; friction scattered into per-thread arrays and
+        // reduced afterwards (velocities identical to the serial order).
+        std::vector<std::vector<double>> part((size_t)threads, std::vector<double>(ns, 0.));
+#pragma omp parallel num_threads(threads)
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t l = leaf_begin; l < leaf_end; l++) diffusive_leaf(c, (int)l, part[(size_t)tid].data());
+        }
+        for (size_t s = 0; s < ns; s++)
+            for (int t = 0; t < threads; t++) c->segs[s].fric += part[(size_t)t][s];
+    }
+    return 0;
+}
+
+int vvo_direct_sum(const double* objs6, int64_t n, const int64_t* targets, int64_t nt, double* vout, int threads) {
+    const Obj* o = reinterpret_cast<const Obj*>(objs6);
+    if (threads <= 0) threads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+    for (int64_t t = 0; t < nt; t++) {
+        const Obj& a = o[targets ? targets[t] : t];
+        double sx = 0, sy = 0, cx = 0, cy = 0;   // Neumaier compensation
+        for (int64_t j = 0; j < n; j++) {
+            const Obj& s = o[j];
+            if (!s.g) continue;
+            double dx = a.x - s.x, dy = a.y - s.y;
+            double e = 1. / s.inv_eps;
+            double f = s.g / (dx * dx + dy * dy + e * e);
+            double tx = -dy * f, ty = dx * f;
+            double ux = sx + tx;
+            cx += (std::fabs(sx) >= std::fabs(tx)) ? (sx - ux) + tx : (tx - ux) + sx;
+            sx = ux;
+            double uy = sy + ty;
+            cy += (std::fabs(sy) >= std::fabs(ty)) ? (sy - uy) + ty : (ty - uy) + sy;
+            sy = uy;
+        }
+        vout[2 * t] = (sx + cx) * k1_2Pi;
+        vout[2 * t + 1] = (sy + cy) * k1_2Pi;
+    }
+    return 0;
+}
+
+// MFlowmove::move_and_clean (MFlowmove.cpp:11-215) for rigid bodies without
+// collisions or root bodies; heat and streak lists are empty.
+int vvo_move_and_clean(vvo_ctx* c, int remove, int64_t* cleaned) {
+    int64_t cl = 0;
+    const double dt = c->dt;
+    // Body moves (TBody::move, TBody.cpp:179-198)
+    for (Body& b : c->bodies) {
+        double dH[3] = {b.speed_c[0] * dt, b.speed_c[1] * dt, b.speed_c[2] * dt};
+        double dB[3] = {b.speed_slae[0] * dt, b.speed_slae[1] * dt, b.speed_slae[2] * dt};
+        V2 axis = b.axis();
+        double cs = std::cos(dB[2]), sn = std::sin(dB[2]);
+        for (int k = 0; k < b.n; k++) {
+            Seg& s = c->segs[b.first + k];
+            V2 dr = s.corner() - axis;
+            V2 nc = axis + V2{dB[0], dB[1]} + dr * cs + rotl(dr) * sn;
+            s.cx = nc.x;
+            s.cy = nc.y;
+        }
+        b.holder[0] += dH[0];
+        b.holder[1] += dH[1];
+        b.holder[2] += dH[2];
+        b.dpos[0] += dB[0] - dH[0];
+        b.dpos[1] += dB[1] - dH[1];
+        b.dpos[2] += dB[2] - dH[2];
+        for (int i = 0; i < 3; i++) b.speed_slae_prev[i] = b.speed_slae[i];
+        update_segments(c, b);
+        fill_properties(c, b);
+    }
+    for (Obj& o : c->vort) {
+        o.x += o.vx * dt;
+        o.y += o.vy * dt;
+    }
+    std::vector<Obj> keep;
+    keep.reserve(c->vort.size());
+    for (const Obj& o : c->vort)
+        if (!(std::fabs(o.g) < kRemoveEps)) keep.push_back(o);
+    if (remove) {
+        std::vector<Obj> keep2;
+        keep2.reserve(keep.size());
+        for (const Obj& o : keep) {
+            int bad = -1, bb = -1;
+            for (int b = 0; b < (int)c->bodies.size(); b++) {
+                bad = point_invalid(c, c->bodies[b], V2{o.x, o.y});
+                if (bad >= 0) {
+                    bb = b;
+                    break;
+                }
+            }
+            if (bb < 0) {
+                keep2.push_back(o);
+                continue;
+            }
+            Body& B = c->bodies[bb];
+            V2 t = rotl(V2{o.x, o.y}) * o.g;
+            B.fdt_dead[0] += t.x;
+            B.fdt_dead[1] += t.y;
+            B.fdt_dead[2] += abs2(V2{o.x, o.y} - B.axis()) * o.g;
+            c->segs[bad].gsum -= o.g;
+            B.g_dead += o.g;
+            cl++;
+        }
+        keep.swap(keep2);
+    }
+    c->vort.swap(keep);
+    for (Obj& o : c->vort) {
+        o.vx = 0.;
+        o.vy = 0.;
+    }
+    for (Body& b : c->bodies) {
+        V2 axis = b.axis();
+        for (int k = 0; k < b.n; k++) {
+            Seg& s = c->segs[b.first + k];
+            if (std::fabs(s.g) < kRemoveEps || s.slip) {
+                V2 t = rotl(s.corner()) * s.g;
+                b.fdt_dead[0] += t.x;
+                b.fdt_dead[1] += t.y;
+                b.fdt_dead[2] += abs2(s.corner() - axis) * s.g;
+                s.gsum -= s.g;
+                b.g_dead += s.g;
+            }
+        }
+    }
+    c->built = false;
+    if (cleaned) *cleaned = cl;
+    return 0;
+}
+
+// MFlowmove::vortex_shed (MFlowmove.cpp:217-235)
+int vvo_vortex_shed(vvo_ctx* c) {
+    c->built = false;
+    for (Body& b : c->bodies) {
+        for (int k = 0; k < b.n; k++) {
+            Seg& s = c->segs[b.first + k];
+            s.gsum += s.g;
+            if (std::fabs(s.g) >= kRemoveEps && !s.slip) {
+                V2 p = s.corner() + rotl(s.dl()) * 1e-4;
+                c->vort.push_back(Obj{p.x, p.y, s.g, 0, 0, 0});
+            }
+        }
+    }
+    return 0;
+}
+
+int vvo_slae_size(vvo_ctx* c) { return (int)c->segs.size() + 9 * (int)c->bodies.size(); }
+
+// MConvectiveFast::calc_circulation (MConvectiveFast.cpp:235-273) with
+// Matrix::solveUsingInverseMatrix (TMatrix.cpp:166-259).
+int vvo_calc_circulation(vvo_ctx* c, double* A_out, double* rhs_out) {
+    if (!c->built) {
+        c->err = "calc_circulation: tree is not built";
+        return 1;
+    }
+    Slae S;
+    S.c = c;
+    S.n = vvo_slae_size(c);
+    S.A.assign((size_t)S.n * S.n, 0.);
+    S.rhs.assign((size_t)S.n, 0.);
+    int col = 0;
+    for (const Body& b : c->bodies) {
+        S.base.push_back(col);
+        col += b.n + 9;
+    }
+    fill_matrix(S);
+    if (A_out) std::memcpy(A_out, S.A.data(), S.A.size() * sizeof(double));
+    if (rhs_out) std::memcpy(rhs_out, S.rhs.data(), S.rhs.size() * sizeof(double));
+    bool use_inverse = (c->time > 0);
+    for (const Body& b : c->bodies) {
+        if (!(std::fabs(b.speed_c[0]) + std::fabs(b.speed_c[1]) + std::fabs(b.speed_c[2]) < 1E-10)) use_inverse = false;
+        for (int i = 0; i < 3; i++)
+            if (!Body::isrigid(b.kspring[i])) use_inverse = false;
+    }
+    std::vector<double> x(S.rhs);
+    if (use_inverse) {
+        if (!c->inv_cache_valid || c->inv_cache_src != S.A) {
+            std::vector<double> lu(S.A);
+            std::vector<int> piv;
+            if (int info = lu_factor(lu, S.n, piv)) {
+                c->err = "calc_circulation: singular matrix, column " + std::to_string(info);
+                return 2;
+            }
+            c->inv_cache.assign((size_t)S.n * S.n, 0.);
+            std::vector<double> e((size_t)S.n);
+            for (int j = 0; j < S.n; j++) {
+                std::fill(e.begin(), e.end(), 0.);
+                e[j] = 1.;
+                lu_solve(lu, S.n, piv, e.data());
+                for (int i = 0; i < S.n; i++) c->inv_cache[(size_t)i * S.n + j] = e[i];
+            }
+            c->inv_cache_src = S.A;
+            c->inv_cache_valid = true;
+        }
+        for (int i = 0; i < S.n; i++) {
+            double s = 0;
+            for (int j = 0; j < S.n; j++) s += c->inv_cache[(size_t)i * S.n + j] * S.rhs[j];
+            x[i] = s;
+        }
+    } else {
+        std::vector<double> lu(S.A);
+        std::vector<int> piv;
+        if (int info = lu_factor(lu, S.n, piv)) {
+            c->err = "calc_circulation: singular matrix, column " + std::to_string(info);
+            return 2;
+        }
+        lu_solve(lu, S.n, piv, x.data());
+    }
+    for (size_t b = 0; b < c->bodies.size(); b++) {
+        Body& B = c->bodies[b];
+        for (int k = 0; k < B.n; k++) c->segs[B.first + k].g = x[S.col_g((int)b, k)];
+        for (int i = 0; i < 3; i++) {
+            B.speed_slae[i] = x[S.col_speed((int)b, i)];
+            B.force_hydro[i] = x[S.col_hydro((int)b, i)];
+            B.force_holder[i] = x[S.col_holder((int)b, i)];
+        }
+    }
+    return 0;
+}
+
+// Space::calc_forces (TSpace.cpp:94-120)
+void vvo_calc_forces(vvo_ctx* c) {
+    const double re = c->re, dt = c->dt, pr = 0;
+    const double C_NyuDt_Pi = dt / (M_PI * re);
+    const double C_Nyu_Pi = 1. / (M_PI * re);
+    for (Body& b : c->bodies) {
+        double tmp_gsum = 0;
+        b.friction[0] = b.friction[1] = b.friction[2] = 0;
+        for (int k = 0; k < b.n; k++) {
+            Seg& s = c->segs[b.first + k];
+            tmp_gsum += s.gsum;
+            s.Cp += tmp_gsum;
+            s.Fr += s.fric * C_NyuDt_Pi;
+            s.Nu += s.hsum * (re * pr / vabs(s.dl()));
+            V2 t = s.dl() * (s.fric * C_Nyu_Pi / vabs(s.dl()));
+            b.friction[0] -= t.x;
+            b.friction[1] -= t.y;
+            b.friction[2] -= dot(rotl(s.r()), s.dl()) * (s.fric * C_Nyu_Pi / vabs(s.dl()));
+            b.nusselt += s.hsum * (re * pr);
+        }
+        b.nusselt /= dt;
+    }
+}
+
+// Space::zero_forces (TSpace.cpp:122-143)
+void vvo_zero_forces(vvo_ctx* c) {
+    for (Body& b : c->bodies) {
+        for (int k = 0; k < b.n; k++) {
+            Seg& s = c->segs[b.first + k];
+            s.gsum = s.fric = s.hsum = 0;
+        }
+        for (int i = 0; i < 3; i++) {
+            b.force_hydro[i] = 0;
+            b.force_holder[i] = 0;
+            b.friction_prev[i] = b.friction[i];
+            b.friction[i] = 0;
+            b.fdt_dead[i] = 0;
+        }
+        b.g_dead = 0;
+        b.nusselt = 0.;
+    }
+}
+
+// Space::gsum / integral / hydrodynamic_momentum / gmax (TSpace.hpp:85-117)
+void vvo_integrals(vvo_ctx* c, double* out) {
+    double gs = 0, in = 0, mx = 0, my = 0, gm = 0;
+    for (const Obj& o : c->vort) {
+        gs += o.g;
+        in += o.g * (o.x * o.x + o.y * o.y);
+        mx += o.g * o.x;
+        my += o.g * o.y;
+        if (std::fabs(o.g) > std::fabs(gm)) gm = o.g;
+    }
+    out[0] = gs;
+    out[1] = in;
+    out[2] = mx;
+    out[3] = my;
+    out[4] = gm;
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+/*
+ * vvhd_oracle.h -- C API of the FP64 CPU restatement of libvvhd's per-step
+ * particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py may load it; the product
+ * (vvflow_amd) never links, loads or calls it.
+ *
+ * Every function restates the reference algorithm it names (file:line of
+ * vvflow/vvflow, libvvhd/src/...).  Arithmetic is FP64 with the reference's
+ * operation order and no FMA contraction, so results are bit-identical to
+ * the reference wherever the reference uses only +,-,*,/,sqrt and glibc libm.
+ *
+ * Record layout: a vortex is 6 doubles {x, y, g, vx, vy, inv_eps}, the
+ * layout of the reference's TObj (libvvhd/headers/TObj.hpp:10-35).
+ */
+#ifndef VVHD_ORACLE_H
+#define VVHD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vvo_ctx vvo_ctx;
+
+/* One body segment as handed over by the caller: corner and slip flag
+ * (TAtt(TVec corner, bool slip), libvvhd/headers/TBody.hpp:188-199). */
+typedef struct vvo_corner { double x, y; uint32_t slip; uint32_t pad; } vvo_corner;
+
+/* Per-segment state (the TAtt fields the hot path reads or writes). */
+typedef struct vvo_seg_state {
+    double x, y;          /* centre r                       */
+    double g;             /* bound circulation              */
+    double inv_eps;       /* 3/|dl|                         */
+    double cx, cy;        /* corner                         */
+    double dlx, dly;      /* segment vector                 */
+    double gsum, hsum, fric, Cp, Fr, Nu;
+    uint32_t slip, eq_no;
+} vvo_seg_state;
+
+/* Per-body state (TBody fields used by SLAE, flow-move and forces). */
+typedef struct vvo_body_state {
+    double holder[3], dpos[3], speed_slae[3], speed_slae_prev[3];
+    double force_hydro[3], force_holder[3], friction[3], friction_prev[3];
+    double fdt_dead[3];
+    double g_dead, nusselt;
+    double slen, area, cofm[2], moi_cofm;
+    int32_t first_seg, nseg;
+    int32_t special_segment_no, kutta;
+} vvo_body_state;
+
+vvo_ctx* vvo_create(void);
+void     vvo_destroy(vvo_ctx*);
+const char* vvo_last_error(vvo_ctx*);
+
+/* ---- Space ---------------------------------------------------------- */
+void    vvo_set_params(vvo_ctx*, double re, double dt, double inf_vx, double inf_vy,
+                       double inf_g, double time);
+void    vvo_set_vortices(vvo_ctx*, const double* objs6, int64_t n);
+int64_t vvo_num_vortices(vvo_ctx*);
+void    vvo_get_vortices(vvo_ctx*, double* objs6);
+/* Adds a rigid, static body (kspring = inf, speed = 0) from its corners;
+ * performs doUpdateSegments + doFillProperties (TBody.cpp:200-215, 283-343). */
+int     vvo_add_body(vvo_ctx*, const vvo_corner* corners, int32_t n);
+int32_t vvo_num_bodies(vvo_ctx*);
+int32_t vvo_num_segments(vvo_ctx*);
+void    vvo_get_segments(vvo_ctx*, vvo_seg_state* out);
+void    vvo_set_segments(vvo_ctx*, const vvo_seg_state* in);
+void    vvo_get_body(vvo_ctx*, int32_t b, vvo_body_state* out);
+void    vvo_set_body(vvo_ctx*, int32_t b, const vvo_body_state* in);
+/* TBody::isPointInvalid (TBody.cpp:217-281): returns segment index or -1. */
+int32_t vvo_point_invalid(vvo_ctx*, int32_t body, double x, double y);
+
+/* ---- Tree (TSortedTree.cpp) ----------------------------------------- */
+int     vvo_tree_build(vvo_ctx*, int far_criteria, double min_node, double max_node);
+void    vvo_tree_destroy(vvo_ctx*);
+int32_t vvo_tree_num_nodes(vvo_ctx*);
+int32_t vvo_tree_num_leaves(vvo_ctx*);
+/* Node table in creation order (root = 0, children created in pairs).
+ * geom: 4 doubles per node {x, y, h, w}; ints: 6 int32 per node
+ * {v0, v1, ch1, parent, depth, nseg}; cm: 6 doubles {CMp.x, CMp.y, CMp.g,
+ * CMm.x, CMm.y, CMm.g}; cm_inv_eps: 2 doubles {CMp._1_eps, CMm._1_eps}. */
+void    vvo_tree_nodes(vvo_ctx*, double* geom, int32_t* ints, double* cm, double* cm_inv_eps);
+/* Leaves in DFS order (getBottomNodes). */
+void    vvo_tree_leaves(vvo_ctx*, int32_t* leaf_nodes);
+/* Near / far lists per leaf (DFS order) in CSR form: off has nleaves+1. */
+int64_t vvo_tree_list_size(vvo_ctx*, int far);
+void    vvo_tree_lists(vvo_ctx*, int far, int64_t* off, int32_t* nodes);
+/* Segment lists of leaves (CSR, DFS order of leaves). */
+void    vvo_tree_leaf_segs(vvo_ctx*, int64_t* off, int32_t* segs);
+/* stree::findNode (TSortedTree.cpp:284-303): node id or -1. */
+int32_t vvo_tree_find(vvo_ctx*, double x, double y);
+
+/* ---- Epsilon (MEpsilonFast.cpp:11-63) --------------------------------- */
+/* nn1/nn2 (optional, length N) receive the indices of the 1st/2nd nearest
+ * vortex found by the final epsv call for each target (-1 if none / skipped).
+ * merges (optional, 2*max_merges) receives (target, partner) pairs in order. */
+int     vvo_epsilon(vvo_ctx*, int merge, int32_t* nn1, int32_t* nn2,
+                    int32_t* merges, int32_t max_merges, int32_t* merged);
+
+/* ---- Convective (MConvectiveFast.cpp:36-151) -------------------------- */
+/* threads: 0 = the reference's own structure (serial leaf loop);
+ * >0 = leaves distributed over that many OpenMP threads (identical results). */
+int     vvo_convective(vvo_ctx*, int threads, int64_t leaf_begin, int64_t leaf_end);
+/* MConvectiveFast::velocity(p) (:20-34) for n points. */
+int     vvo_velocity(vvo_ctx*, const double* xy, int64_t n, double* vout);
+
+/* ---- Diffusive (MDiffusiveFast.cpp:8-48, 93-123) ---------------------- */
+int     vvo_diffusive(vvo_ctx*, int threads, int64_t leaf_begin, int64_t leaf_end);
+
+/* ---- FP64 direct sum of the induced convective velocity ---------------
+ * v_i = 1/(2 pi) sum_{j, g_j != 0} g_j rotl(r_i - r_j) / (|r_i - r_j|^2 + eps_j^2)
+ * with eps_j = 1/_1_eps_j (the Lamb kernel of MConvectiveFast.cpp:116-120
+ * applied to every pair; no tree).  Compensated (Neumaier) FP64 sums. */
+int     vvo_direct_sum(const double* objs6, int64_t n, const int64_t* targets,
+                       int64_t nt, double* vout, int threads);
+
+/* ---- Flow move (MFlowmove.cpp:11-215, 217-235) ------------------------ */
+int     vvo_move_and_clean(vvo_ctx*, int remove, int64_t* cleaned);
+int     vvo_vortex_shed(vvo_ctx*);
+
+/* ---- SLAE (MConvectiveFast.cpp:235-273, 882-1017; TMatrix.cpp:166-259) */
+/* Fills and solves the boundary-condition system; writes back seg g,
+ * speed_slae, force_hydro, force_holder.  A/rhs (optional) receive the
+ * filled row-major matrix and right column (n*n and n doubles). */
+int     vvo_slae_size(vvo_ctx*);
+int     vvo_calc_circulation(vvo_ctx*, double* A, double* rhs);
+
+/* ---- Integrals (TSpace.cpp:94-143, TSpace.hpp:85-117) ----------------- */
+void    vvo_calc_forces(vvo_ctx*);
+void    vvo_zero_forces(vvo_ctx*);
+/* out: {gsum, integral, momentum.x, momentum.y, gmax} */
+void    vvo_integrals(vvo_ctx*, double* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
