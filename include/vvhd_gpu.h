@@ -1,0 +1,180 @@
+/*
+ * vvhd_gpu.h -- C ABI of the MI355X (gfx950) implementation of libvvhd's
+ * per-step particle hot path: tree build, epsilon search with vortex merging,
+ * convective (Biot-Savart) velocity and vortex-domain diffusive velocity,
+ * plus flow-move, the FP64 boundary-condition SLAE and the FP64 integrals.
+ *
+ * Boundary.  The reference exposes this path as C++ classes over a borrowed
+ * Space* (vvflow/vvflow libvvhd/headers/TSortedTree.hpp:60-92,
+ * MEpsilonFast.hpp:7-31, MConvectiveFast.hpp:9-27, MDiffusiveFast.hpp:7-18,
+ * MFlowmove.hpp:5-19).  This header is the flat C layer beneath a
+ * replacement of those classes (vvflow_amd/csrc/shim): plain pointers and
+ * sizes, an opaque device context, int status codes, no exceptions.
+ *
+ * Conventions.
+ *  - Every call returns VVHD_OK (0) or a VVHD_E* code; vvhd_last_error()
+ *    then holds a message.  Nothing throws across the ABI.
+ *  - Host arrays of vortices use the reference's TObj record: 6 doubles
+ *    {r.x, r.y, g, v.x, v.y, _1_eps} per vortex (TObj.hpp:13-16).  On the
+ *    device the context keeps them structure-of-arrays (FP64 master copy).
+ *  - All device work of one context is ordered on one HIP stream
+ *    (vvhd_stream).  Calls block until their results are on the host only
+ *    when they return host data.
+ *  - One host thread per context.
+ */
+#ifndef VVHD_GPU_H
+#define VVHD_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VVHD_OK = 0,
+    VVHD_EINVAL = 1,       /* invalid argument                               */
+    VVHD_ESTATE = 2,       /* call out of order (e.g. tree not built)        */
+    VVHD_EHIP = 3,         /* HIP runtime error                              */
+    VVHD_ENOMEM = 4,       /* device allocation failed                       */
+    VVHD_ESINGULAR = 5,    /* singular boundary-condition matrix             */
+    VVHD_EUNSUPPORTED = 6  /* input outside what this implementation covers  */
+};
+
+typedef struct vvhd_ctx vvhd_ctx;
+
+/* One body segment (the TAtt fields the path reads and writes,
+ * libvvhd/headers/TBody.hpp:17-55).  Same layout for upload and download. */
+typedef struct vvhd_seg {
+    double x, y;          /* centre r                                  */
+    double g;             /* bound circulation (SLAE solution)         */
+    double inv_eps;       /* _1_eps = 3/|dl|                           */
+    double cx, cy;        /* corner                                    */
+    double dlx, dly;      /* segment vector dl                         */
+    double gsum, hsum, fric, Cp, Fr, Nu;
+    uint32_t slip, eq_no;
+} vvhd_seg;
+
+/* One body (TBody fields used by the SLAE, flow-move and force integrals,
+ * libvvhd/headers/TBody.hpp:67-158).  Bodies are rigid and static
+ * (kspring = inf, prescribed speed 0): the configurations of this path. */
+typedef struct vvhd_body {
+    double holder[3], dpos[3], speed_slae[3], speed_slae_prev[3];
+    double force_hydro[3], force_holder[3], friction[3], friction_prev[3];
+    double fdt_dead[3];
+    double g_dead, nusselt;
+    double slen, area, cofm[2], moi_cofm;
+    int32_t first_seg, nseg;
+    int32_t special_segment_no, kutta;
+} vvhd_body;
+
+/* ---- context -------------------------------------------------------- */
+int  vvhd_create(int device, vvhd_ctx** out);
+void vvhd_destroy(vvhd_ctx* ctx);
+const char* vvhd_last_error(const vvhd_ctx* ctx);
+/* The hipStream_t all work of this context is ordered on. */
+void* vvhd_stream(vvhd_ctx* ctx);
+/* Space::re, Space::dt, Space::inf_g, Space::time (TSpace.hpp:24-36). */
+int  vvhd_set_params(vvhd_ctx* ctx, double re, double dt, double inf_g, double time);
+
+/* ---- particles (Space::VortexList, TSpace.hpp:39) -------------------- */
+int     vvhd_upload_vortices(vvhd_ctx* ctx, const double* objs6, int64_t n);
+int     vvhd_download_vortices(vvhd_ctx* ctx, double* objs6, int64_t n);
+int64_t vvhd_num_vortices(vvhd_ctx* ctx);
+/* Device-side copy of the vortex list and its restore (benchmarks start
+ * every repeat from the same input because merges mutate it). */
+int     vvhd_snapshot(vvhd_ctx* ctx);
+int     vvhd_restore(vvhd_ctx* ctx);
+
+/* ---- bodies (Space::BodyList, TSpace.hpp:38) ------------------------- */
+int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg,
+                       const vvhd_body* bodies, int32_t nbody);
+int vvhd_download_bodies(vvhd_ctx* ctx, vvhd_seg* segs, int32_t nseg,
+                         vvhd_body* bodies, int32_t nbody);
+
+/* ---- tree: stree (TSortedTree.hpp:60-92, TSortedTree.cpp:232-303) ----
+ * Builds the adaptive binary tree over the vortex list and the body
+ * segments, reordering the vortex list in place exactly as the reference's
+ * Hoare partitions do, then the per-sign monopoles and the near/far
+ * interaction lists of every leaf.  perm_out (optional, host, n entries)
+ * receives for every new position the vortex's index before the build. */
+int vvhd_tree_build(vvhd_ctx* ctx, int far_criteria, double min_node_size,
+                    double max_node_size, int32_t* perm_out);
+int vvhd_tree_destroy(vvhd_ctx* ctx);
+/* Leaf table in DFS order (getBottomNodes):
+ *   geom: 4 doubles per leaf {x, y, h, w}; ints: 4 int32 {v0, v1, depth,
+ *   nseg}; path: uint64 path code (bit per level, 1 = second child).
+ * Any pointer may be NULL.  Returns the leaf count through nleaves. */
+int vvhd_tree_leaves(vvhd_ctx* ctx, int32_t* nleaves, double* geom,
+                     int32_t* ints, uint64_t* path);
+/* Near (far = 0) or far (far = 1) lists of all leaves, flattened in leaf DFS
+ * order; each entry is a node given by {depth, path code} plus, for far
+ * entries, its monopoles cm = {CMp.x, CMp.y, CMp.g, CMm.x, CMm.y, CMm.g}.
+ * Call with entries == NULL to get the total count. */
+int vvhd_tree_lists(vvhd_ctx* ctx, int far, int64_t* count, int64_t* offsets,
+                    int32_t* depth, uint64_t* path, double* cm);
+/* stree::findNode for n points: leaf index (DFS order) or -1. */
+int vvhd_tree_find(vvhd_ctx* ctx, const double* xy, int64_t n, int32_t* leaf_out);
+
+/* ---- epsilon: MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:11-63) --
+ * Sets _1_eps of every vortex with g != 0 from its two nearest neighbours
+ * in the near leaves, merging vortices with the reference's criterion and in
+ * the reference's order when merge != 0.  nn1/nn2 (optional, host, n
+ * entries, tree order) receive the neighbour indices of the final search of
+ * each target (-1 where none).  merged (optional) receives Merged(). */
+int vvhd_epsilon(vvhd_ctx* ctx, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
+
+/* ---- convective: MConvectiveFast::process_all_lists (:36-114) ---------
+ * v += U_inf + near-field Lamb-kernel sum (FP32 pair sums, Kahan-compensated
+ * when kahan != 0, in leaf-local coordinates) + far-field Taylor terms
+ * (FP64).  inf_x/inf_y: Space::inf_speed() evaluated by the caller. */
+int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan);
+/* MConvectiveFast::velocity(TVec) (:20-34) for n points (host xy, host out). */
+int vvhd_velocity_points(vvhd_ctx* ctx, const double* xy, int64_t n,
+                         double inf_x, double inf_y, double* v_out);
+
+/* ---- diffusive: MDiffusiveFast::process_vort_list (:8-48) ------------ */
+int vvhd_diffusive(vvhd_ctx* ctx);
+
+/* ---- flow-move: MFlowmove::move_and_clean / vortex_shed (:11-235) ---- */
+int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned);
+int vvhd_vortex_shed(vvhd_ctx* ctx);
+
+/* ---- boundary-condition SLAE: MConvectiveFast::calc_circulation -------
+ * (:235-273, fill :882-1017) with Matrix::solveUsingInverseMatrix
+ * (TMatrix.cpp:166-259): FP64 fill and LU solve on the device; writes the
+ * bound circulations, speed_slae, force_hydro and force_holder. */
+int vvhd_calc_circulation(vvhd_ctx* ctx, double inf_x, double inf_y);
+
+/* ---- FP64 integrals: Space::calc_forces / zero_forces (TSpace.cpp:94-143)
+ * and Space::gsum / integral / hydrodynamic_momentum / gmax
+ * (TSpace.hpp:85-117); out5 = {gsum, integral, momentum.x, momentum.y, gmax}. */
+int vvhd_calc_forces(vvhd_ctx* ctx);
+int vvhd_zero_forces(vvhd_ctx* ctx);
+int vvhd_integrals(vvhd_ctx* ctx, double* out5);
+
+/* ---- measurement (no counterpart in the reference) -------------------
+ * FP64 direct sum of the induced Lamb-kernel velocity over all vortices for
+ * nt targets (tree order indices; NULL = all), into host v_out (2*nt). */
+int vvhd_direct_sum(vvhd_ctx* ctx, const int64_t* targets, int64_t nt, double* v_out);
+/* Device time in ms of the phases of the last calls, measured with HIP
+ * events on the context stream: {tree, epsilon, convective, diffusive,
+ * move_and_clean, slae}. */
+int vvhd_phase_times(vvhd_ctx* ctx, float* ms6);
+/* Algorithmic work counted on the interaction lists of the current tree:
+ * {near convective pairs, far (leaf, node) terms, epsilon pairs scanned,
+ *  diffusive same-sign pairs, diffusive segment pairs, leaves, nodes}. */
+int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7);
+
+/* ---- synthetic inputs (host) -------------------------------------------
+ * Karman-street wake: 400 Gaussian clusters (sigma 0.05) in two staggered
+ * rows at y = +-0.2, SplitMix64 seeded, Box-Muller normals; objs6 receives
+ * n TObj records. */
+int vvhd_gen_wake(int64_t n, uint64_t seed, double* objs6);
+/* Cylinder corners as gen_cylinder{R, N} makes them (gen_body.cpp:74-88). */
+int vvhd_gen_cylinder(double radius, int32_t n, double* corners_xy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

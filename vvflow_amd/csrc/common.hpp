@@ -1,0 +1,252 @@
+// common.hpp -- shared infrastructure of the gfx950 implementation: the
+// device context, error handling, device buffers and small wave-level
+// helpers.  Host and device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vvhd_gpu.h"
+
+namespace vvhd {
+
+// ----------------------------------------------------------------- errors --
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define VVHD_HIP(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::vvhd::Error(VVHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define VVHD_CHECK(cond, code, msg)                         \
+    do {                                                    \
+        if (!(cond)) throw ::vvhd::Error((code), (msg));    \
+    } while (0)
+
+// ---------------------------------------------------------------- buffers --
+
+// Device buffer that only grows; contents are not preserved across grow().
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    void grow(size_t n) {
+        if (n <= cap) return;
+        size_t nc = std::max<size_t>(n, cap + cap / 2);
+        if (p) VVHD_HIP(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, nc * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            throw Error(VVHD_ENOMEM, "hipMalloc of " + std::to_string(nc * sizeof(T)) + " bytes failed");
+        }
+        cap = nc;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// ------------------------------------------------------------ device utils --
+
+constexpr int kWave = 64;
+constexpr int kLeafMaxList = 16;   // TSortedTree.cpp:10
+
+// Doubles mapped to uint64 so that unsigned order equals numeric order
+// (NaN excluded); used for atomic min/max of bounding boxes.
+__host__ __device__ inline uint64_t ord_from_double(double d) {
+    uint64_t u = __double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double double_from_ord(uint64_t u) {
+    uint64_t v = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+    return __longlong_as_double(v);
+}
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ inline T wave_bcast(T v, int src) {
+    return __shfl(v, src, kWave);
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ inline int wave_sum_i(int v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+// Inclusive scan across the wave.
+__device__ inline int wave_incl_scan(int v) {
+    const int l = lane_id();
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(v, o, kWave);
+        if (l >= o) v += t;
+    }
+    return v;
+}
+
+// Maps a linear block index so that consecutive logical blocks land on the
+// same XCD (blocks are dispatched round-robin over the 8 XCDs of MI355X):
+// neighbouring leaves then share that XCD's L2.
+__device__ inline int xcd_block(int b, int nblocks) {
+    const int kXcd = 8;
+    const int per = nblocks / kXcd;
+    const int rem = nblocks % kXcd;
+    const int xcd = b % kXcd;
+    const int idx = b / kXcd;
+    // XCD x owns logical blocks [start(x), start(x) + size(x))
+    const int start = xcd * per + min(xcd, rem);
+    return start + idx;
+}
+
+// ------------------------------------------------------------------ context --
+
+struct TreeArrays {
+    // node table (capacity ncap)
+    DBuf<double> x, y, h, w;              // Stretch() geometry
+    DBuf<uint64_t> blx, bly, trx, try_;   // ordered-uint64 bbox (atomics)
+    DBuf<int32_t> v0, v1, ch1, parent, depth, nseg;
+    DBuf<uint64_t> path;                  // path code (bit per level)
+    DBuf<double> cmp_x, cmp_y, cmp_g, cmm_x, cmm_y, cmm_g;
+    DBuf<float> cmp_ie, cmm_ie;           // _1_eps of monopoles: +inf or 0
+    DBuf<int32_t> nleaves, arrive, leaf_rank;
+    DBuf<int32_t> split;                  // per node: 0 undecided,1 leaf,2 split
+    // levels
+    DBuf<int32_t> counters;               // [0]=node count, [1]=leaf count, [2..] misc
+    // leaves (DFS order)
+    DBuf<int32_t> leaf_node;              // leaf -> node id
+    DBuf<int64_t> near_off, far_off;      // CSR offsets per leaf
+    DBuf<int32_t> near_list, far_list;    // node ids
+    DBuf<int64_t> seg_off;                // per leaf
+    DBuf<int32_t> seg_list;               // segment ids
+    DBuf<int32_t> seg_leaf;               // per segment: leaf (DFS)
+    DBuf<int32_t> vleaf;                  // per vortex: leaf (DFS)
+    DBuf<int32_t> node_L, node_m, leaf_tmp, seg_node;   // build scratch
+    DBuf<int32_t> cnt_near, cnt_far, err;               // list scratch
+    int32_t nnodes = 0, nleaves_h = 0;
+    int64_t nnear = 0, nfar = 0;
+    int max_depth = 0;
+    bool built = false;
+    int far_criteria = 8;
+    double min_node = 0, max_node = DBL_MAX;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double re = INFINITY, dt = 1, inf_g = 0, time = 0;
+
+    // vortices, structure of arrays, FP64 master copy
+    int64_t n = 0;
+    DBuf<double> x, y, g, vx, vy, ie;
+    DBuf<double> x2, y2, g2, vx2, vy2, ie2;   // double buffers
+    DBuf<int32_t> id, id2;                   // index before the tree build
+    DBuf<float2> src_ge;                     // {g, eps^2} FP32 pair-kernel copy
+    DBuf<double2> near_v;                    // near-field sums per vortex
+    DBuf<double4> taylor;                    // far-field Taylor terms per leaf
+    // snapshot for benchmarks
+    DBuf<double> snap;
+    int64_t snap_n = -1;
+
+    // segments and bodies (host copy is authoritative between calls)
+    std::vector<vvhd_seg> segs_h;
+    std::vector<vvhd_body> bodies_h;
+    DBuf<vvhd_seg> segs;
+    DBuf<double> fric;                       // per segment friction accumulator
+    DBuf<int32_t> body_range;                // per body: first segment, count
+
+    TreeArrays t;
+
+    // scratch
+    DBuf<int32_t> iscratch0, iscratch1, iscratch2, iscratch3;
+    DBuf<int64_t> lscratch0;
+    DBuf<uint8_t> tmp;
+
+    // epsilon results
+    DBuf<int32_t> nn1, nn2;
+    DBuf<double> d1, d2;
+    DBuf<double> leaf_crit, leaf_restr;
+    DBuf<uint8_t> eflag;                     // per target: bit0 processed, bit1 merge flag
+    DBuf<int32_t> flag_list;                 // targets flagged by the parallel pass
+    DBuf<int32_t> ecount;                    // [0]=flags, [1]=merged, ...
+    DBuf<int64_t> rnear_off;                 // transposed near lists (repair)
+    DBuf<int32_t> rnear_list, rnear_cnt, leaf_stamp;
+
+    // timing
+    hipEvent_t ev[14] = {};
+    bool phase_recorded[6] = {false, false, false, false, false, false};
+
+    // work counts of the current tree
+    int64_t work[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+// Read-only device view of the built tree used by the physics kernels.
+struct TreeDev {
+    const double *nx, *ny, *nh, *nw;
+    const uint64_t *blx, *bly, *trx, *try_;
+    const int32_t *v0, *v1, *ch1, *leaf_rank, *leaf_node;
+    const int64_t *near_off, *far_off, *seg_off;
+    const int32_t *near_list, *far_list, *seg_list, *vleaf;
+    const double *cmp_x, *cmp_y, *cmp_g, *cmm_x, *cmm_y, *cmm_g;
+    const float *cmp_ie, *cmm_ie;
+    int nleaf;
+};
+
+inline TreeDev tree_dev(const TreeArrays& t) {
+    return TreeDev{t.x.p,       t.y.p,      t.h.p,       t.w.p,       t.blx.p,     t.bly.p,     t.trx.p,
+                   t.try_.p,    t.v0.p,     t.v1.p,      t.ch1.p,     t.leaf_rank.p, t.leaf_node.p, t.near_off.p,
+                   t.far_off.p, t.seg_off.p, t.near_list.p, t.far_list.p, t.seg_list.p, t.vleaf.p,  t.cmp_x.p,
+                   t.cmp_y.p,   t.cmp_g.p,  t.cmm_x.p,   t.cmm_y.p,   t.cmm_g.p,   t.cmp_ie.p,  t.cmm_ie.p,
+                   t.nleaves_h};
+}
+
+// Box-to-box squared distance of two nodes (0 when the boxes touch or
+// overlap).  Rounded operations are monotone, so the result never exceeds the
+// computed squared distance of any pair of points inside the two boxes.
+__device__ inline double box_dist2(const TreeDev& T, int a, int b) {
+    const double ablx = double_from_ord(T.blx[a]), atrx = double_from_ord(T.trx[a]);
+    const double ably = double_from_ord(T.bly[a]), atry = double_from_ord(T.try_[a]);
+    const double bblx = double_from_ord(T.blx[b]), btrx = double_from_ord(T.trx[b]);
+    const double bbly = double_from_ord(T.bly[b]), btry = double_from_ord(T.try_[b]);
+    double dx = fmax(0.0, fmax(bblx - atrx, ablx - btrx));
+    double dy = fmax(0.0, fmax(bbly - atry, ably - btry));
+    return dx * dx + dy * dy;
+}
+
+// host helpers implemented in ctx.hip
+void ensure_vortex_capacity(Ctx& c, int64_t n);
+void sync(Ctx& c);
+
+// phase entry points (one translation unit each)
+void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int32_t* perm_out);
+void tree_lists(Ctx& c);
+void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
+void convective(Ctx& c, double inf_x, double inf_y, int kahan);
+void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out);
+void diffusive(Ctx& c);
+void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out);
+void pack_sources(Ctx& c);
+
+}  // namespace vvhd

@@ -1,0 +1,385 @@
+// convective.hip -- MConvectiveFast::process_all_lists (MConvectiveFast.cpp:
+// 36-114), MConvectiveFast::velocity (:20-34) and the FP64 direct sum used as
+// the accuracy yardstick.
+//
+// process_all_lists runs as three kernels:
+//  1. k_far_taylor, one wave per leaf: the reference's first-order Taylor
+//     expansion about the leaf centre of the per-sign monopoles of the far
+//     nodes (:48-69), FP64, lanes over the far list.
+//  2. k_near, one wave per leaf: the Lamb-kernel sum
+//     g_j rotl(r_i - r_j) / (|r_i - r_j|^2 + eps_j^2) over every vortex of the
+//     near leaves (:116-137).  The near leaves' vortex ranges are concatenated
+//     into one virtual source list that the 64 lanes walk 64 sources at a
+//     time (a lane finds its leaf with a 6-step binary search over prefix
+//     sums held across the wave).  Each lane converts its source once to FP32
+//     coordinates relative to the target leaf centre (the FP64 difference is
+//     taken first, so FP32 keeps ~1e-7 relative accuracy of the offset even
+//     60 units from the origin) and accumulates the pair terms of up to 16
+//     targets whose leaf-relative coordinates sit in scalar registers.
+//     Accumulation is FP32, Kahan-compensated or plain; the 64 lane partials
+//     are combined in FP64.
+//  3. k_conv_finish, one thread per vortex: v += U_inf + near + far terms in
+//     the reference's order, FP64.
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace vvhd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kTargets = 16;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double k1_2Pi = 1. / (2. * kPi);
+constexpr double k1_Pi = 1. / kPi;
+
+inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+__device__ inline float readlane_f(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// Far-field Taylor coefficients of every leaf (MConvectiveFast.cpp:48-69).
+__global__ __launch_bounds__(kBlock) void k_far_taylor(TreeDev T, double4* taylor) {
+    const int w = threadIdx.x / kWave;
+    const int l = blockIdx.x * kWavesPerBlock + w;
+    if (l >= T.nleaf) return;
+    const int L = T.leaf_node[l];
+    const double cx = T.nx[L], cy = T.ny[L];
+    double a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    for (int64_t e = T.far_off[l] + lane_id(); e < T.far_off[l + 1]; e += kWave) {
+        const int F = T.far_list[e];
+        const double dPx = cx - T.cmp_x[F], dPy = cy - T.cmp_y[F];
+        const double dMx = cx - T.cmm_x[F], dMy = cy - T.cmm_y[F];
+        const double aP = dPx * dPx + dPy * dPy, aM = dMx * dMx + dMy * dMy;
+        const double fP1 = T.cmp_g[F] / aP, fM1 = T.cmm_g[F] / aM;
+        const double fP2 = T.cmp_g[F] / (aP * aP), fM2 = T.cmm_g[F] / (aM * aM);
+        a1 += (fP1 * dPy + fM1 * dMy);
+        a2 += (fP1 * dPx + fM1 * dMx);
+        a3 += (fP2 * dPy * dPx + fM2 * dMy * dMx);
+        a4 += (fP2 * (dPy * dPy - dPx * dPx) + fM2 * (dMy * dMy - dMx * dMx));
+    }
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    a3 = wave_sum(a3);
+    a4 = wave_sum(a4);
+    if (lane_id() == 0) taylor[l] = make_double4(-a1 * k1_2Pi, a2 * k1_2Pi, a3 * k1_Pi, a4 * k1_2Pi);
+}
+
+// Only what the near-field kernel reads (keeps its scalar registers free for
+// the target coordinates).
+struct NearArgs {
+    const int32_t* leaf_node;
+    const int32_t* v0;
+    const int32_t* v1;
+    const double* nx;
+    const double* ny;
+    const int64_t* near_off;
+    const int32_t* near_list;
+    int nleaf;
+};
+
+// Near-field sums of one chunk of NT targets (base .. base+nt-1, nt <= NT;
+// slots beyond nt hold far-away dummies whose sums are discarded), with no
+// per-target branches in the pair loop.
+template <int NT, bool KAHAN>
+__device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ x, const double* __restrict__ y,
+                                  const float2* __restrict__ src_ge, double2* __restrict__ near_out, int base, int nt,
+                                  double cx, double cy, int64_t e_beg, int64_t e_end) {
+    const int lane = lane_id();
+    float my_tx = 1e15f, my_ty = 1e15f;
+    if (lane < nt) {
+        my_tx = (float)(x[base + lane] - cx);
+        my_ty = (float)(y[base + lane] - cy);
+    }
+    float tx[NT], ty[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        tx[k] = readlane_f(my_tx, k);
+        ty[k] = readlane_f(my_ty, k);
+    }
+    float sx[NT], sy[NT], qx[NT], qy[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) sx[k] = sy[k] = qx[k] = qy[k] = 0.f;
+
+    for (int64_t eb = e_beg; eb < e_end; eb += kWave) {
+        const int64_t e = eb + lane;
+        int s0 = 0, cnt = 0;
+        if (e < e_end) {
+            const int M = A.near_list[e];
+            s0 = A.v0[M];
+            cnt = A.v1[M] - s0;
+        }
+        const int incl = wave_incl_scan(cnt);
+        const int total = __shfl(incl, kWave - 1, kWave);
+        for (int sb = 0; sb < total; sb += kWave) {
+            const int sp = sb + lane;
+            // entry index: number of entries whose inclusive prefix <= sp
+            int pos = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int v = __shfl(incl, pos + step - 1, kWave);
+                if (v <= sp) pos += step;
+            }
+            pos = min(pos, kWave - 1);
+            const int ex = __shfl(incl, pos, kWave) - __shfl(cnt, pos, kWave);
+            const int j = __shfl(s0, pos, kWave) + (sp - ex);
+            float sxr = 0.f, syr = 0.f, gj = 0.f, e2 = 1.f;
+            if (sp < total) {
+                const float2 ge = src_ge[j];
+                gj = ge.x;
+                e2 = ge.y;
+                sxr = (float)(x[j] - cx);
+                syr = (float)(y[j] - cy);
+            }
+#pragma unroll
+            for (int k = 0; k < NT; k++) {
+                const float dx = tx[k] - sxr;
+                const float dy = ty[k] - syr;
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, e2));
+                const float f = gj * __builtin_amdgcn_rcpf(r2);
+                if (KAHAN) {
+                    const float ux = __builtin_fmaf(-dy, f, -qx[k]);
+                    const float tsx = sx[k] + ux;
+                    qx[k] = (tsx - sx[k]) - ux;
+                    sx[k] = tsx;
+                    const float uy = __builtin_fmaf(dx, f, -qy[k]);
+                    const float tsy = sy[k] + uy;
+                    qy[k] = (tsy - sy[k]) - uy;
+                    sy[k] = tsy;
+                } else {
+                    sx[k] = __builtin_fmaf(-dy, f, sx[k]);
+                    sy[k] = __builtin_fmaf(dx, f, sy[k]);
+                }
+            }
+        }
+    }
+    double nearx = 0, neary = 0;
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        const double px = KAHAN ? (double)sx[k] - (double)qx[k] : (double)sx[k];
+        const double py = KAHAN ? (double)sy[k] - (double)qy[k] : (double)sy[k];
+        const double rx = wave_sum(px), ry = wave_sum(py);
+        if (lane == k) {
+            nearx = rx;
+            neary = ry;
+        }
+    }
+    if (lane < nt) near_out[base + lane] = make_double2(nearx, neary);
+}
+
+template <bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_near(NearArgs A, const double* __restrict__ x,
+                                                 const double* __restrict__ y, const float2* __restrict__ src_ge,
+                                                 double2* __restrict__ near_out) {
+    const int w = threadIdx.x / kWave;
+    const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
+    if (l >= A.nleaf) return;
+    const int L = A.leaf_node[l];
+    const int v0 = A.v0[L], v1 = A.v1[L];
+    const double cx = A.nx[L], cy = A.ny[L];
+    const int64_t e_beg = A.near_off[l], e_end = A.near_off[l + 1];
+    for (int base = v0; base < v1; base += kTargets) {
+        const int nt = min(kTargets, v1 - base);
+        if (nt <= 4)
+            near_chunk<4, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
+        else if (nt <= 8)
+            near_chunk<8, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
+        else if (nt <= 12)
+            near_chunk<12, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
+        else
+            near_chunk<16, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
+    }
+}
+
+// v += U_inf; += near * 1/(2 pi); += sinks (none); += bodies (none);
+// += (T1, T2); += (T3 dx + T4 dy, T4 dx - T3 dy)   (MConvectiveFast.cpp:76-86)
+__global__ void k_conv_finish(const int32_t* vleaf, const double* nx, const double* ny, const int32_t* leaf_node,
+                              const double4* taylor, const double* x, const double* y, const double* g,
+                              const double2* near_in, double* vx, double* vy, int64_t n, double inf_x, double inf_y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || g[i] == 0) return;
+    const int l = vleaf[i];
+    const int L = leaf_node[l];
+    const double4 T = taylor[l];
+    const double2 nr = near_in[i];
+    double ax = vx[i] + inf_x, ay = vy[i] + inf_y;
+    ax += nr.x * k1_2Pi;
+    ay += nr.y * k1_2Pi;
+    ax += 0.0;
+    ay += 0.0;
+    ax += 0.0;
+    ay += 0.0;
+    ax += T.x;
+    ay += T.y;
+    const double dxl = x[i] - nx[L], dyl = y[i] - ny[L];
+    ax += T.z * dxl + T.w * dyl;
+    ay += T.w * dxl + (-T.z) * dyl;
+    vx[i] = ax;
+    vy[i] = ay;
+}
+
+// MConvectiveFast::velocity for arbitrary points, FP64 (one wave per point).
+__global__ void k_velocity_points(TreeDev T, const double* x, const double* y, const double* g, const double* ie,
+                                  const double* pts, int64_t n, double inf_x, double inf_y, double* out) {
+    const int64_t p = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (p >= n) return;
+    const double px = pts[2 * p], py = pts[2 * p + 1];
+    int node = 0;   // stree::findNode
+    while (T.ch1[node] >= 0) {
+        const int c = T.ch1[node];
+        if (T.nh[node] < T.nw[node])
+            node = (px < T.nx[node]) ? c : c + 1;
+        else
+            node = (py < T.ny[node]) ? c : c + 1;
+    }
+    const int l = T.leaf_rank[node];
+    double nx_ = 0, ny_ = 0, fx = 0, fy = 0;
+    for (int64_t e = T.near_off[l]; e < T.near_off[l + 1]; e++) {
+        const int M = T.near_list[e];
+        for (int j = T.v0[M] + lane_id(); j < T.v1[M]; j += kWave) {
+            const double gj = g[j];
+            if (!gj) continue;
+            const double dx = px - x[j], dy = py - y[j];
+            const double e1 = 1. / ie[j];
+            const double f = gj / (dx * dx + dy * dy + e1 * e1);
+            nx_ += -dy * f;
+            ny_ += dx * f;
+        }
+    }
+    for (int64_t e = T.far_off[l] + lane_id(); e < T.far_off[l + 1]; e += kWave) {
+        const int F = T.far_list[e];
+        {
+            const double dx = px - T.cmp_x[F], dy = py - T.cmp_y[F];
+            const double e1 = 1. / (double)T.cmp_ie[F];
+            const double f = T.cmp_g[F] / (dx * dx + dy * dy + e1 * e1);
+            fx += -dy * f;
+            fy += dx * f;
+        }
+        {
+            const double dx = px - T.cmm_x[F], dy = py - T.cmm_y[F];
+            const double e1 = 1. / (double)T.cmm_ie[F];
+            const double f = T.cmm_g[F] / (dx * dx + dy * dy + e1 * e1);
+            fx += -dy * f;
+            fy += dx * f;
+        }
+    }
+    nx_ = wave_sum(nx_);
+    ny_ = wave_sum(ny_);
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    if (lane_id() == 0) {
+        double rx = 0, ry = 0;
+        rx += nx_ * k1_2Pi;
+        ry += ny_ * k1_2Pi;
+        rx += fx * k1_2Pi;
+        ry += fy * k1_2Pi;
+        rx += 0.0;
+        ry += 0.0;
+        rx += inf_x;
+        ry += inf_y;
+        out[2 * p] = rx;
+        out[2 * p + 1] = ry;
+    }
+}
+
+// FP64 direct sum of the induced velocity, compensated; sources staged in
+// LDS, one thread per target.
+constexpr int kDsBlock = 256;
+__global__ __launch_bounds__(kDsBlock) void k_direct_sum(const double* x, const double* y, const double* g,
+                                                         const double* ie, int64_t n, const int64_t* targets,
+                                                         int64_t nt, double* out) {
+    __shared__ double sx[kDsBlock], sy[kDsBlock], sg[kDsBlock], se[kDsBlock];
+    const int64_t t = (int64_t)blockIdx.x * kDsBlock + threadIdx.x;
+    const int64_t i = (t < nt) ? (targets ? targets[t] : t) : 0;
+    const double px = (t < nt) ? x[i] : 0, py = (t < nt) ? y[i] : 0;
+    double ax = 0, ay = 0, cxs = 0, cys = 0;
+    for (int64_t b = 0; b < n; b += kDsBlock) {
+        const int64_t j = b + threadIdx.x;
+        if (j < n && g[j] != 0) {
+            const double e1 = 1. / ie[j];
+            sx[threadIdx.x] = x[j];
+            sy[threadIdx.x] = y[j];
+            sg[threadIdx.x] = g[j];
+            se[threadIdx.x] = e1 * e1;
+        } else {
+            sx[threadIdx.x] = 0;
+            sy[threadIdx.x] = 0;
+            sg[threadIdx.x] = 0;
+            se[threadIdx.x] = 1;
+        }
+        __syncthreads();
+        const int m = (int)std::min<int64_t>(kDsBlock, n - b);
+        for (int q = 0; q < m; q++) {
+            const double dx = px - sx[q], dy = py - sy[q];
+            const double f = sg[q] / (dx * dx + dy * dy + se[q]);
+            const double tx = -dy * f - cxs;
+            const double ux = ax + tx;
+            cxs = (ux - ax) - tx;
+            ax = ux;
+            const double ty = dx * f - cys;
+            const double uy = ay + ty;
+            cys = (uy - ay) - ty;
+            ay = uy;
+        }
+        __syncthreads();
+    }
+    if (t < nt) {
+        out[2 * t] = ax * k1_2Pi;
+        out[2 * t + 1] = ay * k1_2Pi;
+    }
+}
+
+}  // namespace
+
+void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
+    const TreeDev T = tree_dev(c.t);
+    if (T.nleaf == 0) return;
+    const int nl = T.nleaf;
+    c.taylor.grow(nl + 1);
+    c.near_v.grow(c.n + 1);
+    const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, c.taylor.p);
+    NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
+    if (kahan)
+        hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.x.p, c.y.p, c.src_ge.p,
+                           c.near_v.p);
+    else
+        hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.x.p, c.y.p, c.src_ge.p,
+                           c.near_v.p);
+    hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
+                       c.taylor.p, c.x.p, c.y.p, c.g.p, c.near_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
+    VVHD_HIP(hipGetLastError());
+}
+
+void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out) {
+    if (n == 0) return;
+    const TreeDev T = tree_dev(c.t);
+    c.tmp.grow((size_t)n * 32);
+    double* dpts = (double*)c.tmp.p;
+    double* dout = dpts + 2 * n;
+    VVHD_HIP(hipMemcpyAsync(dpts, xy, (size_t)n * 16, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_velocity_points, dim3((unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock)), dim3(kBlock), 0,
+                       c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p, dpts, n, inf_x, inf_y, dout);
+    VVHD_HIP(hipMemcpyAsync(v_out, dout, (size_t)n * 16, hipMemcpyDeviceToHost, c.stream));
+    VVHD_HIP(hipStreamSynchronize(c.stream));
+}
+
+void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out) {
+    if (nt == 0) return;
+    c.lscratch0.grow((size_t)nt + 1);
+    c.tmp.grow((size_t)nt * 16);
+    if (targets) VVHD_HIP(hipMemcpyAsync(c.lscratch0.p, targets, (size_t)nt * 8, hipMemcpyHostToDevice, c.stream));
+    double* dout = (double*)c.tmp.p;
+    hipLaunchKernelGGL(k_direct_sum, dim3((unsigned)((nt + kDsBlock - 1) / kDsBlock)), dim3(kDsBlock), 0, c.stream,
+                       c.x.p, c.y.p, c.g.p, c.ie.p, c.n, targets ? c.lscratch0.p : nullptr, nt, dout);
+    VVHD_HIP(hipMemcpyAsync(v_out, dout, (size_t)nt * 16, hipMemcpyDeviceToHost, c.stream));
+    VVHD_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace vvhd

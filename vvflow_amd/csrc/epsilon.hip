@@ -1,0 +1,598 @@
+// epsilon.hip -- MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:11-63) on
+// the GPU, with the reference's sequential merge semantics.
+//
+// The reference visits targets leaf by leaf in DFS order, i.e. in array
+// order, and for each searches the two nearest vortices (g != 0, not itself)
+// of its near leaves; ties go to the first found.  Because leaves in DFS
+// order hold increasing array ranges, "first found" is "smaller index", so
+// the result is the top-2 under the order (d^2, index): order independent,
+// hence computable in parallel.  Distances are FP64 with the reference's
+// operation order (no contraction), so indices are bit-exact.
+//
+// A merge (:111-126) mutates the target and zeroes its partner; every later
+// target sees the mutated state.  Pass 1 computes every target on the
+// unmodified state and flags the targets that would merge.  When flags exist,
+// a single-workgroup pass commits merges in target order: after each commit
+// it updates the targets that come later and have the merged leaves in their
+// near lists -- in O(1) when the moved vortex only enters their top-2, by a
+// full rescan when one of their two neighbours changed -- and re-evaluates
+// their merge flags, exactly reproducing the sequential outcome.
+//
+// Pass 1 prunes near leaves by box distance: leaves touching the target leaf
+// are scanned first, then only leaves whose box distance does not exceed the
+// largest second-neighbour distance found.  Box distances are computed with
+// the same monotone rounded operations, so pruning never drops a candidate.
+
+#include "common.hpp"
+
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace vvhd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+__device__ inline double sgn(double x) { return (x > 0) ? 1 : ((x < 0) ? -1 : 0); }
+
+// Top-2 state under the order (d, index); index -1 marks an empty slot whose
+// distance is DBL_MAX (the reference's initial res1/res2).
+struct Top2 {
+    double d1, d2;
+    int i1, i2;
+};
+
+__device__ inline bool less_di(double d, int j, double dc, int ic) {
+    return d < dc || (d == dc && ic >= 0 && j < ic);
+}
+
+__device__ inline void top2_insert(Top2& t, double d, int j) {
+    if (less_di(d, j, t.d1, t.i1)) {
+        t.d2 = t.d1;
+        t.i2 = t.i1;
+        t.d1 = d;
+        t.i1 = j;
+    } else if (less_di(d, j, t.d2, t.i2)) {
+        t.d2 = d;
+        t.i2 = j;
+    }
+}
+
+__device__ inline void top2_merge(Top2& t, const Top2& o) {
+    if (o.i1 >= 0) top2_insert(t, o.d1, o.i1);
+    if (o.i2 >= 0) top2_insert(t, o.d2, o.i2);
+}
+
+__device__ inline Top2 top2_shfl_xor(const Top2& t, int mask) {
+    Top2 o;
+    o.d1 = __shfl_xor(t.d1, mask, kWave);
+    o.d2 = __shfl_xor(t.d2, mask, kWave);
+    o.i1 = __shfl_xor(t.i1, mask, kWave);
+    o.i2 = __shfl_xor(t.i2, mask, kWave);
+    return o;
+}
+
+__device__ inline Top2 top2_empty() { return Top2{DBL_MAX, DBL_MAX, -1, -1}; }
+
+// (lv->r - lobj->r).abs2() of MEpsilonFast.cpp:141
+__device__ inline double dist2(double tx, double ty, double sx, double sy) {
+    const double dx = tx - sx, dy = ty - sy;
+    return dx * dx + dy * dy;
+}
+
+// nearestBodySegment (MEpsilonFast.cpp:214-253) for every leaf centre, then
+// the merge criterion and the epsilon floor (:29-47).
+__global__ void k_leaf_criteria(TreeDev T, const vvhd_seg* segs, const int32_t* body_range, int nbody, int merge,
+                                double* crit_out, double* restr_out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= T.nleaf) return;
+    if (nbody == 0) {   // no segment anywhere: criterion 0 (or NaN), floor 0
+        crit_out[l] = merge ? 0.0 : __builtin_nan("");
+        restr_out[l] = 0;
+        return;
+    }
+    const int L = T.leaf_node[l];
+    const double px = T.nx[L], py = T.ny[L];
+    int att = -1;
+    double res = DBL_MAX;
+    for (int64_t e = T.near_off[l]; e < T.near_off[l + 1]; e++) {
+        const int m = T.leaf_rank[T.near_list[e]];
+        for (int64_t q = T.seg_off[m]; q < T.seg_off[m + 1]; q++) {
+            const int s = T.seg_list[q];
+            const double d2 = dist2(px, py, segs[s].x, segs[s].y);
+            if (d2 < res) {
+                res = d2;
+                att = s;
+            }
+        }
+    }
+    if (att < 0) {
+        for (int b = 0; b < nbody; b++) {
+            const int f = body_range[2 * b], cnt = body_range[2 * b + 1];
+            for (int k = 0; k < cnt; k++) {
+                const double d2 = dist2(px, py, segs[f + k].x, segs[f + k].y);
+                if (d2 < res) {
+                    res = d2;
+                    att = f + k;
+                } else {
+                    k += 9;   // the reference's "speed up" skip
+                }
+            }
+        }
+    }
+    double crit;
+    if (merge) {
+        if (att >= 0) {
+            const double dlx = segs[att].dlx, dly = segs[att].dly;
+            const double ddx = px - segs[att].x, ddy = py - segs[att].y;
+            crit = 0.16 * (dlx * dlx + dly * dly) * (1. + sqrt(ddx * ddx + ddy * ddy));
+        } else {
+            crit = 0;
+        }
+    } else {
+        crit = __builtin_nan("");
+    }
+    double restr = 0;
+    if (att >= 0) {
+        const double dlx = segs[att].dlx, dly = segs[att].dly;
+        restr = sqrt(dlx * dlx + dly * dly) * (1.0 / 3.0);
+    }
+    crit_out[l] = crit;
+    restr_out[l] = restr;
+}
+
+// Scans the sources of the near leaves of leaf l whose box distance d2b to
+// the leaf satisfies lo < d2b <= hi (lo = -1: include touching leaves) for
+// the 16 targets of the current chunk.  Lane = (target k, slot s): slot s
+// visits every 4th source of a leaf.
+__device__ inline void scan_sources(const TreeDev& T, int l, int L, const double* x, const double* y, const double* g,
+                                    double lo, double hi, int slot, bool active, int i, double tx, double ty,
+                                    Top2& st) {
+    const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
+    for (int64_t eb = e0; eb < e1; eb += kWave) {
+        const int64_t e = eb + lane_id();
+        int M = -1;
+        bool q = false;
+        if (e < e1) {
+            M = T.near_list[e];
+            const double bd = box_dist2(T, L, M);
+            q = (bd > lo) && (bd <= hi);
+        }
+        uint64_t mask = __ballot(q);
+        while (mask) {
+            const int b = (int)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            const int Mb = __shfl(M, b, kWave);
+            const int s0 = T.v0[Mb], s1 = T.v1[Mb];
+            for (int j = s0 + slot; j < s1; j += 4) {
+                const double gj = g[j];
+                if (active && gj != 0 && j != i) top2_insert(st, dist2(tx, ty, x[j], y[j]), j);
+            }
+        }
+    }
+}
+
+// Pass 1: top-2 of every target on the unmodified state (one wave per leaf).
+__global__ __launch_bounds__(kBlock) void k_eps_top2(TreeDev T, const double* x, const double* y, const double* g,
+                                                     int merge, const double* leaf_crit, int32_t* nn1, int32_t* nn2,
+                                                     double* d1o, double* d2o, uint8_t* eflag, int32_t* ecount,
+                                                     int32_t* flag_list) {
+    const int w = threadIdx.x / kWave;
+    const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
+    if (l >= T.nleaf) return;
+    const int lane = lane_id();
+    const int k = lane & 15, slot = lane >> 4;
+    const int L = T.leaf_node[l];
+    const int v0 = T.v0[L], v1 = T.v1[L];
+    const double crit = leaf_crit[l];
+    for (int base = v0; base < v1; base += 16) {
+        const int i = base + k;
+        const bool valid = i < v1;
+        const double tx = valid ? x[i] : 0, ty = valid ? y[i] : 0, tg = valid ? g[i] : 0;
+        const bool active = valid && tg != 0;
+        Top2 st = top2_empty();
+        scan_sources(T, l, L, x, y, g, -1.0, 0.0, slot, active, i, tx, ty, st);
+        // combine the 4 slots of each target into a bound
+        Top2 c = st;
+        top2_merge(c, top2_shfl_xor(c, 16));
+        top2_merge(c, top2_shfl_xor(c, 32));
+        double bound = active ? (c.i2 >= 0 ? c.d2 : INFINITY) : -INFINITY;
+        bound = wave_max(bound);
+        if (bound > 0) scan_sources(T, l, L, x, y, g, 0.0, bound, slot, active, i, tx, ty, st);
+        top2_merge(st, top2_shfl_xor(st, 16));
+        top2_merge(st, top2_shfl_xor(st, 32));
+        if (slot == 0 && valid) {
+            uint8_t f = 0;
+            int n1 = -1, n2 = -1;
+            if (active) {
+                f = 1;
+                n1 = st.i1;
+                n2 = st.i2;
+                if (n1 >= 0 && n2 >= 0 && !isnan(crit)) {
+                    const double s1 = sgn(g[n1]), s2 = sgn(g[n2]), si = sgn(tg);
+                    if ((st.d1 < crit) || ((s1 == s2) && (s1 != si))) {
+                        f |= 2;
+                        flag_list[atomicAdd(&ecount[0], 1)] = i;
+                    }
+                }
+            }
+            nn1[i] = n1;
+            nn2[i] = n2;
+            d1o[i] = st.d1;
+            d2o[i] = st.d2;
+            eflag[i] = f;
+        }
+    }
+}
+
+// Squared distance from a point to a node's box (0 inside); never exceeds
+// the computed squared distance to any point of the box.
+__device__ inline double point_box_dist2(const TreeDev& T, int M, double px, double py) {
+    const double dx = fmax(0.0, fmax(double_from_ord(T.blx[M]) - px, px - double_from_ord(T.trx[M])));
+    const double dy = fmax(0.0, fmax(double_from_ord(T.bly[M]) - py, py - double_from_ord(T.try_[M])));
+    return dx * dx + dy * dy;
+}
+
+// Sources of the near leaves of leaf l selected by sel(M) for one target i,
+// lanes over each selected leaf's vortices.
+template <typename Sel>
+__device__ inline void scan_one(const TreeDev& T, int l, const double* x, const double* y, const double* g, int i,
+                                double tx, double ty, Top2& st, Sel sel) {
+    const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
+    for (int64_t eb = e0; eb < e1; eb += kWave) {
+        const int64_t e = eb + lane_id();
+        int M = -1;
+        bool q = false;
+        if (e < e1) {
+            M = T.near_list[e];
+            q = sel(M);
+        }
+        uint64_t mask = __ballot(q);
+        while (mask) {
+            const int b = (int)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            const int Mb = __shfl(M, b, kWave);
+            const int s0 = T.v0[Mb], s1 = T.v1[Mb];
+            for (int j = s0 + lane_id(); j < s1; j += kWave) {
+                const double gj = g[j];
+                if (gj != 0 && j != i) top2_insert(st, dist2(tx, ty, x[j], y[j]), j);
+            }
+        }
+    }
+}
+
+__device__ inline Top2 wave_top2(Top2 st) {
+    for (int o = kWave / 2; o > 0; o >>= 1) top2_merge(st, top2_shfl_xor(st, o));
+    return st;
+}
+
+// Full top-2 search of one target by one wave over its near leaves: leaves
+// touching the target's leaf first, then those within the bound found.
+__device__ Top2 rescan_target(const TreeDev& T, const double* x, const double* y, const double* g, int i) {
+    const int l = T.vleaf[i];
+    const int L = T.leaf_node[l];
+    const double tx = x[i], ty = y[i];
+    Top2 st = top2_empty();
+    scan_one(T, l, x, y, g, i, tx, ty, st, [&](int M) { return box_dist2(T, L, M) <= 0.0; });
+    const Top2 c = wave_top2(st);
+    const double bound = c.i2 >= 0 ? c.d2 : INFINITY;
+    scan_one(T, l, x, y, g, i, tx, ty, st,
+             [&](int M) { return box_dist2(T, L, M) > 0.0 && point_box_dist2(T, M, tx, ty) <= bound; });
+    return wave_top2(st);
+}
+
+__device__ inline bool merge_flag(const double* g, int i, const Top2& st, double crit) {
+    if (st.i1 < 0 || st.i2 < 0 || isnan(crit)) return false;
+    const double s1 = sgn(g[st.i1]), s2 = sgn(g[st.i2]), si = sgn(g[i]);
+    return (st.d1 < crit) || ((s1 == s2) && (s1 != si));
+}
+
+// Transposed near lists: for every leaf M the leaves L with M in near(L).
+__global__ void k_rnear_count(TreeDev T, int32_t* cnt) {
+    const int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (l >= T.nleaf) return;
+    for (int64_t e = T.near_off[l] + lane_id(); e < T.near_off[l + 1]; e += kWave)
+        atomicAdd(&cnt[T.leaf_rank[T.near_list[e]]], 1);
+}
+
+__global__ void k_rnear_fill(TreeDev T, const int64_t* roff, int32_t* fillc, int32_t* rlist) {
+    const int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (l >= T.nleaf) return;
+    for (int64_t e = T.near_off[l] + lane_id(); e < T.near_off[l + 1]; e += kWave) {
+        const int m = T.leaf_rank[T.near_list[e]];
+        rlist[roff[m] + atomicAdd(&fillc[m], 1)] = l;
+    }
+}
+
+constexpr int kRepairThreads = 1024;
+constexpr int kRepairWaves = kRepairThreads / kWave;
+constexpr int kMaxHard = 2048;
+constexpr int kMaxPend = 8192;
+
+// Pass 2: commits merges in target order (single workgroup).  Candidates are
+// kept in an LDS list (the flags of pass 1 plus targets flagged by later
+// updates); the next commit is the smallest listed index at or after the
+// cursor whose flag is still set.  If the list overflows, the remaining
+// commits find their target by scanning the flags.
+__global__ __launch_bounds__(kRepairThreads) void k_eps_repair(TreeDev T, double* x, double* y, double* g, int64_t n,
+                                                               const double* leaf_crit, const int64_t* roff,
+                                                               const int32_t* rlist, int32_t* stamp, int32_t* nn1,
+                                                               int32_t* nn2, double* d1a, double* d2a, uint8_t* eflag,
+                                                               int32_t* ecount, const int32_t* flag_list,
+                                                               uint64_t* box_lo_x, uint64_t* box_lo_y,
+                                                               uint64_t* box_hi_x, uint64_t* box_hi_y) {
+    __shared__ int s_q;
+    __shared__ int s_nhard;
+    __shared__ int s_hard[kMaxHard];
+    __shared__ int s_overflow;
+    __shared__ int s_np;
+    __shared__ int s_pend[kMaxPend];
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    int64_t cursor = 0;
+    int commit = 0;
+    if (tid == 0) {
+        s_overflow = 0;
+        s_np = min(ecount[0], kMaxPend);
+    }
+    __syncthreads();
+    const bool scan_mode0 = ecount[0] > kMaxPend;
+    for (int k = tid; k < s_np; k += kRepairThreads) s_pend[k] = flag_list[k];
+    __syncthreads();
+    while (true) {
+        if (tid == 0) s_q = INT32_MAX;
+        __syncthreads();
+        const bool scan_mode = scan_mode0 || s_np > kMaxPend;
+        if (!scan_mode) {
+            for (int k = tid; k < s_np; k += kRepairThreads) {
+                const int i = s_pend[k];
+                if (i >= cursor && (eflag[i] & 2)) atomicMin(&s_q, i);
+            }
+            __syncthreads();
+        } else {
+            for (int64_t b = cursor; b < n; b += kRepairThreads) {
+                const int64_t i = b + tid;
+                if (i < n && (eflag[i] & 2)) atomicMin(&s_q, (int)i);
+                __syncthreads();
+                if (s_q != INT32_MAX) break;
+                __syncthreads();
+            }
+            __syncthreads();
+        }
+        const int q = s_q;
+        if (q == INT32_MAX) break;
+        commit++;
+        const int p = nn1[q];
+        // MergeVortexes(q, p) (MEpsilonFast.cpp:111-126)
+        if (tid == 0) {
+            const double gq = g[q], gp = g[p];
+            if (sgn(gq) == sgn(gp)) {
+                const double inv = 1. / (gq + gp);
+                const double nx = (x[q] * gq + x[p] * gp) * inv;
+                const double ny = (y[q] * gq + y[p] * gp) * inv;
+                x[q] = nx;
+                y[q] = ny;
+            } else if (fabs(gq) < fabs(gp)) {
+                x[q] = x[p];
+                y[q] = y[p];
+            }
+            g[q] = gq + gp;
+            g[p] = 0;
+            ecount[1] += 1;
+            // the merged vortex may leave its leaf's box: grow the box so box
+            // distances stay lower bounds (pruned rescans, diffusive cutoff)
+            const int Lq = T.leaf_node[T.vleaf[q]];
+            if (!isnan(x[q])) {
+                atomicMin((unsigned long long*)&box_lo_x[Lq], (unsigned long long)ord_from_double(x[q]));
+                atomicMax((unsigned long long*)&box_hi_x[Lq], (unsigned long long)ord_from_double(x[q]));
+            }
+            if (!isnan(y[q])) {
+                atomicMin((unsigned long long*)&box_lo_y[Lq], (unsigned long long)ord_from_double(y[q]));
+                atomicMax((unsigned long long*)&box_hi_y[Lq], (unsigned long long)ord_from_double(y[q]));
+            }
+            if (p > q) {   // skipped later by the reference (g == 0)
+                eflag[p] = 0;
+                nn1[p] = -1;
+                nn2[p] = -1;
+            }
+            s_nhard = 0;
+        }
+        __threadfence_block();
+        __syncthreads();
+        // the target itself: search again without merging
+        if (wave == 0) {
+            const Top2 st = rescan_target(T, x, y, g, q);
+            if (lane_id() == 0) {
+                nn1[q] = st.i1;
+                nn2[q] = st.i2;
+                d1a[q] = st.d1;
+                d2a[q] = st.d2;
+                eflag[q] = 1;
+            }
+        }
+        // mark the leaves whose near list contains leaf(q)
+        const int lq = T.vleaf[q], lp = T.vleaf[p];
+        for (int64_t e = roff[lq] + tid; e < roff[lq + 1]; e += kRepairThreads) stamp[rlist[e]] = commit;
+        __threadfence_block();
+        __syncthreads();
+        const double qx = x[q], qy = y[q];
+        // later targets that see leaf(q) (then those that see only leaf(p))
+        for (int pass = 0; pass < 2; pass++) {
+            const int lm = pass == 0 ? lq : lp;
+            if (pass == 1 && lp == lq) break;
+            // 64 leaves at a time, 16 threads per leaf
+            for (int64_t e0 = roff[lm]; e0 < roff[lm + 1]; e0 += kRepairThreads / 16) {
+                const int64_t e = e0 + tid / 16;
+                if (e >= roff[lm + 1]) continue;
+                const int L = rlist[e];
+                if (pass == 1 && stamp[L] == commit) continue;
+                const int Ln = T.leaf_node[L];
+                const int a0 = T.v0[Ln], a1 = T.v1[Ln];
+                for (int i = a0 + (tid & 15); i < a1; i += 16) {
+                    if (i <= q || !(eflag[i] & 1)) continue;
+                    const bool hard = nn1[i] == q || nn1[i] == p || nn2[i] == q || nn2[i] == p;
+                    if (hard) {
+                        const int h = atomicAdd(&s_nhard, 1);
+                        if (h < kMaxHard)
+                            s_hard[h] = i;
+                        else
+                            s_overflow = 1;
+                        continue;
+                    }
+                    if (pass == 1) continue;   // q is not in the near set of i
+                    Top2 st{d1a[i], d2a[i], nn1[i], nn2[i]};
+                    top2_insert(st, dist2(x[i], y[i], qx, qy), q);
+                    nn1[i] = st.i1;
+                    nn2[i] = st.i2;
+                    d1a[i] = st.d1;
+                    d2a[i] = st.d2;
+                    const bool f = merge_flag(g, i, st, leaf_crit[L]);
+                    eflag[i] = 1 | (f ? 2 : 0);
+                    if (f) {
+                        const int k = atomicAdd(&s_np, 1);
+                        if (k < kMaxPend) s_pend[k] = i;
+                    }
+                }
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        const int nh = min(s_nhard, kMaxHard);
+        for (int h = wave; h < nh; h += kRepairWaves) {
+            const int i = s_hard[h];
+            const Top2 st = rescan_target(T, x, y, g, i);
+            if (lane_id() == 0) {
+                nn1[i] = st.i1;
+                nn2[i] = st.i2;
+                d1a[i] = st.d1;
+                d2a[i] = st.d2;
+                const bool f = merge_flag(g, i, st, leaf_crit[T.vleaf[i]]);
+                eflag[i] = 1 | (f ? 2 : 0);
+                if (f) {
+                    const int k = atomicAdd(&s_np, 1);
+                    if (k < kMaxPend) s_pend[k] = i;
+                }
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        cursor = q + 1;
+    }
+    if (tid == 0 && s_overflow) ecount[2] = 1;
+}
+
+// Applies _1_eps = 1/max(eps, floor) to every processed target (:49-54) and
+// packs the FP32 {g, eps^2} copy the pair kernels read.
+__global__ void k_eps_apply(const TreeDev T, const double* g, double* ie, int64_t n, const int32_t* nn1,
+                            const int32_t* nn2, const double* d1a, const double* d2a, const uint8_t* eflag,
+                            const double* leaf_restr, float2* src_ge) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double iv = ie[i];
+    if (eflag[i] & 1) {
+        double eps;
+        if (nn1[i] < 0)
+            eps = DBL_MIN;
+        else if (nn2[i] < 0)
+            eps = sqrt(d1a[i]);
+        else
+            eps = sqrt(d2a[i]);
+        const double r = leaf_restr[T.vleaf[i]];
+        iv = 1.0 / ((eps < r) ? r : eps);
+        ie[i] = iv;
+    }
+    const double gi = g[i];
+    if (gi != 0) {
+        const double e = 1. / iv;
+        src_ge[i] = make_float2((float)gi, (float)(e * e));
+    } else {
+        src_ge[i] = make_float2(0.f, 1.f);
+    }
+}
+
+__global__ void k_pack(const double* g, const double* ie, int64_t n, float2* src_ge) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double gi = g[i];
+    if (gi != 0) {
+        const double e = 1. / ie[i];
+        src_ge[i] = make_float2((float)gi, (float)(e * e));
+    } else {
+        src_ge[i] = make_float2(0.f, 1.f);
+    }
+}
+
+}  // namespace
+
+void pack_sources(Ctx& c) {
+    if (c.n)
+        hipLaunchKernelGGL(k_pack, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, c.g.p, c.ie.p, c.n, c.src_ge.p);
+}
+
+void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* merged_out) {
+    const int64_t n = c.n;
+    const TreeDev T = tree_dev(c.t);
+    const int nl = T.nleaf;
+    hipStream_t s = c.stream;
+    c.nn1.grow(n + 1);
+    c.nn2.grow(n + 1);
+    c.d1.grow(n + 1);
+    c.d2.grow(n + 1);
+    c.eflag.grow(n + 1);
+    c.flag_list.grow(n + 1);
+    c.ecount.grow(8);
+    c.leaf_crit.grow(nl + 1);
+    c.leaf_restr.grow(nl + 1);
+    VVHD_HIP(hipMemsetAsync(c.ecount.p, 0, 8 * 4, s));
+    const int nbody = (int)c.bodies_h.size();
+    hipLaunchKernelGGL(k_leaf_criteria, dim3(nblocks(nl)), dim3(kBlock), 0, s, T, c.segs.p, c.body_range.p, nbody,
+                       merge, c.leaf_crit.p, c.leaf_restr.p);
+    if (nl > 0)
+        hipLaunchKernelGGL(k_eps_top2, dim3((nl + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0, s, T, c.x.p,
+                           c.y.p, c.g.p, merge, c.leaf_crit.p, c.nn1.p, c.nn2.p, c.d1.p, c.d2.p, c.eflag.p,
+                           c.ecount.p, c.flag_list.p);
+    int32_t hc[4] = {0, 0, 0, 0};
+    VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 16, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipStreamSynchronize(s));
+    if (hc[0] > 0) {
+        c.rnear_off.grow(nl + 1);
+        c.rnear_cnt.grow(nl + 1);
+        c.leaf_stamp.grow(nl + 1);
+        c.rnear_list.grow(c.t.nnear + 1);
+        VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
+        VVHD_HIP(hipMemsetAsync(c.leaf_stamp.p, 0, (size_t)(nl + 1) * 4, s));
+        hipLaunchKernelGGL(k_rnear_count, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_cnt.p);
+        {
+            size_t bytes = 0;
+            VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
+                                             rocprim::plus<int64_t>(), s));
+            c.tmp.grow(bytes + 16);
+            VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
+                                             rocprim::plus<int64_t>(), s));
+        }
+        VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
+        hipLaunchKernelGGL(k_rnear_fill, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_off.p, c.rnear_cnt.p,
+                           c.rnear_list.p);
+        hipLaunchKernelGGL(k_eps_repair, dim3(1), dim3(kRepairThreads), 0, s, T, c.x.p, c.y.p, c.g.p, n,
+                           c.leaf_crit.p, c.rnear_off.p, c.rnear_list.p, c.leaf_stamp.p, c.nn1.p, c.nn2.p, c.d1.p,
+                           c.d2.p, c.eflag.p, c.ecount.p, c.flag_list.p, c.t.blx.p, c.t.bly.p, c.t.trx.p,
+                           c.t.try_.p);
+    }
+    hipLaunchKernelGGL(k_eps_apply, dim3(nblocks(n)), dim3(kBlock), 0, s, T, c.g.p, c.ie.p, n, c.nn1.p, c.nn2.p,
+                       c.d1.p, c.d2.p, c.eflag.p, c.leaf_restr.p, c.src_ge.p);
+    VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 16, hipMemcpyDeviceToHost, s));
+    if (nn1_out && n) VVHD_HIP(hipMemcpyAsync(nn1_out, c.nn1.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    if (nn2_out && n) VVHD_HIP(hipMemcpyAsync(nn2_out, c.nn2.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipStreamSynchronize(s));
+    VVHD_CHECK(hc[2] == 0, VVHD_EUNSUPPORTED, "epsilon: too many targets to rescan after one merge");
+    if (merged_out) *merged_out = hc[1];
+}
+
+}  // namespace vvhd

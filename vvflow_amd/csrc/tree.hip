@@ -1,0 +1,699 @@
+// tree.hip -- the adaptive binary tree of TSortedTree.cpp rebuilt
+// level-synchronously on the GPU.
+//
+// stree::build (TSortedTree.cpp:232-265) splits nodes recursively at the
+// centre of their tight bounding box with an in-place Hoare partition
+// (DistributeContent, :81-99).  That partition pairs the k-th element left
+// of the split position that belongs right with the k-th element right of
+// it, counted from the end, that belongs left; everything else stays.  Each
+// level is therefore replayed exactly with one exclusive scan of the
+// "goes left" flags: misplaced elements get their rank on their side and
+// swap with the partner of complementary rank.  Bounding boxes are exact
+// FP64 min/max (atomics on an order-preserving integer image), so node
+// centres, leaf decisions and the resulting permutation are bit-identical to
+// the reference.  Segments (body panels) follow a stable partition
+// (:139-148), i.e. each segment descends like findNode and keeps its order.
+//
+// Node ids are allocated level by level, so every level is a contiguous id
+// range.  After the levels: subtree leaf counts (bottom-up) and leaf DFS
+// ranks (top-down) give the bottomNodes order; per-sign monopoles
+// (CalculateCMass, :150-197) are combined bottom-up level by level; and the
+// near/far lists of every leaf (FindNearNodes, :199-217) come from one
+// traversal per wave of 64 DFS-consecutive leaves with a shared stack.
+
+#include "common.hpp"
+
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace vvhd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxStack = 128;
+
+inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+// --------------------------------------------------------------- kernels --
+
+struct NodeView {
+    double *x, *y, *h, *w;
+    uint64_t *blx, *bly, *trx, *try_;
+    int32_t *v0, *v1, *ch1, *parent, *depth, *nseg, *split;
+    uint64_t* path;
+};
+
+NodeView view(TreeArrays& t) {
+    return NodeView{t.x.p,  t.y.p,   t.h.p,      t.w.p,     t.blx.p,   t.bly.p,  t.trx.p,   t.try_.p,
+                    t.v0.p, t.v1.p,  t.ch1.p,    t.parent.p, t.depth.p, t.nseg.p, t.split.p, t.path.p};
+}
+
+__device__ inline void init_node(NodeView nv, int id, int parent, int depth, uint64_t path, int v0, int v1) {
+    nv.parent[id] = parent;
+    nv.depth[id] = depth;
+    nv.path[id] = path;
+    nv.v0[id] = v0;
+    nv.v1[id] = v1;
+    nv.ch1[id] = -1;
+    nv.nseg[id] = 0;
+    nv.split[id] = 0;
+    nv.blx[id] = ord_from_double(DBL_MAX);
+    nv.bly[id] = ord_from_double(DBL_MAX);
+    nv.trx[id] = ord_from_double(-DBL_MAX);
+    nv.try_[id] = ord_from_double(-DBL_MAX);
+}
+
+__global__ void k_init_root(NodeView nv, int n, int nseg, int32_t* counters) {
+    init_node(nv, 0, -1, 0, 0, 0, n);
+    nv.nseg[0] = nseg;
+    counters[0] = 1;   // nodes
+    counters[1] = 0;   // leaves
+    counters[2] = 0;   // level begin
+    counters[3] = 1;   // level end
+    counters[4] = 0;   // overflow flag
+    counters[5] = 0;   // max depth
+}
+
+// Adds a point to a node's bounding box; NaN coordinates are ignored, as by
+// std::max/std::min in Stretch.
+__device__ inline void bbox_add(NodeView nv, int node, double px, double py) {
+    if (!isnan(px)) {
+        atomicMin((unsigned long long*)&nv.blx[node], (unsigned long long)ord_from_double(px));
+        atomicMax((unsigned long long*)&nv.trx[node], (unsigned long long)ord_from_double(px));
+    }
+    if (!isnan(py)) {
+        atomicMin((unsigned long long*)&nv.bly[node], (unsigned long long)ord_from_double(py));
+        atomicMax((unsigned long long*)&nv.try_[node], (unsigned long long)ord_from_double(py));
+    }
+}
+
+// Wave-aggregated bbox update: lanes targeting the same node reduce first and
+// one lane performs the atomics, for up to four distinct nodes per wave; the
+// rest fall back to per-lane atomics.  Called by all lanes of the wave.
+__device__ inline void bbox_add_wave(NodeView nv, int node, double px, double py, bool active) {
+    uint64_t left = __ballot(active);
+    for (int round = 0; round < 4 && left; round++) {
+        const int leader = (int)__builtin_ctzll(left);
+        const int lead_node = __shfl(node, leader, kWave);
+        const bool mine = active && ((left >> lane_id()) & 1ull) && node == lead_node;
+        const uint64_t m_same = __ballot(mine);
+        const bool vx = mine && !isnan(px), vy = mine && !isnan(py);
+        double lx = vx ? px : DBL_MAX, hx = vx ? px : -DBL_MAX;
+        double ly = vy ? py : DBL_MAX, hy = vy ? py : -DBL_MAX;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            lx = fmin(lx, __shfl_xor(lx, o, kWave));
+            hx = fmax(hx, __shfl_xor(hx, o, kWave));
+            ly = fmin(ly, __shfl_xor(ly, o, kWave));
+            hy = fmax(hy, __shfl_xor(hy, o, kWave));
+        }
+        if (lane_id() == leader) {
+            if (lx <= hx) {
+                atomicMin((unsigned long long*)&nv.blx[lead_node], (unsigned long long)ord_from_double(lx));
+                atomicMax((unsigned long long*)&nv.trx[lead_node], (unsigned long long)ord_from_double(hx));
+            }
+            if (ly <= hy) {
+                atomicMin((unsigned long long*)&nv.bly[lead_node], (unsigned long long)ord_from_double(ly));
+                atomicMax((unsigned long long*)&nv.try_[lead_node], (unsigned long long)ord_from_double(hy));
+            }
+        }
+        left &= ~m_same;
+    }
+    if (active && ((left >> lane_id()) & 1ull)) bbox_add(nv, node, px, py);
+}
+
+// Root bounding box over all vortices and segments.
+__global__ void k_root_bbox(NodeView nv, const double* x, const double* y, int n, const vvhd_seg* segs, int nseg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool av = i < n;
+    bbox_add_wave(nv, 0, av ? x[i] : 0, av ? y[i] : 0, av);
+    if (i < nseg) bbox_add(nv, 0, segs[i].x, segs[i].y);
+}
+
+// snode::Stretch tail + DivideNode decision (TSortedTree.cpp:36-56, 122-125)
+// for the nodes of the current level; split nodes get a pair of children.
+__global__ void k_decide(NodeView nv, int32_t* counters, int32_t* leaf_tmp, int cap, double min_node,
+                         double max_node) {
+    const int lb = counters[2], le = counters[3];
+    for (int id = lb + blockIdx.x * blockDim.x + threadIdx.x; id < le; id += gridDim.x * blockDim.x) {
+        const double blx = double_from_ord(nv.blx[id]), bly = double_from_ord(nv.bly[id]);
+        const double trx = double_from_ord(nv.trx[id]), try_ = double_from_ord(nv.try_[id]);
+        const double x = (blx + trx) * 0.5;
+        const double y = (bly + try_) * 0.5;
+        const double h = try_ - bly;
+        const double w = trx - blx;
+        nv.x[id] = x;
+        nv.y[id] = y;
+        nv.h[id] = h;
+        nv.w[id] = w;
+        const double mx = (h < w) ? w : h;   // std::max(h, w)
+        const double mn = (w < h) ? w : h;   // std::min(h, w)
+        const int nvort = nv.v1[id] - nv.v0[id];
+        const int nlist = max(nvort, nv.nseg[id]);
+        const bool leaf = (mx < max_node && mn <= min_node) || (mx < max_node && nlist < kLeafMaxList);
+        if (leaf) {
+            nv.split[id] = 1;
+            const int li = atomicAdd(&counters[1], 1);
+            leaf_tmp[li] = id;
+            continue;
+        }
+        const int c = atomicAdd(&counters[0], 2);
+        if (c + 2 > cap) {
+            counters[4] = 1;
+            nv.split[id] = 1;   // stop here; reported as an error by the host
+            continue;
+        }
+        nv.split[id] = 2;
+        nv.ch1[id] = c;
+        const int d = nv.depth[id] + 1;
+        atomicMax(&counters[5], d);
+        init_node(nv, c, id, d, nv.path[id] << 1, nv.v0[id], nv.v0[id]);
+        init_node(nv, c + 1, id, d, (nv.path[id] << 1) | 1ull, nv.v1[id], nv.v1[id]);
+    }
+}
+
+__device__ inline bool goes_left(const NodeView& nv, int node, double px, double py) {
+    return (nv.h[node] < nv.w[node]) ? (px < nv.x[node]) : (py < nv.y[node]);
+}
+
+// Per element: "goes left" flag of elements in split nodes (else 0).
+__global__ void k_flags(NodeView nv, const int32_t* enode, const double* x, const double* y, int n, int32_t* flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    int f = 0;
+    if (k < n) {
+        const int node = enode[k];
+        if (nv.split[node] == 2) f = goes_left(nv, node, x[k], y[k]) ? 1 : 0;
+    }
+    flag[k] = f;
+}
+
+// Per split node: split position and number of misplaced pairs.
+// P is the exclusive prefix of the flags over all elements (n+1 entries).
+__global__ void k_node_counts(NodeView nv, const int32_t* counters, const int32_t* P, int32_t* node_L, int32_t* node_m) {
+    const int lb = counters[2], le = counters[3];
+    for (int id = lb + blockIdx.x * blockDim.x + threadIdx.x; id < le; id += gridDim.x * blockDim.x) {
+        if (nv.split[id] != 2) continue;
+        const int v0 = nv.v0[id], v1 = nv.v1[id];
+        const int Lc = P[v1] - P[v0];
+        const int Labs = v0 + Lc;
+        const int m = Lc - (P[Labs] - P[v0]);   // left-goers sitting right of Labs
+        node_L[id] = Labs;
+        node_m[id] = m;
+        const int c = nv.ch1[id];
+        nv.v0[c] = v0;
+        nv.v1[c] = Labs;
+        nv.v0[c + 1] = Labs;
+        nv.v1[c + 1] = v1;
+    }
+}
+
+// Ranks of misplaced elements: left_pos[v0 + r] / right_pos[v0 + q].
+__global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* flag, const int32_t* P, int n,
+                        const int32_t* node_L, int32_t* left_pos, int32_t* right_pos) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int node = enode[k];
+    if (nv.split[node] != 2) return;
+    const int v0 = nv.v0[node];   // a split node's own range is unchanged
+    const int Labs = node_L[node];
+    const int f = flag[k];
+    if (k < Labs && !f) {
+        const int r = (k - v0) - (P[k] - P[v0]);
+        left_pos[v0 + r] = k;
+    } else if (k >= Labs && f) {
+        const int q = P[k] - P[Labs];
+        right_pos[v0 + q] = k;
+    }
+}
+
+// Hoare swaps, child bounding boxes and element -> child node.
+__global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const int32_t* P, int n,
+                       const int32_t* node_L, const int32_t* node_m, const int32_t* left_pos,
+                       const int32_t* right_pos, double* x, double* y, int32_t* id) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = false;
+    int node = 0, Labs = 0, v0 = 0, f = 0, c = 0;
+    if (k < n) {
+        node = enode[k];
+        active = nv.split[node] == 2;
+    }
+    if (active) {
+        v0 = nv.v0[node];
+        Labs = node_L[node];
+        f = flag[k];
+        c = nv.ch1[node];
+    }
+    const bool left_mis = active && k < Labs && !f;
+    const bool right_mis = active && k >= Labs && f;
+    // Bounding boxes by element identity: each element's own flag decides its
+    // child.  A misplaced pair is accounted by its left thread.
+    const double px = active ? x[k] : 0, py = active ? y[k] : 0;
+    double qx = 0, qy = 0;
+    int partner = -1;
+    if (left_mis) {
+        const int r = (k - v0) - (P[k] - P[v0]);
+        partner = right_pos[v0 + node_m[node] - 1 - r];
+        qx = x[partner];
+        qy = y[partner];
+    }
+    const bool own = active && !right_mis;
+    bbox_add_wave(nv, c + (f ? 0 : 1), px, py, own);
+    bbox_add_wave(nv, c, qx, qy, left_mis);
+    if (active) enode[k] = (k < Labs) ? c : c + 1;
+    // Only the left thread of a misplaced pair writes either position.
+    if (left_mis) {
+        const int32_t ik = id[k], ip = id[partner];
+        x[k] = qx;
+        y[k] = qy;
+        id[k] = ip;
+        x[partner] = px;
+        y[partner] = py;
+        id[partner] = ik;
+    }
+}
+
+// Segments descend one level (stable partition == per-segment descent).
+__global__ void k_seg_step(NodeView nv, const vvhd_seg* segs, int nseg, int32_t* seg_node) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const int node = seg_node[s];
+    if (nv.split[node] != 2) return;
+    const int c = nv.ch1[node] + (goes_left(nv, node, segs[s].x, segs[s].y) ? 0 : 1);
+    seg_node[s] = c;
+    atomicAdd(&nv.nseg[c], 1);
+    bbox_add(nv, c, segs[s].x, segs[s].y);
+}
+
+__global__ void k_next_level(int32_t* counters) {
+    counters[2] = counters[3];
+    counters[3] = counters[0];
+}
+
+// Gathers the non-key vortex fields into tree order.
+__global__ void k_gather(const int32_t* id, int n, const double* g, const double* vx, const double* vy,
+                         const double* ie, double* g2, double* vx2, double* vy2, double* ie2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = id[k];
+    g2[k] = g[s];
+    vx2[k] = vx[s];
+    vy2[k] = vy[s];
+    ie2[k] = ie[s];
+}
+
+struct CMView {
+    double *px, *py, *pg, *mx, *my, *mg;
+    float *pie, *mie;
+};
+
+// Bottom-up over one level: subtree leaf counts, and monopoles --
+// CalculateCMassFromScratch (TSortedTree.cpp:176-197) for leaves (array
+// order) and CalculateCMass (:150-174) from the children otherwise.
+__global__ void k_up_level(NodeView nv, CMView cm, int lb, int le, const double* x, const double* y, const double* g,
+                           int32_t* nleaves) {
+    const int id = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= le) return;
+    if (nv.split[id] != 2) {
+        nleaves[id] = 1;
+        double px = 0, py = 0, pg = 0, mx = 0, my = 0, mg = 0;
+        for (int k = nv.v0[id]; k < nv.v1[id]; k++) {
+            const double gk = g[k];
+            if (gk > 0) {
+                px += x[k] * gk;
+                py += y[k] * gk;
+                pg += gk;
+            } else {
+                mx += x[k] * gk;
+                my += y[k] * gk;
+                mg += gk;
+            }
+        }
+        if (pg) {
+            px *= (1. / pg);
+            py *= (1. / pg);
+        } else {
+            px = nv.x[id];
+            py = nv.y[id];
+        }
+        if (mg) {
+            mx *= (1. / mg);
+            my *= (1. / mg);
+        } else {
+            mx = nv.x[id];
+            my = nv.y[id];
+        }
+        cm.px[id] = px;
+        cm.py[id] = py;
+        cm.pg[id] = pg;
+        cm.mx[id] = mx;
+        cm.my[id] = my;
+        cm.mg[id] = mg;
+        cm.pie[id] = INFINITY;
+        cm.mie[id] = INFINITY;
+        return;
+    }
+    const int a = nv.ch1[id], b = a + 1;
+    nleaves[id] = nleaves[a] + nleaves[b];
+    {
+        const double ag = cm.pg[a], bg = cm.pg[b];
+        const double s = ag + bg;
+        if (s) {
+            const double inv = 1. / s;
+            cm.px[id] = (cm.px[a] * ag + cm.px[b] * bg) * inv;
+            cm.py[id] = (cm.py[a] * ag + cm.py[b] * bg) * inv;
+            cm.pg[id] = s;
+            cm.pie[id] = INFINITY;
+        } else {   // TObj(x, y, 0): _1_eps = 0
+            cm.px[id] = nv.x[id];
+            cm.py[id] = nv.y[id];
+            cm.pg[id] = 0;
+            cm.pie[id] = 0;
+        }
+    }
+    {
+        const double ag = cm.mg[a], bg = cm.mg[b];
+        const double s = ag + bg;
+        if (s) {
+            const double inv = 1. / s;
+            cm.mx[id] = (cm.mx[a] * ag + cm.mx[b] * bg) * inv;
+            cm.my[id] = (cm.my[a] * ag + cm.my[b] * bg) * inv;
+            cm.mg[id] = s;
+            cm.mie[id] = INFINITY;
+        } else {
+            cm.mx[id] = nv.x[id];
+            cm.my[id] = nv.y[id];
+            cm.mg[id] = 0;
+            cm.mie[id] = 0;
+        }
+    }
+}
+
+// Top-down over one level: first leaf rank of every subtree (DFS order).
+__global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves, int32_t* first, int32_t* leaf_rank,
+                             int32_t* leaf_node, int32_t* vleaf) {
+    const int id = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= le) return;
+    const int f = first[id];
+    if (nv.split[id] == 2) {
+        const int a = nv.ch1[id];
+        first[a] = f;
+        first[a + 1] = f + nleaves[a];
+    } else {
+        leaf_rank[id] = f;
+        leaf_node[f] = id;
+        for (int k = nv.v0[id]; k < nv.v1[id]; k++) vleaf[k] = f;
+    }
+}
+
+// FindNearNodes (TSortedTree.cpp:199-217) for the 64 DFS-consecutive leaves
+// of one wave, traversing the tree once with a shared (uniform) stack: a
+// node is expanded while any lane still needs it; a lane that classifies a
+// node as far ignores the rest of that subtree.  Children are pushed ch2
+// then ch1, so each lane sees its nodes in the reference's DFS order.  With
+// the list arrays NULL only counts.
+__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const int32_t* leaf_node, int nleaf, double fc,
+                                                  const int64_t* near_off, const int64_t* far_off,
+                                                  int32_t* near_list, int32_t* far_list, int32_t* near_cnt,
+                                                  int32_t* far_cnt, int32_t* err) {
+    __shared__ int s_stack[kBlock / kWave][kMaxStack];
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const int l = (blockIdx.x * (kBlock / kWave) + w) * kWave + lane;
+    const bool valid = l < nleaf;
+    if (__ballot(valid) == 0) return;
+    int* stack = s_stack[w];
+    const int L = valid ? leaf_node[l] : 0;
+    const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];
+    int64_t pn = (valid && near_off) ? near_off[l] : 0, pf = (valid && far_off) ? far_off[l] : 0;
+    int cn = 0, cf = 0;
+    int skip_depth = INT32_MAX;
+    if (lane == 0) stack[0] = 0;
+    int sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        const int T = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
+        sp--;
+        const int dT = nv.depth[T];
+        if (skip_depth != INT32_MAX && dT <= skip_depth) skip_depth = INT32_MAX;
+        const bool act = valid && skip_depth == INT32_MAX;
+        const double dx = nv.x[T] - lx, dy = nv.y[T] - ly;
+        const double hp = nv.h[T] + nv.w[T] + lh + lw;
+        const bool far = act && (dx * dx + dy * dy > fc * hp * hp);
+        if (far) {
+            if (far_list) far_list[pf++] = T;
+            cf++;
+            skip_depth = dT;
+        }
+        const int c = nv.ch1[T];
+        const bool need = act && !far;
+        if (c >= 0) {
+            if (__ballot(need) != 0) {
+                if (sp + 2 > kMaxStack) {
+                    if (lane == 0) *err = 1;
+                    break;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) {
+                    stack[sp] = c + 1;
+                    stack[sp + 1] = c;
+                }
+                __builtin_amdgcn_wave_barrier();
+                sp += 2;
+            }
+        } else if (need) {
+            if (near_list) near_list[pn++] = T;
+            cn++;
+        }
+    }
+    if (valid && near_cnt) near_cnt[l] = cn;
+    if (valid && far_cnt) far_cnt[l] = cf;
+}
+
+// Segments: leaf of each segment, then stable order by (leaf, index).
+__global__ void k_seg_sort(const int32_t* seg_node, const int32_t* leaf_rank, int nseg, int32_t* seg_leaf,
+                           int32_t* seg_list) {
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_leaf[s] = leaf_rank[seg_node[s]];
+    __syncthreads();
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+        const int ls = seg_leaf[s];
+        int pos = 0;
+        for (int t = 0; t < nseg; t++) {
+            const int lt = seg_leaf[t];
+            pos += (lt < ls) || (lt == ls && t < s);
+        }
+        seg_list[pos] = s;
+    }
+}
+
+__global__ void k_seg_off(const int32_t* seg_leaf, const int32_t* seg_list, int nseg, int nleaf, int64_t* seg_off) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > nleaf) return;
+    // first position whose leaf >= l
+    int lo = 0, hi = nseg;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (seg_leaf[seg_list[mid]] < l)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    seg_off[l] = lo;
+}
+
+__global__ void k_iota(int32_t* a, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = k;
+}
+
+__global__ void k_fill_i32(int32_t* a, int n, int32_t v) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
+template <typename T>
+void swap_buf(DBuf<T>& a, DBuf<T>& b) {
+    std::swap(a.p, b.p);
+    std::swap(a.cap, b.cap);
+}
+
+// exclusive scan of int32 counts into int64 offsets (n+1 outputs)
+void scan_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t n) {
+    size_t bytes = 0;
+    VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int64_t)0, (size_t)n + 1, rocprim::plus<int64_t>(),
+                                     c.stream));
+    c.tmp.grow(bytes + 16);
+    VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, in, out, (int64_t)0, (size_t)n + 1, rocprim::plus<int64_t>(),
+                                     c.stream));
+}
+
+void scan_i32(Ctx& c, const int32_t* in, int32_t* out, int64_t n) {
+    size_t bytes = 0;
+    VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int32_t)0, (size_t)n, rocprim::plus<int32_t>(),
+                                     c.stream));
+    c.tmp.grow(bytes + 16);
+    VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, in, out, (int32_t)0, (size_t)n, rocprim::plus<int32_t>(),
+                                     c.stream));
+}
+
+}  // namespace
+
+void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int32_t* perm_out) {
+    TreeArrays& t = c.t;
+    const int n = (int)c.n;
+    const int nseg = (int)c.segs_h.size();
+    VVHD_CHECK(c.n < INT32_MAX / 2, VVHD_EINVAL, "tree_build: too many vortices");
+    VVHD_CHECK(far_criteria >= 0, VVHD_EINVAL, "tree_build: far_criteria must be non-negative");
+    t.far_criteria = far_criteria;
+    t.min_node = min_node;
+    t.max_node = max_node;
+    const int cap = 2 * (n + nseg) + 64;
+    for (auto* b : {&t.x, &t.y, &t.h, &t.w}) b->grow(cap);
+    for (auto* b : {&t.blx, &t.bly, &t.trx, &t.try_, &t.path}) b->grow(cap);
+    for (auto* b : {&t.v0, &t.v1, &t.ch1, &t.parent, &t.depth, &t.nseg, &t.split, &t.nleaves, &t.arrive, &t.leaf_rank})
+        b->grow(cap);
+    for (auto* b : {&t.cmp_x, &t.cmp_y, &t.cmp_g, &t.cmm_x, &t.cmm_y, &t.cmm_g}) b->grow(cap);
+    t.cmp_ie.grow(cap);
+    t.cmm_ie.grow(cap);
+    t.counters.grow(16);
+    t.leaf_node.grow(cap);
+    t.vleaf.grow(n + 1);
+    c.iscratch0.grow(n + 1);   // element node
+    c.iscratch1.grow(n + 2);   // flags
+    c.iscratch2.grow(n + 2);   // prefix
+    c.iscratch3.grow(2 * (size_t)n + 2);   // left/right positions
+    DBuf<int32_t>& enode = c.iscratch0;
+    int32_t* flag = c.iscratch1.p;
+    int32_t* P = c.iscratch2.p;
+    int32_t* left_pos = c.iscratch3.p;
+    int32_t* right_pos = c.iscratch3.p + n + 1;
+    DBuf<int32_t>& node_L = t.node_L;
+    DBuf<int32_t>& node_m = t.node_m;
+    DBuf<int32_t>& leaf_tmp = t.leaf_tmp;
+    DBuf<int32_t>& seg_node = t.seg_node;
+    node_L.grow(cap);
+    node_m.grow(cap);
+    leaf_tmp.grow(cap);
+    seg_node.grow(nseg + 1);
+    NodeView nv = view(t);
+    hipStream_t s = c.stream;
+
+    hipLaunchKernelGGL(k_init_root, dim3(1), dim3(1), 0, s, nv, n, nseg, t.counters.p);
+    hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(kBlock), 0, s, c.id.p, n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(n)), dim3(kBlock), 0, s, enode.p, n, 0);
+    if (nseg) hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nseg)), dim3(kBlock), 0, s, seg_node.p, nseg, 0);
+    hipLaunchKernelGGL(k_root_bbox, dim3(nblocks(std::max(n, nseg))), dim3(kBlock), 0, s, nv, c.x.p, c.y.p, n,
+                       c.segs.p, nseg);
+
+    std::vector<std::pair<int, int>> levels;   // node id range of every level
+    levels.push_back({0, 1});
+    int32_t hc[8];
+    while (true) {
+        const int node_blocks = 1024;
+        hipLaunchKernelGGL(k_decide, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, leaf_tmp.p, cap,
+                           min_node, max_node);
+        hipLaunchKernelGGL(k_flags, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, c.x.p, c.y.p, n, flag);
+        scan_i32(c, flag, P, n + 1);
+        hipLaunchKernelGGL(k_node_counts, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, P, node_L.p,
+                           node_m.p);
+        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, left_pos,
+                           right_pos);
+        hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
+                           left_pos, right_pos, c.x.p, c.y.p, c.id.p);
+        if (nseg)
+            hipLaunchKernelGGL(k_seg_step, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg, seg_node.p);
+        hipLaunchKernelGGL(k_next_level, dim3(1), dim3(1), 0, s, t.counters.p);
+        VVHD_HIP(hipMemcpyAsync(hc, t.counters.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+        VVHD_CHECK(hc[4] == 0, VVHD_ENOMEM, "tree_build: node capacity exceeded");
+        if (hc[3] == hc[2]) break;   // no new nodes
+        levels.push_back({hc[2], hc[3]});
+        VVHD_CHECK((int)levels.size() < kMaxStack - 2, VVHD_EUNSUPPORTED, "tree_build: tree deeper than supported");
+    }
+    t.nnodes = hc[0];
+    t.nleaves_h = hc[1];
+    t.max_depth = hc[5];
+    const int nl = t.nleaves_h;
+
+    // tree order of the remaining fields
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(n)), dim3(kBlock), 0, s, c.id.p, n, c.g.p, c.vx.p, c.vy.p, c.ie.p,
+                       c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p);
+    swap_buf(c.g, c.g2);
+    swap_buf(c.vx, c.vx2);
+    swap_buf(c.vy, c.vy2);
+    swap_buf(c.ie, c.ie2);
+    if (perm_out && n) VVHD_HIP(hipMemcpyAsync(perm_out, c.id.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+
+    // leaf counts and monopoles bottom-up, DFS ranks top-down
+    CMView cm{t.cmp_x.p, t.cmp_y.p, t.cmp_g.p, t.cmm_x.p, t.cmm_y.p, t.cmm_g.p, t.cmp_ie.p, t.cmm_ie.p};
+    for (int d = (int)levels.size() - 1; d >= 0; d--) {
+        const int lb = levels[d].first, le = levels[d].second;
+        hipLaunchKernelGGL(k_up_level, dim3(nblocks(le - lb)), dim3(kBlock), 0, s, nv, cm, lb, le, c.x.p, c.y.p,
+                           c.g.p, t.nleaves.p);
+    }
+    int32_t* first = t.arrive.p;
+    VVHD_HIP(hipMemsetAsync(first, 0, 4, s));
+    for (int d = 0; d < (int)levels.size(); d++) {
+        const int lb = levels[d].first, le = levels[d].second;
+        hipLaunchKernelGGL(k_down_level, dim3(nblocks(le - lb)), dim3(kBlock), 0, s, nv, lb, le, t.nleaves.p, first,
+                           t.leaf_rank.p, t.leaf_node.p, t.vleaf.p);
+    }
+    // segments per leaf
+    t.seg_off.grow(nl + 1);
+    t.seg_leaf.grow(nseg + 1);
+    t.seg_list.grow(nseg + 1);
+    if (nseg) {
+        hipLaunchKernelGGL(k_seg_sort, dim3(1), dim3(1024), 0, s, seg_node.p, t.leaf_rank.p, nseg, t.seg_leaf.p,
+                           t.seg_list.p);
+        hipLaunchKernelGGL(k_seg_off, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.seg_leaf.p, t.seg_list.p, nseg, nl,
+                           t.seg_off.p);
+    } else {
+        VVHD_HIP(hipMemsetAsync(t.seg_off.p, 0, (size_t)(nl + 1) * 8, s));
+    }
+    t.built = true;
+    tree_lists(c);
+}
+
+void tree_lists(Ctx& c) {
+    TreeArrays& t = c.t;
+    const int nl = t.nleaves_h;
+    NodeView nv = view(t);
+    hipStream_t s = c.stream;
+    DBuf<int32_t>& cnt_n = t.cnt_near;
+    DBuf<int32_t>& cnt_f = t.cnt_far;
+    DBuf<int32_t>& err = t.err;
+    cnt_n.grow(nl + 1);
+    cnt_f.grow(nl + 1);
+    err.grow(1);
+    t.near_off.grow(nl + 1);
+    t.far_off.grow(nl + 1);
+    VVHD_HIP(hipMemsetAsync(err.p, 0, 4, s));
+    VVHD_HIP(hipMemsetAsync(cnt_n.p + nl, 0, 4, s));
+    VVHD_HIP(hipMemsetAsync(cnt_f.p + nl, 0, 4, s));
+    const double fc = (double)t.far_criteria;
+    const int grid = nblocks(nl, kBlock);   // one wave per 64 leaves
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, t.leaf_node.p, nl, fc, (const int64_t*)nullptr,
+                       (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, cnt_n.p, cnt_f.p, err.p);
+    scan_i32_to_i64(c, cnt_n.p, t.near_off.p, nl);
+    scan_i32_to_i64(c, cnt_f.p, t.far_off.p, nl);
+    int64_t tot[2];
+    int32_t herr = 0;
+    VVHD_HIP(hipMemcpyAsync(&tot[0], t.near_off.p + nl, 8, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(&tot[1], t.far_off.p + nl, 8, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipStreamSynchronize(s));
+    VVHD_CHECK(herr == 0, VVHD_EUNSUPPORTED, "tree lists: traversal stack exceeded");
+    t.nnear = tot[0];
+    t.nfar = tot[1];
+    t.near_list.grow(t.nnear + 1);
+    t.far_list.grow(t.nfar + 1);
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, t.leaf_node.p, nl, fc,
+                       (const int64_t*)t.near_off.p, (const int64_t*)t.far_off.p, t.near_list.p, t.far_list.p,
+                       (int32_t*)nullptr, (int32_t*)nullptr, err.p);
+}
+
+}  // namespace vvhd
