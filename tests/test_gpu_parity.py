@@ -1,0 +1,257 @@
+"""GPU (gfx950) hot path through the C ABI against the reference's own
+outputs (tests/golden) and the FP64 oracle.
+
+Bars: tree permutation, leaf table, near/far lists, monopoles, epsilon
+neighbour indices, merges and the post-epsilon vortex state are bit-exact.
+Convective velocity: FP32 pair sums with Kahan compensation, relative L2 of
+the induced velocity within 2e-6 of the FP64 oracle (plain FP32 within 1e-5).
+Diffusive velocity: FP32 pair sums, relative L2 within 1e-5 and pointwise
+within 1e-3 (floor 1e-3 rms) of the FP64 reference (exp-cutoff decisions at
+the |dr|/eps = 8 boundary may flip in FP32).  Segment friction: FP64, 1e-12.
+"""
+import numpy as np
+import pytest
+
+import vvflow_amd as V
+from cases import (DBL_MAX, FIXTURES, layer_case, load_golden, max_rel, oracle_for, params, path_code32,
+                   random_cluster_case, rel_l2)
+from oracle_bind import direct_sum
+
+pytestmark = pytest.mark.gpu
+
+CONV_TOL_KAHAN = 2e-6
+CONV_TOL_PLAIN = 1e-5
+DIFF_TOL_L2 = 1e-5
+DIFF_TOL_MAX = 1e-3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = V.Context(0)
+    yield c
+    c.close()
+
+
+def body_records(segs, oracle=None):
+    b = np.zeros(1, dtype=V.BODY_DTYPE)
+    b["first_seg"] = 0
+    b["nseg"] = len(segs)
+    if oracle is not None:
+        ob = oracle.body(0)
+        b["slen"], b["area"] = ob["slen"], ob["area"]
+    return b
+
+
+def load_case(ctx, objs, segs=None, re=600.0):
+    ctx.set_params(re=re, dt=0.005)
+    if segs is None:
+        ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
+    else:
+        ctx.upload_bodies(np.ascontiguousarray(segs, dtype=V.SEG_DTYPE), body_records(segs))
+    ctx.upload(objs)
+
+
+# ------------------------------------------------------------ golden (reference)
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_tree_matches_reference(ctx, name):
+    d = load_golden(name)
+    print(f"case golden {name}", flush=True)
+    load_case(ctx, d["objs"], d["segs"], float(d["re"]))
+    perm = ctx.tree_build(*params(d), perm=True)
+    assert np.array_equal(perm, d["perm"])
+    lv = ctx.tree_leaves()
+    assert np.array_equal(lv["geom"], d["leaf_geom"])
+    assert np.array_equal(lv["ints"][:, :2], d["leaf_ints"][:, :2])
+    assert np.array_equal(lv["ints"][:, 2], d["leaf_ints"][:, 2])
+    assert np.array_equal(path_code32(lv["path"]), d["leaf_ints"][:, 3].astype(np.uint32).astype(np.int64))
+    for far in (False, True):
+        g = ctx.tree_lists(far=far)
+        ref = d["far" if far else "near"].astype(np.int64)
+        assert np.array_equal(g["depth"], ref[:, 0])
+        assert np.array_equal(path_code32(g["path"]), ref[:, 1].astype(np.uint32).astype(np.int64))
+        if far:
+            cm = d["far_cm"]
+            assert np.array_equal(g["cm"], cm[:, [0, 1, 2, 4, 5, 6]])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_epsilon_and_diffusive_match_reference(ctx, name):
+    d = load_golden(name)
+    print(f"case golden eps/diff {name}", flush=True)
+    load_case(ctx, d["objs"], d["segs"], float(d["re"]))
+    ctx.tree_build(*params(d))
+    merged = ctx.epsilon(bool(d["merge"]))
+    assert merged == int(d["merged"])
+    st = ctx.download()
+    assert np.array_equal(st[:, [0, 1, 2, 5]], d["after_eps"])
+    ctx.diffusive()
+    v = ctx.download()[:, 3:5]
+    assert rel_l2(v, d["after_diff_v"]) < DIFF_TOL_L2
+    assert max_rel(v, d["after_diff_v"]) < DIFF_TOL_MAX
+    if d["segs"] is not None:
+        segs, _ = ctx.download_bodies()
+        assert np.allclose(segs["fric"], d["fric"], rtol=1e-12, atol=1e-15 * np.abs(d["fric"]).max())
+
+
+# ------------------------------------------------------------ random vs oracle
+
+def run_both(ctx, objs, corners=None, fc=8, mn=0.0, mx=DBL_MAX, merge=True, kahan=True, inf=(1.0, 0.0)):
+    o = oracle_for(objs, corners=corners, inf=inf)
+    segs = o.segments() if corners is not None else None
+    load_case(ctx, objs, segs)
+    o.tree_build(fc, mn, mx)
+    perm = ctx.tree_build(fc, mn, mx, perm=True)
+    ov = o.vortices()
+    gv = ctx.download()
+    assert np.array_equal(gv[:, :3], ov[:, :3])
+    nn1o, nn2o, _, mo = o.epsilon(merge)
+    nn1, nn2, mg = ctx.epsilon(merge, want_nn=True)
+    assert mg == mo
+    assert np.array_equal(nn1, nn1o) and np.array_equal(nn2, nn2o)
+    ov = o.vortices()
+    gv = ctx.download()
+    assert np.array_equal(gv[:, [0, 1, 2, 5]], ov[:, [0, 1, 2, 5]], equal_nan=True)
+    o.convective(threads=4)
+    ctx.convective(inf, kahan=kahan)
+    ov = o.vortices()
+    gv = ctx.download()
+    ind = ov[:, 3:5] - np.asarray(inf)
+    conv_err = rel_l2(gv[:, 3:5] - np.asarray(inf), ind)
+    o.diffusive()
+    ctx.diffusive()
+    ov2 = o.vortices()
+    gv2 = ctx.download()
+    dv_o = ov2[:, 3:5] - ov[:, 3:5]
+    dv_g = gv2[:, 3:5] - gv[:, 3:5]
+    return conv_err, dv_g, dv_o, o
+
+
+@pytest.mark.parametrize("seed,signs,n", [(1, "mixed", 4000), (2, "cluster", 6000), (3, "positive", 3000),
+                                          (4, "cluster", 12000)])
+def test_random_clusters_vs_oracle(ctx, seed, signs, n):
+    print(f"case random seed={seed} signs={signs} n={n}", flush=True)
+    objs = random_cluster_case(n, seed, signs)
+    conv_err, dv_g, dv_o, _ = run_both(ctx, objs)
+    assert conv_err < CONV_TOL_KAHAN
+    assert rel_l2(dv_g, dv_o) < DIFF_TOL_L2
+    assert max_rel(dv_g, dv_o) < DIFF_TOL_MAX
+
+
+def test_plain_fp32_accumulation(ctx):
+    print("case plain fp32", flush=True)
+    conv_err, _, _, _ = run_both(ctx, random_cluster_case(5000, 7, "cluster"), kahan=False)
+    assert conv_err < CONV_TOL_PLAIN
+
+
+@pytest.mark.parametrize("fc", [2, 30])
+def test_far_criteria_vs_oracle(ctx, fc):
+    print(f"case fc={fc}", flush=True)
+    conv_err, dv_g, dv_o, _ = run_both(ctx, random_cluster_case(5000, 9, "cluster"), fc=fc)
+    assert conv_err < CONV_TOL_KAHAN
+    assert rel_l2(dv_g, dv_o) < DIFF_TOL_L2
+
+
+def test_no_merge_mode(ctx):
+    print("case nomerge", flush=True)
+    run_both(ctx, random_cluster_case(3000, 11, "mixed"), merge=False)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_body_layer_vs_oracle(ctx, seed):
+    print(f"case body layer seed={seed}", flush=True)
+    objs, corners = layer_case(4000, seed)
+    dl = np.pi / 499
+    conv_err, dv_g, dv_o, o = run_both(ctx, objs, corners=corners, mn=5 * dl, mx=100 * dl)
+    assert conv_err < CONV_TOL_KAHAN
+    assert rel_l2(dv_g, dv_o) < DIFF_TOL_L2
+    segs, _ = ctx.download_bodies()
+    fo = o.segments()["fric"]
+    assert np.allclose(segs["fric"], fo, rtol=1e-10, atol=1e-13 * np.abs(fo).max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 40])
+def test_tiny_inputs(ctx, n):
+    print(f"case tiny n={n}", flush=True)
+    run_both(ctx, random_cluster_case(n, 20 + n, "mixed"))
+
+
+def test_duplicate_positions(ctx):
+    print("case duplicates", flush=True)
+    base = random_cluster_case(500, 3, "mixed")
+    dup = np.concatenate([base, base[:60], base[:7]])
+    dup = dup[np.random.default_rng(0).permutation(len(dup))]
+    o = oracle_for(dup)
+    load_case(ctx, dup)
+    o.tree_build(8)
+    perm = ctx.tree_build(8, perm=True)
+    assert np.array_equal(ctx.download()[:, :3], o.vortices()[:, :3])
+    o.epsilon(True)
+    ctx.epsilon(True)
+    assert np.array_equal(ctx.download()[:, [0, 1, 2, 5]], o.vortices()[:, [0, 1, 2, 5]], equal_nan=True)
+
+
+def test_velocity_points_vs_oracle(ctx):
+    print("case velocity points", flush=True)
+    objs = random_cluster_case(5000, 13, "cluster")
+    o = oracle_for(objs)
+    load_case(ctx, objs)
+    o.tree_build(8)
+    ctx.tree_build(8)
+    o.epsilon(True)
+    ctx.epsilon(True)
+    pts = np.random.default_rng(2).uniform(-0.1, 1.1, (300, 2))
+    vo = o.velocity(pts)
+    vg = ctx.velocity(pts, inf=(1.0, 0.0))
+    assert np.allclose(vg, vo, rtol=1e-10, atol=1e-12)
+
+
+def test_direct_sum_vs_oracle(ctx):
+    print("case direct sum", flush=True)
+    objs = random_cluster_case(3000, 15, "mixed")
+    objs[:, 5] = np.random.default_rng(1).uniform(100, 1000, len(objs))
+    load_case(ctx, objs)
+    tg = np.arange(0, 3000, 7, dtype=np.int64)
+    assert np.allclose(ctx.direct_sum(tg), direct_sum(objs, tg), rtol=1e-12, atol=1e-15)
+
+
+def test_repeatable(ctx):
+    print("case repeatability", flush=True)
+    objs = random_cluster_case(8000, 17, "mixed")
+    outs = []
+    for _ in range(2):
+        load_case(ctx, objs)
+        ctx.tree_build(8)
+        ctx.epsilon(True)
+        ctx.convective((1.0, 0.0))
+        ctx.diffusive()
+        outs.append(ctx.download())
+    assert np.array_equal(outs[0], outs[1])
+
+
+# ------------------------------------------------------------ full size
+
+def test_full_size_wake_properties(ctx):
+    """1M-vortex wake: permutation, leaf partition, circulation conservation
+    through merges, agreement with the FP64 direct sum on sampled targets."""
+    print("case wake 1M", flush=True)
+    n = 1_000_000
+    objs = V.gen_wake(n)
+    load_case(ctx, objs)
+    perm = ctx.tree_build(8, perm=True)
+    assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.int32))
+    lv = ctx.tree_leaves()
+    r = lv["ints"][:, :2]
+    assert r[0, 0] == 0 and r[-1, 1] == n and np.all(r[1:, 0] == r[:-1, 1]) and np.all(r[:, 1] - r[:, 0] < 16)
+    g0 = objs[:, 2].sum()
+    merged = ctx.epsilon(True)
+    st = ctx.download()
+    assert abs(st[:, 2].sum() - g0) < 1e-13 * np.abs(objs[:, 2]).sum()
+    assert np.sum(st[:, 2] == 0) == merged
+    ctx.convective((0.0, 0.0))
+    v = ctx.download()[:, 3:5]
+    tg = np.random.default_rng(0).choice(n, 2000, replace=False).astype(np.int64)
+    ref = ctx.direct_sum(tg)
+    err = rel_l2(v[tg], ref)
+    print(f"  merged={merged} tree-method rel L2 vs direct sum (fc=8) = {err:.3e}", flush=True)
+    assert err < 1e-2
