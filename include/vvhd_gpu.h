@@ -159,8 +159,9 @@ int vvhd_integrals(vvhd_ctx* ctx, double* out5);
 int vvhd_direct_sum(vvhd_ctx* ctx, const int64_t* targets, int64_t nt, double* v_out);
 /* Device time in ms of the phases of the last calls, measured with HIP
  * events on the context stream: {tree, epsilon, convective, diffusive,
- * move_and_clean, slae}. */
-int vvhd_phase_times(vvhd_ctx* ctx, float* ms6);
+ * move_and_clean, slae, convective near-field kernel alone, reserved};
+ * -1 for a phase not run yet. */
+int vvhd_phase_times(vvhd_ctx* ctx, float* ms8);
 /* Algorithmic work counted on the interaction lists of the current tree:
  * {near convective pairs, far (leaf, node) terms, epsilon pairs scanned,
  *  diffusive same-sign pairs, diffusive segment pairs, leaves, nodes}. */

@@ -303,7 +303,9 @@ class Context:
         return out
 
     def phase_times(self):
-        out = np.zeros(6, dtype=np.float32)
+        """{tree, epsilon, convective, diffusive, move_and_clean, slae,
+        near-field kernel, reserved} in ms (-1: not run)."""
+        out = np.zeros(8, dtype=np.float32)
         self._check(self.L.vvhd_phase_times(self.h, _ptr(out)))
         return out
 

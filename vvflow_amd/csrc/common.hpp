@@ -195,8 +195,8 @@ struct Ctx {
     DBuf<int32_t> rnear_list, rnear_cnt, leaf_stamp;
 
     // timing
-    hipEvent_t ev[14] = {};
-    bool phase_recorded[6] = {false, false, false, false, false, false};
+    hipEvent_t ev[16] = {};
+    bool phase_recorded[8] = {false, false, false, false, false, false, false, false};
 
     // work counts of the current tree
     int64_t work[7] = {0, 0, 0, 0, 0, 0, 0};

@@ -346,12 +346,15 @@ void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, c.taylor.p);
     NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
+    VVHD_HIP(hipEventRecord(c.ev[12], c.stream));
     if (kahan)
         hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.x.p, c.y.p, c.src_ge.p,
                            c.near_v.p);
     else
         hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.x.p, c.y.p, c.src_ge.p,
                            c.near_v.p);
+    VVHD_HIP(hipEventRecord(c.ev[13], c.stream));
+    c.phase_recorded[6] = true;
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
                        c.taylor.p, c.x.p, c.y.p, c.g.p, c.near_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     VVHD_HIP(hipGetLastError());

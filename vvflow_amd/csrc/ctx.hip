@@ -409,7 +409,7 @@ int vvhd_phase_times(vvhd_ctx* ctx, float* ms6) {
     Ctx& c = ctx->c;
     VVHD_CHECK(ms6, VVHD_EINVAL, "phase_times: null output");
     sync(c);
-    for (int p = 0; p < 6; p++) {
+    for (int p = 0; p < 8; p++) {
         float ms = -1;
         if (c.phase_recorded[p] && hipEventElapsedTime(&ms, c.ev[2 * p], c.ev[2 * p + 1]) != hipSuccess) {
             ms = -1;
