@@ -145,6 +145,12 @@ int vvhd_vortex_shed(vvhd_ctx* ctx);
  * (TMatrix.cpp:166-259): FP64 fill and LU solve on the device; writes the
  * bound circulations, speed_slae, force_hydro and force_holder. */
 int vvhd_calc_circulation(vvhd_ctx* ctx, double inf_x, double inf_y);
+/* The system filled by the last vvhd_calc_circulation: order into *n; when
+ * non-NULL, the matrix row-major into A (n*n) and the right-hand side into
+ * rhs (n).  Rows and columns follow Space::EnumerateBodies (TSpace.cpp:
+ * 189-203): per body its segments, then speed x/y/o, hydro x/y/o and holder
+ * x/y/o.  (Inspection of MConvectiveFast::fillSlaeMatrix, :882-1017.) */
+int vvhd_slae_system(vvhd_ctx* ctx, int32_t* n, double* A, double* rhs);
 
 /* ---- FP64 integrals: Space::calc_forces / zero_forces (TSpace.cpp:94-143)
  * and Space::gsum / integral / hydrodynamic_momentum / gmax

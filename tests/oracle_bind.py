@@ -163,6 +163,9 @@ class Oracle:
     def tree_build(self, far_criteria=8, min_node=0.0, max_node=np.finfo(np.float64).max):
         self._check(self.L.vvo_tree_build(self.h, far_criteria, min_node, max_node))
 
+    def tree_destroy(self):
+        self.L.vvo_tree_destroy(self.h)
+
     def tree(self):
         """Returns a dict describing the tree (node table, leaves, lists)."""
         nn = self.L.vvo_tree_num_nodes(self.h)

@@ -73,6 +73,7 @@ ABI = {
     "vvhd_move_and_clean": (_I, [_P, _I, _VP]),
     "vvhd_vortex_shed": (_I, [_P]),
     "vvhd_calc_circulation": (_I, [_P, _D, _D]),
+    "vvhd_slae_system": (_I, [_P, _VP, _VP, _VP]),
     "vvhd_calc_forces": (_I, [_P]),
     "vvhd_zero_forces": (_I, [_P]),
     "vvhd_integrals": (_I, [_P, _VP]),
@@ -280,6 +281,15 @@ class Context:
 
     def calc_circulation(self, inf=(0.0, 0.0)):
         self._check(self.L.vvhd_calc_circulation(self.h, inf[0], inf[1]))
+
+    def slae_system(self):
+        """(A row-major n x n, rhs) of the last filled boundary-condition system."""
+        n = C.c_int32(0)
+        self._check(self.L.vvhd_slae_system(self.h, C.addressof(n), None, None))
+        A = np.empty((n.value, n.value), dtype=np.float64)
+        b = np.empty(n.value, dtype=np.float64)
+        self._check(self.L.vvhd_slae_system(self.h, C.addressof(n), _ptr(A), _ptr(b)))
+        return A, b
 
     def calc_forces(self):
         self._check(self.L.vvhd_calc_forces(self.h))

@@ -1,6 +1,838 @@
-// bodies.hip -- flow-move, vortex shedding, the boundary-condition SLAE and
-// the force integrals (entry points; kernels follow).
+// bodies.hip -- body geometry, flow-move, vortex shedding, force integrals
+// and the FP64 boundary-condition system, on the device.
+//
+//  * TBody::doUpdateSegments / doFillProperties (TBody.cpp:200-215, 283-343)
+//    and TBody::move (:179-198): one thread per body, sequential in segment
+//    order, so every FP64 quantity rounds as in the reference.
+//  * MFlowmove::move_and_clean (MFlowmove.cpp:11-215) for rigid bodies
+//    without collisions: advection (thread per vortex), stable removal of
+//    |g| < 1e-10 and in-body vortices (TBody::isPointInvalid, TBody.cpp:
+//    217-281) by an exclusive scan and a scatter, the bodies' dead force,
+//    moment and circulation accumulated in vortex order, then the zero/slip
+//    segment loop.
+//  * MFlowmove::vortex_shed (MFlowmove.cpp:217-235): appended behind the
+//    list in segment order.
+//  * Space::calc_forces / zero_forces (TSpace.cpp:94-143).
+//  * MConvectiveFast::calc_circulation (MConvectiveFast.cpp:235-273, fill
+//    :459-1017): FP64 fill of the column-major matrix and right-hand side
+//    (influence cells one thread each; the vortex influence on a slip row one
+//    wave per segment, summed in the reference's order; body rows one
+//    thread), LU factorisation and solve with rocSOLVER.  The factors are
+//    reused while the matrix is bit-identical to the factorised one, the
+//    condition under which the reference reuses its inverse
+//    (TMatrix.cpp:225-259).
+
 #include "api.hpp"
+
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace vvhd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double k2Pi = 2. * kPi;
+constexpr double k1_2Pi = 1. / (2. * kPi);
+constexpr double kRemoveEps = 1e-10;   // MFlowmove.hpp:8
+
+inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+__device__ inline double dot2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
+
+// ------------------------------------------------------------ body geometry
+
+// doUpdateSegments + doFillProperties of one body; geo receives the
+// bounding disc radius squared and the bounding box used by isPointInvalid.
+__device__ void body_properties(vvhd_body& B, vvhd_seg* segs, double* geo) {
+    const int f = B.first_seg, n = B.nseg;
+    for (int k = 0; k < n; k++) {
+        vvhd_seg& s = segs[f + k];
+        const vvhd_seg& nx = segs[f + (k + 1) % n];
+        s.dlx = nx.cx - s.cx;
+        s.dly = nx.cy - s.cy;
+        s.inv_eps = 3.0 / sqrt(s.dlx * s.dlx + s.dly * s.dly);
+        s.x = (nx.cx + s.cx) * 0.5;
+        s.y = (nx.cy + s.cy) * 0.5;
+    }
+    double slen = 0, area = 0;
+    for (int k = 0; k < n; k++) {
+        const vvhd_seg& s = segs[f + k];
+        slen += sqrt(s.dlx * s.dlx + s.dly * s.dly);
+        area += s.y * s.dlx;
+    }
+    double cx, cy, moi;
+    if (area == 0) {
+        double Lx = 0., Ly = 0.;
+        for (int k = 0; k < n; k++) {
+            const vvhd_seg& s = segs[f + k];
+            const double a = sqrt(s.dlx * s.dlx + s.dly * s.dly);
+            Lx += s.x * a;
+            Ly += s.y * a;
+        }
+        const double inv = 1. / slen;
+        cx = Lx * inv;
+        cy = Ly * inv;
+        moi = 0.;
+    } else {
+        double Sx = 0., Sy = 0., m12 = 0.;
+        for (int k = 0; k < n; k++) {
+            const vvhd_seg& s = segs[f + k];
+            const vvhd_seg& nx = segs[f + (k + 1) % n];
+            const double cr = dot2(-s.cy, s.cx, nx.cx, nx.cy);
+            Sx -= s.x * cr;
+            Sy -= s.y * cr;
+            m12 -= (dot2(s.cx, s.cy, s.cx, s.cy) + dot2(s.cx, s.cy, nx.cx, nx.cy) + dot2(nx.cx, nx.cy, nx.cx, nx.cy)) *
+                   cr;
+        }
+        const double inv = 1. / (3 * area);
+        cx = Sx * inv;
+        cy = Sy * inv;
+        moi = m12 / 12. - area * (cx * cx + cy * cy);
+    }
+    B.slen = slen;
+    B.area = area;
+    B.cofm[0] = cx;
+    B.cofm[1] = cy;
+    B.moi_cofm = moi;
+    double r2max = 0, blx = INFINITY, bly = INFINITY, trx = -INFINITY, try_ = -INFINITY;
+    for (int k = 0; k < n; k++) {
+        const vvhd_seg& s = segs[f + k];
+        const double r2 = (s.cx - cx) * (s.cx - cx) + (s.cy - cy) * (s.cy - cy);
+        if (r2 > r2max) r2max = r2;
+        if (s.cx > trx) trx = s.cx;
+        if (s.cy > try_) try_ = s.cy;
+        if (s.cx < blx) blx = s.cx;
+        if (s.cy < bly) bly = s.cy;
+    }
+    geo[0] = r2max;
+    geo[1] = blx;
+    geo[2] = bly;
+    geo[3] = trx;
+    geo[4] = try_;
+}
+
+__global__ void k_body_properties(vvhd_body* bodies, int nbody, vvhd_seg* segs, double* geo) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbody) body_properties(bodies[b], segs, geo + 5 * b);
+}
+
+// TBody::move for a body with no prescribed motion, moved by the speed of
+// the last solution (TBody.cpp:179-198, MFlowmove.cpp:200-246).
+__global__ void k_body_move(vvhd_body* bodies, int nbody, vvhd_seg* segs, double* geo, double dt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbody) return;
+    vvhd_body& B = bodies[b];
+    const double dH[3] = {0. * dt, 0. * dt, 0. * dt};
+    const double dB[3] = {B.speed_slae[0] * dt, B.speed_slae[1] * dt, B.speed_slae[2] * dt};
+    const double ax = B.holder[0] + B.dpos[0], ay = B.holder[1] + B.dpos[1];
+    const double cs = cos(dB[2]), sn = sin(dB[2]);
+    for (int k = 0; k < B.nseg; k++) {
+        vvhd_seg& s = segs[B.first_seg + k];
+        const double drx = s.cx - ax, dry = s.cy - ay;
+        const double nx = ax + dB[0] + drx * cs + (-dry) * sn;
+        const double ny = ay + dB[1] + dry * cs + drx * sn;
+        s.cx = nx;
+        s.cy = ny;
+    }
+    for (int i = 0; i < 3; i++) {
+        B.holder[i] += dH[i];
+        B.dpos[i] += dB[i] - dH[i];
+        B.speed_slae_prev[i] = B.speed_slae[i];
+    }
+    body_properties(B, segs, geo + 5 * b);
+}
+
+// TBody::isPointInContour over the corners (TBody.cpp:238-281): index of the
+// nearest segment when p is inside the body, else -1.
+__device__ int point_invalid(const vvhd_body& B, const vvhd_seg* segs, const double* geo, double px, double py) {
+    bool in = B.area <= 0;
+    if (!in && (px < geo[1] || py < geo[2] || px > geo[3] || py > geo[4] ||
+                (px - B.cofm[0]) * (px - B.cofm[0]) + (py - B.cofm[1]) * (py - B.cofm[1]) > geo[0]))
+        return -1;
+    const int f = B.first_seg, n = B.nseg;
+    for (int i = 0, j = n - 1; i < n; j = i++) {
+        const double vix = segs[f + i].cx, viy = segs[f + i].cy, vjx = segs[f + j].cx, vjy = segs[f + j].cy;
+        if (((viy < vjy) && (viy < py) && (py <= vjy) && ((vjy - viy) * (px - vix) > (vjx - vix) * (py - viy))) ||
+            ((viy > vjy) && (viy > py) && (py >= vjy) && ((vjy - viy) * (px - vix) < (vjx - vix) * (py - viy))))
+            in = !in;
+    }
+    if (!in) return -1;
+    int best = -1;
+    double bd = DBL_MAX;
+    for (int k = 0; k < n; k++) {
+        const double dx = segs[f + k].x - px, dy = segs[f + k].y - py;
+        const double d2 = dx * dx + dy * dy;
+        if (d2 < bd) {
+            best = f + k;
+            bd = d2;
+        }
+    }
+    return best;
+}
+
+__device__ inline int body_of_seg(const vvhd_body* bodies, int nbody, int s) {
+    int b = 0;
+    while (b + 1 < nbody && bodies[b + 1].first_seg <= s) b++;
+    return b;
+}
+
+// ---------------------------------------------------------------- flow-move
+
+__global__ void k_advect_flags(double* x, double* y, const double* vx, const double* vy, const double* g, int64_t n,
+                               double dt, int remove, const vvhd_body* bodies, int nbody, const vvhd_seg* segs,
+                               const double* geo, int32_t* keep, int32_t* inbody, int32_t* badseg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double nx = x[i] + vx[i] * dt, ny = y[i] + vy[i] * dt;
+    x[i] = nx;
+    y[i] = ny;
+    int k = 1, ib = 0, bs = -1;
+    if (fabs(g[i]) < kRemoveEps) {
+        k = 0;
+    } else if (remove) {
+        for (int b = 0; b < nbody; b++) {
+            const int s = point_invalid(bodies[b], segs, geo + 5 * b, nx, ny);
+            if (s >= 0) {
+                k = 0;
+                ib = 1;
+                bs = s;
+                break;
+            }
+        }
+    }
+    keep[i] = k;
+    inbody[i] = ib;
+    badseg[i] = bs;
+}
+
+__global__ void k_compact(const int32_t* keep, const int32_t* kpos, const int32_t* inbody, const int32_t* ipos,
+                          int64_t n, const double* x, const double* y, const double* g, const double* ie, double* x2,
+                          double* y2, double* g2, double* vx2, double* vy2, double* ie2, int32_t* dead_list) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (keep[i]) {
+        const int p = kpos[i];
+        x2[p] = x[i];
+        y2[p] = y[i];
+        g2[p] = g[i];
+        vx2[p] = 0.;
+        vy2[p] = 0.;
+        ie2[p] = ie[i];
+    } else if (inbody[i]) {
+        dead_list[ipos[i]] = (int)i;
+    }
+}
+
+// Dead force, moment and circulation of the removed vortices in vortex order
+// (MFlowmove.cpp:265-285), then the zero/slip segment loop (:342-355).
+__global__ void k_dead(vvhd_body* bodies, int nbody, vvhd_seg* segs, const int32_t* dead_list, const int32_t* ndead,
+                       const int32_t* badseg, const double* x, const double* y, const double* g) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int nd = *ndead;
+    for (int q = 0; q < nd; q++) {
+        const int i = dead_list[q];
+        const int s = badseg[i];
+        vvhd_body& B = bodies[body_of_seg(bodies, nbody, s)];
+        const double gi = g[i], xi = x[i], yi = y[i];
+        B.fdt_dead[0] += (-yi) * gi;
+        B.fdt_dead[1] += xi * gi;
+        const double ax = B.holder[0] + B.dpos[0], ay = B.holder[1] + B.dpos[1];
+        B.fdt_dead[2] += ((xi - ax) * (xi - ax) + (yi - ay) * (yi - ay)) * gi;
+        segs[s].gsum -= gi;
+        B.g_dead += gi;
+    }
+    for (int b = 0; b < nbody; b++) {
+        vvhd_body& B = bodies[b];
+        const double ax = B.holder[0] + B.dpos[0], ay = B.holder[1] + B.dpos[1];
+        for (int k = 0; k < B.nseg; k++) {
+            vvhd_seg& s = segs[B.first_seg + k];
+            if (fabs(s.g) < kRemoveEps || s.slip) {
+                B.fdt_dead[0] += (-s.cy) * s.g;
+                B.fdt_dead[1] += s.cx * s.g;
+                B.fdt_dead[2] += ((s.cx - ax) * (s.cx - ax) + (s.cy - ay) * (s.cy - ay)) * s.g;
+                s.gsum -= s.g;
+                B.g_dead += s.g;
+            }
+        }
+    }
+}
+
+// Where each segment's shed vortex lands: a flag per segment, positions by an
+// exclusive scan, so the new vortices follow the segment order.
+__global__ void k_shed_flags(vvhd_seg* segs, int nseg, int32_t* flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nseg) return;
+    if (s == nseg) {
+        flag[s] = 0;
+        return;
+    }
+    vvhd_seg& r = segs[s];
+    r.gsum += r.g;
+    flag[s] = (fabs(r.g) >= kRemoveEps && !r.slip) ? 1 : 0;
+}
+
+__global__ void k_shed_write(const vvhd_seg* segs, int nseg, const int32_t* flag, const int32_t* pos, int64_t n,
+                             double* x, double* y, double* g, double* vx, double* vy, double* ie) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg || !flag[s]) return;
+    const vvhd_seg& r = segs[s];
+    const int64_t w = n + pos[s];
+    x[w] = r.cx + (-r.dly) * 1e-4;
+    y[w] = r.cy + r.dlx * 1e-4;
+    g[w] = r.g;
+    vx[w] = 0;
+    vy[w] = 0;
+    ie[w] = 0;
+}
+
+// Space::calc_forces (TSpace.cpp:94-120); fric holds the diffusive friction.
+__global__ void k_calc_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double* fric, double re, double dt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbody) return;
+    const double pr = 0;
+    const double C_NyuDt_Pi = dt / (M_PI * re);
+    const double C_Nyu_Pi = 1. / (M_PI * re);
+    vvhd_body& B = bodies[b];
+    double tmp = 0;
+    B.friction[0] = B.friction[1] = B.friction[2] = 0;
+    for (int k = 0; k < B.nseg; k++) {
+        vvhd_seg& s = segs[B.first_seg + k];
+        s.fric = fric[B.first_seg + k];
+        const double a = sqrt(s.dlx * s.dlx + s.dly * s.dly);
+        tmp += s.gsum;
+        s.Cp += tmp;
+        s.Fr += s.fric * C_NyuDt_Pi;
+        s.Nu += s.hsum * (re * pr / a);
+        const double c = s.fric * C_Nyu_Pi / a;
+        B.friction[0] -= s.dlx * c;
+        B.friction[1] -= s.dly * c;
+        B.friction[2] -= dot2(-s.y, s.x, s.dlx, s.dly) * c;
+        B.nusselt += s.hsum * (re * pr);
+    }
+    B.nusselt /= dt;
+}
+
+// Space::zero_forces (TSpace.cpp:122-143).
+__global__ void k_zero_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, double* fric) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbody) return;
+    vvhd_body& B = bodies[b];
+    for (int k = 0; k < B.nseg; k++) {
+        vvhd_seg& s = segs[B.first_seg + k];
+        s.gsum = s.fric = s.hsum = 0;
+        fric[B.first_seg + k] = 0;
+    }
+    for (int i = 0; i < 3; i++) {
+        B.force_hydro[i] = 0;
+        B.force_holder[i] = 0;
+        B.friction_prev[i] = B.friction[i];
+        B.friction[i] = 0;
+        B.fdt_dead[i] = 0;
+    }
+    B.g_dead = 0;
+    B.nusselt = 0.;
+}
+
+// ------------------------------------------------------------------- SLAE
+
+// MConvectiveFast::_2PI_Xi_g and its near / dist parts (MConvectiveFast.cpp:
+// 275-310), without the 2 pi factor.
+__device__ inline double xi_g_near(double px, double py, double cx, double cy, double dlx, double dly, double rd) {
+    return dot2(cx - px, cy - py, dlx, dly) / (rd * rd);
+}
+__device__ inline double xi_g_dist(double px, double py, double x1, double y1, double x2, double y2) {
+    const double a = (px - x2) * (px - x2) + (py - y2) * (py - y2);
+    const double b = (px - x1) * (px - x1) + (py - y1) * (py - y1);
+    return 0.5 * log(a / b);
+}
+__device__ double xi_g(double px, double py, const vvhd_seg& s, double rd) {
+    const double rd_sqr = rd * rd;
+    const double dr1 = (px - s.cx) * (px - s.cx) + (py - s.cy) * (py - s.cy);
+    const double ex = px - s.cx - s.dlx, ey = py - s.cy - s.dly;
+    const double dr2 = ex * ex + ey * ey;
+    if (dr1 >= rd_sqr && dr2 >= rd_sqr) return 0.5 * log(dr2 / dr1);
+    if (dr1 <= rd_sqr && dr2 <= rd_sqr) return xi_g_near(px, py, s.x, s.y, s.dlx, s.dly, rd);
+    const double a0 = s.dlx * s.dlx + s.dly * s.dly;
+    const double b0 = dot2(px - s.x, py - s.y, s.dlx, s.dly);
+    const double d = sqrt(b0 * b0 - a0 * (((px - s.x) * (px - s.x) + (py - s.y) * (py - s.y)) - rd * rd));
+    double k = (b0 + d) / a0;
+    if ((k <= -0.5) || (k >= 0.5)) k = (b0 - d) / a0;
+    const double p3x = s.x + k * s.dlx, p3y = s.y + k * s.dly;
+    const double c2x = s.cx + s.dlx, c2y = s.cy + s.dly;
+    if (dr1 < rd_sqr)
+        return xi_g_near(px, py, 0.5 * (p3x + s.cx), 0.5 * (p3y + s.cy), p3x - s.cx, p3y - s.cy, rd) +
+               xi_g_dist(px, py, p3x, p3y, c2x, c2y);
+    return xi_g_dist(px, py, s.cx, s.cy, p3x, p3y) +
+           xi_g_near(px, py, 0.5 * (c2x + p3x), 0.5 * (c2y + p3y), c2x - p3x, c2y - p3y, rd);
+}
+
+struct SlaeMap {
+    const vvhd_body* bodies;
+    const int32_t* base;   // first equation (and column) of each body
+    int nbody;
+    int n;                 // order of the system
+};
+
+// Influence cells of the slip rows: A(row of segment i, column of segment m)
+// = Xi_g(corner_m; segment i, |dl_m| / 4) / 2 pi  (:459-482).
+__global__ void k_slip_cells(SlaeMap M, const vvhd_seg* segs, int nseg, double* A) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nseg * nseg) return;
+    const int i = (int)(t / nseg), m = (int)(t % nseg);
+    const int bi = body_of_seg(M.bodies, M.nbody, i);
+    const vvhd_body& Bi = M.bodies[bi];
+    if (i - Bi.first_seg == Bi.special_segment_no) return;
+    const int row = M.base[bi] + (i - Bi.first_seg);
+    const int bm = body_of_seg(M.bodies, M.nbody, m);
+    const int col = M.base[bm] + (m - M.bodies[bm].first_seg);
+    const vvhd_seg& sm = segs[m];
+    const double rd = sqrt(sm.dlx * sm.dlx + sm.dly * sm.dly) * 0.25;
+    A[(size_t)col * M.n + row] = xi_g(sm.cx, sm.cy, segs[i], rd) * k1_2Pi;
+}
+
+// Right-hand side of the slip rows, one wave (= one block) per segment:
+// rotl(U) . dl minus NodeInfluence (:398-418) of the leaf holding the
+// segment centre (stree::findNode, TSortedTree.cpp:284-303).  Lanes
+// evaluate 64 terms at a time; lane 0 adds them in the reference's order
+// (near nodes and their vortices in list order, then each far node's
+// positive and negative monopole).  Also the speed cells (_2PI_A123, :375-396)
+// of a rigid body with no prescribed rotation.
+__global__ __launch_bounds__(64) void k_slip_rhs(SlaeMap M, TreeDev T, const vvhd_seg* segs, int nseg,
+                                                 const double* x, const double* y, const double* g, const double* ie,
+                                                 double infx, double infy, double* A, double* rhs) {
+    __shared__ double buf[2 * kWave];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= nseg) return;
+    const int bi = body_of_seg(M.bodies, M.nbody, i);
+    const vvhd_body& Bi = M.bodies[bi];
+    if (i - Bi.first_seg == Bi.special_segment_no) return;
+    const int row = M.base[bi] + (i - Bi.first_seg);
+    const vvhd_seg s = segs[i];
+    int node = 0;
+    while (T.ch1[node] >= 0) {
+        const int c = T.ch1[node];
+        if (T.nh[node] < T.nw[node])
+            node = (s.x < T.nx[node]) ? c : c + 1;
+        else
+            node = (s.y < T.ny[node]) ? c : c + 1;
+    }
+    const int l = T.leaf_rank[node];
+    // res starts at +0 and a sum of two reals rounds to -0 only when both are
+    // -0, so res is never -0 and adding +0 for a skipped vortex is exact.
+    double res = 0;
+    for (int64_t e = T.near_off[l]; e < T.near_off[l + 1]; e++) {
+        const int Mn = T.near_list[e];
+        const int j1 = T.v1[Mn];
+        for (int j0 = T.v0[Mn]; j0 < j1; j0 += kWave) {
+            const int j = j0 + lane;
+            double t = 0;
+            if (j < j1) {
+                const double gj = g[j];
+                if (gj != 0) t = xi_g(x[j], y[j], s, 1. / ie[j]) * gj;
+            }
+            buf[lane] = t;
+            __syncthreads();
+            if (lane == 0) {
+                const int cnt = min(kWave, j1 - j0);
+                for (int q = 0; q < cnt; q++) res += buf[q];
+            }
+            __syncthreads();
+        }
+    }
+    const double c2x = s.cx + s.dlx, c2y = s.cy + s.dly;
+    const int64_t f1 = T.far_off[l + 1];
+    for (int64_t f0 = T.far_off[l]; f0 < f1; f0 += kWave) {
+        const int64_t e = f0 + lane;
+        double tp = 0, tm = 0;
+        if (e < f1) {
+            const int F = T.far_list[e];
+            tp = xi_g_dist(T.cmp_x[F], T.cmp_y[F], s.cx, s.cy, c2x, c2y) * T.cmp_g[F];
+            tm = xi_g_dist(T.cmm_x[F], T.cmm_y[F], s.cx, s.cy, c2x, c2y) * T.cmm_g[F];
+        }
+        buf[2 * lane] = tp;
+        buf[2 * lane + 1] = tm;
+        __syncthreads();
+        if (lane == 0) {
+            const int cnt = (int)min((int64_t)kWave, f1 - f0);
+            for (int q = 0; q < 2 * cnt; q++) res += buf[q];
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        double r = dot2(-infy, infx, s.dlx, s.dly);
+        r += dot2(-0.0, 0.0, s.dlx, s.dly);   // rotl(body speed 0 / 2 pi) . dl
+        r -= res * k1_2Pi;
+        rhs[row] = r;
+        for (int jb = 0; jb < M.nbody; jb++) {
+            const double A1 = (jb == bi) ? k2Pi * s.dly : 0;
+            const double A2 = (jb == bi) ? -k2Pi * s.dlx : 0;
+            const double A3 = 0;
+            const int sc = M.base[jb] + M.bodies[jb].nseg;
+            A[(size_t)(sc + 0) * M.n + row] = A1 * k1_2Pi;
+            A[(size_t)(sc + 1) * M.n + row] = A2 * k1_2Pi;
+            A[(size_t)(sc + 2) * M.n + row] = A3 * k1_2Pi;
+        }
+    }
+}
+
+// Special-segment rows and the nine body rows (:484-862, :896-975), one
+// thread: the steady-circulation rows depend on a counter over bodies.
+__global__ void k_body_rows(SlaeMap M, const vvhd_seg* segs, const double* gsum_all, double inf_g, double dt,
+                            double* A, double* rhs) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n = M.n;
+    auto cell = [&](int row, int col) -> double& { return A[(size_t)col * n + row]; };
+    int once = 0;
+    for (int b = 0; b < M.nbody; b++) {
+        const vvhd_body& B = M.bodies[b];
+        const int base = M.base[b], ns = B.nseg;
+        const int cs = base + ns, ch = base + ns + 3, ck = base + ns + 6;   // speed, hydro, holder columns
+        {
+            const int row = base + B.special_segment_no;
+            if (B.kutta) {   // fillZeroEquationForSegment (:484-492)
+                rhs[row] = 0;
+                cell(row, base + B.special_segment_no) = 1;
+            } else if (!once++) {   // fillInfSteadyEquationForSegment (:508-522)
+                rhs[row] = -(*gsum_all) - inf_g;
+                for (int jb = 0; jb < M.nbody; jb++) {
+                    for (int m = 0; m < M.bodies[jb].nseg; m++) cell(row, M.base[jb] + m) = 1;
+                    cell(row, M.base[jb] + M.bodies[jb].nseg + 2) = 2 * M.bodies[jb].area;
+                }
+            } else {   // fillSteadyEquationForSegment (:494-506)
+                rhs[row] = B.g_dead + 2 * B.area * B.speed_slae_prev[2];
+                for (int m = 0; m < ns; m++) cell(row, base + m) = 1;
+                cell(row, cs + 2) = 2 * B.area;
+            }
+        }
+        int eq = base + ns;
+        for (int c = 0; c < 3; c++, eq++) {   // fillSpeed?Equation (:803-862), rigid
+            rhs[eq] = 0.0;
+            cell(eq, cs + c) = 1;
+        }
+        const double _1_dt = 1 / dt, _1_2dt = 0.5 / dt;
+        const double axx = B.holder[0] + B.dpos[0], axy = B.holder[1] + B.dpos[1];
+        const double rx = B.cofm[0] - axx, ry = B.cofm[1] - axy;
+        const double area = B.area, rho = 1.0;
+        const double* sp = B.speed_slae_prev;
+        const double moi_axis =
+            B.moi_cofm + ((axx - B.cofm[0]) * (axx - B.cofm[0]) + (axy - B.cofm[1]) * (axy - B.cofm[1])) * area;
+        // fillHydroXEquation (:533-562)
+        rhs[eq] = +area * _1_dt * sp[0] + area * _1_dt * (-(2 * B.cofm[1] + ry)) * sp[2] + (sp[2] * sp[2]) * area * rx -
+                  B.friction_prev[0] + B.fdt_dead[0] * _1_dt;
+        for (int m = 0; m < ns; m++) cell(eq, base + m) = _1_dt * (-segs[B.first_seg + m].cy);
+        cell(eq, cs + 0) = area * _1_dt;
+        cell(eq, cs + 1) = 0;
+        cell(eq, cs + 2) = area * _1_dt * (-2 * B.cofm[1] - ry);
+        cell(eq, ch + 0) = -1;
+        eq++;
+        // fillHydroYEquation (:564-593)
+        rhs[eq] = +area * _1_dt * sp[1] + area * _1_dt * (2 * B.cofm[0] + rx) * sp[2] + (sp[2] * sp[2]) * area * ry -
+                  B.friction_prev[1] + B.fdt_dead[1] * _1_dt;
+        for (int m = 0; m < ns; m++) cell(eq, base + m) = _1_dt * (segs[B.first_seg + m].cx);
+        cell(eq, cs + 0) = 0;
+        cell(eq, cs + 1) = area * _1_dt;
+        cell(eq, cs + 2) = area * _1_dt * (2 * B.cofm[0] + rx);
+        cell(eq, ch + 1) = -1;
+        eq++;
+        // fillHydroOEquation (:595-623)
+        {
+            const double c = area * _1_dt;
+            rhs[eq] = +((-ry) * c * sp[0] + rx * c * sp[1]) + 2 * moi_axis * _1_dt * sp[2] - B.friction_prev[2] +
+                      B.fdt_dead[2] * _1_2dt;
+        }
+        for (int m = 0; m < ns; m++) {
+            const vvhd_seg& sm = segs[B.first_seg + m];
+            cell(eq, base + m) = _1_2dt * ((sm.cx - axx) * (sm.cx - axx) + (sm.cy - axy) * (sm.cy - axy));
+        }
+        cell(eq, cs + 0) = area * _1_dt * (-ry);
+        cell(eq, cs + 1) = area * _1_dt * (rx);
+        cell(eq, cs + 2) = 2 * moi_axis * _1_dt;
+        cell(eq, ch + 2) = -1;
+        eq++;
+        // fillNewtonXEquation (:634-667); no gravity, no child bodies
+        rhs[eq] = -area * _1_dt * rho * sp[0] + area * _1_dt * rho * ry * sp[2] - area * rho * rx * (sp[2] * sp[2]) -
+                  (rho - 1.0) * 0. * area;
+        cell(eq, cs + 0) = -area * _1_dt * rho;
+        cell(eq, cs + 1) = 0;
+        cell(eq, cs + 2) = area * _1_dt * rho * ry;
+        cell(eq, ch + 0) = 1;
+        cell(eq, ck + 0) = -1;
+        eq++;
+        // fillNewtonYEquation (:669-702)
+        rhs[eq] = -area * _1_dt * rho * sp[1] - area * _1_dt * rho * rx * sp[2] - area * rho * ry * (sp[2] * sp[2]) -
+                  (rho - 1.0) * 0. * area;
+        cell(eq, cs + 0) = 0;
+        cell(eq, cs + 1) = -area * _1_dt * rho;
+        cell(eq, cs + 2) = -area * _1_dt * rho * rx;
+        cell(eq, ch + 1) = 1;
+        cell(eq, ck + 1) = -1;
+        eq++;
+        // fillNewtonOEquation (:704-739)
+        rhs[eq] = -area * _1_dt * rho * ((-ry) * sp[0] + rx * sp[1]) - moi_axis * _1_dt * rho * sp[2] -
+                  (rho - 1.0) * ((-ry) * 0.0 + rx * 0.0) * area;
+        cell(eq, cs + 0) = +area * _1_dt * rho * ry;
+        cell(eq, cs + 1) = -area * _1_dt * rho * rx;
+        cell(eq, cs + 2) = -moi_axis * _1_dt * rho;
+        cell(eq, ch + 2) = 1;
+        cell(eq, ck + 2) = -1;
+    }
+}
+
+// Total vortex circulation (Space::gsum, TSpace.hpp:93-99): per-thread
+// Neumaier sums over contiguous chunks combined in a fixed tree, so the
+// result is deterministic and at least as accurate as the serial sum.
+__global__ __launch_bounds__(1024) void k_gsum(const double* g, int64_t n, double* out) {
+    __shared__ double ss[1024], sc[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t a = min(n, (int64_t)t * chunk), b = min(n, a + chunk);
+    double s = 0, c = 0;
+    for (int64_t i = a; i < b; i++) {
+        const double v = g[i];
+        const double u = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - u) + v) : ((v - u) + s);
+        s = u;
+    }
+    ss[t] = s;
+    sc[t] = c;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) {
+            const double s1 = ss[t], s2 = ss[t + o];
+            const double u = s1 + s2;
+            const double e = (fabs(s1) >= fabs(s2)) ? ((s1 - u) + s2) : ((s2 - u) + s1);
+            ss[t] = u;
+            sc[t] = sc[t] + sc[t + o] + e;
+        }
+        __syncthreads();
+    }
+    if (t == 0) *out = ss[0] + sc[0];
+}
+
+__global__ void k_differ(const double* a, const double* b, size_t n, int32_t* differ) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && __double_as_longlong(a[i]) != __double_as_longlong(b[i])) atomicOr(differ, 1);
+}
+
+__global__ void k_writeback(SlaeMap M, vvhd_body* bodies, vvhd_seg* segs, const double* xsol) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= M.nbody) return;
+    vvhd_body& B = bodies[b];
+    const int base = M.base[b];
+    for (int k = 0; k < B.nseg; k++) segs[B.first_seg + k].g = xsol[base + k];
+    for (int i = 0; i < 3; i++) {
+        B.speed_slae[i] = xsol[base + B.nseg + i];
+        B.force_hydro[i] = xsol[base + B.nseg + 3 + i];
+        B.force_holder[i] = xsol[base + B.nseg + 6 + i];
+    }
+}
+
+void sync_bodies_to_host(Ctx& c) {
+    const int ns = (int)c.segs_h.size(), nb = (int)c.bodies_h.size();
+    if (ns)
+        VVHD_HIP(hipMemcpyAsync(c.segs_h.data(), c.segs.p, (size_t)ns * sizeof(vvhd_seg), hipMemcpyDeviceToHost,
+                                c.stream));
+    if (nb)
+        VVHD_HIP(hipMemcpyAsync(c.bodies_h.data(), c.bodies_d.p, (size_t)nb * sizeof(vvhd_body),
+                                hipMemcpyDeviceToHost, c.stream));
+    VVHD_HIP(hipStreamSynchronize(c.stream));
+}
+
+void exclusive_scan_i32(Ctx& c, const int32_t* in, int32_t* out, size_t n) {
+    size_t bytes = 0;
+    VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int32_t)0, n, rocprim::plus<int32_t>(), c.stream));
+    c.tmp.grow(bytes + 16);
+    VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, in, out, (int32_t)0, n, rocprim::plus<int32_t>(), c.stream));
+}
+
+}  // namespace
+
+// Geometry of every uploaded body from its corners (the reference computes
+// it when a body is loaded: TBody::doUpdateSegments + doFillProperties).
+void bodies_prepare(Ctx& c) {
+    const int nb = (int)c.bodies_h.size();
+    c.bodies_d.grow(std::max(nb, 1));
+    c.body_geo.grow(5 * (size_t)std::max(nb, 1));
+    if (!nb) return;
+    VVHD_HIP(hipMemcpyAsync(c.bodies_d.p, c.bodies_h.data(), (size_t)nb * sizeof(vvhd_body), hipMemcpyHostToDevice,
+                            c.stream));
+    hipLaunchKernelGGL(k_body_properties, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, c.bodies_d.p, nb, c.segs.p,
+                       c.body_geo.p);
+    sync_bodies_to_host(c);
+}
+
+void bodies_release(Ctx& c) {
+    c.bodies_d.release();
+    c.body_geo.release();
+    for (auto* b : {&c.slae_A, &c.slae_Acached, &c.slae_LU, &c.slae_b, &c.slae_x}) b->release();
+    for (auto* b : {&c.slae_base, &c.slae_ipiv, &c.slae_info, &c.iscratch4}) b->release();
+    if (c.rocblas) (void)rocblas_destroy_handle((rocblas_handle)c.rocblas);
+    c.rocblas = nullptr;
+    c.slae_n = -1;
+}
+
+void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
+    const int nb = (int)c.bodies_h.size();
+    hipStream_t s = c.stream;
+    if (nb)
+        hipLaunchKernelGGL(k_body_move, dim3(nblocks(nb, 64)), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p,
+                           c.body_geo.p, c.dt);
+    const int64_t n = c.n;
+    c.iscratch0.grow(n + 1);
+    c.iscratch1.grow(n + 1);
+    c.iscratch2.grow(n + 1);
+    c.iscratch3.grow(2 * (size_t)n + 2);
+    c.iscratch4.grow(n + 1);
+    int32_t* keep = c.iscratch0.p;
+    int32_t* inbody = c.iscratch1.p;
+    int32_t* badseg = c.iscratch2.p;
+    int32_t* kpos = c.iscratch3.p;
+    int32_t* ipos = c.iscratch3.p + n + 1;
+    int32_t* dead_list = c.iscratch4.p;
+    VVHD_HIP(hipMemsetAsync(keep + n, 0, 4, s));
+    VVHD_HIP(hipMemsetAsync(inbody + n, 0, 4, s));
+    hipLaunchKernelGGL(k_advect_flags, dim3(nblocks(n)), dim3(kBlock), 0, s, c.x.p, c.y.p, c.vx.p, c.vy.p, c.g.p, n,
+                       c.dt, remove, c.bodies_d.p, nb, c.segs.p, c.body_geo.p, keep, inbody, badseg);
+    exclusive_scan_i32(c, keep, kpos, (size_t)n + 1);
+    exclusive_scan_i32(c, inbody, ipos, (size_t)n + 1);
+    hipLaunchKernelGGL(k_compact, dim3(nblocks(n)), dim3(kBlock), 0, s, keep, kpos, inbody, ipos, n, c.x.p, c.y.p,
+                       c.g.p, c.ie.p, c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p, dead_list);
+    if (nb)
+        hipLaunchKernelGGL(k_dead, dim3(1), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p, dead_list, ipos + n, badseg,
+                           c.x.p, c.y.p, c.g.p);
+    int32_t hn[2] = {0, 0};
+    VVHD_HIP(hipMemcpyAsync(&hn[0], kpos + n, 4, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(&hn[1], ipos + n, 4, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipStreamSynchronize(s));
+    if (n) {
+        std::swap(c.x.p, c.x2.p);
+        std::swap(c.y.p, c.y2.p);
+        std::swap(c.g.p, c.g2.p);
+        std::swap(c.vx.p, c.vx2.p);
+        std::swap(c.vy.p, c.vy2.p);
+        std::swap(c.ie.p, c.ie2.p);
+    }
+    c.n = hn[0];
+    if (cleaned) *cleaned = hn[1];
+    c.t.built = false;
+    if (nb) sync_bodies_to_host(c);
+}
+
+void vortex_shed(Ctx& c) {
+    const int nb = (int)c.bodies_h.size();
+    if (!nb) return;
+    const int nseg = (int)c.segs_h.size();
+    reserve_vortices(c, c.n + nseg + 1, true);
+    c.iscratch0.grow(nseg + 1);
+    c.iscratch1.grow(nseg + 1);
+    hipLaunchKernelGGL(k_shed_flags, dim3(nblocks(nseg + 1)), dim3(kBlock), 0, c.stream, c.segs.p, nseg,
+                       c.iscratch0.p);
+    exclusive_scan_i32(c, c.iscratch0.p, c.iscratch1.p, (size_t)nseg + 1);
+    hipLaunchKernelGGL(k_shed_write, dim3(nblocks(nseg)), dim3(kBlock), 0, c.stream, c.segs.p, nseg, c.iscratch0.p,
+                       c.iscratch1.p, c.n, c.x.p, c.y.p, c.g.p, c.vx.p, c.vy.p, c.ie.p);
+    int32_t added = 0;
+    VVHD_HIP(hipMemcpyAsync(&added, c.iscratch1.p + nseg, 4, hipMemcpyDeviceToHost, c.stream));
+    VVHD_HIP(hipStreamSynchronize(c.stream));
+    c.n += added;
+    c.t.built = false;
+    sync_bodies_to_host(c);
+}
+
+void calc_forces(Ctx& c) {
+    const int nb = (int)c.bodies_h.size();
+    if (!nb) return;
+    hipLaunchKernelGGL(k_calc_forces, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, c.bodies_d.p, nb, c.segs.p,
+                       c.fric.p, c.re, c.dt);
+    sync_bodies_to_host(c);
+}
+
+void zero_forces(Ctx& c) {
+    const int nb = (int)c.bodies_h.size();
+    if (!nb) return;
+    hipLaunchKernelGGL(k_zero_forces, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, c.bodies_d.p, nb, c.segs.p,
+                       c.fric.p);
+    sync_bodies_to_host(c);
+}
+
+void calc_circulation(Ctx& c, double infx, double infy) {
+    const int nb = (int)c.bodies_h.size();
+    const int nseg = (int)c.segs_h.size();
+    VVHD_CHECK(nb > 0, VVHD_ESTATE, "calc_circulation: no bodies");
+    const int n = nseg + 9 * nb;
+    hipStream_t s = c.stream;
+    std::vector<int32_t> base(nb);
+    int acc = 0;
+    for (int b = 0; b < nb; b++) {
+        base[b] = acc;
+        acc += c.bodies_h[b].nseg + 9;
+    }
+    c.slae_base.grow(nb);
+    VVHD_HIP(hipMemcpyAsync(c.slae_base.p, base.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s));
+    const size_t nn = (size_t)n * n;
+    c.slae_A.grow(nn);
+    c.slae_b.grow(n + 1);
+    c.slae_x.grow(n + 1);
+    VVHD_HIP(hipMemsetAsync(c.slae_A.p, 0, nn * 8, s));
+    VVHD_HIP(hipMemsetAsync(c.slae_b.p, 0, (size_t)(n + 1) * 8, s));
+    SlaeMap M{c.bodies_d.p, c.slae_base.p, nb, n};
+    hipLaunchKernelGGL(k_slip_cells, dim3(nblocks((int64_t)nseg * nseg)), dim3(kBlock), 0, s, M, c.segs.p, nseg,
+                       c.slae_A.p);
+    hipLaunchKernelGGL(k_slip_rhs, dim3(nseg), dim3(kWave), 0, s, M, tree_dev(c.t), c.segs.p, nseg, c.x.p, c.y.p,
+                       c.g.p, c.ie.p, infx, infy, c.slae_A.p, c.slae_b.p);
+    hipLaunchKernelGGL(k_gsum, dim3(1), dim3(1024), 0, s, c.g.p, c.n, c.slae_x.p + n);
+    hipLaunchKernelGGL(k_body_rows, dim3(1), dim3(64), 0, s, M, c.segs.p, c.slae_x.p + n, c.inf_g, c.dt, c.slae_A.p,
+                       c.slae_b.p);
+    c.slae_filled = n;
+    bool refactor = !(c.time > 0) || c.slae_n != n;
+    c.slae_info.grow(4);
+    if (!refactor) {
+        VVHD_HIP(hipMemsetAsync(c.slae_info.p + 1, 0, 4, s));
+        hipLaunchKernelGGL(k_differ, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, s, c.slae_A.p, c.slae_Acached.p, nn,
+                           c.slae_info.p + 1);
+        int32_t differ = 0;
+        VVHD_HIP(hipMemcpyAsync(&differ, c.slae_info.p + 1, 4, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+        refactor = differ != 0;
+    }
+    if (!c.rocblas) {
+        rocblas_handle h = nullptr;
+        VVHD_CHECK(rocblas_create_handle(&h) == rocblas_status_success, VVHD_EHIP, "rocblas_create_handle failed");
+        c.rocblas = h;
+    }
+    rocblas_handle h = (rocblas_handle)c.rocblas;
+    VVHD_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, VVHD_EHIP, "rocblas_set_stream failed");
+    if (refactor) {
+        c.slae_n = -1;
+        c.slae_Acached.grow(nn);
+        c.slae_LU.grow(nn);
+        c.slae_ipiv.grow(n + 1);
+        VVHD_HIP(hipMemcpyAsync(c.slae_Acached.p, c.slae_A.p, nn * 8, hipMemcpyDeviceToDevice, s));
+        VVHD_HIP(hipMemcpyAsync(c.slae_LU.p, c.slae_A.p, nn * 8, hipMemcpyDeviceToDevice, s));
+        VVHD_CHECK(rocsolver_dgetrf(h, n, n, c.slae_LU.p, n, c.slae_ipiv.p, c.slae_info.p) == rocblas_status_success,
+                   VVHD_EHIP, "rocsolver_dgetrf failed");
+        int32_t info = 0;
+        VVHD_HIP(hipMemcpyAsync(&info, c.slae_info.p, 4, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+        if (info != 0) throw Error(VVHD_ESINGULAR, "calc_circulation: singular matrix, column " + std::to_string(info));
+        c.slae_n = n;
+    }
+    VVHD_HIP(hipMemcpyAsync(c.slae_x.p, c.slae_b.p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    VVHD_CHECK(rocsolver_dgetrs(h, rocblas_operation_none, n, 1, c.slae_LU.p, n, c.slae_ipiv.p, c.slae_x.p, n) ==
+                   rocblas_status_success,
+               VVHD_EHIP, "rocsolver_dgetrs failed");
+    hipLaunchKernelGGL(k_writeback, dim3(nblocks(nb, 64)), dim3(64), 0, s, M, c.bodies_d.p, c.segs.p, c.slae_x.p);
+    sync_bodies_to_host(c);
+}
+
+}  // namespace vvhd
 
 using namespace vvhd;
 
@@ -8,36 +840,74 @@ extern "C" {
 
 int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned) {
     REQUIRE_CTX(ctx);
-    (void)remove;
-    (void)cleaned;
-    ctx->c.err = "move_and_clean: not available in this build";
-    return VVHD_EUNSUPPORTED;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_HIP(hipSetDevice(c.device));
+    VVHD_HIP(hipEventRecord(c.ev[8], c.stream));
+    move_and_clean(c, remove, cleaned);
+    VVHD_HIP(hipEventRecord(c.ev[9], c.stream));
+    c.phase_recorded[4] = true;
+    API_END(ctx)
 }
 
 int vvhd_vortex_shed(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
-    ctx->c.err = "vortex_shed: not available in this build";
-    return VVHD_EUNSUPPORTED;
+    API_BEGIN
+    VVHD_HIP(hipSetDevice(ctx->c.device));
+    vortex_shed(ctx->c);
+    API_END(ctx)
 }
 
 int vvhd_calc_circulation(vvhd_ctx* ctx, double inf_x, double inf_y) {
     REQUIRE_CTX(ctx);
-    (void)inf_x;
-    (void)inf_y;
-    ctx->c.err = "calc_circulation: not available in this build";
-    return VVHD_EUNSUPPORTED;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_HIP(hipSetDevice(c.device));
+    VVHD_CHECK(c.t.built, VVHD_ESTATE, "calc_circulation: tree is not built");
+    VVHD_HIP(hipEventRecord(c.ev[10], c.stream));
+    calc_circulation(c, inf_x, inf_y);
+    VVHD_HIP(hipEventRecord(c.ev[11], c.stream));
+    c.phase_recorded[5] = true;
+    API_END(ctx)
+}
+
+int vvhd_slae_system(vvhd_ctx* ctx, int32_t* n, double* A, double* rhs) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_CHECK(n, VVHD_EINVAL, "slae_system: null order");
+    VVHD_CHECK(c.slae_filled > 0, VVHD_ESTATE, "slae_system: no system has been filled");
+    const int m = c.slae_filled;
+    *n = m;
+    VVHD_HIP(hipSetDevice(c.device));
+    if (A) {
+        std::vector<double> cm((size_t)m * m);
+        VVHD_HIP(hipMemcpyAsync(cm.data(), c.slae_A.p, cm.size() * 8, hipMemcpyDeviceToHost, c.stream));
+        VVHD_HIP(hipStreamSynchronize(c.stream));
+        for (int r = 0; r < m; r++)
+            for (int q = 0; q < m; q++) A[(size_t)r * m + q] = cm[(size_t)q * m + r];
+    }
+    if (rhs) {
+        VVHD_HIP(hipMemcpyAsync(rhs, c.slae_b.p, (size_t)m * 8, hipMemcpyDeviceToHost, c.stream));
+        VVHD_HIP(hipStreamSynchronize(c.stream));
+    }
+    API_END(ctx)
 }
 
 int vvhd_calc_forces(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
-    ctx->c.err = "calc_forces: not available in this build";
-    return VVHD_EUNSUPPORTED;
+    API_BEGIN
+    VVHD_HIP(hipSetDevice(ctx->c.device));
+    calc_forces(ctx->c);
+    API_END(ctx)
 }
 
 int vvhd_zero_forces(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
-    ctx->c.err = "zero_forces: not available in this build";
-    return VVHD_EUNSUPPORTED;
+    API_BEGIN
+    VVHD_HIP(hipSetDevice(ctx->c.device));
+    zero_forces(ctx->c);
+    API_END(ctx)
 }
 
 }  // extern "C"

@@ -147,6 +147,7 @@ struct TreeArrays {
     int32_t nnodes = 0, nleaves_h = 0;
     int64_t nnear = 0, nfar = 0;
     int max_depth = 0;
+    std::vector<std::pair<int, int>> levels;   // node id range per depth
     bool built = false;
     int far_criteria = 8;
     double min_node = 0, max_node = DBL_MAX;
@@ -176,11 +177,20 @@ struct Ctx {
     DBuf<vvhd_seg> segs;
     DBuf<double> fric;                       // per segment friction accumulator
     DBuf<int32_t> body_range;                // per body: first segment, count
+    DBuf<vvhd_body> bodies_d;                // device copy of the body records
+    DBuf<double> body_geo;                   // per body: r2max, bbox (blx, bly, trx, try)
+
+    // boundary-condition system (column-major, LAPACK convention)
+    DBuf<double> slae_A, slae_Acached, slae_LU, slae_b, slae_x;
+    DBuf<int32_t> slae_base, slae_ipiv, slae_info;
+    int slae_n = -1;                         // order of the cached factorisation
+    int slae_filled = 0;                     // order of the last filled system
+    void* rocblas = nullptr;                 // rocblas_handle
 
     TreeArrays t;
 
     // scratch
-    DBuf<int32_t> iscratch0, iscratch1, iscratch2, iscratch3;
+    DBuf<int32_t> iscratch0, iscratch1, iscratch2, iscratch3, iscratch4;
     DBuf<int64_t> lscratch0;
     DBuf<uint8_t> tmp;
 
@@ -237,6 +247,7 @@ __device__ inline double box_dist2(const TreeDev& T, int a, int b) {
 
 // host helpers implemented in ctx.hip
 void ensure_vortex_capacity(Ctx& c, int64_t n);
+void reserve_vortices(Ctx& c, int64_t n, bool preserve);
 void sync(Ctx& c);
 
 // phase entry points (one translation unit each)
@@ -246,6 +257,8 @@ void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 void convective(Ctx& c, double inf_x, double inf_y, int kahan);
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out);
 void diffusive(Ctx& c);
+void bodies_prepare(Ctx& c);
+void bodies_release(Ctx& c);
 void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out);
 void pack_sources(Ctx& c);
 

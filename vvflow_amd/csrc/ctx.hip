@@ -156,6 +156,8 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     c.tmp.release();
     c.eflag.release();
     c.segs.release();
+    c.body_range.release();
+    bodies_release(c);
     TreeArrays& t = c.t;
     for (auto* b : {&t.x, &t.y, &t.h, &t.w, &t.cmp_x, &t.cmp_y, &t.cmp_g, &t.cmm_x, &t.cmm_y, &t.cmm_g}) b->release();
     for (auto* b : {&t.blx, &t.bly, &t.trx, &t.try_, &t.path}) b->release();
@@ -292,6 +294,7 @@ int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg, const 
         VVHD_HIP(hipMemcpyAsync(c.fric.p, fr.data(), (size_t)nseg * 8, hipMemcpyHostToDevice, c.stream));
         VVHD_HIP(hipStreamSynchronize(c.stream));
     }
+    bodies_prepare(c);
     c.t.built = false;
     API_END(ctx)
 }
