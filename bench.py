@@ -1,24 +1,33 @@
-"""Single-step benchmark of the hot path on one MI355X.
+"""Single-GPU benchmark of the hot path: learned vs plain far field at a
+matched convective error.
 
 Workload: the 1M-vortex Karman-street wake (two staggered rows of 200
 Gaussian clusters, sigma 0.05, SplitMix64 seed 0x5EED0001), Re = 600,
 U = (1, 0), merging on.  One step = tree build + epsilon (with merges) +
-convective + diffusive velocity, starting from a device-resident copy of the
-input (merges mutate it, so every step restores the input first; the
-restore is a device-to-device copy inside the timed region).
+convective + diffusive velocity from a device-resident copy of the input
+(merges mutate it, so every step restores the input first; the restore is a
+device-to-device copy inside the timed region).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
-                    [--far-criteria F] [--plain] [--no-cpu] [--error-sample S]
+                    [--learned-fc F] [--plain-fcs 512,1024,2048] [--no-plain]
+                    [--no-cpu] [--error-sample S] [--model PATH]
 
-Prints one JSON line.  value = N * K / (time of K steps) in particle-steps/s.
-roofline: the near-field convective kernel (dominant), algorithmic FP32
-flops = 11 per near pair (MConvectiveFast.cpp:116-120) x the near pairs of the
-built tree, over that kernel's duration measured with HIP events on the
-library's stream, against the MI355X FP32 peak of 157.3 TFLOP/s.
-cpu_baseline: the FP64 oracle (the reference's algorithm) on this host:
-full tree build and epsilon (serial, as the reference), convective and
-diffusive on a deterministic subset of leaves spread over all host cores,
-scaled to the full step.
+value: particle-steps/s of the learned path (far criterion --learned-fc,
+learned far-field correction, FP32 Kahan pair sums) = N * K / (time of K
+steps).  The plain tree path (the reference's method) is timed at the
+smallest far criterion of --plain-fcs whose convective error reaches 1e-5,
+and at the reference's far criterion 8.  Errors: relative L2 of the induced
+convective velocity against the FP64 direct sum on --error-sample targets.
+roofline: the kernel with the largest device time in the learned step --
+the near-field pair kernel (11 FP32 flops per near pair,
+MConvectiveFast.cpp:116-120, against the FP32 vector peak 157.3 TFLOP/s) or
+the learned far-field kernel (9984 bf16 matrix flops per (vortex, far node)
+pair against the dense bf16 peak 2.5 PFLOP/s) -- measured with HIP events on
+the library's stream.
+cpu_baseline: the FP64 oracle (the reference's algorithm, plain path at far
+criterion 8) on this host: full tree build and epsilon (serial, as the
+reference), convective and diffusive on a subset of leaves spread over the
+host cores, scaled to the full step.
 """
 import argparse
 import json
@@ -32,22 +41,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 FLOPS_PER_NEAR_PAIR = 11
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "near_kernel_traffic_wake1M.txt")
+FLOPS_PER_FAR_PAIR = 2 * (12 * 64 + 64 * 64 + 64 * 2)
+TARGET_ERR = 1e-5
+TRAFFIC_FILES = {"k_near": os.path.join(ROOT, "profiles", "near_kernel_traffic_wake1M.txt"),
+                 "k_far_learned": os.path.join(ROOT, "profiles", "far_learned_traffic_wake1M.txt")}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--far-criteria", type=int, default=8)
-    ap.add_argument("--plain", action="store_true", help="plain FP32 accumulation (default: Kahan)")
+    ap.add_argument("--learned-fc", type=int, default=16)
+    ap.add_argument("--plain-fcs", default="512,1024,2048,4096")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="time the plain path at --learned-fc instead")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--error-sample", type=int, default=4000)
     ap.add_argument("--cpu-fraction", type=float, default=0.05)
+    ap.add_argument("--model", default=os.path.join(ROOT, "vvflow_amd", "farfield_mlp.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -55,7 +71,7 @@ def parse():
 
 def copy_calibration():
     """A plain device-to-device copy kernel of known size: FETCH_SIZE and
-    WRITE_SIZE of this kernel calibrate the same counters on k_near."""
+    WRITE_SIZE of this kernel calibrate the same counters on the kernels."""
     import torch
     nbytes = 256 << 20
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -66,14 +82,14 @@ def copy_calibration():
     print(json.dumps({"copy_kernel_bytes_read": nbytes, "copy_kernel_bytes_written": nbytes, "launches": 5}))
 
 
-def traffic_bytes():
-    """HBM bytes per launch of the near-field kernel, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+def traffic_bytes(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated)."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILES[kernel]):
             if line.startswith("traffic_bytes_per_launch"):
                 return float(line.split("=")[1].split()[0])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
@@ -94,8 +110,7 @@ def cpu_baseline(objs, fc, fraction):
     o.epsilon(True)
     t2 = time.perf_counter()
     nl = o.L.vvo_tree_num_leaves(o.h)
-    # a contiguous block of leaves in the middle of the DFS order
-    cnt = max(1, int(nl * fraction))
+    cnt = max(1, int(nl * fraction))   # a contiguous block of leaves in the middle of the DFS order
     lb = (nl - cnt) // 2
     le = lb + cnt
     before = o.vortices()
@@ -104,13 +119,70 @@ def cpu_baseline(objs, fc, fraction):
     o.diffusive(threads=threads, leaf_begin=lb, leaf_end=le)
     t4 = time.perf_counter()
     after = o.vortices()
-    touched = int(np.sum(np.any(after[:, 3:5] != before[:, 3:5], axis=1)))
-    touched = max(touched, 1)
+    touched = max(int(np.sum(np.any(after[:, 3:5] != before[:, 3:5], axis=1))), 1)
     t_step = (t1 - t0) + (t2 - t1) + (t4 - t3) * (n / touched)
     sample = (f"full tree build ({t1 - t0:.2f} s) and epsilon ({t2 - t1:.2f} s), serial; convective + diffusive "
               f"on {cnt} of {nl} leaves ({touched} targets, {t4 - t3:.2f} s on {threads} threads) scaled by "
               f"{n / touched:.1f}")
     return n / t_step, threads, sample
+
+
+class Bench:
+    def __init__(self, V, ctx, n, inf=(1.0, 0.0)):
+        self.V, self.ctx, self.n, self.inf = V, ctx, n, inf
+
+    def step(self, fc, learned):
+        c = self.ctx
+        c.restore()
+        c.tree_build(fc)
+        c.epsilon(True)
+        if learned:
+            c.convective_learned(self.inf, kahan=True)
+        else:
+            c.convective(self.inf, kahan=True)
+        c.diffusive()
+
+    def time(self, fc, learned, steps, warmup):
+        for _ in range(warmup):
+            self.step(fc, learned)
+        self.ctx.phase_times()   # synchronises
+        phases = np.zeros(8)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step(fc, learned)
+            phases += self.ctx.phase_times()
+        el = time.perf_counter() - t0
+        return el, phases / steps
+
+    def error(self, fc, learned, nsample):
+        """Convective (induced, U = 0) error against the FP64 direct sum of
+        the same post-epsilon state, and the pair counts of this tree."""
+        c = self.ctx
+        c.restore()
+        c.tree_build(fc)
+        merged = c.epsilon(True)
+        if learned:
+            c.convective_learned((0.0, 0.0), kahan=True)
+        else:
+            c.convective((0.0, 0.0), kahan=True)
+        st = c.download()
+        v = st[:, 3:5]
+        live = np.nonzero(st[:, 2] != 0)[0]   # merge partners (g = 0) are skipped by the method
+        tg = np.random.default_rng(0).choice(live, min(nsample, len(live)), replace=False).astype(np.int64)
+        ref = c.direct_sum(tg)
+        err = float(np.linalg.norm(v[tg] - ref) / np.linalg.norm(ref))
+        nr = np.linalg.norm(ref, axis=1)
+        err_max = float(np.max(np.linalg.norm(v[tg] - ref, axis=1) / np.maximum(nr, 1e-3 * np.sqrt(np.mean(nr ** 2)))))
+        work = c.work_counts()
+        return dict(rel_l2=err, max_rel=err_max, merged=int(merged), near_pairs=int(work[0]),
+                    far_pairs=int(work[2]))
+
+
+def roofline(name, achieved, peak, traffic, extra):
+    out = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+           "frac": achieved / peak, "traffic": traffic}
+    out.update(extra)
+    return out
 
 
 def main():
@@ -131,55 +203,56 @@ def main():
     ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
     ctx.upload(objs)
     ctx.snapshot()
-    kahan = not a.plain
-    inf = (1.0, 0.0)
+    B = Bench(V, ctx, n)
+    learned = not a.plain_only
+    if learned:
+        from vvflow_amd.train_farfield import load_model
+        ctx.far_model_set(load_model(a.model))
+    fc = a.learned_fc
 
-    def step():
-        ctx.restore()
-        ctx.tree_build(a.far_criteria)
-        ctx.epsilon(True)
-        ctx.convective(inf, kahan=kahan)
-        ctx.diffusive()
-
-    for _ in range(a.warmup):
-        step()
-    ctx.phase_times()   # synchronises
-    near_ms, phases = [], np.zeros(8)
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-        pt = ctx.phase_times()
-        phases += pt
-        near_ms.append(pt[6])
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed, ph = B.time(fc, learned, a.steps, a.warmup)
     value = n * a.steps / elapsed
-    phases /= a.steps
+    acc = B.error(fc, learned, a.error_sample)
+    near_ms, far_ms = float(ph[6]), float(ph[7]) if learned else 0.0
+    near_tf = acc["near_pairs"] * FLOPS_PER_NEAR_PAIR / (near_ms * 1e-3) / 1e12
+    rl_near = roofline("k_near (convective near field, FP32 VALU)", near_tf, FP32_PEAK_TFLOPS, traffic_bytes("k_near"),
+                       {"compute_unit": "FP32 VALU (no matrix instructions); peak is the published FP32 figure",
+                        "algorithmic_flops": acc["near_pairs"] * FLOPS_PER_NEAR_PAIR, "near_pairs": acc["near_pairs"],
+                        "kernel_ms": near_ms})
+    rls = [rl_near]
+    if learned:
+        far_tf = acc["far_pairs"] * FLOPS_PER_FAR_PAIR / (far_ms * 1e-3) / 1e12
+        rls.append(roofline("k_far_learned (learned far field, bf16 MFMA)", far_tf, BF16_PEAK_TFLOPS,
+                            traffic_bytes("k_far_learned"),
+                            {"compute_unit": "bf16 matrix cores, dense peak",
+                             "algorithmic_flops": acc["far_pairs"] * FLOPS_PER_FAR_PAIR,
+                             "far_pairs": acc["far_pairs"], "kernel_ms": far_ms}))
+    rls.sort(key=lambda r: -r["kernel_ms"])
 
-    work = ctx.work_counts()
-    near_pairs = int(work[0])
-    flops = near_pairs * FLOPS_PER_NEAR_PAIR
-    near_avg_ms = float(np.mean(near_ms))
-    achieved = flops / (near_avg_ms * 1e-3) / 1e12
-
-    # accuracy of this step against the FP64 direct sum (induced velocity)
-    ctx.restore()
-    ctx.tree_build(a.far_criteria)
-    merged = ctx.epsilon(True)
-    ctx.convective((0.0, 0.0), kahan=kahan)
-    v = ctx.download()[:, 3:5]
-    tg = np.random.default_rng(0).choice(n, min(a.error_sample, n), replace=False).astype(np.int64)
-    ref = ctx.direct_sum(tg)
-    err_l2 = float(np.linalg.norm(v[tg] - ref) / np.linalg.norm(ref))
-    nr = np.linalg.norm(ref, axis=1)
-    err_max = float(np.max(np.linalg.norm(v[tg] - ref, axis=1) / np.maximum(nr, 1e-3 * np.sqrt(np.mean(nr ** 2)))))
+    plain = None
+    if learned and not a.no_plain:
+        plain = {"reference_fc8": None, "at_target": None, "sweep": []}
+        e8 = B.error(8, False, a.error_sample)
+        el8, _ = B.time(8, False, a.plain_steps, 1)
+        plain["reference_fc8"] = {"far_criteria": 8, "value": n * a.plain_steps / el8, "rel_l2": e8["rel_l2"],
+                                  "ms_per_step": el8 / a.plain_steps * 1e3}
+        for f in [int(x) for x in a.plain_fcs.split(",")]:
+            e = B.error(f, False, a.error_sample)
+            rec = {"far_criteria": f, "rel_l2": e["rel_l2"], "near_pairs": e["near_pairs"]}
+            plain["sweep"].append(rec)
+            print(f"plain fc={f}: rel L2 {e['rel_l2']:.3e}", file=sys.stderr, flush=True)
+            if e["rel_l2"] <= TARGET_ERR:
+                elp, _ = B.time(f, False, a.plain_steps, 1)
+                rec["value"] = n * a.plain_steps / elp
+                rec["ms_per_step"] = elp / a.plain_steps * 1e3
+                plain["at_target"] = rec
+                break
 
     cpu = None
     if not a.no_cpu:
-        cv, cores, sample = cpu_baseline(objs, a.far_criteria, a.cpu_fraction)
+        cv, cores, sample = cpu_baseline(objs, 8, a.cpu_fraction)
         cpu = {"value": cv, "unit": "particle-steps/s", "cores": cores, "kind": "port", "sample": sample}
 
-    tb = traffic_bytes()
     out = {
         "metric": "particle-steps/s at N=1M (conv+eps+diff), learned vs plain at <=1e-5 rel err",
         "value": value,
@@ -195,27 +268,19 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"karman-street wake, {n} vortices, one step: tree + epsilon (merge) + convective + "
-                        f"diffusive, plain tree path",
-            "n": n, "far_criteria": a.far_criteria, "re": 600.0, "accumulation": "kahan" if kahan else "plain",
-            "pair_dtype": "fp32", "tree_epsilon_dtype": "fp64",
+                        f"diffusive, {'learned far field' if learned else 'plain tree path'} at far criterion {fc}",
+            "n": n, "far_criteria": fc, "re": 600.0, "accumulation": "kahan", "pair_dtype": "fp32",
+            "tree_epsilon_dtype": "fp64", "far_field": "learned (bf16 MLP)" if learned else "plain",
         },
-        "phases_ms": {"tree": float(phases[0]), "epsilon": float(phases[1]), "convective": float(phases[2]),
-                      "diffusive": float(phases[3]), "near_kernel": float(phases[6])},
-        "accuracy": {"rel_l2_vs_fp64_direct_sum": err_l2, "max_rel": err_max, "sample_targets": int(len(tg)),
-                     "merged": int(merged)},
-        "roofline": {
-            "kernel": "k_near (convective near field)",
-            "bound": "mfma",
-            "compute_unit": "FP32 VALU (no matrix instructions); peak is the published FP32 figure",
-            "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": tb,
-            "algorithmic_flops": flops,
-            "near_pairs": near_pairs,
-            "kernel_ms": near_avg_ms,
-        },
+        "phases_ms": {"tree": float(ph[0]), "epsilon": float(ph[1]), "convective": float(ph[2]),
+                      "diffusive": float(ph[3]), "near_kernel": near_ms, "far_learned_kernel": far_ms},
+        "accuracy": {"rel_l2_vs_fp64_direct_sum": acc["rel_l2"], "max_rel": acc["max_rel"],
+                     "sample_targets": min(a.error_sample, n), "merged": acc["merged"]},
+        "roofline": rls[0],
+        "roofline_other": rls[1:],
+        "plain": plain,
+        "speedup_vs_plain_at_target": (value / plain["at_target"]["value"]
+                                       if plain and plain["at_target"] and acc["rel_l2"] <= TARGET_ERR else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

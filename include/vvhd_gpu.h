@@ -115,6 +115,9 @@ int vvhd_tree_lists(vvhd_ctx* ctx, int far, int64_t* count, int64_t* offsets,
                     int32_t* depth, uint64_t* path, double* cm);
 /* stree::findNode for n points: leaf index (DFS order) or -1. */
 int vvhd_tree_find(vvhd_ctx* ctx, const double* xy, int64_t n, int32_t* leaf_out);
+/* Every node of the built tree in device numbering (inspection):
+ * geom {x, y, h, w}, ints {v0, v1, first child or -1, depth}, path code. */
+int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints, uint64_t* path);
 
 /* ---- epsilon: MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:11-63) --
  * Sets _1_eps of every vortex with g != 0 from its two nearest neighbours
@@ -132,6 +135,48 @@ int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan);
 /* MConvectiveFast::velocity(TVec) (:20-34) for n points (host xy, host out). */
 int vvhd_velocity_points(vvhd_ctx* ctx, const double* xy, int64_t n,
                          double inf_x, double inf_y, double* v_out);
+
+/* ---- learned far-field correction (no counterpart in the reference; it
+ * replaces the far-field part of MConvectiveFast::process_all_lists,
+ * :48-86, and keeps its near field and call order) -----------------------
+ * Model: an MLP 12 -> 64 -> 64 -> 2 with SiLU, VVHD_FAR_PARAMS floats in this
+ * order: W1[64][12], b1[64], W2[64][64] (out, in), b2[64], W3[2][64], b3[2]
+ * (the VVHD_FAR_TRAINABLE trainable ones), then the input means mu[12], the
+ * input deviations sd[12] and the output scales ysd[2].  Inference and
+ * training use bf16 weights and activations with FP32 accumulation. */
+#define VVHD_FAR_TRAINABLE 5122
+#define VVHD_FAR_PARAMS 5148
+int vvhd_far_model_set(vvhd_ctx* ctx, const float* params, int32_t n);
+int vvhd_far_model_get(vvhd_ctx* ctx, float* params, int32_t n);
+/* Convective velocity with the learned far field: the same near field as
+ * vvhd_convective, and for every (vortex, far node) pair the field of the
+ * node's per-sign monopoles (moments taken after the epsilon pass) plus the
+ * MLP's correction.  Requires a model (VVHD_ESTATE otherwise). */
+int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan);
+/* Per-node moments after the epsilon pass of the built tree: *nnodes, and
+ * when out != NULL 12 arrays of nnodes doubles: G+, G-, z+ (x, y), z- (x, y),
+ * Q+ (re, im), Q- (re, im), sum(g eps^2) for + and -. */
+int vvhd_far_moments(vvhd_ctx* ctx, int32_t* nnodes, double* out);
+/* Training pairs of the built tree: every (vortex with g != 0, far node of
+ * its leaf) pair kept with probability rate (hash of seed, vortex, node).
+ * *npairs receives the count; with X != NULL (cap >= count) also the raw
+ * features X[12], the normalised FP64 residual labels Y[2], the scale
+ * S d^2 / (2 pi r^3) (0 for a degenerate pair) and, optionally, the vortex
+ * (tree order) and node of each pair. */
+int vvhd_far_sample(vvhd_ctx* ctx, double rate, uint64_t seed, int64_t cap, int64_t* npairs,
+                    float* X, float* Y, float* scale, int32_t* target, int32_t* node);
+/* Raw network outputs (before ysd) for n raw feature rows. */
+int vvhd_far_forward(vvhd_ctx* ctx, const float* X, int64_t n, float* Y);
+/* Gradient of the loss sum_j W_j |net(X_j) - Y_j / ysd|^2 / n with respect
+ * to the VVHD_FAR_TRAINABLE parameters (same kernels as training). */
+int vvhd_far_gradients(vvhd_ctx* ctx, const float* X, const float* Y, const float* W, int64_t n,
+                       float* grad, float* loss);
+/* Training set kept on the device, then `steps` Adam steps on batches of
+ * `batch` samples drawn with replacement (hash of seed and step); lr holds
+ * one learning rate per step; loss (optional) the batch loss per step. */
+int vvhd_far_train_data(vvhd_ctx* ctx, const float* X, const float* Y, const float* W, int64_t n);
+int vvhd_far_train(vvhd_ctx* ctx, int32_t steps, int32_t batch, const float* lr, uint64_t seed,
+                   float* loss);
 
 /* ---- diffusive: MDiffusiveFast::process_vort_list (:8-48) ------------ */
 int vvhd_diffusive(vvhd_ctx* ctx);
@@ -165,12 +210,12 @@ int vvhd_integrals(vvhd_ctx* ctx, double* out5);
 int vvhd_direct_sum(vvhd_ctx* ctx, const int64_t* targets, int64_t nt, double* v_out);
 /* Device time in ms of the phases of the last calls, measured with HIP
  * events on the context stream: {tree, epsilon, convective, diffusive,
- * move_and_clean, slae, convective near-field kernel alone, reserved};
- * -1 for a phase not run yet. */
+ * move_and_clean, slae, convective near-field kernel alone, learned
+ * far-field kernel alone}; -1 for a phase not run yet. */
 int vvhd_phase_times(vvhd_ctx* ctx, float* ms8);
 /* Algorithmic work counted on the interaction lists of the current tree:
- * {near convective pairs, far (leaf, node) terms, epsilon pairs scanned,
- *  diffusive same-sign pairs, diffusive segment pairs, leaves, nodes}. */
+ * {near convective pairs, far (leaf, node) terms, learned far-field
+ *  (vortex, far node) pairs, -1, -1, leaves, nodes}. */
 int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7);
 
 /* ---- synthetic inputs (host) -------------------------------------------

@@ -1,0 +1,124 @@
+"""far_mlp.py -- CPU restatement of the learned far-field network (numpy).
+
+TEST INFRASTRUCTURE ONLY: the checker for the device's MLP kernels
+(vvflow_amd/csrc/farfield.hip).  Only tests/, the smoke entry point and the
+CPU-baseline leg of bench.py may import it.
+
+The network and its flat parameter vector (include/vvhd_gpu.h):
+    W1[64][12] b1[64] W2[64][64] b2[64] W3[2][64] b3[2] | mu[12] sd[12] ysd[2]
+    y = W3 silu(W2 silu(W1 xs + b1) + b2) + b3,   xs = (x - mu) / sd.
+With mirror=True the bf16 rounding points of the device are reproduced
+(round to nearest even): standardised inputs, weights, hidden activations
+fed to the next product, and the deltas fed to the backward products.  With
+mirror=False everything is float64 (for finite-difference checks).
+This network has no counterpart in the reference, so no reference line is
+cited; the checks that pin it are the finite-difference gradient test and the
+closed-form cases in tests/test_far_oracle.py.
+"""
+import numpy as np
+
+N_IN, N_HID, N_OUT = 12, 64, 2
+O_W1 = 0
+O_B1 = O_W1 + N_HID * N_IN
+O_W2 = O_B1 + N_HID
+O_B2 = O_W2 + N_HID * N_HID
+O_W3 = O_B2 + N_HID
+O_B3 = O_W3 + N_OUT * N_HID
+N_TRAIN = O_B3 + N_OUT
+O_MU = N_TRAIN
+O_SD = O_MU + N_IN
+O_YSD = O_SD + N_IN
+N_PARAMS = O_YSD + N_OUT
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+def unpack(p):
+    p = np.asarray(p)
+    if p.dtype != np.float64:   # float64 parameters stay float64 (finite differences)
+        p = p.astype(np.float32)
+    W1 = p[O_W1:O_B1].reshape(N_HID, N_IN)
+    b1 = p[O_B1:O_W2]
+    W2 = p[O_W2:O_B2].reshape(N_HID, N_HID)
+    b2 = p[O_B2:O_W3]
+    W3 = p[O_W3:O_B3].reshape(N_OUT, N_HID)
+    b3 = p[O_B3:N_TRAIN]
+    return W1, b1, W2, b2, W3, b3, p[O_MU:O_SD], p[O_SD:O_YSD], p[O_YSD:N_PARAMS]
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s + z * s * (1.0 - s)
+
+
+def init_params(rng, mu=None, sd=None, ysd=None):
+    """He-style initialisation; mu/sd/ysd default to 0/1/1."""
+    p = np.zeros(N_PARAMS, dtype=np.float32)
+    p[O_W1:O_B1] = rng.normal(0, np.sqrt(2.0 / N_IN), N_HID * N_IN)
+    p[O_W2:O_B2] = rng.normal(0, np.sqrt(2.0 / N_HID), N_HID * N_HID)
+    p[O_W3:O_B3] = rng.normal(0, np.sqrt(1.0 / N_HID), N_OUT * N_HID)
+    p[O_MU:O_SD] = 0 if mu is None else mu
+    p[O_SD:O_YSD] = 1 if sd is None else sd
+    p[O_YSD:N_PARAMS] = 1 if ysd is None else ysd
+    return p
+
+
+def forward(p, X, mirror=True):
+    """Raw outputs (before ysd) and the cache for backward."""
+    W1, b1, W2, b2, W3, b3, mu, sd, _ = unpack(p)
+    r = bf16 if mirror else (lambda a: np.asarray(a, dtype=np.float64))
+    X = np.asarray(X, dtype=np.float32 if mirror else np.float64)
+    xs = r(((X - mu) / sd).astype(np.float32) if mirror else (X - mu) / sd)
+    xs = xs.astype(np.float64)
+    W1r, W2r, W3r = (r(W).astype(np.float64) for W in (W1, W2, W3))
+    z1 = xs @ W1r.T + b1
+    h1 = r(silu(z1).astype(np.float32) if mirror else silu(z1)).astype(np.float64)
+    z2 = h1 @ W2r.T + b2
+    h2 = r(silu(z2).astype(np.float32) if mirror else silu(z2)).astype(np.float64)
+    y = h2 @ W3r.T + b3
+    return y, dict(xs=xs, z1=z1, h1=h1, z2=z2, h2=h2, W=(W1r, W2r, W3r), mirror=mirror)
+
+
+def loss_and_grad(p, X, Y, W, mirror=True):
+    """Loss sum_j W_j |net(X_j) - Y_j/ysd|^2 / n and its gradient with
+    respect to the N_TRAIN trainable parameters (the device's definition)."""
+    n = len(X)
+    y, c = forward(p, X, mirror)
+    ysd = unpack(p)[8].astype(np.float64)
+    W = np.asarray(W, dtype=np.float64)[:, None]
+    res = y - np.asarray(Y, dtype=np.float64) / ysd
+    loss = float(np.sum(W * res * res) / n)
+    dy = 2.0 * W * res / n
+    r = (lambda a: bf16(a.astype(np.float32)).astype(np.float64)) if mirror else (lambda a: a)
+    W1r, W2r, W3r = c["W"]
+    dyr = r(dy)
+    dz2 = (dyr @ W3r) * dsilu(c["z2"])
+    dz2r = r(dz2)
+    dz1 = (dz2r @ W2r) * dsilu(c["z1"])
+    dz1r = r(dz1)
+    g = np.zeros(N_TRAIN)
+    g[O_W3:O_B3] = (dyr.T @ c["h2"]).ravel()
+    g[O_B3:N_TRAIN] = dy.sum(0)
+    g[O_W2:O_B2] = (dz2r.T @ c["h1"]).ravel()
+    g[O_B2:O_W3] = dz2.sum(0)
+    g[O_W1:O_B1] = (dz1r.T @ c["xs"]).ravel()
+    g[O_B1:O_W2] = dz1.sum(0)
+    return loss, g
+
+
+def correction(p, X, scale, ux, uy):
+    """Velocity correction (u, v) of pairs from their features: the network
+    output times ysd, scaled and rotated back to the target direction."""
+    y, _ = forward(p, X, mirror=True)
+    ysd = unpack(p)[8].astype(np.float64)
+    a, b = y[:, 0] * ysd[0], y[:, 1] * ysd[1]
+    return np.stack([(a * ux - b * uy) * scale, (a * uy + b * ux) * scale], 1)

@@ -1559,6 +1559,163 @@ void vvo_zero_forces(vvo_ctx* c) {
     }
 }
 
+// ----------------------------------------------------- learned far field
+// Moments: a leaf sums its vortices per sign (centroid, then the second
+// complex moment about it, smoothing moment sum g eps^2 with eps = 1/_1_eps);
+// an internal node combines its two children with the parallel-axis rule.
+// Children are created after their parent, so a reverse sweep over the node
+// ids sees every child first.
+int32_t vvo_far_moments(vvo_ctx* c, double* out) {
+    const int nn = (int)c->nodes.size();
+    if (!out || !c->built) return nn;
+    auto M = [&](int k, int id) -> double& { return out[(size_t)k * nn + id]; };
+    for (int id = nn - 1; id >= 0; id--) {
+        const Node& nd = c->nodes[id];
+        for (int s = 0; s < 2; s++) {
+            double G = 0, zx, zy, qr = 0, qi = 0, E = 0;
+            if (nd.ch1 < 0) {
+                double sx = 0, sy = 0;
+                for (int k = nd.v0; k < nd.v1; k++) {
+                    const Obj& o = c->vort[k];
+                    if (s == 0 ? !(o.g > 0) : !(o.g < 0)) continue;
+                    G += o.g;
+                    sx += o.g * o.x;
+                    sy += o.g * o.y;
+                    const double ek = 1. / o.inv_eps;
+                    E += o.g * (ek * ek);
+                }
+                if (G != 0) {
+                    zx = sx / G;
+                    zy = sy / G;
+                    for (int k = nd.v0; k < nd.v1; k++) {
+                        const Obj& o = c->vort[k];
+                        if (s == 0 ? !(o.g > 0) : !(o.g < 0)) continue;
+                        const double dx = o.x - zx, dy = o.y - zy;
+                        qr += o.g * (dx * dx - dy * dy);
+                        qi += o.g * (2 * dx * dy);
+                    }
+                } else {
+                    zx = nd.x;
+                    zy = nd.y;
+                }
+            } else {
+                const int a = nd.ch1, b = a + 1;
+                const double Ga = M(s, a), Gb = M(s, b);
+                G = Ga + Gb;
+                E = M(10 + s, a) + M(10 + s, b);
+                if (G != 0) {
+                    const double ax = M(2 + 2 * s, a), ay = M(3 + 2 * s, a);
+                    const double bx = M(2 + 2 * s, b), by = M(3 + 2 * s, b);
+                    zx = (Ga * ax + Gb * bx) / G;
+                    zy = (Ga * ay + Gb * by) / G;
+                    const double dax = ax - zx, day = ay - zy, dbx = bx - zx, dby = by - zy;
+                    qr = M(6 + 2 * s, a) + M(6 + 2 * s, b) + Ga * (dax * dax - day * day) + Gb * (dbx * dbx - dby * dby);
+                    qi = M(7 + 2 * s, a) + M(7 + 2 * s, b) + Ga * (2 * dax * day) + Gb * (2 * dbx * dby);
+                } else {
+                    zx = nd.x;
+                    zy = nd.y;
+                }
+            }
+            M(s, id) = G;
+            M(2 + 2 * s, id) = zx;
+            M(3 + 2 * s, id) = zy;
+            M(6 + 2 * s, id) = qr;
+            M(7 + 2 * s, id) = qi;
+            M(10 + s, id) = E;
+        }
+    }
+    return nn;
+}
+
+int vvo_far_pairs(vvo_ctx* c, const int32_t* target, const int32_t* node, int64_t np, double* X, double* Y,
+                  double* scale, double* base, double* exact) {
+    if (!c->built) {
+        c->err = "far_pairs: tree is not built";
+        return 1;
+    }
+    const int nn = (int)c->nodes.size();
+    std::vector<double> mom((size_t)12 * nn);
+    vvo_far_moments(c, mom.data());
+    auto M = [&](int k, int id) { return mom[(size_t)k * nn + id]; };
+    for (int64_t p = 0; p < np; p++) {
+        const Obj& t = c->vort[target[p]];
+        const int F = node[p];
+        const Node& nd = c->nodes[F];
+        double su = 0, sv = 0;
+        for (int j = nd.v0; j < nd.v1; j++) {
+            const Obj& o = c->vort[j];
+            if (o.g == 0) continue;
+            const double dx = t.x - o.x, dy = t.y - o.y;
+            const double ej = 1. / o.inv_eps;
+            const double fac = o.g / (dx * dx + dy * dy + ej * ej);
+            su += -dy * fac;
+            sv += dx * fac;
+        }
+        su *= k1_2Pi;
+        sv *= k1_2Pi;
+        double bu = 0, bv = 0;
+        for (int s = 0; s < 2; s++) {
+            const double G = M(s, F);
+            if (G == 0) continue;
+            const double dx = t.x - M(2 + 2 * s, F), dy = t.y - M(3 + 2 * s, F);
+            const double fac = G / (dx * dx + dy * dy);
+            bu += -dy * fac * k1_2Pi;
+            bv += dx * fac * k1_2Pi;
+        }
+        if (base) {
+            base[2 * p] = bu;
+            base[2 * p + 1] = bv;
+        }
+        if (exact) {
+            exact[2 * p] = su;
+            exact[2 * p + 1] = sv;
+        }
+        // features (the device computes the same FP64 expressions, then rounds to FP32)
+        double f[12] = {0};
+        double ux = 0, uy = 0, sc = 0;
+        bool ok = false;
+        const double gp = M(0, F), gm = M(1, F);
+        const double S = std::fabs(gp) + std::fabs(gm);
+        const double dx = t.x - nd.x, dy = t.y - nd.y;
+        const double r = std::sqrt(dx * dx + dy * dy);
+        if (S > 0 && r > 0) {
+            const double ep = M(10, F), em = M(11, F);
+            const double d2 = nd.h * nd.h + nd.w * nd.w + (ep - em) / S;
+            if (d2 > 0 && std::isfinite(d2)) {
+                ok = true;
+                ux = dx / r;
+                uy = dy / r;
+                const double sd = std::sqrt(d2);
+                const double inv1 = 1. / (S * sd), inv2 = 1. / (S * d2);
+                double ax = M(2, F) - nd.x, ay = M(3, F) - nd.y;
+                f[2] = gp * (ax * ux + ay * uy) * inv1;
+                f[3] = gp * (ay * ux - ax * uy) * inv1;
+                ax = M(4, F) - nd.x;
+                ay = M(5, F) - nd.y;
+                f[4] = gm * (ax * ux + ay * uy) * inv1;
+                f[5] = gm * (ay * ux - ax * uy) * inv1;
+                const double qr = M(6, F) + M(8, F), qi = M(7, F) + M(9, F);
+                const double c2r = ux * ux - uy * uy, c2i = -2 * ux * uy;
+                f[6] = (qr * c2r - qi * c2i) * inv2;
+                f[7] = (qr * c2i + qi * c2r) * inv2;
+                f[0] = gp / S;
+                f[1] = gm / S;
+                f[8] = ep * inv2;
+                f[9] = em * inv2;
+                f[10] = sd / r;
+                f[11] = (nd.h - nd.w) / std::fmax(nd.h + nd.w, 1e-300);
+                sc = S * d2 / (2 * kPi * r * r * r);
+            }
+        }
+        for (int k = 0; k < 12; k++) X[12 * p + k] = f[k];
+        const double ru = su - bu, rv = sv - bv;
+        Y[2 * p] = ok ? (ru * ux + rv * uy) / sc : 0.;
+        Y[2 * p + 1] = ok ? (rv * ux - ru * uy) / sc : 0.;
+        scale[p] = sc;
+    }
+    return 0;
+}
+
 // Space::gsum / integral / hydrodynamic_momentum / gmax (TSpace.hpp:85-117)
 void vvo_integrals(vvo_ctx* c, double* out) {
     double gs = 0, in = 0, mx = 0, my = 0, gm = 0;

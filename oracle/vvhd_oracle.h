@@ -128,6 +128,22 @@ int     vvo_vortex_shed(vvo_ctx*);
 int     vvo_slae_size(vvo_ctx*);
 int     vvo_calc_circulation(vvo_ctx*, double* A, double* rhs);
 
+/* ---- Learned far field (checker of the device's learned path; no
+ * counterpart in the reference) -------------------------------------------
+ * Per-node moments of the built tree from the current vortex state (after
+ * the epsilon pass): 12 arrays of nnodes doubles {G+, G-, z+x, z+y, z-x,
+ * z-y, Q+re, Q+im, Q-re, Q-im, E+, E-} with Q the second complex moment
+ * sum g (z - z_s)^2 about the sign's own centroid and E = sum g eps^2.
+ * Leaves sum their vortices; internal nodes combine their children.
+ * Returns the node count (out may be NULL). */
+int32_t vvo_far_moments(vvo_ctx*, double* out);
+/* For np (vortex, node) pairs of the built tree (oracle node ids): the 12
+ * model features, the normalised residual labels, the scale
+ * S d^2 / (2 pi r^3) (0 for a degenerate pair), the per-sign monopole field
+ * and the direct Lamb-kernel field of the node's vortices (u, v). */
+int     vvo_far_pairs(vvo_ctx*, const int32_t* target, const int32_t* node, int64_t np, double* X,
+                      double* Y, double* scale, double* base, double* exact);
+
 /* ---- Integrals (TSpace.cpp:94-143, TSpace.hpp:85-117) ----------------- */
 void    vvo_calc_forces(vvo_ctx*);
 void    vvo_zero_forces(vvo_ctx*);

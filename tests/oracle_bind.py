@@ -96,6 +96,8 @@ class Oracle:
             _sig(L, "vvo_calc_forces", None, [P])
             _sig(L, "vvo_zero_forces", None, [P])
             _sig(L, "vvo_integrals", None, [P, dp])
+            _sig(L, "vvo_far_moments", I32, [P, C.c_void_p])
+            _sig(L, "vvo_far_pairs", C.c_int, [P, ip, ip, I64, dp, dp, dp, C.c_void_p, C.c_void_p])
             cls._lib = L
         return cls._lib
 
@@ -248,6 +250,23 @@ class Oracle:
 
     def zero_forces(self):
         self.L.vvo_zero_forces(self.h)
+
+    def far_moments(self):
+        nn = self.L.vvo_far_moments(self.h, None)
+        out = np.zeros((12, nn))
+        self.L.vvo_far_moments(self.h, out.ctypes.data)
+        return out
+
+    def far_pairs(self, target, node):
+        """Features, labels, scale, monopole field and exact field of pairs
+        (tree-order vortex, oracle node id)."""
+        target = np.ascontiguousarray(target, dtype=np.int32)
+        node = np.ascontiguousarray(node, dtype=np.int32)
+        n = len(target)
+        X = np.zeros((n, 12)); Y = np.zeros((n, 2)); sc = np.zeros(n)
+        base = np.zeros((n, 2)); exact = np.zeros((n, 2))
+        self._check(self.L.vvo_far_pairs(self.h, target, node, n, X, Y, sc, base.ctypes.data, exact.ctypes.data))
+        return X, Y, sc, base, exact
 
     def integrals(self):
         out = np.zeros(5)

@@ -1,0 +1,247 @@
+"""Learned far field on the GPU against the CPU oracle: node moments, the
+pair sampler's features and FP64 labels, the bf16 MFMA network (forward and
+gradients against oracle/far_mlp.py with the same rounding points), training,
+and the learned convective path end to end.
+
+Bars: moments and features bit-exact (same FP64 expressions; features
+rounded once to FP32); labels within 1e-5 relative (FP32 rounding of an FP64
+difference whose sums run in another order); network outputs within 1e-3
+(relative L2) of the oracle with the same bf16 rounding points (the device's
+exp/rcp and FP32 accumulation order can flip a bf16 rounding); gradients
+within 1e-2 per layer; learned-path velocities within 5e-6 (monopole part,
+FP32 Kahan) and 1e-5 (with a network) relative L2 of the oracle's.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import vvflow_amd as V
+from cases import oracle_for, random_cluster_case, rel_l2
+from oracle_bind import direct_sum
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import far_mlp as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = V.Context(0)
+    yield c
+    c.close()
+
+
+def node_map(ctx, o):
+    """Device node id -> oracle node id, by (depth, path code)."""
+    gn = ctx.tree_nodes()
+    T = o.tree()
+    key = {(int(d), int(p)): k for k, (d, p) in enumerate(zip(T["ints"][:, 4], T["path"]))}
+    m = np.array([key[(int(d), int(p))] for d, p in zip(gn["ints"][:, 3], gn["path"].astype(np.int64))])
+    assert len(np.unique(m)) == len(m) == len(T["ints"])
+    return m
+
+
+@pytest.fixture(scope="module")
+def case(ctx):
+    print("case far-field 12000 clusters fc=8", flush=True)
+    objs = random_cluster_case(12000, 31, "cluster")
+    o = oracle_for(objs, inf=(0.0, 0.0))
+    ctx.set_params(re=600.0, dt=0.005)
+    ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
+    ctx.upload(objs)
+    o.tree_build(8)
+    ctx.tree_build(8)
+    o.epsilon(True)
+    ctx.epsilon(True)
+    assert np.array_equal(ctx.download(), o.vortices(), equal_nan=True)
+    return o, node_map(ctx, o)
+
+
+def sample_model(X, Y, seed=3):
+    rng = np.random.default_rng(seed)
+    mu, sd = X.mean(0), X.std(0) + 1e-6
+    ysd = Y.std(0) + 1e-12
+    return M.init_params(rng, mu=mu, sd=sd, ysd=ysd)
+
+
+def test_moments_match_oracle(ctx, case):
+    o, m = case
+    mg = ctx.far_moments()
+    mo = o.far_moments()
+    assert np.array_equal(mg, mo[:, m], equal_nan=True)
+
+
+def test_sample_matches_oracle(ctx, case):
+    o, m = case
+    s = ctx.far_sample(0.05, seed=7)
+    n = len(s["target"])
+    print(f"  sampled pairs {n}", flush=True)
+    assert n > 10000
+    v = o.vortices()
+    assert np.all(v[s["target"], 2] != 0)
+    # every pair is (vortex, far node of its leaf)
+    T = o.tree()
+    foff, flst = T["far"]
+    leaf_of = np.empty(len(v), int)
+    for l, L in enumerate(T["leaves"]):
+        leaf_of[T["ints"][L, 0]:T["ints"][L, 1]] = l
+    on = m[s["node"]]
+    for p in range(0, n, 97):
+        l = leaf_of[s["target"][p]]
+        assert on[p] in flst[foff[l]:foff[l + 1]]
+    X, Y, sc, base, exact = o.far_pairs(s["target"], on)
+    assert np.array_equal(s["X"], X.astype(np.float32))
+    assert np.allclose(s["scale"], sc, rtol=1e-6, atol=0)
+    ok = sc > 0
+    assert np.allclose(s["Y"][ok], Y[ok], rtol=1e-5, atol=1e-6 * np.abs(Y).max())
+
+
+def test_forward_matches_oracle(ctx, case):
+    s = ctx.far_sample(0.02, seed=9)
+    X = s["X"][:20000]
+    p = sample_model(X, s["Y"])
+    ctx.far_model_set(p)
+    yg = ctx.far_forward(X)
+    yo, _ = M.forward(p, X, mirror=True)
+    assert rel_l2(yg, yo) < 1e-3
+    assert np.abs(yg - yo).max() < 2e-2 * np.abs(yo).max()
+    assert np.array_equal(ctx.far_model_get(), p)
+
+
+def test_gradients_match_oracle(ctx, case):
+    s = ctx.far_sample(0.02, seed=11)
+    n = 4096
+    X, Y = s["X"][:n], s["Y"][:n]
+    W = np.random.default_rng(4).uniform(0.5, 1.5, n).astype(np.float32)
+    p = sample_model(s["X"], s["Y"], seed=5)
+    ctx.far_model_set(p)
+    lg, gg = ctx.far_gradients(X, Y, W)
+    lo, go = M.loss_and_grad(p, X, Y, W, mirror=True)
+    assert np.isclose(lg, lo, rtol=1e-3)
+    for a, b in ((M.O_W1, M.O_B1), (M.O_B1, M.O_W2), (M.O_W2, M.O_B2), (M.O_B2, M.O_W3), (M.O_W3, M.O_B3),
+                 (M.O_B3, M.N_TRAIN)):
+        assert rel_l2(gg[a:b], go[a:b]) < 1e-2, (a, b, rel_l2(gg[a:b], go[a:b]))
+
+
+def test_training_reduces_loss(ctx, case):
+    s = ctx.far_sample(0.05, seed=13)
+    p = sample_model(s["X"], s["Y"], seed=6)
+    ctx.far_model_set(p)
+    W = np.ones(len(s["X"]), np.float32)
+    ctx.far_train_data(s["X"], s["Y"], W)
+    l0, _ = ctx.far_gradients(s["X"][:8192], s["Y"][:8192], W[:8192])
+    loss = ctx.far_train(np.full(400, 2e-3, np.float32), batch=4096, seed=3)
+    l1, _ = ctx.far_gradients(s["X"][:8192], s["Y"][:8192], W[:8192])
+    print(f"  loss {l0:.4e} -> {l1:.4e} (batch loss {loss[0]:.3e} -> {loss[-1]:.3e})", flush=True)
+    assert np.all(np.isfinite(loss))
+    assert l1 < 0.2 * l0
+
+
+def _oracle_learned(o, targets, p=None):
+    """Induced velocity of the learned path for the given targets, on the
+    oracle: exact near field + per far node (monopoles + network correction)."""
+    T = o.tree()
+    v = o.vortices()
+    noff, nlst = T["near"]
+    foff, flst = T["far"]
+    ints = T["ints"]
+    leaf_of = np.empty(len(v), int)
+    for l, L in enumerate(T["leaves"]):
+        leaf_of[ints[L, 0]:ints[L, 1]] = l
+    out = np.zeros((len(targets), 2))
+    for q, t in enumerate(targets):
+        l = leaf_of[t]
+        near = nlst[noff[l]:noff[l + 1]]
+        src = np.concatenate([v[ints[N, 0]:ints[N, 1]] for N in near])
+        tgt = v[t:t + 1].copy()
+        tgt[0, 2] = 0.0
+        out[q] = direct_sum(np.concatenate([src, tgt]), targets=np.array([len(src)]), threads=1)[0]
+        far = flst[foff[l]:foff[l + 1]]
+        X, Y, sc, base, exact = o.far_pairs(np.full(len(far), t), far)
+        out[q] += base.sum(0)
+        if p is not None:
+            ok = sc > 0
+            D = v[t, :2] - T["geom"][far, :2]
+            r = np.hypot(D[:, 0], D[:, 1])
+            corr = M.correction(p, X[ok].astype(np.float32), sc[ok], D[ok, 0] / r[ok], D[ok, 1] / r[ok])
+            out[q] += corr.sum(0)
+    return out
+
+
+def test_learned_path_monopoles(ctx, case):
+    """A network with a zero output layer leaves the monopole field: the
+    learned path is then exact near field + per-target monopoles."""
+    o, _ = case
+    s = ctx.far_sample(0.01, seed=17)
+    p = sample_model(s["X"], s["Y"], seed=8)
+    p[M.O_W3:M.N_TRAIN] = 0
+    ctx.far_model_set(p)
+    v0 = ctx.download()
+    ctx.convective_learned((0.0, 0.0))
+    vg = ctx.download()[:, 3:5] - v0[:, 3:5]
+    tg = np.random.default_rng(1).choice(np.nonzero(v0[:, 2] != 0)[0], 150, replace=False)
+    want = _oracle_learned(o, tg)
+    err = rel_l2(vg[tg], want)
+    print(f"  monopole path rel L2 {err:.2e}", flush=True)
+    assert err < 5e-6
+
+
+def test_learned_path_with_network(ctx, case):
+    o, _ = case
+    s = ctx.far_sample(0.05, seed=19)
+    p = sample_model(s["X"], s["Y"], seed=9)
+    ctx.far_model_set(p)
+    ctx.far_train_data(s["X"], s["Y"], np.ones(len(s["X"]), np.float32))
+    ctx.far_train(np.full(300, 2e-3, np.float32), batch=4096, seed=5)
+    p = ctx.far_model_get()
+    v0 = ctx.download()   # convective adds into v: compare the increment
+    ctx.convective_learned((0.0, 0.0))
+    vg = ctx.download()[:, 3:5] - v0[:, 3:5]
+    tg = np.random.default_rng(2).choice(np.nonzero(v0[:, 2] != 0)[0], 150, replace=False)
+    want = _oracle_learned(o, tg, p)
+    err = rel_l2(vg[tg], want)
+    print(f"  learned path rel L2 {err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "vvflow_amd", "farfield_mlp.npy")
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained far-field model in the tree")
+def test_trained_network_gain_on_wake(ctx):
+    """What the trained network gains: on a 200k-vortex wake (not a training
+    snapshot), convective velocity error against the FP64 direct sum for the
+    plain tree path (the reference's far field), the learned path without the
+    network (per-target monopoles only) and the learned path."""
+    from vvflow_amd.train_farfield import load_model
+    print("case trained network gain, wake 200000 fc=16", flush=True)
+    objs = V.gen_wake(200_000, 0x5EED0002)
+    ctx.set_params(re=600.0, dt=0.005)
+    ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
+    ctx.upload(objs)
+    ctx.tree_build(16)
+    ctx.epsilon(True)
+    v0 = ctx.download()
+    tg = np.random.default_rng(3).choice(np.nonzero(v0[:, 2] != 0)[0], 2000, replace=False).astype(np.int64)
+    ref = ctx.direct_sum(tg)
+
+    def err_of(run):
+        before = ctx.download()[:, 3:5]
+        run()
+        return rel_l2(ctx.download()[tg, 3:5] - before[tg], ref)
+
+    e_plain = err_of(lambda: ctx.convective((0.0, 0.0)))
+    p = load_model(MODEL)
+    p0 = p.copy()
+    p0[M.O_W3:M.N_TRAIN] = 0
+    ctx.far_model_set(p0)
+    e_mono = err_of(lambda: ctx.convective_learned((0.0, 0.0)))
+    ctx.far_model_set(p)
+    e_learn = err_of(lambda: ctx.convective_learned((0.0, 0.0)))
+    print(f"  rel L2 vs direct sum: plain {e_plain:.2e}  monopoles {e_mono:.2e}  learned {e_learn:.2e}", flush=True)
+    assert e_learn < 1.5e-5
+    assert e_learn < e_mono / 5
+    assert e_learn < e_plain / 10

@@ -30,6 +30,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvvhd_gpu.so")
 
 VVHD_OK, VVHD_EINVAL, VVHD_ESTATE, VVHD_EHIP, VVHD_ENOMEM, VVHD_ESINGULAR, VVHD_EUNSUPPORTED = range(7)
+FAR_TRAINABLE, FAR_PARAMS = 5122, 5148   # VVHD_FAR_TRAINABLE, VVHD_FAR_PARAMS
 
 SEG_DTYPE = np.dtype([(n, np.float64) for n in
                       ("x", "y", "g", "inv_eps", "cx", "cy", "dlx", "dly",
@@ -66,6 +67,7 @@ ABI = {
     "vvhd_tree_leaves": (_I, [_P, _VP, _VP, _VP, _VP]),
     "vvhd_tree_lists": (_I, [_P, _I, _VP, _VP, _VP, _VP, _VP]),
     "vvhd_tree_find": (_I, [_P, _VP, _I64, _VP]),
+    "vvhd_tree_nodes": (_I, [_P, _VP, _VP, _VP, _VP]),
     "vvhd_epsilon": (_I, [_P, _I, _VP, _VP, _VP]),
     "vvhd_convective": (_I, [_P, _D, _D, _I]),
     "vvhd_velocity_points": (_I, [_P, _VP, _I64, _D, _D, _VP]),
@@ -76,6 +78,15 @@ ABI = {
     "vvhd_slae_system": (_I, [_P, _VP, _VP, _VP]),
     "vvhd_calc_forces": (_I, [_P]),
     "vvhd_zero_forces": (_I, [_P]),
+    "vvhd_far_model_set": (_I, [_P, _VP, _I32]),
+    "vvhd_far_model_get": (_I, [_P, _VP, _I32]),
+    "vvhd_convective_learned": (_I, [_P, _D, _D, _I]),
+    "vvhd_far_moments": (_I, [_P, _VP, _VP]),
+    "vvhd_far_sample": (_I, [_P, _D, _U64, _I64, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vvhd_far_forward": (_I, [_P, _VP, _I64, _VP]),
+    "vvhd_far_gradients": (_I, [_P, _VP, _VP, _VP, _I64, _VP, _VP]),
+    "vvhd_far_train_data": (_I, [_P, _VP, _VP, _VP, _I64]),
+    "vvhd_far_train": (_I, [_P, _I32, _I32, _VP, _U64, _VP]),
     "vvhd_integrals": (_I, [_P, _VP]),
     "vvhd_direct_sum": (_I, [_P, _VP, _I64, _VP]),
     "vvhd_phase_times": (_I, [_P, _VP]),
@@ -244,6 +255,16 @@ class Context:
             out["cm"] = cm[:tot]
         return out
 
+    def tree_nodes(self):
+        nn = C.c_int32(0)
+        self._check(self.L.vvhd_tree_nodes(self.h, C.addressof(nn), None, None, None))
+        n = nn.value
+        geom = np.zeros((n, 4))
+        ints = np.zeros((n, 4), dtype=np.int32)
+        path = np.zeros(n, dtype=np.uint64)
+        self._check(self.L.vvhd_tree_nodes(self.h, C.addressof(nn), _ptr(geom), _ptr(ints), _ptr(path)))
+        return dict(geom=geom, ints=ints, path=path)
+
     def tree_find(self, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
         out = np.zeros(len(pts), dtype=np.int32)
@@ -296,6 +317,76 @@ class Context:
 
     def zero_forces(self):
         self._check(self.L.vvhd_zero_forces(self.h))
+
+    # ---- learned far field
+    def far_model_set(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.L.vvhd_far_model_set(self.h, _ptr(params), len(params)))
+
+    def far_model_get(self):
+        out = np.zeros(FAR_PARAMS, dtype=np.float32)
+        self._check(self.L.vvhd_far_model_get(self.h, _ptr(out), FAR_PARAMS))
+        return out
+
+    def convective_learned(self, inf=(0.0, 0.0), kahan=True):
+        self._check(self.L.vvhd_convective_learned(self.h, inf[0], inf[1], int(kahan)))
+
+    def far_moments(self):
+        nn = C.c_int32(0)
+        self._check(self.L.vvhd_far_moments(self.h, C.addressof(nn), None))
+        out = np.zeros((12, nn.value))
+        self._check(self.L.vvhd_far_moments(self.h, C.addressof(nn), _ptr(out)))
+        return out
+
+    def far_sample(self, rate, seed=1):
+        """Training pairs of the built tree: dict of X (n,12) float32, Y (n,2),
+        scale (n,), target (n,) tree-order vortex, node (n,) device node id."""
+        npairs = C.c_int64(0)
+        self._check(self.L.vvhd_far_sample(self.h, rate, seed, 0, C.addressof(npairs), None, None, None, None,
+                                           None))
+        n = npairs.value
+        X = np.zeros((n, 12), dtype=np.float32)
+        Y = np.zeros((n, 2), dtype=np.float32)
+        sc = np.zeros(n, dtype=np.float32)
+        tg = np.zeros(n, dtype=np.int32)
+        nd = np.zeros(n, dtype=np.int32)
+        self._check(self.L.vvhd_far_sample(self.h, rate, seed, n, C.addressof(npairs), _ptr(X), _ptr(Y), _ptr(sc),
+                                           _ptr(tg), _ptr(nd)))
+        return dict(X=X, Y=Y, scale=sc, target=tg, node=nd)
+
+    def far_pair_count(self):
+        """Number of (vortex with g != 0, far node of its leaf) pairs of the built tree."""
+        npairs = C.c_int64(0)
+        self._check(self.L.vvhd_far_sample(self.h, 1.0, 0, 0, C.addressof(npairs), None, None, None, None, None))
+        return npairs.value
+
+    def far_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 12)
+        Y = np.zeros((len(X), 2), dtype=np.float32)
+        self._check(self.L.vvhd_far_forward(self.h, _ptr(X), len(X), _ptr(Y)))
+        return Y
+
+    def far_gradients(self, X, Y, W):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 12)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 2)
+        W = np.ascontiguousarray(W, dtype=np.float32).reshape(-1)
+        g = np.zeros(FAR_TRAINABLE, dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(self.L.vvhd_far_gradients(self.h, _ptr(X), _ptr(Y), _ptr(W), len(X), _ptr(g),
+                                              C.addressof(loss)))
+        return loss.value, g
+
+    def far_train_data(self, X, Y, W):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 12)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 2)
+        W = np.ascontiguousarray(W, dtype=np.float32).reshape(-1)
+        self._check(self.L.vvhd_far_train_data(self.h, _ptr(X), _ptr(Y), _ptr(W), len(X)))
+
+    def far_train(self, lr, batch=8192, seed=1):
+        lr = np.ascontiguousarray(lr, dtype=np.float32).reshape(-1)
+        loss = np.zeros(len(lr), dtype=np.float32)
+        self._check(self.L.vvhd_far_train(self.h, len(lr), batch, _ptr(lr), seed, _ptr(loss)))
+        return loss
 
     def integrals(self):
         out = np.zeros(5)

@@ -204,6 +204,20 @@ struct Ctx {
     DBuf<int64_t> rnear_off;                 // transposed near lists (repair)
     DBuf<int32_t> rnear_list, rnear_cnt, leaf_stamp;
 
+    // learned far-field correction
+    DBuf<double> fm;                         // per-node moments after epsilon: kMomArrays x node capacity
+    size_t fm_stride = 0;
+    DBuf<double2> far_v;                     // per-vortex far-field velocity of the learned path
+    DBuf<float> mlp_params;                  // FP32 master parameters (kMlpParams)
+    DBuf<uint16_t> mlp_frags;                // bf16 MFMA fragments + FP32 tail packed from the parameters
+    bool mlp_loaded = false;
+    DBuf<float> tr_x, tr_y, tr_w;            // training set on the device
+    int64_t tr_n = 0;
+    DBuf<float> adam_m, adam_v, grad_part, grad_out;
+    int64_t adam_t = 0;
+    DBuf<int64_t> pair_off;
+    DBuf<int32_t> pair_cnt, pair_t, pair_f;
+
     // timing
     hipEvent_t ev[16] = {};
     bool phase_recorded[8] = {false, false, false, false, false, false, false, false};
@@ -255,10 +269,12 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
 void tree_lists(Ctx& c);
 void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 void convective(Ctx& c, double inf_x, double inf_y, int kahan);
+void convective_near(Ctx& c, int kahan);
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out);
 void diffusive(Ctx& c);
 void bodies_prepare(Ctx& c);
 void bodies_release(Ctx& c);
+void farfield_release(Ctx& c);
 void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out);
 void pack_sources(Ctx& c);
 

@@ -337,14 +337,14 @@ __global__ __launch_bounds__(kDsBlock) void k_direct_sum(const double* x, const 
 
 }  // namespace
 
-void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
+// Near-field sums of every vortex into c.near_v (shared by the plain and the
+// learned path).
+void convective_near(Ctx& c, int kahan) {
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf == 0) return;
     const int nl = T.nleaf;
-    c.taylor.grow(nl + 1);
     c.near_v.grow(c.n + 1);
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, c.taylor.p);
     NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
     VVHD_HIP(hipEventRecord(c.ev[12], c.stream));
     if (kahan)
@@ -355,6 +355,16 @@ void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
                            c.near_v.p);
     VVHD_HIP(hipEventRecord(c.ev[13], c.stream));
     c.phase_recorded[6] = true;
+}
+
+void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
+    const TreeDev T = tree_dev(c.t);
+    if (T.nleaf == 0) return;
+    const int nl = T.nleaf;
+    c.taylor.grow(nl + 1);
+    const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, c.taylor.p);
+    convective_near(c, kahan);
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
                        c.taylor.p, c.x.p, c.y.p, c.g.p, c.near_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     VVHD_HIP(hipGetLastError());

@@ -158,6 +158,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     c.segs.release();
     c.body_range.release();
     bodies_release(c);
+    farfield_release(c);
     TreeArrays& t = c.t;
     for (auto* b : {&t.x, &t.y, &t.h, &t.w, &t.cmp_x, &t.cmp_y, &t.cmp_g, &t.cmm_x, &t.cmm_y, &t.cmm_g}) b->release();
     for (auto* b : {&t.blx, &t.bly, &t.trx, &t.try_, &t.path}) b->release();

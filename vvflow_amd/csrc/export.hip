@@ -50,6 +50,7 @@ __global__ void k_work(TreeDev T, const double* g, unsigned long long* acc) {
     }
     atomicAdd(&acc[0], (unsigned long long)(src * nt));
     atomicAdd(&acc[1], (unsigned long long)((T.far_off[l + 1] - T.far_off[l]) * (nt ? 1 : 0)));
+    atomicAdd(&acc[2], (unsigned long long)((T.far_off[l + 1] - T.far_off[l]) * (T.v1[L] - T.v0[L])));
 }
 
 }  // namespace
@@ -88,6 +89,38 @@ int vvhd_tree_leaves(vvhd_ctx* ctx, int32_t* nleaves, double* geom, int32_t* int
             ints[4 * l + 3] = ns[L];
         }
         if (path) path[l] = pa[L];
+    }
+    API_END(ctx)
+}
+
+int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints, uint64_t* path) {
+    if (!ctx) return VVHD_EINVAL;
+    API_BEGIN
+    Ctx& c = ctx->c;
+    TreeArrays& t = c.t;
+    VVHD_CHECK(t.built, VVHD_ESTATE, "tree_nodes: tree is not built");
+    const int nn = t.nnodes;
+    if (nnodes) *nnodes = nn;
+    if (!geom && !ints && !path) return VVHD_OK;
+    auto x = d2h(c, t.x.p, nn), y = d2h(c, t.y.p, nn), h = d2h(c, t.h.p, nn), w = d2h(c, t.w.p, nn);
+    auto v0 = d2h(c, t.v0.p, nn), v1 = d2h(c, t.v1.p, nn), ch = d2h(c, t.ch1.p, nn), dp = d2h(c, t.depth.p, nn);
+    auto sp = d2h(c, t.split.p, nn);
+    auto pa = d2h(c, t.path.p, nn);
+    sync(c);
+    for (int k = 0; k < nn; k++) {
+        if (geom) {
+            geom[4 * k] = x[k];
+            geom[4 * k + 1] = y[k];
+            geom[4 * k + 2] = h[k];
+            geom[4 * k + 3] = w[k];
+        }
+        if (ints) {
+            ints[4 * k] = v0[k];
+            ints[4 * k + 1] = v1[k];
+            ints[4 * k + 2] = sp[k] == 2 ? ch[k] : -1;
+            ints[4 * k + 3] = dp[k];
+        }
+        if (path) path[k] = pa[k];
     }
     API_END(ctx)
 }
@@ -166,7 +199,7 @@ int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7) {
     sync(c);
     out7[0] = acc[0];
     out7[1] = acc[1];
-    out7[2] = -1;
+    out7[2] = acc[2];
     out7[3] = -1;
     out7[4] = -1;
     out7[5] = c.t.nleaves_h;

@@ -589,7 +589,8 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     hipLaunchKernelGGL(k_root_bbox, dim3(nblocks(std::max(n, nseg))), dim3(kBlock), 0, s, nv, c.x.p, c.y.p, n,
                        c.segs.p, nseg);
 
-    std::vector<std::pair<int, int>> levels;   // node id range of every level
+    std::vector<std::pair<int, int>>& levels = t.levels;   // node id range of every level
+    levels.clear();
     levels.push_back({0, 1});
     int32_t hc[8];
     while (true) {

@@ -1,0 +1,159 @@
+"""train_farfield.py -- trains the learned far-field correction on the GPU.
+
+    python -m vvflow_amd.train_farfield [--out PATH] [--pairs 10000000]
+        [--steps 20000] [--batch 8192] [--re 200,400,800,1400,2000]
+        [--sim-steps 400] [--keep-every 100] [--fc 12,16,24]
+
+1. Snapshots: impulsively started cylinders (R = 0.5, 500 segments,
+   dt = 0.005) run on the device at each Reynolds number; the vortex list is
+   kept every `keep_every` steps.
+2. Pairs: every snapshot is rebuilt as a free vortex cloud with the far
+   criteria listed (no body, unbounded node sizes, as for the wake
+   benchmark), the epsilon pass runs, and (vortex, far node) pairs are
+   sampled with their FP64 direct-sum residual labels (vvhd_far_sample).
+3. Training: feature means / deviations and output scales from the pairs,
+   then Adam on batches drawn on the device, cosine learning-rate decay, with
+   the device's bf16 MFMA forward and backward kernels.  A held-out 5 % of
+   the pairs gives the relative error of the predicted correction.
+The model (VVHD_FAR_PARAMS float32 values) is written with numpy.save.
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import FAR_PARAMS, Context
+from .simulate import CylinderRun
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL_PATH = os.path.join(HERE, "farfield_mlp.npy")
+
+N_IN, N_HID = 12, 64
+
+
+def init_params(rng, mu, sd, ysd):
+    p = np.zeros(FAR_PARAMS, dtype=np.float32)
+    o = 0
+    p[o:o + N_HID * N_IN] = rng.normal(0, np.sqrt(2.0 / N_IN), N_HID * N_IN)
+    o += N_HID * N_IN + N_HID
+    p[o:o + N_HID * N_HID] = rng.normal(0, np.sqrt(2.0 / N_HID), N_HID * N_HID)
+    o += N_HID * N_HID + N_HID
+    p[o:o + 2 * N_HID] = rng.normal(0, np.sqrt(1.0 / N_HID), 2 * N_HID)
+    o += 2 * N_HID + 2
+    p[o:o + N_IN] = mu
+    p[o + N_IN:o + 2 * N_IN] = sd
+    p[o + 2 * N_IN:o + 2 * N_IN + 2] = ysd
+    return p
+
+
+def load_model(path=MODEL_PATH):
+    p = np.load(path, allow_pickle=False)
+    if p.shape != (FAR_PARAMS,) or p.dtype != np.float32:
+        raise ValueError(f"{path}: expected {FAR_PARAMS} float32 parameters")
+    return p
+
+
+def snapshots(ctx, re_list, sim_steps, keep_every, progress=100):
+    out = []
+    for re in re_list:
+        run = CylinderRun(ctx, re=re)
+        for v in run.run(sim_steps, keep_every=keep_every, progress=progress):
+            out.append((re, v))
+    return out
+
+
+def sample_pairs(ctx, snaps, fcs, target_pairs, seed=1):
+    from . import BODY_DTYPE, SEG_DTYPE
+    ctx.upload_bodies(np.zeros(0, SEG_DTYPE), np.zeros(0, BODY_DTYPE))
+    # rate from a first count so that the total lands near target_pairs
+    counts = []
+    for k, (re, v) in enumerate(snaps):
+        ctx.set_params(re=re, dt=0.005)
+        ctx.upload(v)
+        ctx.tree_build(fcs[k % len(fcs)])
+        ctx.epsilon(True)
+        counts.append(ctx.far_pair_count())
+    rate = min(1.0, target_pairs / max(1, sum(counts)))
+    Xs, Ys, Ss = [], [], []
+    for k, (re, v) in enumerate(snaps):
+        ctx.set_params(re=re, dt=0.005)
+        ctx.upload(v)
+        ctx.tree_build(fcs[k % len(fcs)])
+        ctx.epsilon(True)
+        s = ctx.far_sample(rate, seed=seed + k)
+        ok = s["scale"] > 0
+        Xs.append(s["X"][ok])
+        Ys.append(s["Y"][ok])
+        Ss.append(s["scale"][ok])
+    return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ss), rate, sum(counts)
+
+
+def correction_error(ctx, X, Y, S):
+    """Relative L2 error of the predicted velocity correction (scale-weighted)."""
+    p = ctx.far_model_get()
+    ysd = p[-2:]
+    y = ctx.far_forward(X) * ysd
+    num = np.sqrt(np.sum((S[:, None] * (y - Y)) ** 2))
+    den = np.sqrt(np.sum((S[:, None] * Y) ** 2))
+    return float(num / den)
+
+
+def train(ctx, X, Y, steps, batch, lr0=2e-3, lr1=2e-5, seed=7, log_every=2000):
+    rng = np.random.default_rng(seed)
+    mu = X.mean(0)
+    sd = X.std(0) + 1e-6
+    ysd = Y.std(0) + 1e-12
+    ctx.far_model_set(init_params(rng, mu, sd, ysd))
+    ctx.far_train_data(X, Y, np.ones(len(X), np.float32))
+    k = np.arange(steps)
+    lr = (lr1 + 0.5 * (lr0 - lr1) * (1 + np.cos(np.pi * k / max(1, steps - 1)))).astype(np.float32)
+    losses = []
+    for a in range(0, steps, log_every):
+        b = min(steps, a + log_every)
+        ls = ctx.far_train(lr[a:b], batch=batch, seed=seed * 1000003 + a)
+        losses.append(float(np.mean(ls)))
+        print(f"  train steps {b}/{steps} batch loss {losses[-1]:.4e} lr {lr[b - 1]:.2e}", flush=True)
+    return losses
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--pairs", type=int, default=10_000_000)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--re", default="200,400,800,1400,2000")
+    ap.add_argument("--sim-steps", type=int, default=400)
+    ap.add_argument("--keep-every", type=int, default=100)
+    ap.add_argument("--fc", default="12,16,24")
+    a = ap.parse_args()
+    t0 = time.perf_counter()
+    ctx = Context(0)
+    re_list = [float(r) for r in a.re.split(",")]
+    snaps = snapshots(ctx, re_list, a.sim_steps, a.keep_every)
+    print(f"snapshots: {len(snaps)}, vortices {[len(v) for _, v in snaps]} ({time.perf_counter() - t0:.0f} s)",
+          flush=True)
+    fcs = [int(f) for f in a.fc.split(",")]
+    X, Y, S, rate, total = sample_pairs(ctx, snaps, fcs, a.pairs)
+    print(f"pairs: {len(X)} of {total} (rate {rate:.4f}) ({time.perf_counter() - t0:.0f} s)", flush=True)
+    perm = np.random.default_rng(0).permutation(len(X))
+    nval = max(1, len(X) // 20)
+    va, tr = perm[:nval], perm[nval:]
+    losses = train(ctx, X[tr], Y[tr], a.steps, a.batch)
+    err_tr = correction_error(ctx, X[tr[:200000]], Y[tr[:200000]], S[tr[:200000]])
+    err_va = correction_error(ctx, X[va], Y[va], S[va])
+    p = ctx.far_model_get()
+    np.save(a.out, p, allow_pickle=False)
+    info = dict(pairs=int(len(X)), total_pairs=int(total), rate=rate, steps=a.steps, batch=a.batch,
+                re=re_list, fc=fcs, sim_steps=a.sim_steps, keep_every=a.keep_every,
+                correction_rel_err_train=err_tr, correction_rel_err_heldout=err_va, final_loss=losses[-1],
+                seconds=time.perf_counter() - t0)
+    with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
+        json.dump(info, f, indent=1)
+    print(json.dumps(info), flush=True)
+
+
+if __name__ == "__main__":
+    main()
