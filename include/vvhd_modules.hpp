@@ -1,0 +1,249 @@
+/*
+ * vvhd_modules.hpp -- the libvvhd module classes over the device context.
+ *
+ * The classes keep the names and the public signatures of the reference's
+ * modules so that its time loop (utils/vvflow/vvflow.cpp:214-262) compiles
+ * against them unchanged:
+ *   stree / TSortedTree   libvvhd/headers/TSortedTree.hpp:60-92
+ *   MConvectiveFast       libvvhd/headers/MConvectiveFast.hpp:9-27
+ *   MEpsilonFast          libvvhd/headers/MEpsilonFast.hpp:7-31
+ *   MDiffusiveFast        libvvhd/headers/MDiffusiveFast.hpp:7-18
+ *   MFlowmove             libvvhd/headers/MFlowmove.hpp:5-19
+ *   Space (the members the hot path reads and writes), TObj, TAtt, TBody,
+ *   TVec                  libvvhd/headers/TSpace.hpp, TObj.hpp, TBody.hpp
+ *
+ * The vortex list and the bodies live on the device between calls; the host
+ * containers of Space are a mirror that is synchronised explicitly:
+ *   Space::push()  host -> device, after the caller edited VortexList or
+ *                  BodyList (stree::build pushes once by itself if nothing
+ *                  has been pushed yet);
+ *   Space::pull()  device -> host, before the caller reads them (saving,
+ *                  sensors, scripting).
+ * Errors of the device library are thrown as std::invalid_argument for
+ * calls out of order (the reference's own convention for an unbuilt tree)
+ * and std::runtime_error otherwise.  Bodies are rigid and static (the
+ * configurations of this hot path); heat and streak lists are not carried.
+ */
+#ifndef VVHD_MODULES_HPP
+#define VVHD_MODULES_HPP
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "vvhd_gpu.h"
+
+struct TVec {
+    double x = 0, y = 0;
+    TVec() = default;
+    TVec(double x_, double y_) : x(x_), y(y_) {}
+    TVec operator+(TVec o) const { return {x + o.x, y + o.y}; }
+    TVec operator-(TVec o) const { return {x - o.x, y - o.y}; }
+    TVec operator*(double c) const { return {x * c, y * c}; }
+    TVec& operator+=(TVec o) {
+        x += o.x;
+        y += o.y;
+        return *this;
+    }
+    double abs2() const { return x * x + y * y; }
+    double abs() const { return std::sqrt(abs2()); }
+};
+
+struct TVec3D {
+    TVec r;
+    double o = 0;
+};
+
+class TObj {
+   public:
+    TVec r;
+    double g = 0;
+    TVec v;
+    double _1_eps = 0;
+    TObj() = default;
+    TObj(double rx, double ry, double g_) : r(rx, ry), g(g_) {}
+    TObj(TVec r_, double g_) : r(r_), g(g_) {}
+};
+static_assert(sizeof(TObj) == 6 * sizeof(double), "TObj is the 48-byte record of the device ABI");
+
+class TAtt : public TObj {
+   public:
+    TVec corner;
+    uint32_t slip = 0;
+    TVec dl;
+    double gsum = 0, hsum = 0, fric = 0, Cp = 0, Fr = 0, Nu = 0;
+    uint32_t eq_no = 0;
+    TAtt(double x, double y, bool slip_ = false) : corner(x, y), slip(slip_) {}
+    TAtt(TVec c, bool slip_ = false) : TAtt(c.x, c.y, slip_) {}
+};
+
+class TBody {
+   public:
+    std::vector<TAtt> alist;
+    TVec3D holder, dpos, speed_slae, speed_slae_prev;
+    TVec3D force_hydro, force_holder, friction, friction_prev, fdt_dead;
+    double g_dead = 0, nusselt = 0;
+    int special_segment_no = 0;
+    bool kutta = false;
+    // geometry, computed on the device (TBody::doFillProperties)
+    double get_slen() const { return slen_; }
+    double get_area() const { return area_; }
+    TVec get_cofm() const { return cofm_; }
+    double get_moi_cofm() const { return moi_cofm_; }
+    TVec get_axis() const { return holder.r + dpos.r; }
+
+   private:
+    friend class Space;
+    double slen_ = 0, area_ = 0, moi_cofm_ = 0;
+    TVec cofm_;
+};
+
+class Space {
+   public:
+    explicit Space(int device = 0);
+    ~Space();
+    Space(const Space&) = delete;
+    Space& operator=(const Space&) = delete;
+
+    double re = std::numeric_limits<double>::infinity();
+    double pr = 0;
+    double inf_g = 0;
+    TVec inf_vel{0, 0};   // U_inf (the reference evaluates inf_vx/inf_vy expressions)
+    double time = 0;
+    double dt = 1;
+    double finish = 0;
+    std::vector<std::shared_ptr<TBody>> BodyList;
+    std::vector<TObj> VortexList;
+
+    TVec inf_speed() const { return inf_vel; }
+    void calc_forces();
+    void zero_forces();
+    void finish_step() { time = time + dt; }
+    double average_segment_length() const;
+    // sums over the vortex list on the device (TSpace.hpp:85-117)
+    double gsum();
+    double gmax();
+
+    // device synchronisation (no counterpart in the reference)
+    void push();
+    void pull();
+    vvhd_ctx* device() const { return ctx_; }
+    int64_t vortex_count() const;
+
+   private:
+    friend class stree;
+    friend class MConvectiveFast;
+    friend class MEpsilonFast;
+    friend class MDiffusiveFast;
+    friend class MFlowmove;
+    vvhd_ctx* ctx_ = nullptr;
+    bool pushed_ = false;
+    int32_t nseg_ = 0;
+    void ensure_pushed();
+    void set_params();
+    void pull_bodies();
+    void check(int rc, bool state_error = false) const;
+};
+
+// Host view of one leaf of the device tree (the fields the reference's
+// callers read from TSortedNode: centre, size, depth i, path code j and the
+// vortex range as indices into Space::VortexList after Space::pull()).
+class TSortedNode {
+   public:
+    double x = 0, y = 0, h = 0, w = 0;
+    int i = 0, j = 0;
+    int64_t v0 = 0, v1 = 0;
+};
+
+class stree {
+   public:
+    stree(Space* sS, int sFarCriteria, double sMinNodeSize,
+          double sMaxNodeSize = std::numeric_limits<double>::max());
+    stree() = delete;
+    stree(const stree&) = delete;
+    stree& operator=(const stree&) = delete;
+
+    void build(bool IncludeVortexes = true, bool IncludeBody = true, bool IncludeHeat = true);
+    void destroy();
+    const std::vector<TSortedNode*>& getBottomNodes() const;
+    const TSortedNode* findNode(TVec p) const;
+    void printBottomNodes(FILE* f, bool PrintDepth = false) const;
+
+    int farCriteria() const { return farCriteria_; }
+
+   private:
+    Space* S;
+    int farCriteria_;
+    double minNodeSize, maxNodeSize;
+    bool built_ = false;
+    mutable std::vector<TSortedNode> leaves_;
+    mutable std::vector<TSortedNode*> leaf_ptrs_;
+    mutable bool leaves_valid_ = false;
+    void load_leaves() const;
+};
+typedef stree TSortedTree;
+
+class MConvectiveFast {
+   public:
+    MConvectiveFast() = delete;
+    MConvectiveFast(Space* S, const TSortedTree* tree);
+    MConvectiveFast(const MConvectiveFast&) = delete;
+    MConvectiveFast& operator=(const MConvectiveFast&) = delete;
+    void process_all_lists();
+    TVec velocity(TVec p) const;
+    void calc_circulation(const void** collision);
+
+    // Learned far field (no counterpart in the reference): with a model
+    // set, process_all_lists uses it; clear_learned() restores the plain path.
+    void set_learned(const std::vector<float>& params);
+    void clear_learned() { learned_ = false; }
+    // Pair sums: Kahan-compensated FP32 (default) or plain FP32.
+    void set_kahan(bool k) { kahan_ = k; }
+
+   private:
+    Space* S;
+    const TSortedTree* tree;
+    bool learned_ = false, kahan_ = true;
+};
+
+class MEpsilonFast {
+   public:
+    MEpsilonFast(Space* S_, const TSortedTree* Tree_) : S(S_), Tree(Tree_), merged_(0) {}
+    void CalcEpsilonFast(bool merge);
+    int Merged() { return merged_; }
+
+   private:
+    Space* S;
+    const TSortedTree* Tree;
+    int merged_;
+};
+
+class MDiffusiveFast {
+   public:
+    MDiffusiveFast(Space* S_, const TSortedTree* tree_) : S(S_), tree(tree_) {}
+    void process_vort_list();
+    void process_heat_list() {}   // no heat list in this implementation
+
+   private:
+    Space* S;
+    const TSortedTree* tree;
+};
+
+class MFlowmove {
+   public:
+    MFlowmove(Space* S_, double remove_eps_ = 1E-10);
+    void move_and_clean(bool remove, const void** collision, size_t* cleaned_v = nullptr);
+    void vortex_shed();
+    void streak_shed() {}   // no streak lists in this implementation
+    void heat_shed() {}     // no heat list in this implementation
+    void heat_crop(double scale = 16) { (void)scale; }
+
+   private:
+    Space* S;
+};
+
+#endif

@@ -6,10 +6,22 @@
 // |r_i - r_j| * (1/eps_i) <= 8 (the reference's exp-argument cutoff), so near
 // leaves whose box lies farther than 8 eps from the whole target leaf are
 // skipped without loss.  Vortex sums (S1, S2) are FP32 with Kahan
-// compensation in leaf-local coordinates; body-segment terms (S0, S3 and the
-// segment friction) are FP64.  The clamp and the velocity update are FP64.
+// compensation in leaf-local coordinates; body-segment terms (S0, S3) are
+// FP64.  The clamp and the velocity update are FP64.
+//
+// Segment friction (segment_influence's fric term, :120-122) runs as its own
+// pass so that every segment adds its contributions in the reference's
+// serial order: target leaves in DFS order, their targets in list order.
+// The leaves that have a segment's leaf in their near list are collected
+// per segment leaf (a transposed near list restricted to leaves holding
+// segments) and sorted; one wave per segment evaluates 64 terms at a time
+// and lane 0 adds them in order.
 
 #include "common.hpp"
+
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -47,7 +59,7 @@ __device__ inline double slot_sum(double v) {
 __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ g,
                                                       const double* __restrict__ ie, const vvhd_seg* segs, double* vx,
-                                                      double* vy, double* fric, double re) {
+                                                      double* vy, double re) {
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
@@ -120,7 +132,6 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
                     s3x += dSx * expres;
                     s3y += dSy * expres;
                     s0 += (drabs * iei + 1) / drabs2 * (drx * dSx + dry * dSy) * expres;
-                    atomicAdd(&fric[sid], iei * iei * gi * expres * sqrt(dSx * dSx + dSy * dSy));
                 }
             }
         }
@@ -142,14 +153,119 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
     }
 }
 
+// Transposed near lists restricted to leaves that hold segments: for every
+// near entry (leaf l, near leaf M) with segments in M, record l under M.
+__global__ void k_fric_count(TreeDev T, int32_t* cnt, const int64_t* off, int32_t* fillc, int32_t* list) {
+    const int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (l >= T.nleaf) return;
+    for (int64_t e = T.near_off[l] + lane_id(); e < T.near_off[l + 1]; e += kWave) {
+        const int m = T.leaf_rank[T.near_list[e]];
+        if (T.seg_off[m + 1] == T.seg_off[m]) continue;
+        if (off)
+            list[off[m] + atomicAdd(&fillc[m], 1)] = l;
+        else
+            atomicAdd(&cnt[m], 1);
+    }
+}
+
+// Friction of one segment per wave, summed in (leaf, target) order.
+__global__ __launch_bounds__(kBlock) void k_friction(TreeDev T, const double* __restrict__ x,
+                                                     const double* __restrict__ y, const double* __restrict__ g,
+                                                     const double* __restrict__ ie, const vvhd_seg* segs,
+                                                     const int32_t* seg_leaf, const int64_t* off,
+                                                     const int32_t* list, int nseg, double* fric) {
+    __shared__ double buf[kWavesPerBlock][kWave];
+    const int w = threadIdx.x / kWave;
+    const int sid = blockIdx.x * kWavesPerBlock + w;
+    const int lane = lane_id();
+    if (sid >= nseg) return;
+    const vvhd_seg sg = segs[sid];
+    const int m = seg_leaf[sid];
+    const double dSx = -sg.dly, dSy = sg.dlx;
+    const double dlen = sqrt(dSx * dSx + dSy * dSy);
+    double acc = fric[sid];
+    for (int64_t e = off[m]; e < off[m + 1]; e++) {
+        const int L = T.leaf_node[list[e]];
+        const int v0 = T.v0[L], v1 = T.v1[L];
+        for (int b = v0; b < v1; b += kWave) {
+            const int i = b + lane;
+            double term = 0;
+            bool has = false;
+            if (i < v1) {
+                const double gi = g[i];
+                if (gi != 0) {
+                    const double iei = ie[i];
+                    const double drx = x[i] - sg.x, dry = y[i] - sg.y;
+                    const double drabs = sqrt(drx * drx + dry * dry);
+                    const double exparg = -drabs * iei;
+                    if (!(exparg < kExpArgLimit)) {
+                        term = iei * iei * gi * exp(exparg) * dlen;
+                        has = true;
+                    }
+                }
+            }
+            buf[w][lane] = term;
+            const uint64_t any = __ballot(has);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            if (lane == 0) {
+                const int cnt = min(kWave, v1 - b);
+                for (int q = 0; q < cnt; q++)
+                    if ((any >> q) & 1) acc += buf[w][q];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (lane == 0) fric[sid] = acc;
+}
+
 }  // namespace
 
 void diffusive(Ctx& c) {
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf == 0) return;
-    const int grid = (T.nleaf + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int nl = T.nleaf;
+    const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_diffusive, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p,
-                       c.vx.p, c.vy.p, c.fric.p, c.re);
+                       c.vx.p, c.vy.p, c.re);
+    const int nseg = (int)c.segs_h.size();
+    if (nseg) {
+        hipStream_t s = c.stream;
+        c.iscratch0.grow(nl + 1);
+        c.iscratch1.grow(nl + 1);
+        c.lscratch0.grow(nl + 1);
+        int32_t* cnt = c.iscratch0.p;
+        int32_t* fillc = c.iscratch1.p;
+        int64_t* off = c.lscratch0.p;
+        VVHD_HIP(hipMemsetAsync(cnt, 0, (size_t)(nl + 1) * 4, s));
+        VVHD_HIP(hipMemsetAsync(fillc, 0, (size_t)(nl + 1) * 4, s));
+        hipLaunchKernelGGL(k_fric_count, dim3(grid), dim3(kBlock), 0, s, T, cnt, (const int64_t*)nullptr, fillc,
+                           (int32_t*)nullptr);
+        size_t bytes = 0;
+        VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, cnt, off, (int64_t)0, (size_t)nl + 1,
+                                         rocprim::plus<int64_t>(), s));
+        c.tmp.grow(bytes + 16);
+        VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, cnt, off, (int64_t)0, (size_t)nl + 1,
+                                         rocprim::plus<int64_t>(), s));
+        int64_t total = 0;
+        VVHD_HIP(hipMemcpyAsync(&total, off + nl, 8, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+        c.iscratch2.grow(total + 1);
+        c.iscratch3.grow(total + 1);
+        hipLaunchKernelGGL(k_fric_count, dim3(grid), dim3(kBlock), 0, s, T, cnt, (const int64_t*)off, fillc,
+                           c.iscratch2.p);
+        if (total) {
+            bytes = 0;
+            VVHD_HIP(rocprim::segmented_radix_sort_keys(nullptr, bytes, c.iscratch2.p, c.iscratch3.p, (unsigned)total,
+                                                        (unsigned)nl, off, off + 1, 0, 32, s));
+            c.tmp.grow(bytes + 16);
+            VVHD_HIP(rocprim::segmented_radix_sort_keys(c.tmp.p, bytes, c.iscratch2.p, c.iscratch3.p,
+                                                        (unsigned)total, (unsigned)nl, off, off + 1, 0, 32, s));
+        }
+        hipLaunchKernelGGL(k_friction, dim3((nseg + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0, s, T,
+                           c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p, c.t.seg_leaf.p, off, c.iscratch3.p, nseg, c.fric.p);
+    }
     VVHD_HIP(hipGetLastError());
 }
 

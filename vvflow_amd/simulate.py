@@ -6,8 +6,9 @@ One step, in the reference's order:
     vortex_shed; calc_forces; (stepdata); zero_forces;
     tree build; epsilon (merge when viscous); convective; diffusive;
     tree destroy; move_and_clean(remove = True); time += dt.
-Tree parameters follow vvflow.cpp:200-203: far criterion 8, minimum node
-size 5 dl and maximum 100 dl with dl the body's average segment length.
+Tree parameters follow vvflow.cpp:200-203: one tree object (far criterion
+8 by default) for both builds, minimum node size 5 dl and maximum 100 dl with
+dl the body's average segment length.
 
 The convective step can use the learned far field instead of the plain
 tree path (``learned=True``; a model must be loaded into the context).
@@ -46,7 +47,7 @@ class CylinderRun:
     def step(self):
         c = self.ctx
         c.set_params(re=self.re, dt=self.dt, time=self.time)
-        c.tree_build(8, self.min_node, self.max_node)
+        c.tree_build(self.fc, self.min_node, self.max_node)
         c.calc_circulation(self.inf)
         c.tree_destroy()
         c.vortex_shed()
