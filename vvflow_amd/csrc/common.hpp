@@ -144,6 +144,8 @@ struct TreeArrays {
     DBuf<int32_t> vleaf;                  // per vortex: leaf (DFS)
     DBuf<int32_t> node_L, node_m, leaf_tmp, seg_node;   // build scratch
     DBuf<int32_t> cnt_near, cnt_far, err;               // list scratch
+    DBuf<int64_t> slot_near, slot_far;                  // per (leaf, top entry) list offsets
+    DBuf<uint8_t> entries;                              // top-of-tree entry table of the list builder
     int32_t nnodes = 0, nleaves_h = 0;
     int64_t nnear = 0, nfar = 0;
     int max_depth = 0;

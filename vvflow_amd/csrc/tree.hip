@@ -410,68 +410,130 @@ __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves
     }
 }
 
-// FindNearNodes (TSortedTree.cpp:199-217) for the 64 DFS-consecutive leaves
-// of one wave, traversing the tree once with a shared (uniform) stack: a
-// node is expanded while any lane still needs it; a lane that classifies a
-// node as far ignores the rest of that subtree.  Children are pushed ch2
-// then ch1, so each lane sees its nodes in the reference's DFS order.  With
-// the list arrays NULL only counts.
-__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const int32_t* leaf_node, int nleaf, double fc,
-                                                  const int64_t* near_off, const int64_t* far_off,
-                                                  int32_t* near_list, int32_t* far_list, int32_t* near_cnt,
-                                                  int32_t* far_cnt, int32_t* err) {
-    __shared__ int s_stack[kBlock / kWave][kMaxStack];
-    const int w = threadIdx.x / kWave;
-    const int lane = lane_id();
-    const int l = (blockIdx.x * (kBlock / kWave) + w) * kWave + lane;
-    const bool valid = l < nleaf;
-    if (__ballot(valid) == 0) return;
-    int* stack = s_stack[w];
-    const int L = valid ? leaf_node[l] : 0;
-    const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];
-    int64_t pn = (valid && near_off) ? near_off[l] : 0, pf = (valid && far_off) ? far_off[l] : 0;
-    int cn = 0, cf = 0;
-    int skip_depth = INT32_MAX;
-    if (lane == 0) stack[0] = 0;
-    int sp = 1;
-    __builtin_amdgcn_wave_barrier();
+// FindNearNodes (TSortedTree.cpp:199-217) split over the top of the tree:
+// the "entries" are the nodes at depth kSplitDepth plus the shallower
+// leaves, in DFS order (at most 2^kSplitDepth).  One lane handles one
+// (leaf, entry) pair: it tests the entry's ancestors, then walks the entry's
+// subtree depth-first without a stack (first child; else the right sibling
+// of the nearest ancestor that is a left child, from the parent links), so
+// many more independent traversals are in flight than there are leaves.
+// Emissions follow the reference's preorder: a far ancestor is emitted by
+// the first entry beneath it only, and the per-(leaf, entry) segments are
+// concatenated in entry order.  Lanes of one wave share the entry and take
+// 64 DFS-consecutive leaves, so their traversals are alike.
+constexpr int kSplitDepth = 3;
+constexpr int kMaxEntries = 1 << kSplitDepth;
+
+struct ListEntries {
+    int n;                                 // number of entries
+    int node[kMaxEntries];                 // entry node
+    int depth[kMaxEntries];                // its depth (<= kSplitDepth)
+    int anc[kMaxEntries][kSplitDepth];     // ancestors, root first
+    int lead[kMaxEntries];                 // bit d: first entry beneath anc[d]
+};
+
+__global__ void k_list_entries(NodeView nv, ListEntries* E) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int stack[2 * kMaxEntries + 2];
+    int sp = 0, n = 0;
+    stack[sp++] = 0;
     while (sp > 0) {
-        const int T = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
-        sp--;
-        const int dT = nv.depth[T];
-        if (skip_depth != INT32_MAX && dT <= skip_depth) skip_depth = INT32_MAX;
-        const bool act = valid && skip_depth == INT32_MAX;
-        const double dx = nv.x[T] - lx, dy = nv.y[T] - ly;
-        const double hp = nv.h[T] + nv.w[T] + lh + lw;
-        const bool far = act && (dx * dx + dy * dy > fc * hp * hp);
-        if (far) {
-            if (far_list) far_list[pf++] = T;
-            cf++;
-            skip_depth = dT;
-        }
-        const int c = nv.ch1[T];
-        const bool need = act && !far;
-        if (c >= 0) {
-            if (__ballot(need) != 0) {
-                if (sp + 2 > kMaxStack) {
-                    if (lane == 0) *err = 1;
-                    break;
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) {
-                    stack[sp] = c + 1;
-                    stack[sp + 1] = c;
-                }
-                __builtin_amdgcn_wave_barrier();
-                sp += 2;
+        const int T = stack[--sp];
+        const int d = nv.depth[T];
+        if (d == kSplitDepth || nv.ch1[T] < 0) {
+            E->node[n] = T;
+            E->depth[n] = d;
+            int a = T, lead = 0;
+            for (int k = d - 1; k >= 0; k--) {
+                const int p = nv.parent[a];
+                E->anc[n][k] = p;
+                // T is the first entry beneath p when every step from p down is to a first child
+                a = p;
             }
-        } else if (need) {
-            if (near_list) near_list[pn++] = T;
-            cn++;
+            int c = T;
+            bool first = true;
+            for (int k = d - 1; k >= 0; k--) {
+                const int p = nv.parent[c];
+                first = first && (nv.ch1[p] == c);
+                if (first) lead |= 1 << k;
+                c = p;
+            }
+            E->lead[n] = lead;
+            n++;
+        } else {
+            stack[sp++] = nv.ch1[T] + 1;
+            stack[sp++] = nv.ch1[T];
         }
     }
-    if (valid && near_cnt) near_cnt[l] = cn;
-    if (valid && far_cnt) far_cnt[l] = cf;
+    E->n = n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries* __restrict__ E,
+                                                  const int32_t* leaf_node, int nleaf, double fc,
+                                                  const int64_t* near_off, const int64_t* far_off,
+                                                  int32_t* near_list, int32_t* far_list, int32_t* near_cnt,
+                                                  int32_t* far_cnt) {
+    const int wave = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int q = wave % kMaxEntries;          // entry of this wave
+    const int l = (wave / kMaxEntries) * kWave + lane_id();
+    if (l >= nleaf || q >= E->n) return;
+    const int L = leaf_node[l];
+    const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];
+    const size_t slot = (size_t)l * kMaxEntries + q;
+    int64_t pn = near_off ? near_off[slot] : 0, pf = far_off ? far_off[slot] : 0;
+    int cn = 0, cf = 0;
+    auto is_far = [&](int T) {
+        const double dx = nv.x[T] - lx, dy = nv.y[T] - ly;
+        const double hp = nv.h[T] + nv.w[T] + lh + lw;
+        return dx * dx + dy * dy > fc * hp * hp;
+    };
+    bool cut = false;
+    const int dE = E->depth[q];
+    for (int k = 0; k < dE; k++) {
+        const int A = E->anc[q][k];
+        if (is_far(A)) {
+            if ((E->lead[q] >> k) & 1) {
+                if (far_list) far_list[pf] = A;
+                pf++;
+                cf++;
+            }
+            cut = true;
+            break;
+        }
+    }
+    if (!cut) {
+        const int root = E->node[q];
+        int T = root;
+        while (true) {
+            bool descend = false;
+            if (is_far(T)) {
+                if (far_list) far_list[pf] = T;
+                pf++;
+                cf++;
+            } else if (nv.ch1[T] < 0) {
+                if (near_list) near_list[pn] = T;
+                pn++;
+                cn++;
+            } else {
+                descend = true;
+            }
+            if (descend) {
+                T = nv.ch1[T];
+                continue;
+            }
+            while (T != root && T == nv.ch1[nv.parent[T]] + 1) T = nv.parent[T];
+            if (T == root) break;
+            T = T + 1;
+        }
+    }
+    if (near_cnt) near_cnt[slot] = cn;
+    if (far_cnt) far_cnt[slot] = cf;
+}
+
+// Per-leaf CSR offsets from the per-(leaf, entry) offsets.
+__global__ void k_leaf_offsets(const int64_t* slot_off, int nleaf, int64_t* leaf_off) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l <= nleaf) leaf_off[l] = slot_off[(size_t)l * kMaxEntries];
 }
 
 // Segments: leaf of each segment, then stable order by (leaf, index).
@@ -664,37 +726,41 @@ void tree_lists(Ctx& c) {
     const int nl = t.nleaves_h;
     NodeView nv = view(t);
     hipStream_t s = c.stream;
+    const size_t nslot = (size_t)nl * kMaxEntries;
     DBuf<int32_t>& cnt_n = t.cnt_near;
     DBuf<int32_t>& cnt_f = t.cnt_far;
-    DBuf<int32_t>& err = t.err;
-    cnt_n.grow(nl + 1);
-    cnt_f.grow(nl + 1);
-    err.grow(1);
+    cnt_n.grow(nslot + 1);
+    cnt_f.grow(nslot + 1);
+    t.slot_near.grow(nslot + 1);
+    t.slot_far.grow(nslot + 1);
     t.near_off.grow(nl + 1);
     t.far_off.grow(nl + 1);
-    VVHD_HIP(hipMemsetAsync(err.p, 0, 4, s));
-    VVHD_HIP(hipMemsetAsync(cnt_n.p + nl, 0, 4, s));
-    VVHD_HIP(hipMemsetAsync(cnt_f.p + nl, 0, 4, s));
+    t.entries.grow(sizeof(ListEntries));
+    ListEntries* E = reinterpret_cast<ListEntries*>(t.entries.p);
+    hipLaunchKernelGGL(k_list_entries, dim3(1), dim3(1), 0, s, nv, E);
+    VVHD_HIP(hipMemsetAsync(cnt_n.p, 0, (nslot + 1) * 4, s));
+    VVHD_HIP(hipMemsetAsync(cnt_f.p, 0, (nslot + 1) * 4, s));
     const double fc = (double)t.far_criteria;
-    const int grid = nblocks(nl, kBlock);   // one wave per 64 leaves
-    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, t.leaf_node.p, nl, fc, (const int64_t*)nullptr,
-                       (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, cnt_n.p, cnt_f.p, err.p);
-    scan_i32_to_i64(c, cnt_n.p, t.near_off.p, nl);
-    scan_i32_to_i64(c, cnt_f.p, t.far_off.p, nl);
+    const int waves = ((nl + kWave - 1) / kWave) * kMaxEntries;
+    const int grid = (waves + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E, t.leaf_node.p, nl, fc,
+                       (const int64_t*)nullptr, (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                       cnt_n.p, cnt_f.p);
+    scan_i32_to_i64(c, cnt_n.p, t.slot_near.p, (int)nslot);
+    scan_i32_to_i64(c, cnt_f.p, t.slot_far.p, (int)nslot);
+    hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_near.p, nl, t.near_off.p);
+    hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_far.p, nl, t.far_off.p);
     int64_t tot[2];
-    int32_t herr = 0;
-    VVHD_HIP(hipMemcpyAsync(&tot[0], t.near_off.p + nl, 8, hipMemcpyDeviceToHost, s));
-    VVHD_HIP(hipMemcpyAsync(&tot[1], t.far_off.p + nl, 8, hipMemcpyDeviceToHost, s));
-    VVHD_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(&tot[0], t.slot_near.p + nslot, 8, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(&tot[1], t.slot_far.p + nslot, 8, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
-    VVHD_CHECK(herr == 0, VVHD_EUNSUPPORTED, "tree lists: traversal stack exceeded");
     t.nnear = tot[0];
     t.nfar = tot[1];
     t.near_list.grow(t.nnear + 1);
     t.far_list.grow(t.nfar + 1);
-    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, t.leaf_node.p, nl, fc,
-                       (const int64_t*)t.near_off.p, (const int64_t*)t.far_off.p, t.near_list.p, t.far_list.p,
-                       (int32_t*)nullptr, (int32_t*)nullptr, err.p);
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E, t.leaf_node.p, nl, fc,
+                       (const int64_t*)t.slot_near.p, (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p,
+                       (int32_t*)nullptr, (int32_t*)nullptr);
 }
 
 }  // namespace vvhd
