@@ -345,20 +345,118 @@ __global__ void k_pack_model(const float* P, uint16_t* frag) {
 
 // -------------------------------------------------------------- inference
 
+// Per-node record of the inference path, built once per step from the
+// moments: everything in the features that does not depend on the target,
+// in FP32 (the features are rounded to bf16 afterwards), plus the monopoles
+// relative to the node centre.
+struct FarNode {
+    double cx, cy;
+    float f0, f1;           // G+/S, G-/S
+    float dpx, dpy;         // G+ (z+ - c) / (S d), unrotated
+    float dmx, dmy;         // G- (z- - c) / (S d)
+    float qqx, qqy;         // (Q+ + Q-) / (S d^2)
+    float e8, e9;           // E+ / (S d^2), E- / (S d^2)
+    float sd, asp;          // d, (h - w) / (h + w)
+    float scl, ok;          // S d^2 / (2 pi); 1 when the features are defined
+    float gpk, gmk;         // G+ / (2 pi), G- / (2 pi)
+    float zpx, zpy, zmx, zmy;   // z+ - c, z- - c
+};
+static_assert(sizeof(FarNode) == 96, "FarNode layout");
+
+__global__ void k_far_nodes(TreeDev T, MomView M, int nnodes, FarNode* out) {
+#pragma clang fp contract(off)
+    const int F = blockIdx.x * blockDim.x + threadIdx.x;
+    if (F >= nnodes) return;
+    FarNode r;
+    const double cx = T.nx[F], cy = T.ny[F], h = T.nh[F], w = T.nw[F];
+    const double gp = M(0, F), gm = M(1, F);
+    const double S = fabs(gp) + fabs(gm);
+    const double ep = M(10, F), em = M(11, F);
+    const double d2 = S > 0 ? h * h + w * w + (ep - em) / S : 0.;
+    const bool ok = S > 0 && d2 > 0 && isfinite(d2);
+    const double sd = ok ? sqrt(d2) : 1., inv1 = ok ? 1. / (S * sd) : 0., inv2 = ok ? 1. / (S * d2) : 0.;
+    r.cx = cx;
+    r.cy = cy;
+    r.f0 = ok ? (float)(gp / S) : 0.f;
+    r.f1 = ok ? (float)(gm / S) : 0.f;
+    r.dpx = (float)(gp * (M(2, F) - cx) * inv1);
+    r.dpy = (float)(gp * (M(3, F) - cy) * inv1);
+    r.dmx = (float)(gm * (M(4, F) - cx) * inv1);
+    r.dmy = (float)(gm * (M(5, F) - cy) * inv1);
+    r.qqx = (float)((M(6, F) + M(8, F)) * inv2);
+    r.qqy = (float)((M(7, F) + M(9, F)) * inv2);
+    r.e8 = (float)(ep * inv2);
+    r.e9 = (float)(em * inv2);
+    r.sd = (float)sd;
+    r.asp = (float)((h - w) / fmax(h + w, 1e-300));
+    r.scl = ok ? (float)(S * d2 / (2 * kPi)) : 0.f;
+    r.ok = ok ? 1.f : 0.f;
+    r.gpk = (float)(gp * k1_2Pi);
+    r.gmk = (float)(gm * k1_2Pi);
+    r.zpx = (float)(M(2, F) - cx);
+    r.zpy = (float)(M(3, F) - cy);
+    r.zmx = (float)(M(4, F) - cx);
+    r.zmy = (float)(M(5, F) - cy);
+    out[F] = r;
+}
+
+// Inference forward of one tile; biases of the hidden layers come from LDS
+// (sb1, sb2: 64 floats each), the output biases from scalar arguments.
+// Returns the two outputs in lanes 0..31.
+__device__ inline void mlp_infer(const Weights& W, const float* sb1, const float* sb2, float b30, float b31,
+                                 bf16x8 x, float& y0, float& y1) {
+    const int g = lane_id() >> 5;
+    bf16x8 h[4];
+    for (int b = 0; b < 2; b++) {
+        f32x16 c;
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) c[4 * j + i] = sb1[32 * b + 8 * j + 4 * g + i];
+        c = mfma32(W.a1[b], x, c);
+        for (int e = 0; e < 16; e++) h[2 * b + (e >> 3)][e & 7] = (__bf16)silu_f(c[e]);
+    }
+    bf16x8 h2[4];
+    for (int b = 0; b < 2; b++) {
+        f32x16 c;
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) c[4 * j + i] = sb2[32 * b + 8 * j + 4 * g + i];
+        for (int st = 0; st < 4; st++) c = mfma32(W.a2[4 * b + st], h[st], c);
+        for (int e = 0; e < 16; e++) h2[2 * b + (e >> 3)][e & 7] = (__bf16)silu_f(c[e]);
+    }
+    f32x16 c = {};
+    c[0] = g == 0 ? b30 : 0.f;
+    c[1] = g == 0 ? b31 : 0.f;
+    for (int st = 0; st < 4; st++) c = mfma32(W.a3[st], h2[st], c);
+    y0 = c[0];
+    y1 = c[1];
+}
+
 // Far field of every vortex by the learned path, one wave per leaf; targets
 // in chunks of up to 16, padded to a power of two, so every lane keeps one
 // target for the whole far list and the per-target sums need no atomics.
-__global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, MomView M, const double* __restrict__ x,
-                                                        const double* __restrict__ y, const uint16_t* frag,
-                                                        const float* P, double2* far_out) {
-#pragma clang fp contract(off)
+// Lane group g (lanes 32g..32g+31) builds inputs 8g..8g+7 of pair l%32.
+__global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, const FarNode* __restrict__ nodes,
+                                                        const double* __restrict__ x, const double* __restrict__ y,
+                                                        const uint16_t* frag, const float* P, double2* far_out) {
+    __shared__ float sb1[kHid], sb2[kHid];
+    for (int k = threadIdx.x; k < kHid; k += blockDim.x) {
+        sb1[k] = P[oB1 + k];
+        sb2[k] = P[oB2 + k];
+    }
+    __syncthreads();
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
     const int lane = lane_id(), g = lane >> 5, n = lane & 31;
     Weights W;
     load_weights(frag, W);
+    const float b30 = P[oB3], b31 = P[oB3 + 1];
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
+    float mu[8], isd[8];
+    for (int e = 0; e < 8; e++) {
+        const int k = 8 * g + e;
+        mu[e] = k < kFeat ? P[oMu + k] : 0.f;
+        isd[e] = k < kFeat ? 1.f / P[oSd + k] : 0.f;
+    }
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
@@ -372,28 +470,53 @@ __global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, MomView M, co
         float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;   // Kahan sums (lanes 0..31)
         for (int64_t f0 = fb; f0 < fe; f0 += nper) {
             const int64_t e = f0 + fo;
-            const bool fvalid = e < fe;
-            const int F = fvalid ? T.far_list[e] : 0;
-            PairFeat pf;
-            bool ok = false;
-            if (tvalid && fvalid) ok = far_features(M, F, T.nx[F], T.ny[F], T.nh[F], T.nw[F], tx, ty, pf);
-            const bf16x8 xin = input_frag(pf.f, ok, P);
-            Fwd s;
-            mlp_forward(W, P, xin, s);
-            if (g == 0 && tvalid && fvalid) {
+            const bool valid = tvalid && e < fe;
+            const FarNode& nd = nodes[valid ? T.far_list[e] : 0];
+            const float dx = (float)(tx - nd.cx), dy = (float)(ty - nd.cy);
+            const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
+            const float ux = dx * rinv, uy = dy * rinv;
+            const bool ok = valid && nd.ok != 0.f;
+            float f[8];
+            if (g == 0) {
+                const float c2r = ux * ux - uy * uy, c2i = -2.f * ux * uy;
+                f[0] = nd.f0;
+                f[1] = nd.f1;
+                f[2] = nd.dpx * ux + nd.dpy * uy;
+                f[3] = nd.dpy * ux - nd.dpx * uy;
+                f[4] = nd.dmx * ux + nd.dmy * uy;
+                f[5] = nd.dmy * ux - nd.dmx * uy;
+                f[6] = nd.qqx * c2r - nd.qqy * c2i;
+                f[7] = nd.qqx * c2i + nd.qqy * c2r;
+            } else {
+                f[0] = nd.e8;
+                f[1] = nd.e9;
+                f[2] = nd.sd * rinv;
+                f[3] = nd.asp;
+                f[4] = f[5] = f[6] = f[7] = 0.f;
+            }
+            bf16x8 xin;
+            for (int q = 0; q < 8; q++) xin[q] = (__bf16)(ok ? (f[q] - mu[q]) * isd[q] : 0.f);
+            float y0, y1;
+            mlp_infer(W, sb1, sb2, b30, b31, xin, y0, y1);
+            if (g == 0 && valid) {
                 float u = 0.f, v = 0.f;
-                for (int sg = 0; sg < 2; sg++) {
-                    const double G = M(sg, F);
-                    if (G == 0) continue;
-                    const float ddx = (float)(tx - M(2 + 2 * sg, F)), ddy = (float)(ty - M(3 + 2 * sg, F));
-                    const float fac = (float)(G * k1_2Pi) / (ddx * ddx + ddy * ddy);
-                    u += -ddy * fac;
-                    v += ddx * fac;
+                if (nd.gpk != 0.f) {
+                    const float ex = dx - nd.zpx, ey = dy - nd.zpy;
+                    const float fac = nd.gpk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
+                    u -= ey * fac;
+                    v += ex * fac;
+                }
+                if (nd.gmk != 0.f) {
+                    const float ex = dx - nd.zmx, ey = dy - nd.zmy;
+                    const float fac = nd.gmk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
+                    u -= ey * fac;
+                    v += ex * fac;
                 }
                 if (ok) {
-                    const double a = (double)s.c3[0] * ysd0, b = (double)s.c3[1] * ysd1;
-                    u += (float)((a * pf.ux - b * pf.uy) * pf.scale);
-                    v += (float)((a * pf.uy + b * pf.ux) * pf.scale);
+                    const float sc = nd.scl * rinv * rinv * rinv;
+                    const float a = y0 * ysd0, b = y1 * ysd1;
+                    u += (a * ux - b * uy) * sc;
+                    v += (a * uy + b * ux) * sc;
                 }
                 float yk = u - cu, tk = su + yk;
                 cu = (tk - su) - yk;
@@ -746,6 +869,7 @@ void farfield_release(Ctx& c) {
     c.fm.release();
     c.fm_stride = 0;
     c.far_v.release();
+    c.far_nodes.release();
     c.mlp_params.release();
     c.mlp_frags.release();
     c.mlp_loaded = false;
@@ -965,10 +1089,13 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
     if (T.nleaf > 0) {
         MomView M = moments(c);
         c.far_v.grow(c.n + 1);
+        c.far_nodes.grow((size_t)c.t.nnodes * sizeof(FarNode));
+        FarNode* nodes = reinterpret_cast<FarNode*>(c.far_nodes.p);
+        hipLaunchKernelGGL(k_far_nodes, dim3(nblocks(c.t.nnodes)), dim3(kBlock), 0, c.stream, T, M, c.t.nnodes, nodes);
         VVHD_HIP(hipEventRecord(c.ev[14], c.stream));
         const int grid = (T.nleaf + kWavesPerBlock - 1) / kWavesPerBlock;
-        hipLaunchKernelGGL(k_far_learned, dim3(grid), dim3(kBlock), 0, c.stream, T, M, c.x.p, c.y.p, c.mlp_frags.p,
-                           c.mlp_params.p, c.far_v.p);
+        hipLaunchKernelGGL(k_far_learned, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.x.p,
+                           c.y.p, c.mlp_frags.p, c.mlp_params.p, c.far_v.p);
         VVHD_HIP(hipEventRecord(c.ev[15], c.stream));
         c.phase_recorded[7] = true;
         hipLaunchKernelGGL(k_finish_learned, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, c.g.p, c.near_v.p,

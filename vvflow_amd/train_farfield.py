@@ -2,15 +2,21 @@
 
     python -m vvflow_amd.train_farfield [--out PATH] [--pairs 10000000]
         [--steps 20000] [--batch 8192] [--re 200,400,800,1400,2000]
-        [--sim-steps 400] [--keep-every 100] [--fc 12,16,24]
+        [--sim-steps 400] [--keep-every 20] [--fc 12,16,24]
+        [--wake 250000:0x5EED0011,500000:0x5EED0012,1000000:0x5EED0013]
+        [--wake-pairs 10000000]
 
 1. Snapshots: impulsively started cylinders (R = 0.5, 500 segments,
    dt = 0.005) run on the device at each Reynolds number; the vortex list is
-   kept every `keep_every` steps.
+   kept every `keep_every` steps.  These wakes stay small (a few thousand
+   vortices after 400 steps), so dense Karman-street clouds from the wake
+   generator are added (sizes and seeds of --wake; the benchmark's seed is
+   not among them).
 2. Pairs: every snapshot is rebuilt as a free vortex cloud with the far
    criteria listed (no body, unbounded node sizes, as for the wake
    benchmark), the epsilon pass runs, and (vortex, far node) pairs are
-   sampled with their FP64 direct-sum residual labels (vvhd_far_sample).
+   sampled with their FP64 direct-sum residual labels (vvhd_far_sample),
+   about --pairs from the cylinders and --wake-pairs from the wakes.
 3. Training: feature means / deviations and output scales from the pairs,
    then Adam on batches drawn on the device, cosine learning-rate decay, with
    the device's bf16 MFMA forward and backward kernels.  A held-out 5 % of
@@ -24,7 +30,7 @@ import time
 
 import numpy as np
 
-from . import FAR_PARAMS, Context
+from . import FAR_PARAMS, Context, gen_wake
 from .simulate import CylinderRun
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -126,8 +132,10 @@ def main():
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--re", default="200,400,800,1400,2000")
     ap.add_argument("--sim-steps", type=int, default=400)
-    ap.add_argument("--keep-every", type=int, default=100)
+    ap.add_argument("--keep-every", type=int, default=20)
     ap.add_argument("--fc", default="12,16,24")
+    ap.add_argument("--wake", default="250000:0x5EED0011,500000:0x5EED0012,1000000:0x5EED0013")
+    ap.add_argument("--wake-pairs", type=int, default=10_000_000)
     a = ap.parse_args()
     t0 = time.perf_counter()
     ctx = Context(0)
@@ -137,7 +145,18 @@ def main():
           flush=True)
     fcs = [int(f) for f in a.fc.split(",")]
     X, Y, S, rate, total = sample_pairs(ctx, snaps, fcs, a.pairs)
-    print(f"pairs: {len(X)} of {total} (rate {rate:.4f}) ({time.perf_counter() - t0:.0f} s)", flush=True)
+    print(f"cylinder pairs: {len(X)} of {total} (rate {rate:.4f}) ({time.perf_counter() - t0:.0f} s)", flush=True)
+    wakes = []
+    for item in filter(None, a.wake.split(",")):
+        n, seed = item.split(":")
+        wakes.append((600.0, gen_wake(int(n), int(seed, 0))))
+    if wakes:
+        Xw, Yw, Sw, rate_w, total_w = sample_pairs(ctx, wakes * len(fcs), [f for f in fcs for _ in wakes],
+                                                   a.wake_pairs, seed=1001)
+        print(f"wake pairs: {len(Xw)} of {total_w} (rate {rate_w:.5f}) ({time.perf_counter() - t0:.0f} s)",
+              flush=True)
+        X, Y, S = np.concatenate([X, Xw]), np.concatenate([Y, Yw]), np.concatenate([S, Sw])
+        total += total_w
     perm = np.random.default_rng(0).permutation(len(X))
     nval = max(1, len(X) // 20)
     va, tr = perm[:nval], perm[nval:]
@@ -147,7 +166,7 @@ def main():
     p = ctx.far_model_get()
     np.save(a.out, p, allow_pickle=False)
     info = dict(pairs=int(len(X)), total_pairs=int(total), rate=rate, steps=a.steps, batch=a.batch,
-                re=re_list, fc=fcs, sim_steps=a.sim_steps, keep_every=a.keep_every,
+                re=re_list, fc=fcs, sim_steps=a.sim_steps, keep_every=a.keep_every, wake=a.wake,
                 correction_rel_err_train=err_tr, correction_rel_err_heldout=err_va, final_loss=losses[-1],
                 seconds=time.perf_counter() - t0)
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
