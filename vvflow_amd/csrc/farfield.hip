@@ -434,13 +434,18 @@ __device__ inline void mlp_infer(const Weights& W, const float* sb1, const float
 // in chunks of up to 16, padded to a power of two, so every lane keeps one
 // target for the whole far list and the per-target sums need no atomics.
 // Lane group g (lanes 32g..32g+31) builds inputs 8g..8g+7 of pair l%32.
-__global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, const FarNode* __restrict__ nodes,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_far_learned(
+    TreeDev T, const FarNode* __restrict__ nodes,
                                                         const double* __restrict__ x, const double* __restrict__ y,
                                                         const uint16_t* frag, const float* P, double2* far_out) {
-    __shared__ float sb1[kHid], sb2[kHid];
+    __shared__ float sb1[kHid], sb2[kHid], smu[16], sisd[16];
     for (int k = threadIdx.x; k < kHid; k += blockDim.x) {
         sb1[k] = P[oB1 + k];
         sb2[k] = P[oB2 + k];
+        if (k < 16) {
+            smu[k] = k < kFeat ? P[oMu + k] : 0.f;
+            sisd[k] = k < kFeat ? 1.f / P[oSd + k] : 0.f;
+        }
     }
     __syncthreads();
     const int w = threadIdx.x / kWave;
@@ -451,12 +456,6 @@ __global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, const FarNode
     load_weights(frag, W);
     const float b30 = P[oB3], b31 = P[oB3 + 1];
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
-    float mu[8], isd[8];
-    for (int e = 0; e < 8; e++) {
-        const int k = 8 * g + e;
-        mu[e] = k < kFeat ? P[oMu + k] : 0.f;
-        isd[e] = k < kFeat ? 1.f / P[oSd + k] : 0.f;
-    }
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
@@ -468,10 +467,13 @@ __global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, const FarNode
         const bool tvalid = k < ntc;
         const double tx = tvalid ? x[base + k] : 0., ty = tvalid ? y[base + k] : 0.;
         float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;   // Kahan sums (lanes 0..31)
+        // the node record of the next tile is loaded while this tile computes
+        FarNode nd = nodes[(tvalid && fb + fo < fe) ? T.far_list[fb + fo] : 0];
         for (int64_t f0 = fb; f0 < fe; f0 += nper) {
             const int64_t e = f0 + fo;
             const bool valid = tvalid && e < fe;
-            const FarNode& nd = nodes[valid ? T.far_list[e] : 0];
+            const int64_t en = e + nper;
+            const FarNode nxt = nodes[(tvalid && en < fe) ? T.far_list[en] : 0];
             const float dx = (float)(tx - nd.cx), dy = (float)(ty - nd.cy);
             const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
             const float ux = dx * rinv, uy = dy * rinv;
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, const FarNode
                 f[4] = f[5] = f[6] = f[7] = 0.f;
             }
             bf16x8 xin;
-            for (int q = 0; q < 8; q++) xin[q] = (__bf16)(ok ? (f[q] - mu[q]) * isd[q] : 0.f);
+            for (int q = 0; q < 8; q++) xin[q] = (__bf16)(ok ? (f[q] - smu[8 * g + q]) * sisd[8 * g + q] : 0.f);
             float y0, y1;
             mlp_infer(W, sb1, sb2, b30, b31, xin, y0, y1);
             if (g == 0 && valid) {
@@ -526,6 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_far_learned(TreeDev T, const FarNode
                 cv = (tk - sv) - yk;
                 sv = tk;
             }
+            nd = nxt;
         }
         double du = (double)su - (double)cu, dv = (double)sv - (double)cv;
         for (int o = npad; o < 32; o <<= 1) {

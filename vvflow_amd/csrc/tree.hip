@@ -412,15 +412,16 @@ __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves
 
 // FindNearNodes (TSortedTree.cpp:199-217) split over the top of the tree:
 // the "entries" are the nodes at depth kSplitDepth plus the shallower
-// leaves, in DFS order (at most 2^kSplitDepth).  One lane handles one
-// (leaf, entry) pair: it tests the entry's ancestors, then walks the entry's
-// subtree depth-first without a stack (first child; else the right sibling
-// of the nearest ancestor that is a left child, from the parent links), so
-// many more independent traversals are in flight than there are leaves.
-// Emissions follow the reference's preorder: a far ancestor is emitted by
-// the first entry beneath it only, and the per-(leaf, entry) segments are
-// concatenated in entry order.  Lanes of one wave share the entry and take
-// 64 DFS-consecutive leaves, so their traversals are alike.
+// leaves, in DFS order (at most 2^kSplitDepth).  One wave handles one entry
+// for 64 DFS-consecutive leaves: each lane first tests the entry's
+// ancestors for its leaf, then the wave walks the entry's subtree once with
+// a shared stack (a node is expanded while any lane still needs it; a lane
+// that classifies a node as far ignores the rest of that subtree).  The
+// split gives 2^kSplitDepth times more independent waves than leaf groups,
+// which hides the latency of the dependent node loads.  Emissions follow the
+// reference's preorder: a far ancestor is emitted by the first entry beneath
+// it only, and the per-(leaf, entry) segments are concatenated in entry
+// order.
 constexpr int kSplitDepth = 3;
 constexpr int kMaxEntries = 1 << kSplitDepth;
 
@@ -472,62 +473,78 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries
                                                   const int32_t* leaf_node, int nleaf, double fc,
                                                   const int64_t* near_off, const int64_t* far_off,
                                                   int32_t* near_list, int32_t* far_list, int32_t* near_cnt,
-                                                  int32_t* far_cnt) {
-    const int wave = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    const int q = wave % kMaxEntries;          // entry of this wave
-    const int l = (wave / kMaxEntries) * kWave + lane_id();
-    if (l >= nleaf || q >= E->n) return;
-    const int L = leaf_node[l];
+                                                  int32_t* far_cnt, int32_t* err) {
+    __shared__ int s_stack[kBlock / kWave][kMaxStack];
+    const int w = threadIdx.x / kWave;
+    const int wave = blockIdx.x * (kBlock / kWave) + w;
+    const int q = wave % kMaxEntries;
+    const int lane = lane_id();
+    const int l = (wave / kMaxEntries) * kWave + lane;
+    if (q >= E->n) return;   // uniform over the wave
+    const bool valid = l < nleaf;
+    if (__ballot(valid) == 0) return;
+    int* stack = s_stack[w];
+    const int L = valid ? leaf_node[l] : 0;
     const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];
     const size_t slot = (size_t)l * kMaxEntries + q;
-    int64_t pn = near_off ? near_off[slot] : 0, pf = far_off ? far_off[slot] : 0;
+    int64_t pn = (valid && near_off) ? near_off[slot] : 0, pf = (valid && far_off) ? far_off[slot] : 0;
     int cn = 0, cf = 0;
-    auto is_far = [&](int T) {
-        const double dx = nv.x[T] - lx, dy = nv.y[T] - ly;
-        const double hp = nv.h[T] + nv.w[T] + lh + lw;
-        return dx * dx + dy * dy > fc * hp * hp;
-    };
-    bool cut = false;
+    // ancestors of the entry: the first far one (if any) cuts the whole entry
+    int skip_depth = INT32_MAX;
     const int dE = E->depth[q];
-    for (int k = 0; k < dE; k++) {
+    for (int k = 0; k < dE && valid; k++) {
         const int A = E->anc[q][k];
-        if (is_far(A)) {
+        const double dx = nv.x[A] - lx, dy = nv.y[A] - ly;
+        const double hp = nv.h[A] + nv.w[A] + lh + lw;
+        if (dx * dx + dy * dy > fc * hp * hp) {
             if ((E->lead[q] >> k) & 1) {
-                if (far_list) far_list[pf] = A;
-                pf++;
+                if (far_list) far_list[pf++] = A;
                 cf++;
             }
-            cut = true;
+            skip_depth = -1;   // never reset: nothing below the entry is needed
             break;
         }
     }
-    if (!cut) {
-        const int root = E->node[q];
-        int T = root;
-        while (true) {
-            bool descend = false;
-            if (is_far(T)) {
-                if (far_list) far_list[pf] = T;
-                pf++;
-                cf++;
-            } else if (nv.ch1[T] < 0) {
-                if (near_list) near_list[pn] = T;
-                pn++;
-                cn++;
-            } else {
-                descend = true;
+    if (lane == 0) stack[0] = E->node[q];
+    int sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        const int T = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
+        sp--;
+        const int dT = nv.depth[T];
+        if (skip_depth != INT32_MAX && dT <= skip_depth) skip_depth = INT32_MAX;
+        const bool act = valid && skip_depth == INT32_MAX;
+        const double dx = nv.x[T] - lx, dy = nv.y[T] - ly;
+        const double hp = nv.h[T] + nv.w[T] + lh + lw;
+        const bool far = act && (dx * dx + dy * dy > fc * hp * hp);
+        if (far) {
+            if (far_list) far_list[pf++] = T;
+            cf++;
+            skip_depth = dT;
+        }
+        const int c = nv.ch1[T];
+        const bool need = act && !far;
+        if (c >= 0) {
+            if (__ballot(need) != 0) {
+                if (sp + 2 > kMaxStack) {
+                    if (lane == 0) *err = 1;
+                    break;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) {
+                    stack[sp] = c + 1;
+                    stack[sp + 1] = c;
+                }
+                __builtin_amdgcn_wave_barrier();
+                sp += 2;
             }
-            if (descend) {
-                T = nv.ch1[T];
-                continue;
-            }
-            while (T != root && T == nv.ch1[nv.parent[T]] + 1) T = nv.parent[T];
-            if (T == root) break;
-            T = T + 1;
+        } else if (need) {
+            if (near_list) near_list[pn++] = T;
+            cn++;
         }
     }
-    if (near_cnt) near_cnt[slot] = cn;
-    if (far_cnt) far_cnt[slot] = cf;
+    if (valid && near_cnt) near_cnt[slot] = cn;
+    if (valid && far_cnt) far_cnt[slot] = cf;
 }
 
 // Per-leaf CSR offsets from the per-(leaf, entry) offsets.
@@ -743,24 +760,29 @@ void tree_lists(Ctx& c) {
     const double fc = (double)t.far_criteria;
     const int waves = ((nl + kWave - 1) / kWave) * kMaxEntries;
     const int grid = (waves + kBlock / kWave - 1) / (kBlock / kWave);
+    t.err.grow(1);
+    VVHD_HIP(hipMemsetAsync(t.err.p, 0, 4, s));
     hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E, t.leaf_node.p, nl, fc,
                        (const int64_t*)nullptr, (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
-                       cnt_n.p, cnt_f.p);
+                       cnt_n.p, cnt_f.p, t.err.p);
     scan_i32_to_i64(c, cnt_n.p, t.slot_near.p, (int)nslot);
     scan_i32_to_i64(c, cnt_f.p, t.slot_far.p, (int)nslot);
     hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_near.p, nl, t.near_off.p);
     hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_far.p, nl, t.far_off.p);
     int64_t tot[2];
+    int32_t herr = 0;
     VVHD_HIP(hipMemcpyAsync(&tot[0], t.slot_near.p + nslot, 8, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipMemcpyAsync(&tot[1], t.slot_far.p + nslot, 8, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(&herr, t.err.p, 4, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
+    VVHD_CHECK(herr == 0, VVHD_EUNSUPPORTED, "tree lists: traversal stack exceeded");
     t.nnear = tot[0];
     t.nfar = tot[1];
     t.near_list.grow(t.nnear + 1);
     t.far_list.grow(t.nfar + 1);
     hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E, t.leaf_node.p, nl, fc,
                        (const int64_t*)t.slot_near.p, (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p,
-                       (int32_t*)nullptr, (int32_t*)nullptr);
+                       (int32_t*)nullptr, (int32_t*)nullptr, t.err.p);
 }
 
 }  // namespace vvhd
