@@ -1627,6 +1627,65 @@ int32_t vvo_far_moments(vvo_ctx* c, double* out) {
     return nn;
 }
 
+// Features of the learned far field for the point (tx, ty) and node F (see
+// vvflow_amd/csrc/farfield.hip); false (and zeros) for a degenerate pair.
+static bool point_features(const std::vector<double>& mom, const Node& nd, int F, double tx, double ty, double* f,
+                           double& ux, double& uy, double& sc) {
+    const size_t nn = mom.size() / 12;
+    auto M = [&](int k, int id) { return mom[(size_t)k * nn + id]; };
+    for (int k = 0; k < 12; k++) f[k] = 0;
+    ux = uy = sc = 0;
+    const double gp = M(0, F), gm = M(1, F);
+    const double S = std::fabs(gp) + std::fabs(gm);
+    const double dx = tx - nd.x, dy = ty - nd.y;
+    const double r = std::sqrt(dx * dx + dy * dy);
+    if (!(S > 0 && r > 0)) return false;
+    const double ep = M(10, F), em = M(11, F);
+    const double d2 = nd.h * nd.h + nd.w * nd.w + (ep - em) / S;
+    if (!(d2 > 0 && std::isfinite(d2))) return false;
+    ux = dx / r;
+    uy = dy / r;
+    const double sd = std::sqrt(d2);
+    const double inv1 = 1. / (S * sd), inv2 = 1. / (S * d2);
+    double ax = M(2, F) - nd.x, ay = M(3, F) - nd.y;
+    f[2] = gp * (ax * ux + ay * uy) * inv1;
+    f[3] = gp * (ay * ux - ax * uy) * inv1;
+    ax = M(4, F) - nd.x;
+    ay = M(5, F) - nd.y;
+    f[4] = gm * (ax * ux + ay * uy) * inv1;
+    f[5] = gm * (ay * ux - ax * uy) * inv1;
+    const double qr = M(6, F) + M(8, F), qi = M(7, F) + M(9, F);
+    const double c2r = ux * ux - uy * uy, c2i = -2 * ux * uy;
+    f[6] = (qr * c2r - qi * c2i) * inv2;
+    f[7] = (qr * c2i + qi * c2r) * inv2;
+    f[0] = gp / S;
+    f[1] = gm / S;
+    f[8] = ep * inv2;
+    f[9] = em * inv2;
+    f[10] = sd / r;
+    f[11] = (nd.h - nd.w) / std::fmax(nd.h + nd.w, 1e-300);
+    sc = S * d2 / (2 * kPi * r * r * r);
+    return true;
+}
+
+int vvo_far_point_features(vvo_ctx* c, const double* px, const double* py, const int32_t* node, int64_t np,
+                           double* X, double* scale, double* u) {
+    if (!c->built) {
+        c->err = "far_point_features: tree is not built";
+        return 1;
+    }
+    const int nn = (int)c->nodes.size();
+    std::vector<double> mom((size_t)12 * nn);
+    vvo_far_moments(c, mom.data());
+    for (int64_t p = 0; p < np; p++) {
+        double ux, uy;
+        point_features(mom, c->nodes[node[p]], node[p], px[p], py[p], X + 12 * p, ux, uy, scale[p]);
+        u[2 * p] = ux;
+        u[2 * p + 1] = uy;
+    }
+    return 0;
+}
+
 int vvo_far_pairs(vvo_ctx* c, const int32_t* target, const int32_t* node, int64_t np, double* X, double* Y,
                   double* scale, double* base, double* exact) {
     if (!c->built) {
@@ -1671,42 +1730,8 @@ int vvo_far_pairs(vvo_ctx* c, const int32_t* target, const int32_t* node, int64_
             exact[2 * p + 1] = sv;
         }
         // features (the device computes the same FP64 expressions, then rounds to FP32)
-        double f[12] = {0};
-        double ux = 0, uy = 0, sc = 0;
-        bool ok = false;
-        const double gp = M(0, F), gm = M(1, F);
-        const double S = std::fabs(gp) + std::fabs(gm);
-        const double dx = t.x - nd.x, dy = t.y - nd.y;
-        const double r = std::sqrt(dx * dx + dy * dy);
-        if (S > 0 && r > 0) {
-            const double ep = M(10, F), em = M(11, F);
-            const double d2 = nd.h * nd.h + nd.w * nd.w + (ep - em) / S;
-            if (d2 > 0 && std::isfinite(d2)) {
-                ok = true;
-                ux = dx / r;
-                uy = dy / r;
-                const double sd = std::sqrt(d2);
-                const double inv1 = 1. / (S * sd), inv2 = 1. / (S * d2);
-                double ax = M(2, F) - nd.x, ay = M(3, F) - nd.y;
-                f[2] = gp * (ax * ux + ay * uy) * inv1;
-                f[3] = gp * (ay * ux - ax * uy) * inv1;
-                ax = M(4, F) - nd.x;
-                ay = M(5, F) - nd.y;
-                f[4] = gm * (ax * ux + ay * uy) * inv1;
-                f[5] = gm * (ay * ux - ax * uy) * inv1;
-                const double qr = M(6, F) + M(8, F), qi = M(7, F) + M(9, F);
-                const double c2r = ux * ux - uy * uy, c2i = -2 * ux * uy;
-                f[6] = (qr * c2r - qi * c2i) * inv2;
-                f[7] = (qr * c2i + qi * c2r) * inv2;
-                f[0] = gp / S;
-                f[1] = gm / S;
-                f[8] = ep * inv2;
-                f[9] = em * inv2;
-                f[10] = sd / r;
-                f[11] = (nd.h - nd.w) / std::fmax(nd.h + nd.w, 1e-300);
-                sc = S * d2 / (2 * kPi * r * r * r);
-            }
-        }
+        double f[12], ux, uy, sc;
+        const bool ok = point_features(mom, nd, F, t.x, t.y, f, ux, uy, sc);
         for (int k = 0; k < 12; k++) X[12 * p + k] = f[k];
         const double ru = su - bu, rv = sv - bv;
         Y[2 * p] = ok ? (ru * ux + rv * uy) / sc : 0.;

@@ -143,6 +143,10 @@ int32_t vvo_far_moments(vvo_ctx*, double* out);
  * and the direct Lamb-kernel field of the node's vortices (u, v). */
 int     vvo_far_pairs(vvo_ctx*, const int32_t* target, const int32_t* node, int64_t np, double* X,
                       double* Y, double* scale, double* base, double* exact);
+/* The 12 model features, the scale and the unit direction u (ux, uy) of np
+ * (point, node) pairs, for arbitrary points (px, py). */
+int     vvo_far_point_features(vvo_ctx*, const double* px, const double* py, const int32_t* node, int64_t np,
+                               double* X, double* scale, double* u);
 
 /* ---- Integrals (TSpace.cpp:94-143, TSpace.hpp:85-117) ----------------- */
 void    vvo_calc_forces(vvo_ctx*);

@@ -98,6 +98,7 @@ class Oracle:
             _sig(L, "vvo_integrals", None, [P, dp])
             _sig(L, "vvo_far_moments", I32, [P, C.c_void_p])
             _sig(L, "vvo_far_pairs", C.c_int, [P, ip, ip, I64, dp, dp, dp, C.c_void_p, C.c_void_p])
+            _sig(L, "vvo_far_point_features", C.c_int, [P, dp, dp, ip, I64, dp, dp, dp])
             cls._lib = L
         return cls._lib
 
@@ -267,6 +268,17 @@ class Oracle:
         base = np.zeros((n, 2)); exact = np.zeros((n, 2))
         self._check(self.L.vvo_far_pairs(self.h, target, node, n, X, Y, sc, base.ctypes.data, exact.ctypes.data))
         return X, Y, sc, base, exact
+
+    def far_point_features(self, px, py, node):
+        """Features, scale and unit direction (ux, uy) of (point, oracle node)
+        pairs for arbitrary points."""
+        px = np.ascontiguousarray(px, dtype=np.float64)
+        py = np.ascontiguousarray(py, dtype=np.float64)
+        node = np.ascontiguousarray(node, dtype=np.int32)
+        n = len(node)
+        X = np.zeros((n, 12)); sc = np.zeros(n); u = np.zeros((n, 2))
+        self._check(self.L.vvo_far_point_features(self.h, px, py, node, n, X, sc, u))
+        return X, sc, u
 
     def integrals(self):
         out = np.zeros(5)

@@ -176,3 +176,19 @@ def test_single_vortex_node_residual():
         want = v[j, 2] * np.array([-D[1], D[0]]) * (1 / (r2 + e2) - 1 / r2) / (2 * np.pi)
         assert np.allclose(exact[p] - base[p], want, rtol=1e-6, atol=1e-16)
         assert X[p, 6] == 0 and X[p, 7] == 0
+
+
+def test_point_features_match_pair_features():
+    """Features at arbitrary points (the learned path's three points per
+    leaf) are the pair features when the point is the vortex itself, and u
+    is the unit vector from the node centre to the point."""
+    o = _case(2000, 9)
+    tg, nd, T = _pairs(o, 11)
+    v = o.vortices()
+    X, _, sc, _, _ = o.far_pairs(tg, nd)
+    Xp, scp, u = o.far_point_features(v[tg, 0], v[tg, 1], nd)
+    assert np.array_equal(X, Xp) and np.array_equal(sc, scp)
+    D = v[tg, :2] - T["geom"][nd, :2]
+    ok = sc > 0
+    assert np.allclose(u[ok], D[ok] / np.hypot(D[ok, 0], D[ok, 1])[:, None], rtol=1e-14, atol=1e-15)
+    assert np.all(u[~ok] == 0)

@@ -9,7 +9,8 @@ difference whose sums run in another order); network outputs within 1e-3
 (relative L2) of the oracle with the same bf16 rounding points (the device's
 exp/rcp and FP32 accumulation order can flip a bf16 rounding); gradients
 within 1e-2 per layer; learned-path velocities within 5e-6 (monopole part,
-FP32 Kahan) and 1e-5 (with a network) relative L2 of the oracle's.
+FP32 Kahan) and 1e-5 (with a network: corrections at three points per leaf,
+interpolated to the targets) relative L2 of the oracle's.
 """
 import os
 import sys
@@ -142,12 +143,14 @@ def test_training_reduces_loss(ctx, case):
 
 def _oracle_learned(o, targets, p=None):
     """Induced velocity of the learned path for the given targets, on the
-    oracle: exact near field + per far node (monopoles + network correction)."""
+    oracle: exact near field + per far node monopoles at the target + the
+    network correction of the far nodes at the leaf's three points (centre,
+    centre + (w/2, 0), centre + (0, h/2)), interpolated linearly."""
     T = o.tree()
     v = o.vortices()
     noff, nlst = T["near"]
     foff, flst = T["far"]
-    ints = T["ints"]
+    ints, geom = T["ints"], T["geom"]
     leaf_of = np.empty(len(v), int)
     for l, L in enumerate(T["leaves"]):
         leaf_of[ints[L, 0]:ints[L, 1]] = l
@@ -160,14 +163,18 @@ def _oracle_learned(o, targets, p=None):
         tgt[0, 2] = 0.0
         out[q] = direct_sum(np.concatenate([src, tgt]), targets=np.array([len(src)]), threads=1)[0]
         far = flst[foff[l]:foff[l + 1]]
-        X, Y, sc, base, exact = o.far_pairs(np.full(len(far), t), far)
+        _, _, _, base, _ = o.far_pairs(np.full(len(far), t), far)
         out[q] += base.sum(0)
-        if p is not None:
-            ok = sc > 0
-            D = v[t, :2] - T["geom"][far, :2]
-            r = np.hypot(D[:, 0], D[:, 1])
-            corr = M.correction(p, X[ok].astype(np.float32), sc[ok], D[ok, 0] / r[ok], D[ok, 1] / r[ok])
-            out[q] += corr.sum(0)
+        if p is not None and len(far):
+            cx, cy, h, w = geom[T["leaves"][l]]
+            C = []
+            for px, py in ((cx, cy), (cx + w / 2, cy), (cx, cy + h / 2)):
+                X, sc, u = o.far_point_features(np.full(len(far), px), np.full(len(far), py), far)
+                ok = sc > 0
+                C.append(M.correction(p, X[ok].astype(np.float32), sc[ok], u[ok, 0], u[ok, 1]).sum(0))
+            ax = (v[t, 0] - cx) / (w / 2) if w > 0 else 0.0
+            ay = (v[t, 1] - cy) / (h / 2) if h > 0 else 0.0
+            out[q] += C[0] + ax * (C[1] - C[0]) + ay * (C[2] - C[0])
     return out
 
 

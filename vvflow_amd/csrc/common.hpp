@@ -211,6 +211,7 @@ struct Ctx {
     size_t fm_stride = 0;
     DBuf<double2> far_v;                     // per-vortex far-field velocity of the learned path
     DBuf<uint8_t> far_nodes;                 // per-node feature records of the learned path
+    DBuf<double> far_leaf;                   // per-leaf network correction sums at three points (6 doubles)
     DBuf<float> mlp_params;                  // FP32 master parameters (kMlpParams)
     DBuf<uint16_t> mlp_frags;                // bf16 MFMA fragments + FP32 tail packed from the parameters
     bool mlp_loaded = false;

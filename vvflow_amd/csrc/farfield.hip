@@ -13,7 +13,10 @@
 //   * plus a correction predicted by a small MLP from the node's moments and
 //     the target offset: 12 inputs, two hidden layers of 64 SiLU units, 2
 //     outputs, bf16 weights and activations with FP32 accumulation on the
-//     matrix cores (v_mfma_f32_32x32x16_bf16).
+//     matrix cores (v_mfma_f32_32x32x16_bf16).  The network runs per
+//     (leaf, far node) at three points of the leaf, and each target takes
+//     the linear interpolation of the three corrections (k_far_learned,
+//     k_far_mono).
 // The MLP's target is the residual between the direct Lamb-kernel sum over
 // the node's vortices (FP64) and the monopole field, expressed in the frame
 // of the target direction and scaled by S d^2 / (2 pi r^3) (S = |G+| + |G-|,
@@ -430,14 +433,23 @@ __device__ inline void mlp_infer(const Weights& W, const float* sb1, const float
     y1 = c[1];
 }
 
-// Far field of every vortex by the learned path, one wave per leaf; targets
-// in chunks of up to 16, padded to a power of two, so every lane keeps one
-// target for the whole far list and the per-target sums need no atomics.
-// Lane group g (lanes 32g..32g+31) builds inputs 8g..8g+7 of pair l%32.
+// Network part of the learned far field, one wave per leaf: the correction
+// of every far node of the leaf evaluated at three points of the leaf -- its
+// centre c, c + (w/2, 0) and c + (0, h/2) (w, h the leaf's extent in x and
+// y) -- and summed over the far list.  k_far_mono gives each target the
+// linear interpolation of the three sums, so the network runs three times per
+// (leaf, far node) instead of once per (target, far node); the correction
+// varies slowly over a leaf, which lies far from the node.
+// A tile holds 10 far nodes x 3 points in columns 0..29 (column n: node n/3,
+// point n%3; columns 30, 31 idle).  Lane group g (lanes 32g..32g+31) builds
+// inputs 8g..8g+7 of column l%32.  Output per leaf: 6 doubles (u, v at the
+// three points); zeros for a leaf without far nodes or without a target of
+// non-zero circulation.
+constexpr int kFarPerTile = 10;
+
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_far_learned(
-    TreeDev T, const FarNode* __restrict__ nodes,
-                                                        const double* __restrict__ x, const double* __restrict__ y,
-                                                        const uint16_t* frag, const float* P, double2* far_out) {
+    TreeDev T, const FarNode* __restrict__ nodes, const double* __restrict__ g, const uint16_t* frag, const float* P,
+    double* __restrict__ leaf_corr) {
     __shared__ float sb1[kHid], sb2[kHid], smu[16], sisd[16];
     for (int k = threadIdx.x; k < kHid; k += blockDim.x) {
         sb1[k] = P[oB1 + k];
@@ -451,91 +463,149 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
-    const int lane = lane_id(), g = lane >> 5, n = lane & 31;
+    const int lane = lane_id(), grp = lane >> 5, n = lane & 31;
+    const int L = T.leaf_node[l];
+    const int v0 = T.v0[L], v1 = T.v1[L];
+    const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
+    double* out = leaf_corr + 6 * (size_t)l;
+    bool target = false;
+    for (int i = v0 + lane; i < v1; i += kWave) target |= g[i] != 0;
+    if (fb == fe || __ballot(target) == 0) {
+        if (lane < 6) out[lane] = 0.;
+        return;
+    }
     Weights W;
     load_weights(frag, W);
     const float b30 = P[oB3], b31 = P[oB3 + 1];
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
+    const int j = n / 3, pt = n % 3;
+    const bool col = n < 3 * kFarPerTile;
+    const double px = T.nx[L] + (pt == 1 ? 0.5 * T.nw[L] : 0.);
+    const double py = T.ny[L] + (pt == 2 ? 0.5 * T.nh[L] : 0.);
+    float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;   // Kahan sums (lanes 0..29)
+    // the node record of the next tile is loaded while this tile computes
+    FarNode nd = nodes[(col && fb + j < fe) ? T.far_list[fb + j] : 0];
+    for (int64_t f0 = fb; f0 < fe; f0 += kFarPerTile) {
+        const int64_t e = f0 + j;
+        const bool valid = col && e < fe;
+        const int64_t en = e + kFarPerTile;
+        const FarNode nxt = nodes[(col && en < fe) ? T.far_list[en] : 0];
+        const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
+        const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
+        const float ux = dx * rinv, uy = dy * rinv;
+        const bool ok = valid && nd.ok != 0.f;
+        float f[8];
+        if (grp == 0) {
+            const float c2r = ux * ux - uy * uy, c2i = -2.f * ux * uy;
+            f[0] = nd.f0;
+            f[1] = nd.f1;
+            f[2] = nd.dpx * ux + nd.dpy * uy;
+            f[3] = nd.dpy * ux - nd.dpx * uy;
+            f[4] = nd.dmx * ux + nd.dmy * uy;
+            f[5] = nd.dmy * ux - nd.dmx * uy;
+            f[6] = nd.qqx * c2r - nd.qqy * c2i;
+            f[7] = nd.qqx * c2i + nd.qqy * c2r;
+        } else {
+            f[0] = nd.e8;
+            f[1] = nd.e9;
+            f[2] = nd.sd * rinv;
+            f[3] = nd.asp;
+            f[4] = f[5] = f[6] = f[7] = 0.f;
+        }
+        bf16x8 xin;
+        for (int q = 0; q < 8; q++) xin[q] = (__bf16)(ok ? (f[q] - smu[8 * grp + q]) * sisd[8 * grp + q] : 0.f);
+        float y0, y1;
+        mlp_infer(W, sb1, sb2, b30, b31, xin, y0, y1);
+        if (grp == 0 && ok) {
+            const float sc = nd.scl * rinv * rinv * rinv;
+            const float a = y0 * ysd0, b = y1 * ysd1;
+            const float u = (a * ux - b * uy) * sc, v = (a * uy + b * ux) * sc;
+            float yk = u - cu, tk = su + yk;
+            cu = (tk - su) - yk;
+            su = tk;
+            yk = v - cv;
+            tk = sv + yk;
+            cv = (tk - sv) - yk;
+            sv = tk;
+        }
+        nd = nxt;
+    }
+    // per point, the sum over its ten columns in column order
+    const double du = (double)su - (double)cu, dv = (double)sv - (double)cv;
+    double tu = 0., tv = 0.;
+    for (int m = 0; m < kFarPerTile; m++) {
+        tu += __shfl(du, (lane % 3) + 3 * m, kWave);
+        tv += __shfl(dv, (lane % 3) + 3 * m, kWave);
+    }
+    if (lane < 3) {
+        out[2 * lane] = tu;
+        out[2 * lane + 1] = tv;
+    }
+}
+
+// Far field of every vortex by the learned path: the exact field at the
+// vortex of the far nodes' per-sign monopoles (FP32 terms from FP64
+// differences, Kahan sums, lane partials combined in FP64), plus the leaf's
+// network correction interpolated linearly from its three points.  One wave
+// per leaf; targets in chunks of up to 16 padded to a power of two, the far
+// list split over 64 / npad lanes per target.
+__global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* __restrict__ nodes,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ leaf_corr, double2* far_out) {
+    const int w = threadIdx.x / kWave;
+    const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
+    if (l >= T.nleaf) return;
+    const int lane = lane_id();
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
+    const double cx = T.nx[L], cy = T.ny[L], hx = 0.5 * T.nw[L], hy = 0.5 * T.nh[L];
+    const double* C = leaf_corr + 6 * (size_t)l;
+    const double c0u = C[0], c0v = C[1], c1u = C[2], c1v = C[3], c2u = C[4], c2v = C[5];
     for (int base = v0; base < v1; base += 16) {
         const int ntc = min(16, v1 - base);
         const int npad = ntc <= 1 ? 1 : ntc <= 2 ? 2 : ntc <= 4 ? 4 : ntc <= 8 ? 8 : 16;
-        const int nper = 32 / npad;
-        const int k = n % npad, fo = n / npad;
+        const int nper = kWave / npad;
+        const int k = lane % npad, fo = lane / npad;
         const bool tvalid = k < ntc;
         const double tx = tvalid ? x[base + k] : 0., ty = tvalid ? y[base + k] : 0.;
-        float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;   // Kahan sums (lanes 0..31)
-        // the node record of the next tile is loaded while this tile computes
-        FarNode nd = nodes[(tvalid && fb + fo < fe) ? T.far_list[fb + fo] : 0];
-        for (int64_t f0 = fb; f0 < fe; f0 += nper) {
-            const int64_t e = f0 + fo;
-            const bool valid = tvalid && e < fe;
-            const int64_t en = e + nper;
-            const FarNode nxt = nodes[(tvalid && en < fe) ? T.far_list[en] : 0];
+        float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;
+        for (int64_t e = fb + fo; e < fe; e += nper) {
+            if (!tvalid) break;
+            const FarNode& nd = nodes[T.far_list[e]];
             const float dx = (float)(tx - nd.cx), dy = (float)(ty - nd.cy);
-            const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
-            const float ux = dx * rinv, uy = dy * rinv;
-            const bool ok = valid && nd.ok != 0.f;
-            float f[8];
-            if (g == 0) {
-                const float c2r = ux * ux - uy * uy, c2i = -2.f * ux * uy;
-                f[0] = nd.f0;
-                f[1] = nd.f1;
-                f[2] = nd.dpx * ux + nd.dpy * uy;
-                f[3] = nd.dpy * ux - nd.dpx * uy;
-                f[4] = nd.dmx * ux + nd.dmy * uy;
-                f[5] = nd.dmy * ux - nd.dmx * uy;
-                f[6] = nd.qqx * c2r - nd.qqy * c2i;
-                f[7] = nd.qqx * c2i + nd.qqy * c2r;
-            } else {
-                f[0] = nd.e8;
-                f[1] = nd.e9;
-                f[2] = nd.sd * rinv;
-                f[3] = nd.asp;
-                f[4] = f[5] = f[6] = f[7] = 0.f;
+            float u = 0.f, v = 0.f;
+            if (nd.gpk != 0.f) {
+                const float ex = dx - nd.zpx, ey = dy - nd.zpy;
+                const float fac = nd.gpk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
+                u -= ey * fac;
+                v += ex * fac;
             }
-            bf16x8 xin;
-            for (int q = 0; q < 8; q++) xin[q] = (__bf16)(ok ? (f[q] - smu[8 * g + q]) * sisd[8 * g + q] : 0.f);
-            float y0, y1;
-            mlp_infer(W, sb1, sb2, b30, b31, xin, y0, y1);
-            if (g == 0 && valid) {
-                float u = 0.f, v = 0.f;
-                if (nd.gpk != 0.f) {
-                    const float ex = dx - nd.zpx, ey = dy - nd.zpy;
-                    const float fac = nd.gpk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
-                    u -= ey * fac;
-                    v += ex * fac;
-                }
-                if (nd.gmk != 0.f) {
-                    const float ex = dx - nd.zmx, ey = dy - nd.zmy;
-                    const float fac = nd.gmk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
-                    u -= ey * fac;
-                    v += ex * fac;
-                }
-                if (ok) {
-                    const float sc = nd.scl * rinv * rinv * rinv;
-                    const float a = y0 * ysd0, b = y1 * ysd1;
-                    u += (a * ux - b * uy) * sc;
-                    v += (a * uy + b * ux) * sc;
-                }
-                float yk = u - cu, tk = su + yk;
-                cu = (tk - su) - yk;
-                su = tk;
-                yk = v - cv;
-                tk = sv + yk;
-                cv = (tk - sv) - yk;
-                sv = tk;
+            if (nd.gmk != 0.f) {
+                const float ex = dx - nd.zmx, ey = dy - nd.zmy;
+                const float fac = nd.gmk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
+                u -= ey * fac;
+                v += ex * fac;
             }
-            nd = nxt;
+            float yk = u - cu, tk = su + yk;
+            cu = (tk - su) - yk;
+            su = tk;
+            yk = v - cv;
+            tk = sv + yk;
+            cv = (tk - sv) - yk;
+            sv = tk;
         }
         double du = (double)su - (double)cu, dv = (double)sv - (double)cv;
-        for (int o = npad; o < 32; o <<= 1) {
+        for (int o = npad; o < kWave; o <<= 1) {
             du += __shfl_xor(du, o, kWave);
             dv += __shfl_xor(dv, o, kWave);
         }
-        if (g == 0 && n < ntc) far_out[base + n] = make_double2(du, dv);
+        if (fo == 0 && tvalid) {
+            const double ax = hx > 0 ? (tx - cx) / hx : 0., ay = hy > 0 ? (ty - cy) / hy : 0.;
+            du += c0u + ax * (c1u - c0u) + ay * (c2u - c0u);
+            dv += c0v + ax * (c1v - c0v) + ay * (c2v - c0v);
+            far_out[base + k] = make_double2(du, dv);
+        }
     }
 }
 
@@ -873,6 +943,7 @@ void farfield_release(Ctx& c) {
     c.fm_stride = 0;
     c.far_v.release();
     c.far_nodes.release();
+    c.far_leaf.release();
     c.mlp_params.release();
     c.mlp_frags.release();
     c.mlp_loaded = false;
@@ -1094,13 +1165,16 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
         c.far_v.grow(c.n + 1);
         c.far_nodes.grow((size_t)c.t.nnodes * sizeof(FarNode));
         FarNode* nodes = reinterpret_cast<FarNode*>(c.far_nodes.p);
+        c.far_leaf.grow(6 * (size_t)T.nleaf);
         hipLaunchKernelGGL(k_far_nodes, dim3(nblocks(c.t.nnodes)), dim3(kBlock), 0, c.stream, T, M, c.t.nnodes, nodes);
         VVHD_HIP(hipEventRecord(c.ev[14], c.stream));
         const int grid = (T.nleaf + kWavesPerBlock - 1) / kWavesPerBlock;
-        hipLaunchKernelGGL(k_far_learned, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.x.p,
-                           c.y.p, c.mlp_frags.p, c.mlp_params.p, c.far_v.p);
+        hipLaunchKernelGGL(k_far_learned, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.g.p,
+                           c.mlp_frags.p, c.mlp_params.p, c.far_leaf.p);
         VVHD_HIP(hipEventRecord(c.ev[15], c.stream));
         c.phase_recorded[7] = true;
+        hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.x.p, c.y.p,
+                           c.far_leaf.p, c.far_v.p);
         hipLaunchKernelGGL(k_finish_learned, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, c.g.p, c.near_v.p,
                            c.far_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     }
