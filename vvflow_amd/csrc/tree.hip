@@ -126,11 +126,80 @@ __device__ inline void bbox_add_wave(NodeView nv, int node, double px, double py
     if (active && ((left >> lane_id()) & 1ull)) bbox_add(nv, node, px, py);
 }
 
-// Root bounding box over all vortices and segments.
-__global__ void k_root_bbox(NodeView nv, const double* x, const double* y, int n, const vvhd_seg* segs, int nseg) {
+// Bounding boxes of the nodes of a level with at most kTopNodes nodes (the
+// root, and the children of the first levels), from the elements in their
+// final positions: every element k with enode[k] - base in [0, kTopNodes)
+// adds to that node.  Few nodes share many elements there, so the lanes
+// reduce per node in the wave, then in LDS per block, and one lane per block
+// and node issues the global atomics.  Each block covers kTopChunk
+// consecutive elements.
+constexpr int kTopNodes = 64;
+constexpr int kTopChunk = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t* base_ptr, const int32_t* enode,
+                                                     const double* x, const double* y, int n) {
+    __shared__ unsigned long long s_lx[kTopNodes], s_ly[kTopNodes], s_hx[kTopNodes], s_hy[kTopNodes];
+    const unsigned long long lo0 = ord_from_double(DBL_MAX), hi0 = ord_from_double(-DBL_MAX);
+    for (int t = threadIdx.x; t < kTopNodes; t += blockDim.x) {
+        s_lx[t] = lo0;
+        s_ly[t] = lo0;
+        s_hx[t] = hi0;
+        s_hy[t] = hi0;
+    }
+    __syncthreads();
+    const int base = base_ptr ? *base_ptr : 0;
+    const int k0 = blockIdx.x * kTopChunk;
+    const int k1 = min(n, k0 + kTopChunk);
+    for (int kb = k0; kb < k1; kb += blockDim.x) {
+        const int k = kb + threadIdx.x;
+        int node = -1;
+        double px = 0, py = 0;
+        if (k < k1) {
+            node = (enode ? enode[k] : 0) - base;
+            px = x[k];
+            py = y[k];
+        }
+        bool active = node >= 0 && node < kTopNodes;
+        uint64_t left = __ballot(active);
+        while (left) {
+            const int leader = (int)__builtin_ctzll(left);
+            const int lead_node = __shfl(node, leader, kWave);
+            const bool mine = active && node == lead_node;
+            left &= ~__ballot(mine);
+            const bool vx = mine && !isnan(px), vy = mine && !isnan(py);
+            double lx = vx ? px : DBL_MAX, hx = vx ? px : -DBL_MAX;
+            double ly = vy ? py : DBL_MAX, hy = vy ? py : -DBL_MAX;
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                lx = fmin(lx, __shfl_xor(lx, o, kWave));
+                hx = fmax(hx, __shfl_xor(hx, o, kWave));
+                ly = fmin(ly, __shfl_xor(ly, o, kWave));
+                hy = fmax(hy, __shfl_xor(hy, o, kWave));
+            }
+            if (lane_id() == leader) {
+                if (lx <= hx) {
+                    atomicMin(&s_lx[lead_node], (unsigned long long)ord_from_double(lx));
+                    atomicMax(&s_hx[lead_node], (unsigned long long)ord_from_double(hx));
+                }
+                if (ly <= hy) {
+                    atomicMin(&s_ly[lead_node], (unsigned long long)ord_from_double(ly));
+                    atomicMax(&s_hy[lead_node], (unsigned long long)ord_from_double(hy));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTopNodes; t += blockDim.x) {
+        const int id = base + t;
+        if (s_lx[t] != lo0) atomicMin((unsigned long long*)&nv.blx[id], s_lx[t]);
+        if (s_hx[t] != hi0) atomicMax((unsigned long long*)&nv.trx[id], s_hx[t]);
+        if (s_ly[t] != lo0) atomicMin((unsigned long long*)&nv.bly[id], s_ly[t]);
+        if (s_hy[t] != hi0) atomicMax((unsigned long long*)&nv.try_[id], s_hy[t]);
+    }
+}
+
+// Segments of the root bounding box.
+__global__ void k_root_segs(NodeView nv, const vvhd_seg* segs, int nseg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool av = i < n;
-    bbox_add_wave(nv, 0, av ? x[i] : 0, av ? y[i] : 0, av);
     if (i < nseg) bbox_add(nv, 0, segs[i].x, segs[i].y);
 }
 
@@ -232,9 +301,10 @@ __global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* flag, 
 }
 
 // Hoare swaps, child bounding boxes and element -> child node.
+// With bbox false the child boxes are left to k_bbox_top.
 __global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const int32_t* P, int n,
                        const int32_t* node_L, const int32_t* node_m, const int32_t* left_pos,
-                       const int32_t* right_pos, double* x, double* y, int32_t* id) {
+                       const int32_t* right_pos, double* x, double* y, int32_t* id, bool bbox) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
     int node = 0, Labs = 0, v0 = 0, f = 0, c = 0;
@@ -261,9 +331,11 @@ __global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const i
         qx = x[partner];
         qy = y[partner];
     }
-    const bool own = active && !right_mis;
-    bbox_add_wave(nv, c + (f ? 0 : 1), px, py, own);
-    bbox_add_wave(nv, c, qx, qy, left_mis);
+    if (bbox) {
+        const bool own = active && !right_mis;
+        bbox_add_wave(nv, c + (f ? 0 : 1), px, py, own);
+        bbox_add_wave(nv, c, qx, qy, left_mis);
+    }
     if (active) enode[k] = (k < Labs) ? c : c + 1;
     // Only the left thread of a misplaced pair writes either position.
     if (left_mis) {
@@ -422,7 +494,10 @@ __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves
 // reference's preorder: a far ancestor is emitted by the first entry beneath
 // it only, and the per-(leaf, entry) segments are concatenated in entry
 // order.
-constexpr int kSplitDepth = 3;
+#ifndef VVHD_LIST_SPLIT_DEPTH
+#define VVHD_LIST_SPLIT_DEPTH 6
+#endif
+constexpr int kSplitDepth = VVHD_LIST_SPLIT_DEPTH;
 constexpr int kMaxEntries = 1 << kSplitDepth;
 
 struct ListEntries {
@@ -665,8 +740,10 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(kBlock), 0, s, c.id.p, n);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(n)), dim3(kBlock), 0, s, enode.p, n, 0);
     if (nseg) hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nseg)), dim3(kBlock), 0, s, seg_node.p, nseg, 0);
-    hipLaunchKernelGGL(k_root_bbox, dim3(nblocks(std::max(n, nseg))), dim3(kBlock), 0, s, nv, c.x.p, c.y.p, n,
-                       c.segs.p, nseg);
+    if (n)
+        hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, c.x.p, c.y.p, n);
+    if (nseg) hipLaunchKernelGGL(k_root_segs, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg);
 
     std::vector<std::pair<int, int>>& levels = t.levels;   // node id range of every level
     levels.clear();
@@ -682,8 +759,13 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
                            node_m.p);
         hipLaunchKernelGGL(k_ranks, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, left_pos,
                            right_pos);
+        // the children of level d number at most 2^(d+1)
+        const bool top = (1 << std::min<int>(levels.size(), 30)) <= kTopNodes;
         hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
-                           left_pos, right_pos, c.x.p, c.y.p, c.id.p);
+                           left_pos, right_pos, c.x.p, c.y.p, c.id.p, !top);
+        if (top && n)
+            hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv,
+                               (const int32_t*)(t.counters.p + 3), (const int32_t*)enode.p, c.x.p, c.y.p, n);
         if (nseg)
             hipLaunchKernelGGL(k_seg_step, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg, seg_node.p);
         hipLaunchKernelGGL(k_next_level, dim3(1), dim3(1), 0, s, t.counters.p);
