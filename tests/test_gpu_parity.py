@@ -255,3 +255,52 @@ def test_full_size_wake_properties(ctx):
     err = rel_l2(v[tg], ref)
     print(f"  merged={merged} tree-method rel L2 vs direct sum (fc=8) = {err:.3e}", flush=True)
     assert err < 1e-2
+
+
+# ------------------------------------------------------------ merge commit paths
+
+def _one_flag_chain_case():
+    """A single-leaf set (fewer than 16 vortices) where pass 1 flags one
+    target and its merge flags another: the parallel commit must give way to
+    the sequential pass.  Found by a seeded search checked on the oracle."""
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        n = int(rng.integers(5, 15))
+        v = np.zeros((n, 6))
+        v[:, :2] = rng.uniform(0, 1, (n, 2))
+        v[:, 2] = rng.choice([-1.0, 1.0], n) * rng.uniform(0.5, 1.5, n) * 1e-3
+        d2 = ((v[:, None, :2] - v[None, :, :2]) ** 2).sum(-1)
+        np.fill_diagonal(d2, np.inf)
+        order = np.argsort(d2, axis=1, kind="stable")[:, :2]
+        s = np.sign(v[:, 2])
+        flags = (s[order[:, 0]] == s[order[:, 1]]) & (s[order[:, 0]] != s)
+        if flags.sum() != 1:
+            continue
+        o = oracle_for(v, inf=(0.0, 0.0))
+        o.tree_build(8)
+        if o.epsilon(True)[3] >= 2:
+            return v
+    raise AssertionError("no case found")
+
+
+def test_merge_that_flags_another(ctx):
+    print("case one flag whose merge flags another", flush=True)
+    v = _one_flag_chain_case()
+    run_both(ctx, v, inf=(0.0, 0.0))
+
+
+def test_wake_merges_vs_oracle(ctx):
+    """A 200k wake at far criterion 16 merges a few vortex pairs: neighbour
+    indices, merges and the post-epsilon state against the oracle."""
+    print("case wake 200000 merges fc=16", flush=True)
+    objs = V.gen_wake(200_000, 0x5EED0003)
+    o = oracle_for(objs, inf=(0.0, 0.0))
+    load_case(ctx, objs)
+    o.tree_build(16)
+    ctx.tree_build(16)
+    nn1o, nn2o, _, mo = o.epsilon(True)
+    nn1, nn2, mg = ctx.epsilon(True, want_nn=True)
+    print(f"  merged {mg}", flush=True)
+    assert mg == mo and mg > 0
+    assert np.array_equal(nn1, nn1o) and np.array_equal(nn2, nn2o)
+    assert np.array_equal(ctx.download()[:, [0, 1, 2, 5]], o.vortices()[:, [0, 1, 2, 5]], equal_nan=True)

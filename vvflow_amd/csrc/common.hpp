@@ -205,6 +205,7 @@ struct Ctx {
     DBuf<int32_t> ecount;                    // [0]=flags, [1]=merged, ...
     DBuf<int64_t> rnear_off;                 // transposed near lists (repair)
     DBuf<int32_t> rnear_list, rnear_cnt, leaf_stamp;
+    DBuf<uint64_t> leaf_maxd2;                       // per-leaf bound of the second-neighbour distance
 
     // learned far-field correction
     DBuf<double> fm;                         // per-node moments after epsilon: kMomArrays x node capacity

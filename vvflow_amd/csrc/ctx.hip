@@ -150,6 +150,10 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     for (auto* b : {&c.id, &c.id2, &c.iscratch0, &c.iscratch1, &c.iscratch2, &c.iscratch3, &c.nn1, &c.nn2, &c.ecount,
                     &c.rnear_list})
         b->release();
+    for (auto* b : {&c.flag_list, &c.rnear_cnt, &c.leaf_stamp}) b->release();
+    c.leaf_maxd2.release();
+    c.near_v.release();
+    c.taylor.release();
     c.src_ge.release();
     c.lscratch0.release();
     c.rnear_off.release();
@@ -166,7 +170,8 @@ void vvhd_destroy(vvhd_ctx* ctx) {
                     &t.counters, &t.leaf_node, &t.near_list, &t.far_list, &t.seg_list, &t.seg_leaf, &t.vleaf,
                     &t.node_L, &t.node_m, &t.leaf_tmp, &t.seg_node, &t.cnt_near, &t.cnt_far, &t.err})
         b->release();
-    for (auto* b : {&t.near_off, &t.far_off, &t.seg_off}) b->release();
+    for (auto* b : {&t.near_off, &t.far_off, &t.seg_off, &t.slot_near, &t.slot_far}) b->release();
+    t.entries.release();
     t.cmp_ie.release();
     t.cmm_ie.release();
     for (auto& e : c.ev)

@@ -242,11 +242,12 @@ __device__ inline double point_box_dist2(const TreeDev& T, int M, double px, dou
 
 // Sources of the near leaves of leaf l selected by sel(M) for one target i,
 // lanes over each selected leaf's vortices.
+// Chunks of kWave entries starting at chunk `first`, every `stride`-th.
 template <typename Sel>
 __device__ inline void scan_one(const TreeDev& T, int l, const double* x, const double* y, const double* g, int i,
-                                double tx, double ty, Top2& st, Sel sel) {
+                                double tx, double ty, Top2& st, Sel sel, int first = 0, int stride = 1) {
     const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
-    for (int64_t eb = e0; eb < e1; eb += kWave) {
+    for (int64_t eb = e0 + (int64_t)first * kWave; eb < e1; eb += (int64_t)stride * kWave) {
         const int64_t e = eb + lane_id();
         int M = -1;
         bool q = false;
@@ -273,19 +274,57 @@ __device__ inline Top2 wave_top2(Top2 st) {
     return st;
 }
 
-// Full top-2 search of one target by one wave over its near leaves: leaves
-// touching the target's leaf first, then those within the bound found.
-__device__ Top2 rescan_target(const TreeDev& T, const double* x, const double* y, const double* g, int i) {
-    const int l = T.vleaf[i];
-    const int L = T.leaf_node[l];
-    const double tx = x[i], ty = y[i];
-    Top2 st = top2_empty();
-    scan_one(T, l, x, y, g, i, tx, ty, st, [&](int M) { return box_dist2(T, L, M) <= 0.0; });
-    const Top2 c = wave_top2(st);
-    const double bound = c.i2 >= 0 ? c.d2 : INFINITY;
-    scan_one(T, l, x, y, g, i, tx, ty, st,
-             [&](int M) { return box_dist2(T, L, M) > 0.0 && point_box_dist2(T, M, tx, ty) <= bound; });
-    return wave_top2(st);
+// Full top-2 searches of the targets tg[0..nt) over their near leaves by
+// the NW waves of a workgroup: leaves touching the target's leaf first, then
+// those whose box lies within the second-neighbour distance found.  Groups
+// of G waves (G the largest
+// power of two with G * nt <= NW, at least 1) take one target each; within
+// a group, wave k scans every G-th chunk of the near list, and the partial
+// top-2s combine through LDS (the top-2 under (d, index) does not depend on
+// the order of the candidates).  out(i, st) runs on one lane per target.
+// Every thread of the workgroup must call it; it starts and ends with a
+// barrier.
+template <int NW, typename Out>
+__device__ void group_rescans(const TreeDev& T, const double* x, const double* y, const double* g, const int* tg,
+                              int nt, Top2* s_part, Out out) {
+    int G = 1;
+    while (2 * G * nt <= NW && 2 * G <= NW) G *= 2;
+    const int ngroups = NW / G;
+    const int wave = threadIdx.x / kWave, grp = wave / G, k = wave % G;
+    for (int r0 = 0; r0 < nt; r0 += ngroups) {
+        const int h = r0 + grp;
+        const int i = h < nt ? tg[h] : -1;
+        int l = 0, L = 0;
+        double tx = 0, ty = 0;
+        if (i >= 0) {
+            l = T.vleaf[i];
+            L = T.leaf_node[l];
+            tx = x[i];
+            ty = y[i];
+        }
+        Top2 st = top2_empty();
+        if (i >= 0) scan_one(T, l, x, y, g, i, tx, ty, st, [&](int M) { return box_dist2(T, L, M) <= 0.0; }, k, G);
+        st = wave_top2(st);
+        __syncthreads();
+        if (lane_id() == 0) s_part[wave] = st;
+        __syncthreads();
+        Top2 c = top2_empty();
+        for (int w = grp * G; w < grp * G + G; w++) top2_merge(c, s_part[w]);
+        const double bound = c.i2 >= 0 ? c.d2 : INFINITY;
+        st = top2_empty();
+        if (i >= 0)
+            scan_one(T, l, x, y, g, i, tx, ty, st,
+                     [&](int M) { return box_dist2(T, L, M) > 0.0 && point_box_dist2(T, M, tx, ty) <= bound; }, k, G);
+        st = wave_top2(st);
+        __syncthreads();
+        if (lane_id() == 0) s_part[wave] = st;
+        __syncthreads();
+        if (i >= 0 && k == 0 && lane_id() == 0) {
+            for (int w = grp * G; w < grp * G + G; w++) top2_merge(c, s_part[w]);
+            out(i, c);
+        }
+    }
+    __syncthreads();
 }
 
 __device__ inline bool merge_flag(const double* g, int i, const Top2& st, double crit) {
@@ -312,34 +351,218 @@ __global__ void k_rnear_fill(TreeDev T, const int64_t* roff, int32_t* fillc, int
 }
 
 constexpr int kRepairThreads = 1024;
-constexpr int kRepairWaves = kRepairThreads / kWave;
 constexpr int kMaxHard = 2048;
 constexpr int kMaxPend = 8192;
+constexpr int kMaxLive = 2048;
+
+struct RepairArgs {
+    TreeDev T;
+    double *x, *y, *g;
+    const double* leaf_crit;
+    const int64_t* roff;
+    const int32_t* rlist;
+    int32_t* stamp;
+    int32_t *nn1, *nn2;
+    double *d1a, *d2a;
+    uint8_t* eflag;
+    int32_t* ecount;
+    uint64_t *box_lo_x, *box_lo_y, *box_hi_x, *box_hi_y;
+    uint64_t* maxd2;   // per leaf: upper bound of d2 over its searched targets (ordered image)
+};
+
+struct CommitShared {
+    int nhard;
+    int overflow;
+    int nlive;
+    int hard[kMaxHard];
+    int live[kMaxLive];
+    double qx0, qy0, px0, py0;
+    Top2 part[kRepairThreads / kWave];
+};
+
+// Upper bound of the second-neighbour distance over the searched targets of
+// every leaf, as an ordered integer (-1 for a leaf without one; NaN counts
+// as DBL_MAX).  Commits prune leaves whose box is farther from the merged
+// vortices than this bound: none of their targets can change.
+__device__ inline uint64_t d2_key(double d2) { return ord_from_double(isnan(d2) ? DBL_MAX : d2); }
+
+__global__ void k_leaf_maxd2(TreeDev T, const double* d2a, const uint8_t* eflag, uint64_t* maxd2) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= T.nleaf) return;
+    const int L = T.leaf_node[l];
+    uint64_t m = ord_from_double(-1.0);
+    for (int i = T.v0[L]; i < T.v1[L]; i++)
+        if (eflag[i] & 1) m = max(m, d2_key(d2a[i]));
+    maxd2[l] = m;
+}
+
+// One merge, MergeVortexes(q, nn1[q]) (MEpsilonFast.cpp:111-126), by a
+// workgroup of NT threads, and its consequences for the targets that come
+// later: q itself is searched again without merging; every later target
+// whose near list holds leaf(q) takes q into its top-2 in O(1), and those
+// that had q or its partner p among their two neighbours (or see only
+// leaf(p)) are searched again.  Of the leaves whose near lists hold leaf(q)
+// or leaf(p), only those whose box lies within their targets' largest
+// second-neighbour distance of the old or new position of q or of p are
+// visited: no target elsewhere can take q or have had q or p as neighbour.
+// Targets whose merge flag becomes set are passed to on_flag(i).  `mark`
+// tags the leaves of leaf(q)'s reverse near list in stamp[] for this
+// commit.  Starts and ends with a barrier.
+template <int NT, typename OnFlag>
+__device__ void commit_one(const RepairArgs& A, int q, int mark, CommitShared& sh, OnFlag on_flag) {
+    const TreeDev& T = A.T;
+    const int tid = threadIdx.x;
+    constexpr int kWaves = NT / kWave;
+    double* x = A.x;
+    double* y = A.y;
+    double* g = A.g;
+    const int p = A.nn1[q];
+    __syncthreads();
+    if (tid == 0) {
+        sh.qx0 = x[q];
+        sh.qy0 = y[q];
+        sh.px0 = x[p];
+        sh.py0 = y[p];
+        const double gq = g[q], gp = g[p];
+        if (sgn(gq) == sgn(gp)) {
+            const double inv = 1. / (gq + gp);
+            const double nx = (x[q] * gq + x[p] * gp) * inv;
+            const double ny = (y[q] * gq + y[p] * gp) * inv;
+            x[q] = nx;
+            y[q] = ny;
+        } else if (fabs(gq) < fabs(gp)) {
+            x[q] = x[p];
+            y[q] = y[p];
+        }
+        g[q] = gq + gp;
+        g[p] = 0;
+        atomicAdd(&A.ecount[1], 1);
+        // the merged vortex may leave its leaf's box: grow the box so box
+        // distances stay lower bounds (pruned rescans, diffusive cutoff)
+        const int Lq = T.leaf_node[T.vleaf[q]];
+        if (!isnan(x[q])) {
+            atomicMin((unsigned long long*)&A.box_lo_x[Lq], (unsigned long long)ord_from_double(x[q]));
+            atomicMax((unsigned long long*)&A.box_hi_x[Lq], (unsigned long long)ord_from_double(x[q]));
+        }
+        if (!isnan(y[q])) {
+            atomicMin((unsigned long long*)&A.box_lo_y[Lq], (unsigned long long)ord_from_double(y[q]));
+            atomicMax((unsigned long long*)&A.box_hi_y[Lq], (unsigned long long)ord_from_double(y[q]));
+        }
+        if (p > q) {   // skipped later by the reference (g == 0)
+            A.eflag[p] = 0;
+            A.nn1[p] = -1;
+            A.nn2[p] = -1;
+        }
+        sh.nhard = 0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // the target itself: search again without merging
+    group_rescans<kWaves>(T, x, y, g, &q, 1, sh.part, [&](int i, const Top2& st) {
+        A.nn1[i] = st.i1;
+        A.nn2[i] = st.i2;
+        A.d1a[i] = st.d1;
+        A.d2a[i] = st.d2;
+        A.eflag[i] = 1;
+        atomicMax((unsigned long long*)&A.maxd2[T.vleaf[i]], (unsigned long long)d2_key(st.d2));
+    });
+    // mark the leaves whose near list contains leaf(q)
+    const int lq = T.vleaf[q], lp = T.vleaf[p];
+    for (int64_t e = A.roff[lq] + tid; e < A.roff[lq + 1]; e += NT) A.stamp[A.rlist[e]] = mark;
+    __threadfence_block();
+    __syncthreads();
+    const double qx = x[q], qy = y[q];
+    const double qx0 = sh.qx0, qy0 = sh.qy0, px0 = sh.px0, py0 = sh.py0;
+    // later targets that see leaf(q) (then those that see only leaf(p))
+    for (int pass = 0; pass < 2; pass++) {
+        const int lm = pass == 0 ? lq : lp;
+        if (pass == 1 && lp == lq) break;
+        const int64_t r0 = A.roff[lm], r1 = A.roff[lm + 1];
+        // leaves that can hold a changed target, one thread per leaf
+        if (tid == 0) sh.nlive = 0;
+        __syncthreads();
+        for (int64_t e = r0 + tid; e < r1; e += NT) {
+            const int L = A.rlist[e];
+            if (pass == 1 && A.stamp[L] == mark) continue;
+            const int Ln = T.leaf_node[L];
+            const double m = double_from_ord(A.maxd2[L]);
+            bool need = !(point_box_dist2(T, Ln, qx0, qy0) > m) || !(point_box_dist2(T, Ln, px0, py0) > m);
+            if (pass == 0) need = need || !(point_box_dist2(T, Ln, qx, qy) > m);
+            if (need) {
+                const int k = atomicAdd(&sh.nlive, 1);
+                if (k < kMaxLive) sh.live[k] = L;
+            }
+        }
+        __syncthreads();
+        const int nlive = sh.nlive;
+        const bool all = nlive > kMaxLive;   // list overflow: visit every leaf
+        const int64_t nvis = all ? r1 - r0 : nlive;
+        // NT / 16 leaves at a time, 16 threads per leaf
+        for (int64_t e0 = 0; e0 < nvis; e0 += NT / 16) {
+            const int64_t e = e0 + tid / 16;
+            if (e >= nvis) continue;
+            const int L = all ? A.rlist[r0 + e] : sh.live[e];
+            if (all && pass == 1 && A.stamp[L] == mark) continue;
+            const int Ln = T.leaf_node[L];
+            const int a0 = T.v0[Ln], a1 = T.v1[Ln];
+            for (int i = a0 + (tid & 15); i < a1; i += 16) {
+                if (i <= q || !(A.eflag[i] & 1)) continue;
+                const bool hard = A.nn1[i] == q || A.nn1[i] == p || A.nn2[i] == q || A.nn2[i] == p;
+                if (hard) {
+                    const int h = atomicAdd(&sh.nhard, 1);
+                    if (h < kMaxHard)
+                        sh.hard[h] = i;
+                    else
+                        sh.overflow = 1;
+                    continue;
+                }
+                if (pass == 1) continue;   // q is not in the near set of i
+                Top2 st{A.d1a[i], A.d2a[i], A.nn1[i], A.nn2[i]};
+                top2_insert(st, dist2(x[i], y[i], qx, qy), q);
+                A.nn1[i] = st.i1;
+                A.nn2[i] = st.i2;
+                A.d1a[i] = st.d1;
+                A.d2a[i] = st.d2;   // only decreases: the leaf bound holds
+                const bool f = merge_flag(g, i, st, A.leaf_crit[L]);
+                A.eflag[i] = 1 | (f ? 2 : 0);
+                if (f) on_flag(i);
+            }
+        }
+        __syncthreads();
+    }
+    __threadfence_block();
+    __syncthreads();
+    const int nh = min(sh.nhard, kMaxHard);
+    group_rescans<kWaves>(T, x, y, g, sh.hard, nh, sh.part, [&](int i, const Top2& st) {
+        A.nn1[i] = st.i1;
+        A.nn2[i] = st.i2;
+        A.d1a[i] = st.d1;
+        A.d2a[i] = st.d2;
+        const bool f = merge_flag(g, i, st, A.leaf_crit[T.vleaf[i]]);
+        A.eflag[i] = 1 | (f ? 2 : 0);
+        atomicMax((unsigned long long*)&A.maxd2[T.vleaf[i]], (unsigned long long)d2_key(st.d2));
+        if (f) on_flag(i);
+    });
+    __threadfence_block();
+    __syncthreads();
+}
 
 // Pass 2: commits merges in target order (single workgroup).  Candidates are
 // kept in an LDS list (the flags of pass 1 plus targets flagged by later
 // updates); the next commit is the smallest listed index at or after the
 // cursor whose flag is still set.  If the list overflows, the remaining
 // commits find their target by scanning the flags.
-__global__ __launch_bounds__(kRepairThreads) void k_eps_repair(TreeDev T, double* x, double* y, double* g, int64_t n,
-                                                               const double* leaf_crit, const int64_t* roff,
-                                                               const int32_t* rlist, int32_t* stamp, int32_t* nn1,
-                                                               int32_t* nn2, double* d1a, double* d2a, uint8_t* eflag,
-                                                               int32_t* ecount, const int32_t* flag_list,
-                                                               uint64_t* box_lo_x, uint64_t* box_lo_y,
-                                                               uint64_t* box_hi_x, uint64_t* box_hi_y) {
+__global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, int64_t n, const int32_t* flag_list) {
     __shared__ int s_q;
-    __shared__ int s_nhard;
-    __shared__ int s_hard[kMaxHard];
-    __shared__ int s_overflow;
     __shared__ int s_np;
     __shared__ int s_pend[kMaxPend];
+    __shared__ CommitShared sh;
+    int32_t* ecount = A.ecount;
     const int tid = threadIdx.x;
-    const int wave = tid / kWave;
     int64_t cursor = 0;
     int commit = 0;
     if (tid == 0) {
-        s_overflow = 0;
+        sh.overflow = 0;
         s_np = min(ecount[0], kMaxPend);
     }
     __syncthreads();
@@ -353,13 +576,13 @@ __global__ __launch_bounds__(kRepairThreads) void k_eps_repair(TreeDev T, double
         if (!scan_mode) {
             for (int k = tid; k < s_np; k += kRepairThreads) {
                 const int i = s_pend[k];
-                if (i >= cursor && (eflag[i] & 2)) atomicMin(&s_q, i);
+                if (i >= cursor && (A.eflag[i] & 2)) atomicMin(&s_q, i);
             }
             __syncthreads();
         } else {
             for (int64_t b = cursor; b < n; b += kRepairThreads) {
                 const int64_t i = b + tid;
-                if (i < n && (eflag[i] & 2)) atomicMin(&s_q, (int)i);
+                if (i < n && (A.eflag[i] & 2)) atomicMin(&s_q, (int)i);
                 __syncthreads();
                 if (s_q != INT32_MAX) break;
                 __syncthreads();
@@ -369,123 +592,13 @@ __global__ __launch_bounds__(kRepairThreads) void k_eps_repair(TreeDev T, double
         const int q = s_q;
         if (q == INT32_MAX) break;
         commit++;
-        const int p = nn1[q];
-        // MergeVortexes(q, p) (MEpsilonFast.cpp:111-126)
-        if (tid == 0) {
-            const double gq = g[q], gp = g[p];
-            if (sgn(gq) == sgn(gp)) {
-                const double inv = 1. / (gq + gp);
-                const double nx = (x[q] * gq + x[p] * gp) * inv;
-                const double ny = (y[q] * gq + y[p] * gp) * inv;
-                x[q] = nx;
-                y[q] = ny;
-            } else if (fabs(gq) < fabs(gp)) {
-                x[q] = x[p];
-                y[q] = y[p];
-            }
-            g[q] = gq + gp;
-            g[p] = 0;
-            ecount[1] += 1;
-            // the merged vortex may leave its leaf's box: grow the box so box
-            // distances stay lower bounds (pruned rescans, diffusive cutoff)
-            const int Lq = T.leaf_node[T.vleaf[q]];
-            if (!isnan(x[q])) {
-                atomicMin((unsigned long long*)&box_lo_x[Lq], (unsigned long long)ord_from_double(x[q]));
-                atomicMax((unsigned long long*)&box_hi_x[Lq], (unsigned long long)ord_from_double(x[q]));
-            }
-            if (!isnan(y[q])) {
-                atomicMin((unsigned long long*)&box_lo_y[Lq], (unsigned long long)ord_from_double(y[q]));
-                atomicMax((unsigned long long*)&box_hi_y[Lq], (unsigned long long)ord_from_double(y[q]));
-            }
-            if (p > q) {   // skipped later by the reference (g == 0)
-                eflag[p] = 0;
-                nn1[p] = -1;
-                nn2[p] = -1;
-            }
-            s_nhard = 0;
-        }
-        __threadfence_block();
-        __syncthreads();
-        // the target itself: search again without merging
-        if (wave == 0) {
-            const Top2 st = rescan_target(T, x, y, g, q);
-            if (lane_id() == 0) {
-                nn1[q] = st.i1;
-                nn2[q] = st.i2;
-                d1a[q] = st.d1;
-                d2a[q] = st.d2;
-                eflag[q] = 1;
-            }
-        }
-        // mark the leaves whose near list contains leaf(q)
-        const int lq = T.vleaf[q], lp = T.vleaf[p];
-        for (int64_t e = roff[lq] + tid; e < roff[lq + 1]; e += kRepairThreads) stamp[rlist[e]] = commit;
-        __threadfence_block();
-        __syncthreads();
-        const double qx = x[q], qy = y[q];
-        // later targets that see leaf(q) (then those that see only leaf(p))
-        for (int pass = 0; pass < 2; pass++) {
-            const int lm = pass == 0 ? lq : lp;
-            if (pass == 1 && lp == lq) break;
-            // 64 leaves at a time, 16 threads per leaf
-            for (int64_t e0 = roff[lm]; e0 < roff[lm + 1]; e0 += kRepairThreads / 16) {
-                const int64_t e = e0 + tid / 16;
-                if (e >= roff[lm + 1]) continue;
-                const int L = rlist[e];
-                if (pass == 1 && stamp[L] == commit) continue;
-                const int Ln = T.leaf_node[L];
-                const int a0 = T.v0[Ln], a1 = T.v1[Ln];
-                for (int i = a0 + (tid & 15); i < a1; i += 16) {
-                    if (i <= q || !(eflag[i] & 1)) continue;
-                    const bool hard = nn1[i] == q || nn1[i] == p || nn2[i] == q || nn2[i] == p;
-                    if (hard) {
-                        const int h = atomicAdd(&s_nhard, 1);
-                        if (h < kMaxHard)
-                            s_hard[h] = i;
-                        else
-                            s_overflow = 1;
-                        continue;
-                    }
-                    if (pass == 1) continue;   // q is not in the near set of i
-                    Top2 st{d1a[i], d2a[i], nn1[i], nn2[i]};
-                    top2_insert(st, dist2(x[i], y[i], qx, qy), q);
-                    nn1[i] = st.i1;
-                    nn2[i] = st.i2;
-                    d1a[i] = st.d1;
-                    d2a[i] = st.d2;
-                    const bool f = merge_flag(g, i, st, leaf_crit[L]);
-                    eflag[i] = 1 | (f ? 2 : 0);
-                    if (f) {
-                        const int k = atomicAdd(&s_np, 1);
-                        if (k < kMaxPend) s_pend[k] = i;
-                    }
-                }
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-        const int nh = min(s_nhard, kMaxHard);
-        for (int h = wave; h < nh; h += kRepairWaves) {
-            const int i = s_hard[h];
-            const Top2 st = rescan_target(T, x, y, g, i);
-            if (lane_id() == 0) {
-                nn1[i] = st.i1;
-                nn2[i] = st.i2;
-                d1a[i] = st.d1;
-                d2a[i] = st.d2;
-                const bool f = merge_flag(g, i, st, leaf_crit[T.vleaf[i]]);
-                eflag[i] = 1 | (f ? 2 : 0);
-                if (f) {
-                    const int k = atomicAdd(&s_np, 1);
-                    if (k < kMaxPend) s_pend[k] = i;
-                }
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
+        commit_one<kRepairThreads>(A, q, commit, sh, [&](int i) {
+            const int k = atomicAdd(&s_np, 1);
+            if (k < kMaxPend) s_pend[k] = i;
+        });
         cursor = q + 1;
     }
-    if (tid == 0 && s_overflow) ecount[2] = 1;
+    if (tid == 0 && sh.overflow) ecount[2] = 1;
 }
 
 // Applies _1_eps = 1/max(eps, floor) to every processed target (:49-54) and
@@ -558,7 +671,7 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
         hipLaunchKernelGGL(k_eps_top2, dim3((nl + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0, s, T, c.x.p,
                            c.y.p, c.g.p, merge, c.leaf_crit.p, c.nn1.p, c.nn2.p, c.d1.p, c.d2.p, c.eflag.p,
                            c.ecount.p, c.flag_list.p);
-    int32_t hc[4] = {0, 0, 0, 0};
+    int32_t hc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 16, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
     if (hc[0] > 0) {
@@ -580,14 +693,17 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
         VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
         hipLaunchKernelGGL(k_rnear_fill, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_off.p, c.rnear_cnt.p,
                            c.rnear_list.p);
-        hipLaunchKernelGGL(k_eps_repair, dim3(1), dim3(kRepairThreads), 0, s, T, c.x.p, c.y.p, c.g.p, n,
-                           c.leaf_crit.p, c.rnear_off.p, c.rnear_list.p, c.leaf_stamp.p, c.nn1.p, c.nn2.p, c.d1.p,
-                           c.d2.p, c.eflag.p, c.ecount.p, c.flag_list.p, c.t.blx.p, c.t.bly.p, c.t.trx.p,
-                           c.t.try_.p);
+        c.leaf_maxd2.grow(nl + 1);
+        hipLaunchKernelGGL(k_leaf_maxd2, dim3(nblocks(nl)), dim3(kBlock), 0, s, T, c.d2.p, c.eflag.p, c.leaf_maxd2.p);
+        const RepairArgs A{T,         c.x.p,         c.y.p,       c.g.p,     c.leaf_crit.p, c.rnear_off.p,
+                           c.rnear_list.p, c.leaf_stamp.p, c.nn1.p, c.nn2.p, c.d1.p,        c.d2.p,
+                           c.eflag.p, c.ecount.p,    c.t.blx.p,   c.t.bly.p, c.t.trx.p,     c.t.try_.p,
+                           c.leaf_maxd2.p};
+        hipLaunchKernelGGL(k_eps_repair, dim3(1), dim3(kRepairThreads), 0, s, A, n, c.flag_list.p);
     }
     hipLaunchKernelGGL(k_eps_apply, dim3(nblocks(n)), dim3(kBlock), 0, s, T, c.g.p, c.ie.p, n, c.nn1.p, c.nn2.p,
                        c.d1.p, c.d2.p, c.eflag.p, c.leaf_restr.p, c.src_ge.p);
-    VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 16, hipMemcpyDeviceToHost, s));
+    VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 32, hipMemcpyDeviceToHost, s));
     if (nn1_out && n) VVHD_HIP(hipMemcpyAsync(nn1_out, c.nn1.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     if (nn2_out && n) VVHD_HIP(hipMemcpyAsync(nn2_out, c.nn2.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
