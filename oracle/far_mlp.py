@@ -115,10 +115,72 @@ def loss_and_grad(p, X, Y, W, mirror=True):
     return loss, g
 
 
-def correction(p, X, scale, ux, uy):
+def correction(p, X, scale, ux, uy, mirror=True):
     """Velocity correction (u, v) of pairs from their features: the network
     output times ysd, scaled and rotated back to the target direction."""
-    y, _ = forward(p, X, mirror=True)
+    y, _ = forward(p, X, mirror=mirror)
     ysd = unpack(p)[8].astype(np.float64)
     a, b = y[:, 0] * ysd[0], y[:, 1] * ysd[1]
     return np.stack([(a * ux - b * uy) * scale, (a * uy + b * ux) * scale], 1)
+
+
+def feature_tangents(X, ux, uy, rinv):
+    """Derivatives of the features with respect to the target position,
+    (dX/dpx, dX/dpy): with theta the angle of u, d theta/dp = (-uy, ux)/r;
+    the rotated dipoles turn with theta (d(f2 + i f3)/d theta = f3 - i f2),
+    the quadrupole twice as fast, d/r scales as 1/r, the rest is constant."""
+    X = np.asarray(X, dtype=np.float64)
+    thx, thy = -uy * rinv, ux * rinv
+    rot = np.zeros_like(X)
+    rot[:, 2], rot[:, 3] = X[:, 3], -X[:, 2]
+    rot[:, 4], rot[:, 5] = X[:, 5], -X[:, 4]
+    rot[:, 6], rot[:, 7] = 2 * X[:, 7], -2 * X[:, 6]
+    TX, TY = rot * thx[:, None], rot * thy[:, None]
+    TX[:, 10] = -X[:, 10] * ux * rinv
+    TY[:, 10] = -X[:, 10] * uy * rinv
+    return TX, TY
+
+
+def forward_jvp(p, X, TX, TY, mirror=True):
+    """Raw outputs and their forward-mode tangents along TX and TY (inputs
+    before standardisation).  mirror=True rounds the tangents to bf16 where
+    the device rounds them: standardised inputs and hidden activations."""
+    W1, b1, W2, b2, W3, b3, mu, sd, _ = unpack(p)
+    if mirror:
+        r = lambda a: bf16(np.asarray(a, dtype=np.float32)).astype(np.float64)
+        xs = r(((np.asarray(X, np.float32) - mu) / sd).astype(np.float32))
+        txs = r(np.asarray(TX, np.float64) / sd)
+        tys = r(np.asarray(TY, np.float64) / sd)
+    else:
+        r = lambda a: np.asarray(a, dtype=np.float64)
+        xs, txs, tys = (X - mu) / sd, TX / sd, TY / sd
+    W1r, W2r, W3r = (r(W) for W in (W1, W2, W3))
+
+    def layer(z, tx, ty):
+        s = 1.0 / (1.0 + np.exp(-z))
+        d = s * (1.0 + z * (1.0 - s))
+        return r(z * s), r(d * tx), r(d * ty)
+
+    h1, t1x, t1y = layer(xs @ W1r.T + b1, txs @ W1r.T, tys @ W1r.T)
+    h2, t2x, t2y = layer(h1 @ W2r.T + b2, t1x @ W2r.T, t1y @ W2r.T)
+    return h2 @ W3r.T + b3, t2x @ W3r.T, t2y @ W3r.T
+
+
+def correction_jvp(p, X, TX, TY, scale, ux, uy, rinv, mirror=True):
+    """Correction (u, v) of pairs at their target points and its gradient
+    with respect to the target position: columns u, v, du/dx, dv/dx, du/dy,
+    dv/dy (network tangents, rotation back to the target frame and the 1/r^3
+    of the scale)."""
+    y, yx, yy = forward_jvp(p, X, TX, TY, mirror)
+    ysd = unpack(p)[8].astype(np.float64)
+    a, b = y[:, 0] * ysd[0], y[:, 1] * ysd[1]
+    U, V = (a * ux - b * uy) * scale, (a * uy + b * ux) * scale
+    out = [U, V]
+    for t, (ta, tb) in ((0, yx.T), (1, yy.T)):
+        u_t = ux if t == 0 else uy
+        th = -uy * rinv if t == 0 else ux * rinv
+        k = -3 * u_t * rinv
+        ga, gb = ta * ysd[0], tb * ysd[1]
+        out.append(k * U - th * V + (ga * ux - gb * uy) * scale)
+        out.append(k * V + th * U + (ga * uy + gb * ux) * scale)
+    return np.stack(out, 1)

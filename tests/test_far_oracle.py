@@ -192,3 +192,76 @@ def test_point_features_match_pair_features():
     ok = sc > 0
     assert np.allclose(u[ok], D[ok] / np.hypot(D[ok, 0], D[ok, 1])[:, None], rtol=1e-14, atol=1e-15)
     assert np.all(u[~ok] == 0)
+
+
+def test_feature_tangents_finite_differences():
+    """Feature derivatives with respect to the target point (the learned
+    path's expansion about the leaf centre) against central differences of
+    the oracle's point features."""
+    o = _case(2000, 10)
+    tg, nd, T = _pairs(o, 9)
+    v = o.vortices()
+    X, sc, u = o.far_point_features(v[tg, 0], v[tg, 1], nd)
+    ok = sc > 0
+    D = v[tg, :2] - T["geom"][nd, :2]
+    rinv = 1.0 / np.hypot(D[:, 0], D[:, 1])
+    TX, TY = M.feature_tangents(X, u[:, 0], u[:, 1], rinv)
+    h = 1e-5 / rinv
+    for k, T_ in ((0, TX), (1, TY)):
+        dp = np.zeros((len(tg), 2))
+        dp[:, k] = h
+        Xp = o.far_point_features(v[tg, 0] + dp[:, 0], v[tg, 1] + dp[:, 1], nd)[0]
+        Xm = o.far_point_features(v[tg, 0] - dp[:, 0], v[tg, 1] - dp[:, 1], nd)[0]
+        fd = (Xp - Xm) / (2 * h[:, None])
+        scale = np.abs(X[ok]).max(0) * rinv[ok].max() + 1e-300
+        assert np.all(np.abs(fd[ok] - T_[ok]) <= 1e-6 * scale + 1e-6 * np.abs(T_[ok]))
+
+
+def test_network_jvp_finite_differences():
+    """Forward-mode tangents of the network (float64) against central
+    differences of its outputs along the tangent direction."""
+    rng = np.random.default_rng(12)
+    p = M.init_params(rng).astype(np.float64)
+    p[M.O_B1:M.O_W2] = rng.normal(0, 0.3, M.N_HID)
+    X = rng.normal(size=(50, 12))
+    TX, TY = rng.normal(size=(50, 12)), rng.normal(size=(50, 12))
+    y, yx, yy = M.forward_jvp(p, X, TX, TY, mirror=False)
+    eps = 1e-6
+    for T_, yt in ((TX, yx), (TY, yy)):
+        fd = (M.forward(p, X + eps * T_, mirror=False)[0] - M.forward(p, X - eps * T_, mirror=False)[0]) / (2 * eps)
+        assert np.allclose(fd, yt, rtol=1e-6, atol=1e-8)
+    assert np.allclose(y, M.forward(p, X, mirror=False)[0], rtol=1e-13, atol=1e-15)
+
+
+def test_correction_gradient_finite_differences():
+    """The correction's gradient with respect to the target position
+    (network tangents, rotation, 1/r^3 scale) against central differences of
+    the correction at shifted points (float64 network)."""
+    o = _case(2000, 11)
+    tg, nd, T = _pairs(o, 13)
+    v = o.vortices()
+    rng = np.random.default_rng(5)
+    X0, sc0, u0 = o.far_point_features(v[tg, 0], v[tg, 1], nd)
+    ok = sc0 > 0
+    tg, nd = tg[ok], nd[ok]
+    p = M.init_params(rng, mu=X0[ok].mean(0), sd=X0[ok].std(0) + 1e-6).astype(np.float64)
+    X, sc, u = o.far_point_features(v[tg, 0], v[tg, 1], nd)
+    D = v[tg, :2] - T["geom"][nd, :2]
+    rinv = 1.0 / np.hypot(D[:, 0], D[:, 1])
+    TX, TY = M.feature_tangents(X, u[:, 0], u[:, 1], rinv)
+    C = M.correction_jvp(p, X, TX, TY, sc, u[:, 0], u[:, 1], rinv, mirror=False)
+    assert np.allclose(C[:, :2], M.correction(p, X, sc, u[:, 0], u[:, 1], mirror=False), rtol=1e-12, atol=0)
+    h = 1e-6 / rinv
+    for k in (0, 1):
+        dp = np.zeros((len(tg), 2))
+        dp[:, k] = h
+        cp = M.correction(p, *_point(o, v[tg, :2] + dp, nd), mirror=False)
+        cm = M.correction(p, *_point(o, v[tg, :2] - dp, nd), mirror=False)
+        fd = (cp - cm) / (2 * h[:, None])
+        ref = np.abs(C[:, 2 + 2 * k:4 + 2 * k]).max()
+        assert np.max(np.abs(fd - C[:, 2 + 2 * k:4 + 2 * k])) < 1e-5 * ref
+
+
+def _point(o, P, nd):
+    X, sc, u = o.far_point_features(P[:, 0], P[:, 1], nd)
+    return X, sc, u[:, 0], u[:, 1]

@@ -9,8 +9,8 @@ difference whose sums run in another order); network outputs within 1e-3
 (relative L2) of the oracle with the same bf16 rounding points (the device's
 exp/rcp and FP32 accumulation order can flip a bf16 rounding); gradients
 within 1e-2 per layer; learned-path velocities within 5e-6 (monopole part,
-FP32 Kahan) and 1e-5 (with a network: corrections at three points per leaf,
-interpolated to the targets) relative L2 of the oracle's.
+FP32 Kahan) and 1e-5 (with a network: correction and its gradient at the
+leaf centre, expanded to the targets) relative L2 of the oracle's.
 """
 import os
 import sys
@@ -144,8 +144,8 @@ def test_training_reduces_loss(ctx, case):
 def _oracle_learned(o, targets, p=None):
     """Induced velocity of the learned path for the given targets, on the
     oracle: exact near field + per far node monopoles at the target + the
-    network correction of the far nodes at the leaf's three points (centre,
-    centre + (w/2, 0), centre + (0, h/2)), interpolated linearly."""
+    network correction of the far nodes at the leaf centre, expanded to
+    first order in the target's offset from it (forward-mode gradient)."""
     T = o.tree()
     v = o.vortices()
     noff, nlst = T["near"]
@@ -166,15 +166,14 @@ def _oracle_learned(o, targets, p=None):
         _, _, _, base, _ = o.far_pairs(np.full(len(far), t), far)
         out[q] += base.sum(0)
         if p is not None and len(far):
-            cx, cy, h, w = geom[T["leaves"][l]]
-            C = []
-            for px, py in ((cx, cy), (cx + w / 2, cy), (cx, cy + h / 2)):
-                X, sc, u = o.far_point_features(np.full(len(far), px), np.full(len(far), py), far)
-                ok = sc > 0
-                C.append(M.correction(p, X[ok].astype(np.float32), sc[ok], u[ok, 0], u[ok, 1]).sum(0))
-            ax = (v[t, 0] - cx) / (w / 2) if w > 0 else 0.0
-            ay = (v[t, 1] - cy) / (h / 2) if h > 0 else 0.0
-            out[q] += C[0] + ax * (C[1] - C[0]) + ay * (C[2] - C[0])
+            cx, cy = geom[T["leaves"][l], :2]
+            X, sc, u = o.far_point_features(np.full(len(far), cx), np.full(len(far), cy), far)
+            ok = sc > 0
+            D = np.array([cx, cy]) - geom[far, :2]
+            rinv = 1.0 / np.hypot(D[:, 0], D[:, 1])
+            TX, TY = M.feature_tangents(X, u[:, 0], u[:, 1], rinv)
+            C = M.correction_jvp(p, X[ok], TX[ok], TY[ok], sc[ok], u[ok, 0], u[ok, 1], rinv[ok]).sum(0)
+            out[q] += C[:2] + (v[t, 0] - cx) * C[2:4] + (v[t, 1] - cy) * C[4:6]
     return out
 
 

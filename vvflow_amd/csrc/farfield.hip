@@ -14,9 +14,9 @@
 //     the target offset: 12 inputs, two hidden layers of 64 SiLU units, 2
 //     outputs, bf16 weights and activations with FP32 accumulation on the
 //     matrix cores (v_mfma_f32_32x32x16_bf16).  The network runs per
-//     (leaf, far node) at three points of the leaf, and each target takes
-//     the linear interpolation of the three corrections (k_far_learned,
-//     k_far_mono).
+//     (leaf, far node) at the leaf centre with forward-mode tangents, and
+//     each target takes the first-order expansion of the correction about
+//     the centre (k_far_learned, k_far_mono).
 // The MLP's target is the residual between the direct Lamb-kernel sum over
 // the node's vortices (FP64) and the monopole field, expressed in the frame
 // of the target direction and scaled by S d^2 / (2 pi r^3) (S = |G+| + |G-|,
@@ -433,24 +433,92 @@ __device__ inline void mlp_infer(const Weights& W, const float* sb1, const float
     y1 = c[1];
 }
 
-// Network part of the learned far field, one wave per leaf: the correction
-// of every far node of the leaf evaluated at three points of the leaf -- its
-// centre c, c + (w/2, 0) and c + (0, h/2) (w, h the leaf's extent in x and
-// y) -- and summed over the far list.  k_far_mono gives each target the
-// linear interpolation of the three sums, so the network runs three times per
-// (leaf, far node) instead of once per (target, far node); the correction
-// varies slowly over a leaf, which lies far from the node.
-// A tile holds 10 far nodes x 3 points in columns 0..29 (column n: node n/3,
-// point n%3; columns 30, 31 idle).  Lane group g (lanes 32g..32g+31) builds
-// inputs 8g..8g+7 of column l%32.  Output per leaf: 6 doubles (u, v at the
-// three points); zeros for a leaf without far nodes or without a target of
-// non-zero circulation.
-constexpr int kFarPerTile = 10;
+// Inference forward of one tile with two forward-mode tangents: the tangent
+// inputs tx, ty (standardised by the deviations only) pass through the same
+// weights, and each SiLU scales them by its derivative s (1 + z (1 - s)),
+// s = sigmoid(z); tangents are rounded to bf16 where the activations are.
+// Outputs (value, x tangent, y tangent) in lanes 0..31.  The weight
+// fragments come from LDS (sw: the W1, W2, W3 sets, 64 lanes each).
+__device__ inline void mlp_infer_jvp(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
+                                     bf16x8 x, bf16x8 tx, bf16x8 ty, float (&y)[2], float (&yx)[2],
+                                     float (&yy)[2]) {
+    const int g = lane_id() >> 5, ln = lane_id();
+    auto A1 = [&](int b) { return sw[(fA1 / kSet + b) * kWave + ln]; };
+    auto A2 = [&](int k) { return sw[(fA2 / kSet + k) * kWave + ln]; };
+    auto A3 = [&](int k) { return sw[(fA3 / kSet + k) * kWave + ln]; };
+    bf16x8 h[4], hx[4], hy[4];
+    for (int b = 0; b < 2; b++) {
+        f32x16 c, cx = {}, cy = {};
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) c[4 * j + i] = sb1[32 * b + 8 * j + 4 * g + i];
+        const bf16x8 a = A1(b);
+        c = mfma32(a, x, c);
+        cx = mfma32(a, tx, cx);
+        cy = mfma32(a, ty, cy);
+        for (int e = 0; e < 16; e++) {
+            const float z = c[e], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
+            h[2 * b + (e >> 3)][e & 7] = (__bf16)(z * sg);
+            hx[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cx[e]);
+            hy[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cy[e]);
+        }
+    }
+    bf16x8 h2[4], h2x[4], h2y[4];
+    for (int b = 0; b < 2; b++) {
+        f32x16 c, cx = {}, cy = {};
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) c[4 * j + i] = sb2[32 * b + 8 * j + 4 * g + i];
+        for (int st = 0; st < 4; st++) {
+            const bf16x8 a = A2(4 * b + st);
+            c = mfma32(a, h[st], c);
+            cx = mfma32(a, hx[st], cx);
+            cy = mfma32(a, hy[st], cy);
+        }
+        for (int e = 0; e < 16; e++) {
+            const float z = c[e], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
+            h2[2 * b + (e >> 3)][e & 7] = (__bf16)(z * sg);
+            h2x[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cx[e]);
+            h2y[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cy[e]);
+        }
+    }
+    f32x16 c = {}, cx = {}, cy = {};
+    c[0] = g == 0 ? b30 : 0.f;
+    c[1] = g == 0 ? b31 : 0.f;
+    for (int st = 0; st < 4; st++) {
+        const bf16x8 a = A3(st);
+        c = mfma32(a, h2[st], c);
+        cx = mfma32(a, h2x[st], cx);
+        cy = mfma32(a, h2y[st], cy);
+    }
+    y[0] = c[0];
+    y[1] = c[1];
+    yx[0] = cx[0];
+    yx[1] = cx[1];
+    yy[0] = cy[0];
+    yy[1] = cy[1];
+}
 
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_far_learned(
+// Network part of the learned far field, one wave per leaf: for every far
+// node of the leaf, the correction at the leaf centre c and its gradient
+// with respect to the target position (forward-mode tangents through the
+// network, and the derivatives of the features, of the rotation back to
+// the target frame and of the scale S d^2 / (2 pi r^3)), summed over the far
+// list.  k_far_mono gives each target t the first-order expansion
+// C + (t - c) . grad C, so the network runs once per (leaf, far node)
+// instead of once per (target, far node); the correction varies slowly over
+// a leaf, which lies far from the node.
+// Feature derivatives: with theta the angle of u, d theta / dp = (-uy, ux)/r;
+// the rotated moments turn with theta (d(f2 + i f3)/d theta = f3 - i f2, the
+// quadrupole twice as fast), d/r scales as 1/r, the rest is constant.
+// A tile holds 32 far nodes (column l%32); lane group g (lanes 32g..32g+31)
+// builds inputs 8g..8g+7 and their tangents.  Output per leaf: 6 doubles
+// (u, v, du/dx, dv/dx, du/dy, dv/dy); zeros for a leaf without far nodes or
+// without a target of non-zero circulation.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) void k_far_learned(
     TreeDev T, const FarNode* __restrict__ nodes, const double* __restrict__ g, const uint16_t* frag, const float* P,
     double* __restrict__ leaf_corr) {
+    constexpr int kFwdSets = fA3T / kSet;   // W1, W2, W3 fragment sets
     __shared__ float sb1[kHid], sb2[kHid], smu[16], sisd[16];
+    __shared__ uint4 sw[kFwdSets * kWave];
     for (int k = threadIdx.x; k < kHid; k += blockDim.x) {
         sb1[k] = P[oB1 + k];
         sb2[k] = P[oB2 + k];
@@ -459,6 +527,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             sisd[k] = k < kFeat ? 1.f / P[oSd + k] : 0.f;
         }
     }
+    for (int k = threadIdx.x; k < kFwdSets * kWave; k += blockDim.x) sw[k] = reinterpret_cast<const uint4*>(frag)[k];
     __syncthreads();
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
@@ -474,27 +543,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         if (lane < 6) out[lane] = 0.;
         return;
     }
-    Weights W;
-    load_weights(frag, W);
+    const bf16x8* swf = reinterpret_cast<const bf16x8*>(sw);
     const float b30 = P[oB3], b31 = P[oB3 + 1];
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
-    const int j = n / 3, pt = n % 3;
-    const bool col = n < 3 * kFarPerTile;
-    const double px = T.nx[L] + (pt == 1 ? 0.5 * T.nw[L] : 0.);
-    const double py = T.ny[L] + (pt == 2 ? 0.5 * T.nh[L] : 0.);
-    float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;   // Kahan sums (lanes 0..29)
+    const double px = T.nx[L], py = T.ny[L];
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, cmp[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // Kahan sums
     // the node record of the next tile is loaded while this tile computes
-    FarNode nd = nodes[(col && fb + j < fe) ? T.far_list[fb + j] : 0];
-    for (int64_t f0 = fb; f0 < fe; f0 += kFarPerTile) {
-        const int64_t e = f0 + j;
-        const bool valid = col && e < fe;
-        const int64_t en = e + kFarPerTile;
-        const FarNode nxt = nodes[(col && en < fe) ? T.far_list[en] : 0];
+    FarNode nd = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
+    for (int64_t f0 = fb; f0 < fe; f0 += 32) {
+        const int64_t e = f0 + n;
+        const int64_t en = e + 32;
+        const FarNode nxt = nodes[(en < fe) ? T.far_list[en] : 0];
         const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
         const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
         const float ux = dx * rinv, uy = dy * rinv;
-        const bool ok = valid && nd.ok != 0.f;
-        float f[8];
+        const float thx = -uy * rinv, thy = ux * rinv;   // d theta / dp
+        const bool ok = e < fe && nd.ok != 0.f;
+        float f[8], fx[8], fy[8];
         if (grp == 0) {
             const float c2r = ux * ux - uy * uy, c2i = -2.f * ux * uy;
             f[0] = nd.f0;
@@ -505,48 +570,63 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             f[5] = nd.dmy * ux - nd.dmx * uy;
             f[6] = nd.qqx * c2r - nd.qqy * c2i;
             f[7] = nd.qqx * c2i + nd.qqy * c2r;
+            const float rot[8] = {0.f, 0.f, f[3], -f[2], f[5], -f[4], 2.f * f[7], -2.f * f[6]};
+            for (int q = 0; q < 8; q++) {
+                fx[q] = rot[q] * thx;
+                fy[q] = rot[q] * thy;
+            }
         } else {
             f[0] = nd.e8;
             f[1] = nd.e9;
             f[2] = nd.sd * rinv;
             f[3] = nd.asp;
             f[4] = f[5] = f[6] = f[7] = 0.f;
+            for (int q = 0; q < 8; q++) fx[q] = fy[q] = 0.f;
+            fx[2] = -f[2] * (ux * rinv);
+            fy[2] = -f[2] * (uy * rinv);
         }
-        bf16x8 xin;
-        for (int q = 0; q < 8; q++) xin[q] = (__bf16)(ok ? (f[q] - smu[8 * grp + q]) * sisd[8 * grp + q] : 0.f);
-        float y0, y1;
-        mlp_infer(W, sb1, sb2, b30, b31, xin, y0, y1);
+        bf16x8 xin, txin, tyin;
+        for (int q = 0; q < 8; q++) {
+            const float is = sisd[8 * grp + q];
+            xin[q] = (__bf16)(ok ? (f[q] - smu[8 * grp + q]) * is : 0.f);
+            txin[q] = (__bf16)(ok ? fx[q] * is : 0.f);
+            tyin[q] = (__bf16)(ok ? fy[q] * is : 0.f);
+        }
+        float y[2], yx[2], yy[2];
+        mlp_infer_jvp(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
         if (grp == 0 && ok) {
             const float sc = nd.scl * rinv * rinv * rinv;
-            const float a = y0 * ysd0, b = y1 * ysd1;
-            const float u = (a * ux - b * uy) * sc, v = (a * uy + b * ux) * sc;
-            float yk = u - cu, tk = su + yk;
-            cu = (tk - su) - yk;
-            su = tk;
-            yk = v - cv;
-            tk = sv + yk;
-            cv = (tk - sv) - yk;
-            sv = tk;
+            const float a = y[0] * ysd0, b = y[1] * ysd1;
+            const float U = (a * ux - b * uy) * sc, V = (a * uy + b * ux) * sc;
+            const float ax = yx[0] * ysd0, bx = yx[1] * ysd1, ay = yy[0] * ysd0, by = yy[1] * ysd1;
+            const float kx = -3.f * ux * rinv, ky = -3.f * uy * rinv;
+            float t[6];
+            t[0] = U;
+            t[1] = V;
+            t[2] = kx * U - thx * V + (ax * ux - bx * uy) * sc;
+            t[3] = kx * V + thx * U + (ax * uy + bx * ux) * sc;
+            t[4] = ky * U - thy * V + (ay * ux - by * uy) * sc;
+            t[5] = ky * V + thy * U + (ay * uy + by * ux) * sc;
+            for (int q = 0; q < 6; q++) {
+                const float yk = t[q] - cmp[q], tk = acc[q] + yk;
+                cmp[q] = (tk - acc[q]) - yk;
+                acc[q] = tk;
+            }
         }
         nd = nxt;
     }
-    // per point, the sum over its ten columns in column order
-    const double du = (double)su - (double)cu, dv = (double)sv - (double)cv;
-    double tu = 0., tv = 0.;
-    for (int m = 0; m < kFarPerTile; m++) {
-        tu += __shfl(du, (lane % 3) + 3 * m, kWave);
-        tv += __shfl(dv, (lane % 3) + 3 * m, kWave);
-    }
-    if (lane < 3) {
-        out[2 * lane] = tu;
-        out[2 * lane + 1] = tv;
+    // sums over the 32 columns (lanes 0..31), FP64, fixed order
+    for (int q = 0; q < 6; q++) {
+        double v = grp == 0 ? (double)acc[q] - (double)cmp[q] : 0.;
+        for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        if (lane == 0) out[q] = v;
     }
 }
 
 // Far field of every vortex by the learned path: the exact field at the
 // vortex of the far nodes' per-sign monopoles (FP32 terms from FP64
 // differences, Kahan sums, lane partials combined in FP64), plus the leaf's
-// network correction interpolated linearly from its three points.  One wave
+// network correction expanded to first order about the leaf centre.  One wave
 // per leaf; targets in chunks of up to 16 padded to a power of two, the far
 // list split over 64 / npad lanes per target.
 __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* __restrict__ nodes,
@@ -559,9 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
-    const double cx = T.nx[L], cy = T.ny[L], hx = 0.5 * T.nw[L], hy = 0.5 * T.nh[L];
+    const double cx = T.nx[L], cy = T.ny[L];
     const double* C = leaf_corr + 6 * (size_t)l;
-    const double c0u = C[0], c0v = C[1], c1u = C[2], c1v = C[3], c2u = C[4], c2v = C[5];
+    const double c0u = C[0], c0v = C[1], cxu = C[2], cxv = C[3], cyu = C[4], cyv = C[5];
     for (int base = v0; base < v1; base += 16) {
         const int ntc = min(16, v1 - base);
         const int npad = ntc <= 1 ? 1 : ntc <= 2 ? 2 : ntc <= 4 ? 4 : ntc <= 8 ? 8 : 16;
@@ -601,9 +681,9 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
             dv += __shfl_xor(dv, o, kWave);
         }
         if (fo == 0 && tvalid) {
-            const double ax = hx > 0 ? (tx - cx) / hx : 0., ay = hy > 0 ? (ty - cy) / hy : 0.;
-            du += c0u + ax * (c1u - c0u) + ay * (c2u - c0u);
-            dv += c0v + ax * (c1v - c0v) + ay * (c2v - c0v);
+            const double ox = tx - cx, oy = ty - cy;
+            du += c0u + ox * cxu + oy * cyu;
+            dv += c0v + ox * cxv + oy * cyv;
             far_out[base + k] = make_double2(du, dv);
         }
     }
