@@ -1,13 +1,14 @@
 // diffusive.hip -- MDiffusiveFast::process_vort_list (MDiffusiveFast.cpp:
 // 8-48, vortex_influence :93-105, segment_influence :107-123).
 //
-// One wave per leaf, lane = (target k of 16, slot s of 4).  For target i the
-// vortex terms run over same-sign vortices j of the near leaves with
+// One wave per leaf and chunk of up to 16 targets.  For target i the vortex
+// terms run over same-sign vortices j of the near leaves with
 // |r_i - r_j| * (1/eps_i) <= 8 (the reference's exp-argument cutoff), so near
-// leaves whose box lies farther than 8 eps from the whole target leaf are
-// skipped without loss.  Vortex sums (S1, S2) are FP32 with Kahan
-// compensation in leaf-local coordinates; body-segment terms (S0, S3) are
-// FP64.  The clamp and the velocity update are FP64.
+// leaves whose box lies farther than 8 eps from the whole chunk are skipped
+// without loss; the sources of the others are staged in LDS and shared by
+// all lanes.  Vortex sums (S1, S2) are FP32 with Kahan compensation in
+// leaf-local coordinates; body-segment terms (S0, S3) are FP64.  The clamp
+// and the velocity update are FP64.
 //
 // Segment friction (segment_influence's fric term, :120-122) runs as its own
 // pass so that every segment adds its contributions in the reference's
@@ -50,94 +51,175 @@ struct Kahan {
     __device__ inline double value() const { return (double)s - (double)c; }
 };
 
-__device__ inline double slot_sum(double v) {
-    v += __shfl_xor(v, 16, kWave);
-    v += __shfl_xor(v, 32, kWave);
-    return v;
+// Sum over the slots of target t (lanes t, t + ntc, ..., in slot order).
+__device__ inline double slot_sum(double v, int t, int ntc, int nslot) {
+    double s = 0;
+    for (int q = 0; q < nslot; q++) s += __shfl(v, t + ntc * q, kWave);
+    return s;
 }
 
+constexpr int kStage = 512;   // staged sources per wave
+
+// One wave per leaf and chunk of up to 16 targets.  The sources of the near
+// leaves within the cutoff are staged in LDS (FP32, relative to the leaf
+// centre; circulation as FP32), then lane (target t = l % ntc, slot
+// s = l / ntc) runs over every nslot-th staged source: all 64 lanes work
+// whatever the leaf's occupancy.  Per pair: same sign, |dr|_1 >= 1e-10,
+// |dr| / eps_i <= 8; |dr| and 1/|dr| from one reciprocal square root.
 __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ g,
-                                                      const double* __restrict__ ie, const vvhd_seg* segs, double* vx,
-                                                      double* vy, double re) {
+                                                      const double* __restrict__ ie, const vvhd_seg* segs, int nseg,
+                                                      double* vx, double* vy, double re) {
+    __shared__ float s_x[kWavesPerBlock][kStage], s_y[kWavesPerBlock][kStage], s_g[kWavesPerBlock][kStage];
+    __shared__ int s_off[kWavesPerBlock][kWave], s_j0[kWavesPerBlock][kWave];
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
     const int lane = lane_id();
-    const int k = lane & 15, slot = lane >> 4;
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     if (v0 == v1) return;
     const double cx = T.nx[L], cy = T.ny[L];
     const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
+    float* sx = s_x[w];
+    float* sy = s_y[w];
+    float* sg = s_g[w];
+    int* soff = s_off[w];
+    int* sj0 = s_j0[w];
 
     for (int base = v0; base < v1; base += 16) {
-        const int i = base + k;
-        const bool valid = i < v1;
-        const double gi = valid ? g[i] : 0;
-        const bool active = valid && gi != 0;
-        const double iei = active ? ie[i] : 0;
-        const double xi = valid ? x[i] : 0, yi = valid ? y[i] : 0;
+        const int ntc = min(16, v1 - base);
+        const int nslot = kWave / ntc;
+        const int t = lane % ntc, slot = lane / ntc;
+        const int i = base + t;
+        const double gi = g[i];
+        const bool active = gi != 0 && slot < nslot;
+        const double iei = gi != 0 ? ie[i] : 0;
+        const double xi = x[i], yi = y[i];
         const float tx = (float)(xi - cx), ty = (float)(yi - cy);
         const float ief = (float)iei;
-        const double si = sgn(gi);
+        const bool pos = gi > 0;
         // cutoff radius of the chunk: 8 eps of its targets
-        double R = active ? (-kExpArgLimit) / iei : 0.0;
+        double R = gi != 0 ? (-kExpArgLimit) / iei : 0.0;
         R = wave_max(R);
         const double R2 = R * R * (1.0 + 1e-6);
         Kahan s2x, s2y, s1;
-        double s3x = 0, s3y = 0, s0 = 0;
-        for (int64_t eb = e0; eb < e1; eb += kWave) {
-            const int64_t e = eb + lane;
-            int M = -1;
-            bool q = false;
-            if (e < e1) {
-                M = T.near_list[e];
-                q = box_dist2(T, L, M) <= R2;
-            }
-            uint64_t mask = __ballot(q);
-            while (mask) {
-                const int b = (int)__builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int Mb = __shfl(M, b, kWave);
-                const int j0 = T.v0[Mb], j1 = T.v1[Mb];
-                for (int j = j0 + slot; j < j1; j += 4) {
-                    const double gj = g[j];
-                    if (!active || gj == 0 || sgn(gj) != si) continue;
-                    const float dx = tx - (float)(x[j] - cx);
-                    const float dy = ty - (float)(y[j] - cy);
+        auto compute = [&](int nsrc) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (active) {
+                for (int k = slot; k < nsrc; k += nslot) {
+                    const float gj = sg[k];
+                    if (gj == 0.f || (gj > 0.f) != pos) continue;
+                    const float dx = tx - sx[k];
+                    const float dy = ty - sy[k];
                     if (fabsf(dx) + fabsf(dy) < 1e-10f) continue;
-                    const float drabs = sqrtf(__builtin_fmaf(dx, dx, dy * dy));
-                    const float exparg = -drabs * ief;
+                    const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+                    const float rinv = __builtin_amdgcn_rsqf(d2);
+                    const float exparg = -(d2 * rinv) * ief;
                     if (exparg < (float)kExpArgLimit) continue;
-                    const float i1 = (float)gj * __expf(exparg);
-                    const float f = i1 / drabs;
+                    const float i1 = gj * __expf(exparg);
+                    const float f = i1 * rinv;
                     s2x.add(dx * f);
                     s2y.add(dy * f);
                     s1.add(i1);
                 }
-                // body segments of that leaf (FP64)
-                const int m = T.leaf_rank[Mb];
-                for (int64_t qq = T.seg_off[m] + slot; qq < T.seg_off[m + 1]; qq += 4) {
-                    if (!active) continue;
-                    const int sid = T.seg_list[qq];
-                    const vvhd_seg sg = segs[sid];
-                    const double drx = xi - sg.x, dry = yi - sg.y;
-                    const double drabs2 = drx * drx + dry * dry;
-                    const double drabs = sqrt(drabs2);
-                    const double exparg = -drabs * iei;
-                    if (exparg < kExpArgLimit) continue;
-                    const double expres = exp(exparg);
-                    const double dSx = -sg.dly, dSy = sg.dlx;
-                    s3x += dSx * expres;
-                    s3y += dSy * expres;
-                    s0 += (drabs * iei + 1) / drabs2 * (drx * dSx + dry * dSy) * expres;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        int fill = 0;
+        for (int64_t eb = e0; eb < e1; eb += kWave) {
+            const int64_t e = eb + lane;
+            int cnt = 0, j0 = 0;
+            if (e < e1) {
+                const int M = T.near_list[e];
+                if (box_dist2(T, L, M) <= R2) {
+                    j0 = T.v0[M];
+                    cnt = T.v1[M] - j0;
+                }
+            }
+            const int incl = wave_incl_scan(cnt);
+            const int off = incl - cnt;
+            const int tot = __shfl(incl, kWave - 1, kWave);
+            if (tot == 0) continue;
+            soff[lane] = off;
+            sj0[lane] = j0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int done = 0; done < tot;) {
+                const int take = min(tot - done, kStage - fill);
+                for (int q = lane; q < take; q += kWave) {
+                    const int gidx = done + q;
+                    int lo = 0, hi = kWave - 1;   // last lane whose offset <= gidx
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (soff[mid] <= gidx)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const int j = sj0[lo] + (gidx - soff[lo]);
+                    sx[fill + q] = (float)(x[j] - cx);
+                    sy[fill + q] = (float)(y[j] - cy);
+                    sg[fill + q] = (float)g[j];
+                }
+                fill += take;
+                done += take;
+                if (fill == kStage) {
+                    compute(fill);
+                    fill = 0;
+                }
+            }
+            // the next chunk rewrites the lane table
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (fill) compute(fill);
+        // body segments of the near leaves within the cutoff (FP64)
+        double s3x = 0, s3y = 0, s0 = 0;
+        if (nseg) {
+            for (int64_t eb = e0; eb < e1; eb += kWave) {
+                const int64_t e = eb + lane;
+                int M = -1;
+                bool q = false;
+                if (e < e1) {
+                    M = T.near_list[e];
+                    const int m = T.leaf_rank[M];
+                    q = T.seg_off[m + 1] > T.seg_off[m] && box_dist2(T, L, M) <= R2;
+                }
+                uint64_t mask = __ballot(q);
+                while (mask) {
+                    const int b = (int)__builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const int m = T.leaf_rank[__shfl(M, b, kWave)];
+                    for (int64_t qq = T.seg_off[m] + slot; qq < T.seg_off[m + 1]; qq += nslot) {
+                        if (!active) continue;
+                        const int sid = T.seg_list[qq];
+                        const vvhd_seg sgm = segs[sid];
+                        const double drx = xi - sgm.x, dry = yi - sgm.y;
+                        const double drabs2 = drx * drx + dry * dry;
+                        const double drabs = sqrt(drabs2);
+                        const double exparg = -drabs * iei;
+                        if (exparg < kExpArgLimit) continue;
+                        const double expres = exp(exparg);
+                        const double dSx = -sgm.dly, dSy = sgm.dlx;
+                        s3x += dSx * expres;
+                        s3y += dSy * expres;
+                        s0 += (drabs * iei + 1) / drabs2 * (drx * dSx + dry * dSy) * expres;
+                    }
                 }
             }
         }
-        double S2x = slot_sum(s2x.value()), S2y = slot_sum(s2y.value()), S1 = slot_sum(s1.value());
-        double S3x = slot_sum(s3x), S3y = slot_sum(s3y), S0 = slot_sum(s0);
-        if (slot == 0 && active) {
+        double S2x = slot_sum(s2x.value(), t, ntc, nslot), S2y = slot_sum(s2y.value(), t, ntc, nslot);
+        double S1 = slot_sum(s1.value(), t, ntc, nslot);
+        double S3x = slot_sum(s3x, t, ntc, nslot), S3y = slot_sum(s3y, t, ntc, nslot), S0 = slot_sum(s0, t, ntc, nslot);
+        if (lane < ntc && gi != 0) {
+            const double si = sgn(gi);
             if ((sgn(S1) != si) || (fabs(S1) < fabs(0.1 * gi))) S1 = 0.1 * gi;
             double ax = vx[i], ay = vy[i];
             const double c1 = iei / (re * S1);
@@ -227,9 +309,9 @@ void diffusive(Ctx& c) {
     if (T.nleaf == 0) return;
     const int nl = T.nleaf;
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(k_diffusive, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p,
-                       c.vx.p, c.vy.p, c.re);
     const int nseg = (int)c.segs_h.size();
+    hipLaunchKernelGGL(k_diffusive, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p,
+                       nseg, c.vx.p, c.vy.p, c.re);
     if (nseg) {
         hipStream_t s = c.stream;
         c.iscratch0.grow(nl + 1);
