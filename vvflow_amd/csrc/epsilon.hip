@@ -148,76 +148,134 @@ __global__ void k_leaf_criteria(TreeDev T, const vvhd_seg* segs, const int32_t* 
     restr_out[l] = restr;
 }
 
-// Scans the sources of the near leaves of leaf l whose box distance d2b to
-// the leaf satisfies lo < d2b <= hi (lo = -1: include touching leaves) for
-// the 16 targets of the current chunk.  Lane = (target k, slot s): slot s
-// visits every 4th source of a leaf.
-__device__ inline void scan_sources(const TreeDev& T, int l, int L, const double* x, const double* y, const double* g,
-                                    double lo, double hi, int slot, bool active, int i, double tx, double ty,
-                                    Top2& st) {
-    const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
-    for (int64_t eb = e0; eb < e1; eb += kWave) {
-        const int64_t e = eb + lane_id();
-        int M = -1;
-        bool q = false;
-        if (e < e1) {
-            M = T.near_list[e];
-            const double bd = box_dist2(T, L, M);
-            q = (bd > lo) && (bd <= hi);
-        }
-        uint64_t mask = __ballot(q);
-        while (mask) {
-            const int b = (int)__builtin_ctzll(mask);
-            mask &= mask - 1;
-            const int Mb = __shfl(M, b, kWave);
-            const int s0 = T.v0[Mb], s1 = T.v1[Mb];
-            for (int j = s0 + slot; j < s1; j += 4) {
-                const double gj = g[j];
-                if (active && gj != 0 && j != i) top2_insert(st, dist2(tx, ty, x[j], y[j]), j);
-            }
-        }
-    }
-}
+constexpr int kStageE = 384;   // staged sources per wave (pass 1)
 
-// Pass 1: top-2 of every target on the unmodified state (one wave per leaf).
+// Pass 1: top-2 of every target on the unmodified state, one wave per leaf
+// and chunk of up to 16 targets.  The sources (g != 0) of the selected near
+// leaves are staged in LDS, FP64, and lane (target t = l % ntc, slot
+// s = l / ntc) runs over every nslot-th of them; the slots' top-2s combine
+// afterwards (the top-2 under (d^2, index) does not depend on the order).
+// First the leaves touching the target leaf, then those whose box lies
+// within the largest second-neighbour distance found.
 __global__ __launch_bounds__(kBlock) void k_eps_top2(TreeDev T, const double* x, const double* y, const double* g,
                                                      int merge, const double* leaf_crit, int32_t* nn1, int32_t* nn2,
                                                      double* d1o, double* d2o, uint8_t* eflag, int32_t* ecount,
                                                      int32_t* flag_list) {
+    __shared__ double s_x[kWavesPerBlock][kStageE], s_y[kWavesPerBlock][kStageE];
+    __shared__ int s_j[kWavesPerBlock][kStageE];
+    __shared__ int s_off[kWavesPerBlock][kWave], s_j0[kWavesPerBlock][kWave];
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
     const int lane = lane_id();
-    const int k = lane & 15, slot = lane >> 4;
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const double crit = leaf_crit[l];
+    const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
+    double* sx = s_x[w];
+    double* sy = s_y[w];
+    int* sj = s_j[w];
+    int* soff = s_off[w];
+    int* sj0 = s_j0[w];
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     for (int base = v0; base < v1; base += 16) {
-        const int i = base + k;
-        const bool valid = i < v1;
-        const double tx = valid ? x[i] : 0, ty = valid ? y[i] : 0, tg = valid ? g[i] : 0;
-        const bool active = valid && tg != 0;
+        const int ntc = min(16, v1 - base);
+        const int nslot = kWave / ntc;
+        const int t = lane % ntc, slot = lane / ntc;
+        const int i = base + t;
+        const double tx = x[i], ty = y[i], tg = g[i];
+        const bool active = tg != 0 && slot < nslot;
         Top2 st = top2_empty();
-        scan_sources(T, l, L, x, y, g, -1.0, 0.0, slot, active, i, tx, ty, st);
-        // combine the 4 slots of each target into a bound
-        Top2 c = st;
-        top2_merge(c, top2_shfl_xor(c, 16));
-        top2_merge(c, top2_shfl_xor(c, 32));
-        double bound = active ? (c.i2 >= 0 ? c.d2 : INFINITY) : -INFINITY;
+        auto compute = [&](int nsrc) {
+            wave_sync();
+            if (active)
+                for (int k = slot; k < nsrc; k += nslot) {
+                    const int j = sj[k];
+                    if (j >= 0 && j != i) top2_insert(st, dist2(tx, ty, sx[k], sy[k]), j);
+                }
+            wave_sync();
+        };
+        // sources of the near leaves with lo < box distance <= hi
+        auto stage_scan = [&](double lo, double hi) {
+            int fill = 0;
+            for (int64_t eb = e0; eb < e1; eb += kWave) {
+                const int64_t e = eb + lane;
+                int cnt = 0, j0 = 0;
+                if (e < e1) {
+                    const int M = T.near_list[e];
+                    const double bd = box_dist2(T, L, M);
+                    if (bd > lo && bd <= hi) {
+                        j0 = T.v0[M];
+                        cnt = T.v1[M] - j0;
+                    }
+                }
+                const int incl = wave_incl_scan(cnt);
+                const int tot = __shfl(incl, kWave - 1, kWave);
+                if (tot == 0) continue;
+                soff[lane] = incl - cnt;
+                sj0[lane] = j0;
+                wave_sync();
+                for (int done = 0; done < tot;) {
+                    const int take = min(tot - done, kStageE - fill);
+                    for (int q = lane; q < take; q += kWave) {
+                        const int gidx = done + q;
+                        int a = 0, b = kWave - 1;   // last lane whose offset <= gidx
+                        while (a < b) {
+                            const int mid = (a + b + 1) >> 1;
+                            if (soff[mid] <= gidx)
+                                a = mid;
+                            else
+                                b = mid - 1;
+                        }
+                        const int j = sj0[a] + (gidx - soff[a]);
+                        sx[fill + q] = x[j];
+                        sy[fill + q] = y[j];
+                        sj[fill + q] = g[j] != 0 ? j : -1;
+                    }
+                    fill += take;
+                    done += take;
+                    if (fill == kStageE) {
+                        compute(fill);
+                        fill = 0;
+                    }
+                }
+                wave_sync();
+            }
+            if (fill) compute(fill);
+        };
+        auto combine = [&](Top2 s) {
+            Top2 c = top2_empty();
+            for (int q = 0; q < nslot; q++) {
+                Top2 o;
+                const int src = t + ntc * q;
+                o.d1 = __shfl(s.d1, src, kWave);
+                o.d2 = __shfl(s.d2, src, kWave);
+                o.i1 = __shfl(s.i1, src, kWave);
+                o.i2 = __shfl(s.i2, src, kWave);
+                top2_merge(c, o);
+            }
+            return c;
+        };
+        stage_scan(-1.0, 0.0);
+        const Top2 c1 = combine(st);
+        double bound = (tg != 0) ? (c1.i2 >= 0 ? c1.d2 : INFINITY) : -INFINITY;
         bound = wave_max(bound);
-        if (bound > 0) scan_sources(T, l, L, x, y, g, 0.0, bound, slot, active, i, tx, ty, st);
-        top2_merge(st, top2_shfl_xor(st, 16));
-        top2_merge(st, top2_shfl_xor(st, 32));
-        if (slot == 0 && valid) {
+        if (bound > 0) stage_scan(0.0, bound);
+        const Top2 fin = combine(st);
+        if (lane < ntc) {
             uint8_t f = 0;
             int n1 = -1, n2 = -1;
-            if (active) {
+            if (tg != 0) {
                 f = 1;
-                n1 = st.i1;
-                n2 = st.i2;
+                n1 = fin.i1;
+                n2 = fin.i2;
                 if (n1 >= 0 && n2 >= 0 && !isnan(crit)) {
                     const double s1 = sgn(g[n1]), s2 = sgn(g[n2]), si = sgn(tg);
-                    if ((st.d1 < crit) || ((s1 == s2) && (s1 != si))) {
+                    if ((fin.d1 < crit) || ((s1 == s2) && (s1 != si))) {
                         f |= 2;
                         flag_list[atomicAdd(&ecount[0], 1)] = i;
                     }
@@ -225,8 +283,8 @@ __global__ __launch_bounds__(kBlock) void k_eps_top2(TreeDev T, const double* x,
             }
             nn1[i] = n1;
             nn2[i] = n2;
-            d1o[i] = st.d1;
-            d2o[i] = st.d2;
+            d1o[i] = fin.d1;
+            d2o[i] = fin.d2;
             eflag[i] = f;
         }
     }
