@@ -107,17 +107,20 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
 #pragma unroll
     for (int k = 0; k < NT; k++) sx[k] = sy[k] = qx[k] = qy[k] = 0.f;
 
+    // software pipeline: the near-list entry of the next chunk and the
+    // sources of the next batch are loaded while the current batch computes
+    int M_next = (e_beg + lane < e_end) ? A.near_list[e_beg + lane] : -1;
     for (int64_t eb = e_beg; eb < e_end; eb += kWave) {
-        const int64_t e = eb + lane;
+        const int M = M_next;
+        M_next = (eb + kWave + lane < e_end) ? A.near_list[eb + kWave + lane] : -1;
         int s0 = 0, cnt = 0;
-        if (e < e_end) {
-            const int M = A.near_list[e];
+        if (M >= 0) {
             s0 = A.v0[M];
             cnt = A.v1[M] - s0;
         }
         const int incl = wave_incl_scan(cnt);
         const int total = __shfl(incl, kWave - 1, kWave);
-        for (int sb = 0; sb < total; sb += kWave) {
+        auto fetch = [&](int sb, float& sxr, float& syr, float& gj, float& e2) {
             const int sp = sb + lane;
             // entry index: number of entries whose inclusive prefix <= sp
             int pos = 0;
@@ -129,7 +132,10 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
             pos = min(pos, kWave - 1);
             const int ex = __shfl(incl, pos, kWave) - __shfl(cnt, pos, kWave);
             const int j = __shfl(s0, pos, kWave) + (sp - ex);
-            float sxr = 0.f, syr = 0.f, gj = 0.f, e2 = 1.f;
+            sxr = 0.f;
+            syr = 0.f;
+            gj = 0.f;
+            e2 = 1.f;
             if (sp < total) {
                 const float2 ge = src_ge[j];
                 gj = ge.x;
@@ -137,6 +143,12 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                 sxr = (float)(x[j] - cx);
                 syr = (float)(y[j] - cy);
             }
+        };
+        float sxr, syr, gj, e2;
+        if (total > 0) fetch(0, sxr, syr, gj, e2);
+        for (int sb = 0; sb < total; sb += kWave) {
+            float nsxr = 0.f, nsyr = 0.f, ngj = 0.f, ne2 = 1.f;
+            if (sb + kWave < total) fetch(sb + kWave, nsxr, nsyr, ngj, ne2);
 #pragma unroll
             for (int k = 0; k < NT; k++) {
                 const float dx = tx[k] - sxr;
@@ -157,6 +169,10 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                     sy[k] = __builtin_fmaf(dx, f, sy[k]);
                 }
             }
+            sxr = nsxr;
+            syr = nsyr;
+            gj = ngj;
+            e2 = ne2;
         }
     }
     double nearx = 0, neary = 0;

@@ -628,10 +628,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
 // differences, Kahan sums, lane partials combined in FP64), plus the leaf's
 // network correction expanded to first order about the leaf centre.  One wave
 // per leaf; targets in chunks of up to 16 padded to a power of two, the far
-// list split over 64 / npad lanes per target.
+// list split over 64 / npad lanes per target; the monopoles of 64 far nodes
+// at a time are staged in LDS.
 __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* __restrict__ nodes,
                                                      const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ leaf_corr, double2* far_out) {
+    // far-node monopoles of the current batch of 64 far-list entries
+    __shared__ double s_cx[kWavesPerBlock][kWave], s_cy[kWavesPerBlock][kWave];
+    __shared__ float s_m[kWavesPerBlock][6][kWave];   // gpk, gmk, zpx, zpy, zmx, zmy
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
@@ -642,6 +646,14 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
     const double cx = T.nx[L], cy = T.ny[L];
     const double* C = leaf_corr + 6 * (size_t)l;
     const double c0u = C[0], c0v = C[1], cxu = C[2], cxv = C[3], cyu = C[4], cyv = C[5];
+    double* scx = s_cx[w];
+    double* scy = s_cy[w];
+    float(*sm)[kWave] = s_m[w];
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     for (int base = v0; base < v1; base += 16) {
         const int ntc = min(16, v1 - base);
         const int npad = ntc <= 1 ? 1 : ntc <= 2 ? 2 : ntc <= 4 ? 4 : ntc <= 8 ? 8 : 16;
@@ -650,30 +662,46 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
         const bool tvalid = k < ntc;
         const double tx = tvalid ? x[base + k] : 0., ty = tvalid ? y[base + k] : 0.;
         float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;
-        for (int64_t e = fb + fo; e < fe; e += nper) {
-            if (!tvalid) break;
-            const FarNode& nd = nodes[T.far_list[e]];
-            const float dx = (float)(tx - nd.cx), dy = (float)(ty - nd.cy);
-            float u = 0.f, v = 0.f;
-            if (nd.gpk != 0.f) {
-                const float ex = dx - nd.zpx, ey = dy - nd.zpy;
-                const float fac = nd.gpk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
-                u -= ey * fac;
-                v += ex * fac;
+        for (int64_t eb = fb; eb < fe; eb += kWave) {
+            if (eb + lane < fe) {
+                const FarNode& nd = nodes[T.far_list[eb + lane]];
+                scx[lane] = nd.cx;
+                scy[lane] = nd.cy;
+                sm[0][lane] = nd.gpk;
+                sm[1][lane] = nd.gmk;
+                sm[2][lane] = nd.zpx;
+                sm[3][lane] = nd.zpy;
+                sm[4][lane] = nd.zmx;
+                sm[5][lane] = nd.zmy;
             }
-            if (nd.gmk != 0.f) {
-                const float ex = dx - nd.zmx, ey = dy - nd.zmy;
-                const float fac = nd.gmk * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
-                u -= ey * fac;
-                v += ex * fac;
-            }
-            float yk = u - cu, tk = su + yk;
-            cu = (tk - su) - yk;
-            su = tk;
-            yk = v - cv;
-            tk = sv + yk;
-            cv = (tk - sv) - yk;
-            sv = tk;
+            wave_sync();
+            const int nb = (int)min<int64_t>(kWave, fe - eb);
+            if (tvalid)
+                for (int q = fo; q < nb; q += nper) {
+                    const float dx = (float)(tx - scx[q]), dy = (float)(ty - scy[q]);
+                    const float gp = sm[0][q], gm = sm[1][q];
+                    float u = 0.f, v = 0.f;
+                    if (gp != 0.f) {
+                        const float ex = dx - sm[2][q], ey = dy - sm[3][q];
+                        const float fac = gp * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
+                        u -= ey * fac;
+                        v += ex * fac;
+                    }
+                    if (gm != 0.f) {
+                        const float ex = dx - sm[4][q], ey = dy - sm[5][q];
+                        const float fac = gm * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
+                        u -= ey * fac;
+                        v += ex * fac;
+                    }
+                    float yk = u - cu, tk = su + yk;
+                    cu = (tk - su) - yk;
+                    su = tk;
+                    yk = v - cv;
+                    tk = sv + yk;
+                    cv = (tk - sv) - yk;
+                    sv = tk;
+                }
+            wave_sync();
         }
         double du = (double)su - (double)cu, dv = (double)sv - (double)cv;
         for (int o = npad; o < kWave; o <<= 1) {
