@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--model", default=os.path.join(ROOT, "vvflow_amd", "farfield_mlp.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--precision-sweep", action="store_true",
+                    help="4M-vortex wake: FP32 pair sums plain vs Kahan against the FP64 direct sum, and the "
+                         "2000-segment body SLAE filled and solved in FP64 (one JSON line)")
+    ap.add_argument("--sweep-fcs", default="8,64,256")
     return ap.parse_args()
 
 
@@ -186,10 +190,93 @@ def roofline(name, achieved, peak, traffic, extra):
     return out
 
 
+def precision_sweep(a):
+    """4M-vortex wake (seed 0x5EED0004): convective error of the plain tree
+    path with FP32 pair sums, plain and Kahan-compensated, against the FP64
+    direct sum at several far criteria, with the near kernel's FP32 rate;
+    then the boundary SLAE of a 2000-segment cylinder (n = 2009) filled and
+    solved in FP64: the first call (with the solver library's set-up), a
+    second call at t = 0 (fill, LU, solve) and a later step (fill compared
+    with the factorised matrix, right-hand side and solve with the factors
+    kept), and the relative residual."""
+    import vvflow_amd as V
+    n = 4_000_000
+    objs = V.gen_wake(n, 0x5EED0004)
+    ctx = V.Context(0)
+    ctx.set_params(re=600.0, dt=0.005)
+    ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
+    ctx.upload(objs)
+    ctx.snapshot()
+    rows = []
+    tg = None
+    for fc in [int(f) for f in a.sweep_fcs.split(",")]:
+        for kahan in (False, True):
+            ctx.restore()
+            ctx.tree_build(fc)
+            ctx.epsilon(True)
+            before = ctx.download()[:, 3:5]
+            ctx.convective((0.0, 0.0), kahan=kahan)
+            ph = ctx.phase_times()
+            st = ctx.download()
+            if tg is None:
+                live = np.nonzero(st[:, 2] != 0)[0]
+                tg = np.random.default_rng(0).choice(live, min(a.error_sample, len(live)),
+                                                     replace=False).astype(np.int64)
+                ref = ctx.direct_sum(tg)
+            v = st[tg, 3:5] - before[tg]
+            nr = np.linalg.norm(ref, axis=1)
+            work = ctx.work_counts()
+            near_ms = float(ph[6])
+            rows.append({"far_criteria": fc, "accumulation": "kahan" if kahan else "plain",
+                         "rel_l2": float(np.linalg.norm(v - ref) / np.linalg.norm(ref)),
+                         "rms_rel": float(np.sqrt(np.mean((np.linalg.norm(v - ref, axis=1) /
+                                                           np.maximum(nr, 1e-3 * np.sqrt(np.mean(nr ** 2)))) ** 2))),
+                         "max_rel": float(np.max(np.linalg.norm(v - ref, axis=1) /
+                                                 np.maximum(nr, 1e-3 * np.sqrt(np.mean(nr ** 2))))),
+                         "near_pairs": int(work[0]), "near_kernel_ms": near_ms,
+                         "near_tflops": work[0] * FLOPS_PER_NEAR_PAIR / (near_ms * 1e-3) / 1e12})
+            print(f"fc={fc} {rows[-1]['accumulation']}: rel L2 {rows[-1]['rel_l2']:.3e}", file=sys.stderr, flush=True)
+    # the 2000-segment cylinder, shifted upstream of the wake
+    corners = V.gen_cylinder(0.5, 2000)
+    corners[:, 0] -= 2.0
+    segs = V.segments_from_corners(corners)
+    body = np.zeros(1, dtype=V.BODY_DTYPE)
+    body["nseg"] = len(segs)
+    ctx.restore()
+    ctx.set_params(re=600.0, dt=0.005, time=0.0)
+    ctx.upload_bodies(np.ascontiguousarray(segs, dtype=V.SEG_DTYPE), body)
+    dl = ctx.download_bodies()[1][0]["slen"] / len(segs)
+    ctx.tree_build(8, 5 * dl, 100 * dl)
+    ctx.calc_circulation((1.0, 0.0))
+    first_ms = float(ctx.phase_times()[5])   # includes the solver library's first-use set-up
+    ctx.calc_circulation((1.0, 0.0))
+    refactor_ms = float(ctx.phase_times()[5])   # t = 0: fill, LU and solve again
+    ctx.tree_destroy()
+    ctx.set_params(re=600.0, dt=0.005, time=0.005)
+    ctx.tree_build(8, 5 * dl, 100 * dl)
+    ctx.calc_circulation((1.0, 0.0))
+    cached_ms = float(ctx.phase_times()[5])
+    A, b = ctx.slae_system()
+    sg, bd = ctx.download_bodies()
+    x = np.concatenate([sg["g"], bd[0]["speed_slae"], bd[0]["force_hydro"], bd[0]["force_holder"]])
+    resid = float(np.linalg.norm(A @ x - b) / np.linalg.norm(b))
+    ctx.close()
+    out = {"metric": "relative convective velocity error vs FP64 direct sum, FP32 plain vs Kahan pair sums",
+           "workload": f"karman-street wake, {n} vortices (seed 0x5EED0004), plain tree path, merging epsilon",
+           "sample_targets": int(len(tg)), "sweep": rows,
+           "slae": {"segments": int(len(segs)), "n": int(A.shape[0]), "dtype": "fp64",
+                    "first_call_ms": first_ms, "fill_factor_solve_ms": refactor_ms,
+                    "factor_reuse_ms": cached_ms, "relative_residual": resid}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.precision_sweep:
+        precision_sweep(a)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the hot path runs on a single MI355X (asked for --gpus {a.gpus}); "

@@ -92,6 +92,23 @@ int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg,
 int vvhd_download_bodies(vvhd_ctx* ctx, vvhd_seg* segs, int32_t nseg,
                          vvhd_body* bodies, int32_t nbody);
 
+/* ---- independent simulations in one context (no counterpart in the
+ * reference: one Space per process there) --------------------------------
+ * Declares the uploaded vortex and body lists as the concatenation of nsim
+ * independent simulations: simulation s owns vortices [vortex_off[s],
+ * vortex_off[s+1]) (vortex_off[0] = 0, vortex_off[nsim] = vortex count),
+ * bodies [body_off[s], body_off[s+1]) and Reynolds number re[s].  The tree
+ * build then grows one tree per simulation (a forest, one root each), and
+ * epsilon, convective (plain and learned) and diffusive advance every
+ * simulation in the same launches; no list, merge or sum crosses
+ * simulations, and the direct sum of a target runs over its own simulation.
+ * Holds until the next vortex or body upload (back to one simulation).
+ * calc_circulation, move_and_clean, vortex_shed, calc_forces, zero_forces,
+ * integrals, velocity_points and tree_find, whose reference counterparts
+ * act on the whole Space, return VVHD_EUNSUPPORTED meanwhile. */
+int vvhd_set_batch(vvhd_ctx* ctx, int32_t nsim, const int64_t* vortex_off, const int32_t* body_off,
+                   const double* re);
+
 /* ---- tree: stree (TSortedTree.hpp:60-92, TSortedTree.cpp:232-303) ----
  * Builds the adaptive binary tree over the vortex list and the body
  * segments, reordering the vortex list in place exactly as the reference's

@@ -61,6 +61,7 @@ ABI = {
     "vvhd_snapshot": (_I, [_P]),
     "vvhd_restore": (_I, [_P]),
     "vvhd_upload_bodies": (_I, [_P, _VP, _I32, _VP, _I32]),
+    "vvhd_set_batch": (_I, [_P, _I32, _VP, _VP, _VP]),
     "vvhd_download_bodies": (_I, [_P, _VP, _I32, _VP, _I32]),
     "vvhd_tree_build": (_I, [_P, _I, _D, _D, _VP]),
     "vvhd_tree_destroy": (_I, [_P]),
@@ -191,6 +192,15 @@ class Context:
     def upload(self, objs):
         objs = np.ascontiguousarray(objs, dtype=np.float64).reshape(-1, 6)
         self._check(self.L.vvhd_upload_vortices(self.h, _ptr(objs), objs.shape[0]))
+
+    def set_batch(self, vortex_off, body_off, re):
+        """Declares the uploaded vortices and bodies as len(re) independent
+        simulations: simulation s owns vortices vortex_off[s]:vortex_off[s+1]
+        and bodies body_off[s]:body_off[s+1], Reynolds number re[s]."""
+        vo = np.ascontiguousarray(vortex_off, dtype=np.int64)
+        bo = np.ascontiguousarray(body_off, dtype=np.int32)
+        r = np.ascontiguousarray(re, dtype=np.float64)
+        self._check(self.L.vvhd_set_batch(self.h, len(r), _ptr(vo), _ptr(bo), _ptr(r)))
 
     @property
     def n(self):

@@ -23,3 +23,8 @@ struct vvhd_ctx {
 
 #define REQUIRE_CTX(ctx) \
     if (!(ctx)) return VVHD_EINVAL
+
+// Entry points whose reference counterpart couples the whole Space refuse to
+// run while several simulations share the context.
+#define REQUIRE_ONE_SIM(c, name) \
+    VVHD_CHECK((c).nsim == 1, VVHD_EUNSUPPORTED, name ": not available while several simulations share the context")

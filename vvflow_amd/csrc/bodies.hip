@@ -841,6 +841,7 @@ extern "C" {
 int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "move_and_clean");
     Ctx& c = ctx->c;
     VVHD_HIP(hipSetDevice(c.device));
     VVHD_HIP(hipEventRecord(c.ev[8], c.stream));
@@ -853,6 +854,7 @@ int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned) {
 int vvhd_vortex_shed(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "vortex_shed");
     VVHD_HIP(hipSetDevice(ctx->c.device));
     vortex_shed(ctx->c);
     API_END(ctx)
@@ -861,6 +863,7 @@ int vvhd_vortex_shed(vvhd_ctx* ctx) {
 int vvhd_calc_circulation(vvhd_ctx* ctx, double inf_x, double inf_y) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "calc_circulation");
     Ctx& c = ctx->c;
     VVHD_HIP(hipSetDevice(c.device));
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "calc_circulation: tree is not built");
@@ -897,6 +900,7 @@ int vvhd_slae_system(vvhd_ctx* ctx, int32_t* n, double* A, double* rhs) {
 int vvhd_calc_forces(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "calc_forces");
     VVHD_HIP(hipSetDevice(ctx->c.device));
     calc_forces(ctx->c);
     API_END(ctx)
@@ -905,6 +909,7 @@ int vvhd_calc_forces(vvhd_ctx* ctx) {
 int vvhd_zero_forces(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "zero_forces");
     VVHD_HIP(hipSetDevice(ctx->c.device));
     zero_forces(ctx->c);
     API_END(ctx)

@@ -145,7 +145,11 @@ struct TreeArrays {
     DBuf<int32_t> node_L, node_m, leaf_tmp, seg_node;   // build scratch
     DBuf<int32_t> cnt_near, cnt_far, err;               // list scratch
     DBuf<int64_t> slot_near, slot_far;                  // per (leaf, top entry) list offsets
-    DBuf<uint8_t> entries;                              // top-of-tree entry table of the list builder
+    DBuf<uint8_t> entries;                              // top-of-tree entry tables of the list builder (per root)
+    DBuf<int32_t> groups;                               // list builder leaf groups: {first leaf, count, root}
+    int ngroups = 0;
+    DBuf<int32_t> leaf_sim;                             // per leaf: its simulation (root)
+    DBuf<int32_t> root_first;                           // per root: first leaf (DFS), nroots + 1
     int32_t nnodes = 0, nleaves_h = 0;
     int64_t nnear = 0, nfar = 0;
     int max_depth = 0;
@@ -172,6 +176,18 @@ struct Ctx {
     // snapshot for benchmarks
     DBuf<double> snap;
     int64_t snap_n = -1;
+
+    // independent simulations sharing the context (one tree root each):
+    // simulation s owns vortices [sim_off_h[s], sim_off_h[s+1]) and bodies
+    // [sim_body_h[s], sim_body_h[s+1]), Reynolds number sim_re_h[s]
+    int nsim = 1;
+    std::vector<int64_t> sim_off_h;
+    std::vector<int32_t> sim_body_h;
+    std::vector<double> sim_re_h;
+    DBuf<int64_t> sim_off;                   // device copies, refreshed by the tree build
+    DBuf<int32_t> sim_body;
+    DBuf<double> sim_re;
+    DBuf<int32_t> seg_root;                  // per segment: its simulation
 
     // segments and bodies (host copy is authoritative between calls)
     std::vector<vvhd_seg> segs_h;
@@ -266,6 +282,9 @@ __device__ inline double box_dist2(const TreeDev& T, int a, int b) {
 
 // host helpers implemented in ctx.hip
 void ensure_vortex_capacity(Ctx& c, int64_t n);
+// Refreshes the device copies of the simulation table (one simulation over
+// the whole context unless a batch is declared).
+void sim_sync(Ctx& c);
 void reserve_vortices(Ctx& c, int64_t n, bool preserve);
 void sync(Ctx& c);
 

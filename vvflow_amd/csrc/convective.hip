@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -307,15 +308,16 @@ __global__ void k_velocity_points(TreeDev T, const double* x, const double* y, c
 // FP64 direct sum of the induced velocity, compensated; sources staged in
 // LDS, one thread per target.
 constexpr int kDsBlock = 256;
+// Sources [lo, n).
 __global__ __launch_bounds__(kDsBlock) void k_direct_sum(const double* x, const double* y, const double* g,
-                                                         const double* ie, int64_t n, const int64_t* targets,
-                                                         int64_t nt, double* out) {
+                                                         const double* ie, int64_t lo, int64_t n,
+                                                         const int64_t* targets, int64_t nt, double* out) {
     __shared__ double sx[kDsBlock], sy[kDsBlock], sg[kDsBlock], se[kDsBlock];
     const int64_t t = (int64_t)blockIdx.x * kDsBlock + threadIdx.x;
     const int64_t i = (t < nt) ? (targets ? targets[t] : t) : 0;
     const double px = (t < nt) ? x[i] : 0, py = (t < nt) ? y[i] : 0;
     double ax = 0, ay = 0, cxs = 0, cys = 0;
-    for (int64_t b = 0; b < n; b += kDsBlock) {
+    for (int64_t b = lo; b < n; b += kDsBlock) {
         const int64_t j = b + threadIdx.x;
         if (j < n && g[j] != 0) {
             const double e1 = 1. / ie[j];
@@ -401,12 +403,50 @@ void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double i
 
 void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out) {
     if (nt == 0) return;
+    if (c.nsim > 1) {   // each target sums over its own simulation
+        std::vector<int64_t> all;
+        if (!targets) {
+            all.resize(nt);
+            for (int64_t k = 0; k < nt; k++) all[k] = k;
+            targets = all.data();
+        }
+        std::vector<std::vector<int64_t>> pos(c.nsim);
+        for (int64_t k = 0; k < nt; k++) {
+            VVHD_CHECK(targets[k] >= 0 && targets[k] < c.n, VVHD_EINVAL, "direct_sum: target out of range");
+            const int s = (int)(std::upper_bound(c.sim_off_h.begin(), c.sim_off_h.end(), targets[k]) -
+                                c.sim_off_h.begin()) - 1;
+            pos[s].push_back(k);
+        }
+        c.lscratch0.grow((size_t)nt + 1);
+        c.tmp.grow((size_t)nt * 16);
+        double* dout = (double*)c.tmp.p;
+        std::vector<double> buf;
+        std::vector<int64_t> tg;
+        for (int s = 0; s < c.nsim; s++) {
+            const int64_t m = (int64_t)pos[s].size();
+            if (!m) continue;
+            tg.resize(m);
+            for (int64_t k = 0; k < m; k++) tg[k] = targets[pos[s][k]];
+            VVHD_HIP(hipMemcpyAsync(c.lscratch0.p, tg.data(), (size_t)m * 8, hipMemcpyHostToDevice, c.stream));
+            hipLaunchKernelGGL(k_direct_sum, dim3((unsigned)((m + kDsBlock - 1) / kDsBlock)), dim3(kDsBlock), 0,
+                               c.stream, c.x.p, c.y.p, c.g.p, c.ie.p, c.sim_off_h[s], c.sim_off_h[s + 1],
+                               (const int64_t*)c.lscratch0.p, m, dout);
+            buf.resize(2 * m);
+            VVHD_HIP(hipMemcpyAsync(buf.data(), dout, (size_t)m * 16, hipMemcpyDeviceToHost, c.stream));
+            VVHD_HIP(hipStreamSynchronize(c.stream));
+            for (int64_t k = 0; k < m; k++) {
+                v_out[2 * pos[s][k]] = buf[2 * k];
+                v_out[2 * pos[s][k] + 1] = buf[2 * k + 1];
+            }
+        }
+        return;
+    }
     c.lscratch0.grow((size_t)nt + 1);
     c.tmp.grow((size_t)nt * 16);
     if (targets) VVHD_HIP(hipMemcpyAsync(c.lscratch0.p, targets, (size_t)nt * 8, hipMemcpyHostToDevice, c.stream));
     double* dout = (double*)c.tmp.p;
     hipLaunchKernelGGL(k_direct_sum, dim3((unsigned)((nt + kDsBlock - 1) / kDsBlock)), dim3(kDsBlock), 0, c.stream,
-                       c.x.p, c.y.p, c.g.p, c.ie.p, c.n, targets ? c.lscratch0.p : nullptr, nt, dout);
+                       c.x.p, c.y.p, c.g.p, c.ie.p, (int64_t)0, c.n, targets ? c.lscratch0.p : nullptr, nt, dout);
     VVHD_HIP(hipMemcpyAsync(v_out, dout, (size_t)nt * 16, hipMemcpyDeviceToHost, c.stream));
     VVHD_HIP(hipStreamSynchronize(c.stream));
 }

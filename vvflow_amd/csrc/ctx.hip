@@ -85,6 +85,28 @@ __global__ void k_integrals_final(const double* part, int nb, double* out) {
 
 void reserve_vortices(Ctx& c, int64_t n, bool preserve);
 
+void sim_sync(Ctx& c) {
+    if (c.nsim == 1) {
+        c.sim_off_h = {0, c.n};
+        c.sim_body_h = {0, (int32_t)c.bodies_h.size()};
+        c.sim_re_h = {c.re};
+    }
+    const int ns = c.nsim, nseg = (int)c.segs_h.size();
+    c.sim_off.grow(ns + 1);
+    c.sim_body.grow(ns + 1);
+    c.sim_re.grow(ns);
+    c.seg_root.grow(std::max(nseg, 1));
+    std::vector<int32_t> root(nseg, 0);
+    for (int s = 0; s < ns; s++)
+        for (int b = c.sim_body_h[s]; b < c.sim_body_h[s + 1]; b++)
+            for (int k = 0; k < c.bodies_h[b].nseg; k++) root[c.bodies_h[b].first_seg + k] = s;
+    VVHD_HIP(hipMemcpyAsync(c.sim_off.p, c.sim_off_h.data(), (ns + 1) * 8, hipMemcpyHostToDevice, c.stream));
+    VVHD_HIP(hipMemcpyAsync(c.sim_body.p, c.sim_body_h.data(), (ns + 1) * 4, hipMemcpyHostToDevice, c.stream));
+    VVHD_HIP(hipMemcpyAsync(c.sim_re.p, c.sim_re_h.data(), ns * 8, hipMemcpyHostToDevice, c.stream));
+    if (nseg) VVHD_HIP(hipMemcpyAsync(c.seg_root.p, root.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, c.stream));
+    VVHD_HIP(hipStreamSynchronize(c.stream));
+}
+
 void ensure_vortex_capacity(Ctx& c, int64_t n) { reserve_vortices(c, n, false); }
 
 // Grows every per-vortex array to at least n entries, keeping the first c.n
@@ -150,7 +172,9 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     for (auto* b : {&c.id, &c.id2, &c.iscratch0, &c.iscratch1, &c.iscratch2, &c.iscratch3, &c.nn1, &c.nn2, &c.ecount,
                     &c.rnear_list})
         b->release();
-    for (auto* b : {&c.flag_list, &c.rnear_cnt, &c.leaf_stamp}) b->release();
+    for (auto* b : {&c.flag_list, &c.rnear_cnt, &c.leaf_stamp, &c.seg_root, &c.sim_body}) b->release();
+    c.sim_off.release();
+    c.sim_re.release();
     c.leaf_maxd2.release();
     c.near_v.release();
     c.taylor.release();
@@ -171,6 +195,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
                     &t.node_L, &t.node_m, &t.leaf_tmp, &t.seg_node, &t.cnt_near, &t.cnt_far, &t.err})
         b->release();
     for (auto* b : {&t.near_off, &t.far_off, &t.seg_off, &t.slot_near, &t.slot_far}) b->release();
+    for (auto* b : {&t.groups, &t.leaf_sim, &t.root_first}) b->release();
     t.entries.release();
     t.cmp_ie.release();
     t.cmm_ie.release();
@@ -204,6 +229,7 @@ int vvhd_upload_vortices(vvhd_ctx* ctx, const double* objs6, int64_t n) {
     ensure_vortex_capacity(c, std::max<int64_t>(n, 1));
     c.n = n;
     c.t.built = false;
+    c.nsim = 1;
     if (n) {
         // stage through the x2..ie2 double buffers (6 * n doubles fit in x2+y2+g2+vx2+vy2+ie2 only
         // as separate arrays, so use the snapshot buffer as staging)
@@ -281,6 +307,7 @@ int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg, const 
     VVHD_CHECK(expect == nseg, VVHD_EINVAL, "upload_bodies: segment count mismatch");
     c.segs_h.assign(segs, segs + nseg);
     c.bodies_h.assign(bodies, bodies + nbody);
+    c.nsim = 1;
     c.body_range.grow(2 * std::max(nbody, 1));
     if (nbody) {
         std::vector<int32_t> br(2 * nbody);
@@ -301,6 +328,29 @@ int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg, const 
         VVHD_HIP(hipStreamSynchronize(c.stream));
     }
     bodies_prepare(c);
+    c.t.built = false;
+    API_END(ctx)
+}
+
+int vvhd_set_batch(vvhd_ctx* ctx, int32_t nsim, const int64_t* vortex_off, const int32_t* body_off,
+                   const double* re) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_CHECK(nsim >= 1 && vortex_off && body_off && re, VVHD_EINVAL, "set_batch: bad arguments");
+    VVHD_CHECK(vortex_off[0] == 0 && vortex_off[nsim] == c.n, VVHD_EINVAL,
+               "set_batch: vortex offsets must run from 0 to the vortex count");
+    VVHD_CHECK(body_off[0] == 0 && body_off[nsim] == (int32_t)c.bodies_h.size(), VVHD_EINVAL,
+               "set_batch: body offsets must run from 0 to the body count");
+    for (int s = 0; s < nsim; s++) {
+        VVHD_CHECK(vortex_off[s + 1] >= vortex_off[s] && body_off[s + 1] >= body_off[s], VVHD_EINVAL,
+                   "set_batch: offsets must not decrease");
+        VVHD_CHECK(re[s] > 0, VVHD_EINVAL, "set_batch: re must be positive");
+    }
+    c.nsim = nsim;
+    c.sim_off_h.assign(vortex_off, vortex_off + nsim + 1);
+    c.sim_body_h.assign(body_off, body_off + nsim + 1);
+    c.sim_re_h.assign(re, re + nsim);
     c.t.built = false;
     API_END(ctx)
 }
@@ -369,6 +419,7 @@ int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan) {
 int vvhd_velocity_points(vvhd_ctx* ctx, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "velocity_points");
     Ctx& c = ctx->c;
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "TTree::findNode(): tree is not built");
     VVHD_CHECK(n >= 0 && (n == 0 || (xy && v_out)), VVHD_EINVAL, "velocity_points: bad arguments");
@@ -391,6 +442,7 @@ int vvhd_diffusive(vvhd_ctx* ctx) {
 int vvhd_integrals(vvhd_ctx* ctx, double* out5) {
     REQUIRE_CTX(ctx);
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "integrals");
     Ctx& c = ctx->c;
     VVHD_CHECK(out5, VVHD_EINVAL, "integrals: null output");
     const int nb = 256;

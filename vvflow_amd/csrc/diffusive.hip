@@ -69,7 +69,8 @@ constexpr int kStage = 512;   // staged sources per wave
 __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ g,
                                                       const double* __restrict__ ie, const vvhd_seg* segs, int nseg,
-                                                      double* vx, double* vy, double re) {
+                                                      double* vx, double* vy, double re_one, const double* sim_re,
+                                                      const int32_t* leaf_sim) {
     __shared__ float s_x[kWavesPerBlock][kStage], s_y[kWavesPerBlock][kStage], s_g[kWavesPerBlock][kStage];
     __shared__ int s_off[kWavesPerBlock][kWave], s_j0[kWavesPerBlock][kWave];
     const int w = threadIdx.x / kWave;
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
     if (v0 == v1) return;
     const double cx = T.nx[L], cy = T.ny[L];
     const int64_t e0 = T.near_off[l], e1 = T.near_off[l + 1];
+    const double re = sim_re ? sim_re[leaf_sim[l]] : re_one;   // Reynolds number of the leaf's simulation
     float* sx = s_x[w];
     float* sy = s_y[w];
     float* sg = s_g[w];
@@ -310,8 +312,10 @@ void diffusive(Ctx& c) {
     const int nl = T.nleaf;
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nseg = (int)c.segs_h.size();
+    const bool batch = c.nsim > 1;
     hipLaunchKernelGGL(k_diffusive, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p,
-                       nseg, c.vx.p, c.vy.p, c.re);
+                       nseg, c.vx.p, c.vy.p, c.re, batch ? (const double*)c.sim_re.p : (const double*)nullptr,
+                       (const int32_t*)c.t.leaf_sim.p);
     if (nseg) {
         hipStream_t s = c.stream;
         c.iscratch0.grow(nl + 1);

@@ -88,12 +88,15 @@ __device__ inline double dist2(double tx, double ty, double sx, double sy) {
 }
 
 // nearestBodySegment (MEpsilonFast.cpp:214-253) for every leaf centre, then
-// the merge criterion and the epsilon floor (:29-47).
-__global__ void k_leaf_criteria(TreeDev T, const vvhd_seg* segs, const int32_t* body_range, int nbody, int merge,
-                                double* crit_out, double* restr_out) {
+// the merge criterion and the epsilon floor (:29-47).  The bodies searched
+// are those of the leaf's simulation, [sim_body[s], sim_body[s+1]).
+__global__ void k_leaf_criteria(TreeDev T, const vvhd_seg* segs, const int32_t* body_range, const int32_t* sim_body,
+                                const int32_t* leaf_sim, int merge, double* crit_out, double* restr_out) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= T.nleaf) return;
-    if (nbody == 0) {   // no segment anywhere: criterion 0 (or NaN), floor 0
+    const int sim = leaf_sim[l];
+    const int b0 = sim_body[sim], b1 = sim_body[sim + 1];
+    if (b0 == b1) {   // no segment in the simulation: criterion 0 (or NaN), floor 0
         crit_out[l] = merge ? 0.0 : __builtin_nan("");
         restr_out[l] = 0;
         return;
@@ -114,7 +117,7 @@ __global__ void k_leaf_criteria(TreeDev T, const vvhd_seg* segs, const int32_t* 
         }
     }
     if (att < 0) {
-        for (int b = 0; b < nbody; b++) {
+        for (int b = b0; b < b1; b++) {
             const int f = body_range[2 * b], cnt = body_range[2 * b + 1];
             for (int k = 0; k < cnt; k++) {
                 const double d2 = dist2(px, py, segs[f + k].x, segs[f + k].y);
@@ -605,28 +608,37 @@ __device__ void commit_one(const RepairArgs& A, int q, int mark, CommitShared& s
     __syncthreads();
 }
 
-// Pass 2: commits merges in target order (single workgroup).  Candidates are
-// kept in an LDS list (the flags of pass 1 plus targets flagged by later
-// updates); the next commit is the smallest listed index at or after the
-// cursor whose flag is still set.  If the list overflows, the remaining
-// commits find their target by scanning the flags.
-__global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, int64_t n, const int32_t* flag_list) {
+// Pass 2: commits merges in target order, one workgroup per simulation
+// (targets [sim_off[s], sim_off[s+1]); simulations share no leaf).
+// Candidates are kept in an LDS list (the flags of pass 1 plus targets
+// flagged by later updates); the next commit is the smallest listed index at
+// or after the cursor whose flag is still set.  If the list overflows, the
+// remaining commits find their target by scanning the flags.
+__global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, const int64_t* sim_off,
+                                                               const int32_t* flag_list) {
     __shared__ int s_q;
     __shared__ int s_np;
     __shared__ int s_pend[kMaxPend];
     __shared__ CommitShared sh;
     int32_t* ecount = A.ecount;
     const int tid = threadIdx.x;
-    int64_t cursor = 0;
+    const int64_t lo = sim_off[blockIdx.x], n = sim_off[blockIdx.x + 1];
+    int64_t cursor = lo;
     int commit = 0;
     if (tid == 0) {
         sh.overflow = 0;
-        s_np = min(ecount[0], kMaxPend);
+        s_np = 0;
     }
     __syncthreads();
-    const bool scan_mode0 = ecount[0] > kMaxPend;
-    for (int k = tid; k < s_np; k += kRepairThreads) s_pend[k] = flag_list[k];
+    for (int k = tid; k < ecount[0]; k += kRepairThreads) {
+        const int i = flag_list[k];
+        if (i >= lo && i < n) {
+            const int idx = atomicAdd(&s_np, 1);
+            if (idx < kMaxPend) s_pend[idx] = i;
+        }
+    }
     __syncthreads();
+    const bool scan_mode0 = s_np > kMaxPend;
     while (true) {
         if (tid == 0) s_q = INT32_MAX;
         __syncthreads();
@@ -722,8 +734,8 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
     c.leaf_crit.grow(nl + 1);
     c.leaf_restr.grow(nl + 1);
     VVHD_HIP(hipMemsetAsync(c.ecount.p, 0, 8 * 4, s));
-    const int nbody = (int)c.bodies_h.size();
-    hipLaunchKernelGGL(k_leaf_criteria, dim3(nblocks(nl)), dim3(kBlock), 0, s, T, c.segs.p, c.body_range.p, nbody,
+    hipLaunchKernelGGL(k_leaf_criteria, dim3(nblocks(nl)), dim3(kBlock), 0, s, T, c.segs.p, c.body_range.p,
+                       (const int32_t*)c.sim_body.p, (const int32_t*)c.t.leaf_sim.p,
                        merge, c.leaf_crit.p, c.leaf_restr.p);
     if (nl > 0)
         hipLaunchKernelGGL(k_eps_top2, dim3((nl + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0, s, T, c.x.p,
@@ -757,7 +769,8 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
                            c.rnear_list.p, c.leaf_stamp.p, c.nn1.p, c.nn2.p, c.d1.p,        c.d2.p,
                            c.eflag.p, c.ecount.p,    c.t.blx.p,   c.t.bly.p, c.t.trx.p,     c.t.try_.p,
                            c.leaf_maxd2.p};
-        hipLaunchKernelGGL(k_eps_repair, dim3(1), dim3(kRepairThreads), 0, s, A, n, c.flag_list.p);
+        hipLaunchKernelGGL(k_eps_repair, dim3(c.nsim), dim3(kRepairThreads), 0, s, A, (const int64_t*)c.sim_off.p,
+                           c.flag_list.p);
     }
     hipLaunchKernelGGL(k_eps_apply, dim3(nblocks(n)), dim3(kBlock), 0, s, T, c.g.p, c.ie.p, n, c.nn1.p, c.nn2.p,
                        c.d1.p, c.d2.p, c.eflag.p, c.leaf_restr.p, c.src_ge.p);

@@ -170,6 +170,7 @@ int vvhd_tree_lists(vvhd_ctx* ctx, int far, int64_t* count, int64_t* offsets, in
 int vvhd_tree_find(vvhd_ctx* ctx, const double* xy, int64_t n, int32_t* leaf_out) {
     if (!ctx) return VVHD_EINVAL;
     API_BEGIN
+    REQUIRE_ONE_SIM(ctx->c, "tree_find");
     Ctx& c = ctx->c;
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "TTree::findNode(): tree is not built");
     VVHD_CHECK(n >= 0 && (n == 0 || (xy && leaf_out)), VVHD_EINVAL, "tree_find: bad arguments");

@@ -68,15 +68,41 @@ __device__ inline void init_node(NodeView nv, int id, int parent, int depth, uin
     nv.try_[id] = ord_from_double(-DBL_MAX);
 }
 
-__global__ void k_init_root(NodeView nv, int n, int nseg, int32_t* counters) {
-    init_node(nv, 0, -1, 0, 0, 0, n);
-    nv.nseg[0] = nseg;
-    counters[0] = 1;   // nodes
-    counters[1] = 0;   // leaves
-    counters[2] = 0;   // level begin
-    counters[3] = 1;   // level end
-    counters[4] = 0;   // overflow flag
-    counters[5] = 0;   // max depth
+// One root per simulation: root s holds the vortices [sim_off[s], sim_off[s+1]).
+__global__ void k_init_roots(NodeView nv, int nroots, const int64_t* sim_off, int32_t* counters) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nroots) init_node(nv, s, -1, 0, 0, (int)sim_off[s], (int)sim_off[s + 1]);
+    if (s == 0) {
+        counters[0] = nroots;   // nodes
+        counters[1] = 0;        // leaves
+        counters[2] = 0;        // level begin
+        counters[3] = nroots;   // level end
+        counters[4] = 0;        // overflow flag
+        counters[5] = 0;        // max depth
+    }
+}
+
+// Element -> root (its simulation).
+__global__ void k_fill_roots(int32_t* enode, int n, const int64_t* sim_off, int nroots) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int lo = 0, hi = nroots - 1;   // last root whose offset <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sim_off[mid] <= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    enode[k] = lo;
+}
+
+// Segments start at the root of their body's simulation.
+__global__ void k_seg_roots(NodeView nv, const int32_t* seg_root, int nseg, int32_t* seg_node) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    seg_node[i] = seg_root[i];
+    atomicAdd(&nv.nseg[seg_root[i]], 1);
 }
 
 // Adds a point to a node's bounding box; NaN coordinates are ignored, as by
@@ -197,10 +223,44 @@ __global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t*
     }
 }
 
-// Segments of the root bounding box.
-__global__ void k_root_segs(NodeView nv, const vvhd_seg* segs, int nseg) {
+// Segments of the root bounding boxes.
+__global__ void k_root_segs(NodeView nv, const vvhd_seg* segs, const int32_t* seg_root, int nseg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nseg) bbox_add(nv, 0, segs[i].x, segs[i].y);
+    if (i < nseg) bbox_add(nv, seg_root[i], segs[i].x, segs[i].y);
+}
+
+// Root bounding boxes when there are more roots than k_bbox_top handles.
+__global__ void k_bbox_elems(NodeView nv, const int32_t* enode, const double* x, const double* y, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool a = k < n;
+    bbox_add_wave(nv, a ? enode[k] : 0, a ? x[k] : 0, a ? y[k] : 0, a);
+}
+
+// First leaf (DFS rank) of every root, from the subtree leaf counts.
+__global__ void k_root_first(const int32_t* nleaves, int nroots, int32_t* first, int32_t* root_first) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int f = 0;
+    for (int s = 0; s < nroots; s++) {
+        first[s] = f;
+        root_first[s] = f;
+        f += nleaves[s];
+    }
+    root_first[nroots] = f;
+}
+
+// Leaf -> simulation.
+__global__ void k_leaf_sim(const int32_t* root_first, int nroots, int nleaf, int32_t* leaf_sim) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf) return;
+    int lo = 0, hi = nroots - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (root_first[mid] <= l)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    leaf_sim[l] = lo;
 }
 
 // snode::Stretch tail + DivideNode decision (TSortedTree.cpp:36-56, 122-125)
@@ -508,11 +568,14 @@ struct ListEntries {
     int lead[kMaxEntries];                 // bit d: first entry beneath anc[d]
 };
 
-__global__ void k_list_entries(NodeView nv, ListEntries* E) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Entry table of every root (thread r: root r).
+__global__ void k_list_entries(NodeView nv, int nroots, ListEntries* Etab) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nroots) return;
+    ListEntries* E = Etab + r;
     int stack[2 * kMaxEntries + 2];
     int sp = 0, n = 0;
-    stack[sp++] = 0;
+    stack[sp++] = r;
     while (sp > 0) {
         const int T = stack[--sp];
         const int d = nv.depth[T];
@@ -544,8 +607,9 @@ __global__ void k_list_entries(NodeView nv, ListEntries* E) {
     E->n = n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries* __restrict__ E,
-                                                  const int32_t* leaf_node, int nleaf, double fc,
+__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries* __restrict__ Etab,
+                                                  const int32_t* __restrict__ groups, int ngroups,
+                                                  const int32_t* leaf_node, double fc,
                                                   const int64_t* near_off, const int64_t* far_off,
                                                   int32_t* near_list, int32_t* far_list, int32_t* near_cnt,
                                                   int32_t* far_cnt, int32_t* err) {
@@ -553,10 +617,14 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries
     const int w = threadIdx.x / kWave;
     const int wave = blockIdx.x * (kBlock / kWave) + w;
     const int q = wave % kMaxEntries;
+    const int grp = wave / kMaxEntries;
+    if (grp >= ngroups) return;
     const int lane = lane_id();
-    const int l = (wave / kMaxEntries) * kWave + lane;
+    // a group: up to 64 DFS-consecutive leaves of one root
+    const int l = groups[3 * grp] + lane;
+    const ListEntries* E = Etab + groups[3 * grp + 2];
     if (q >= E->n) return;   // uniform over the wave
-    const bool valid = l < nleaf;
+    const bool valid = lane < groups[3 * grp + 1];
     if (__ballot(valid) == 0) return;
     int* stack = s_stack[w];
     const int L = valid ? leaf_node[l] : 0;
@@ -705,7 +773,9 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     t.far_criteria = far_criteria;
     t.min_node = min_node;
     t.max_node = max_node;
-    const int cap = 2 * (n + nseg) + 64;
+    sim_sync(c);
+    const int nroots = c.nsim;
+    const int cap = 2 * (n + nseg) + 64 + nroots;
     for (auto* b : {&t.x, &t.y, &t.h, &t.w}) b->grow(cap);
     for (auto* b : {&t.blx, &t.bly, &t.trx, &t.try_, &t.path}) b->grow(cap);
     for (auto* b : {&t.v0, &t.v1, &t.ch1, &t.parent, &t.depth, &t.nseg, &t.split, &t.nleaves, &t.arrive, &t.leaf_rank})
@@ -736,18 +806,27 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     NodeView nv = view(t);
     hipStream_t s = c.stream;
 
-    hipLaunchKernelGGL(k_init_root, dim3(1), dim3(1), 0, s, nv, n, nseg, t.counters.p);
+    hipLaunchKernelGGL(k_init_roots, dim3(nblocks(nroots)), dim3(kBlock), 0, s, nv, nroots, (const int64_t*)c.sim_off.p,
+                       t.counters.p);
     hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(kBlock), 0, s, c.id.p, n);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(n)), dim3(kBlock), 0, s, enode.p, n, 0);
-    if (nseg) hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nseg)), dim3(kBlock), 0, s, seg_node.p, nseg, 0);
-    if (n)
+    hipLaunchKernelGGL(k_fill_roots, dim3(nblocks(n)), dim3(kBlock), 0, s, enode.p, n, (const int64_t*)c.sim_off.p,
+                       nroots);
+    if (nseg)
+        hipLaunchKernelGGL(k_seg_roots, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, (const int32_t*)c.seg_root.p,
+                           nseg, seg_node.p);
+    if (n && nroots <= kTopNodes)
         hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv, (const int32_t*)nullptr,
-                           (const int32_t*)nullptr, c.x.p, c.y.p, n);
-    if (nseg) hipLaunchKernelGGL(k_root_segs, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg);
+                           (const int32_t*)enode.p, c.x.p, c.y.p, n);
+    else if (n)
+        hipLaunchKernelGGL(k_bbox_elems, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, (const int32_t*)enode.p, c.x.p,
+                           c.y.p, n);
+    if (nseg)
+        hipLaunchKernelGGL(k_root_segs, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p,
+                           (const int32_t*)c.seg_root.p, nseg);
 
     std::vector<std::pair<int, int>>& levels = t.levels;   // node id range of every level
     levels.clear();
-    levels.push_back({0, 1});
+    levels.push_back({0, nroots});
     int32_t hc[8];
     while (true) {
         const int node_blocks = 1024;
@@ -759,8 +838,8 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
                            node_m.p);
         hipLaunchKernelGGL(k_ranks, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, left_pos,
                            right_pos);
-        // the children of level d number at most 2^(d+1)
-        const bool top = (1 << std::min<int>(levels.size(), 30)) <= kTopNodes;
+        // the children of level d number at most nroots 2^(d+1)
+        const bool top = ((int64_t)nroots << std::min<int>(levels.size(), 30)) <= kTopNodes;
         hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
                            left_pos, right_pos, c.x.p, c.y.p, c.id.p, !top);
         if (top && n)
@@ -798,12 +877,33 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
                            c.g.p, t.nleaves.p);
     }
     int32_t* first = t.arrive.p;
-    VVHD_HIP(hipMemsetAsync(first, 0, 4, s));
+    t.root_first.grow(nroots + 1);
+    hipLaunchKernelGGL(k_root_first, dim3(1), dim3(1), 0, s, t.nleaves.p, nroots, first, t.root_first.p);
     for (int d = 0; d < (int)levels.size(); d++) {
         const int lb = levels[d].first, le = levels[d].second;
         hipLaunchKernelGGL(k_down_level, dim3(nblocks(le - lb)), dim3(kBlock), 0, s, nv, lb, le, t.nleaves.p, first,
                            t.leaf_rank.p, t.leaf_node.p, t.vleaf.p);
     }
+    // leaf -> simulation and the list builder's leaf groups (64 leaves of one root)
+    t.leaf_sim.grow(nl + 1);
+    std::vector<int32_t> grp;
+    if (nroots == 1) {
+        if (nl) hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nl)), dim3(kBlock), 0, s, t.leaf_sim.p, nl, 0);
+        for (int f = 0; f < nl; f += kWave) grp.insert(grp.end(), {f, std::min(kWave, nl - f), 0});
+    } else {
+        std::vector<int32_t> rf(nroots + 1);
+        VVHD_HIP(hipMemcpyAsync(rf.data(), t.root_first.p, (nroots + 1) * 4, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+        if (nl)
+            hipLaunchKernelGGL(k_leaf_sim, dim3(nblocks(nl)), dim3(kBlock), 0, s, (const int32_t*)t.root_first.p,
+                               nroots, nl, t.leaf_sim.p);
+        for (int r = 0; r < nroots; r++)
+            for (int f = rf[r]; f < rf[r + 1]; f += kWave) grp.insert(grp.end(), {f, std::min(kWave, rf[r + 1] - f), r});
+    }
+    t.ngroups = (int)grp.size() / 3;
+    t.groups.grow(grp.size() + 3);
+    if (!grp.empty())
+        VVHD_HIP(hipMemcpyAsync(t.groups.p, grp.data(), grp.size() * 4, hipMemcpyHostToDevice, s));
     // segments per leaf
     t.seg_off.grow(nl + 1);
     t.seg_leaf.grow(nseg + 1);
@@ -834,19 +934,20 @@ void tree_lists(Ctx& c) {
     t.slot_far.grow(nslot + 1);
     t.near_off.grow(nl + 1);
     t.far_off.grow(nl + 1);
-    t.entries.grow(sizeof(ListEntries));
+    const int nroots = c.nsim;
+    t.entries.grow(nroots * sizeof(ListEntries));
     ListEntries* E = reinterpret_cast<ListEntries*>(t.entries.p);
-    hipLaunchKernelGGL(k_list_entries, dim3(1), dim3(1), 0, s, nv, E);
+    hipLaunchKernelGGL(k_list_entries, dim3(nblocks(nroots, 64)), dim3(64), 0, s, nv, nroots, E);
     VVHD_HIP(hipMemsetAsync(cnt_n.p, 0, (nslot + 1) * 4, s));
     VVHD_HIP(hipMemsetAsync(cnt_f.p, 0, (nslot + 1) * 4, s));
     const double fc = (double)t.far_criteria;
-    const int waves = ((nl + kWave - 1) / kWave) * kMaxEntries;
-    const int grid = (waves + kBlock / kWave - 1) / (kBlock / kWave);
+    const int64_t waves = (int64_t)t.ngroups * kMaxEntries;
+    const int grid = (int)std::max<int64_t>(1, (waves + kBlock / kWave - 1) / (kBlock / kWave));
     t.err.grow(1);
     VVHD_HIP(hipMemsetAsync(t.err.p, 0, 4, s));
-    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E, t.leaf_node.p, nl, fc,
-                       (const int64_t*)nullptr, (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
-                       cnt_n.p, cnt_f.p, t.err.p);
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E,
+                       (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, (const int64_t*)nullptr,
+                       (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, cnt_n.p, cnt_f.p, t.err.p);
     scan_i32_to_i64(c, cnt_n.p, t.slot_near.p, (int)nslot);
     scan_i32_to_i64(c, cnt_f.p, t.slot_far.p, (int)nslot);
     hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_near.p, nl, t.near_off.p);
@@ -862,9 +963,10 @@ void tree_lists(Ctx& c) {
     t.nfar = tot[1];
     t.near_list.grow(t.nnear + 1);
     t.far_list.grow(t.nfar + 1);
-    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E, t.leaf_node.p, nl, fc,
-                       (const int64_t*)t.slot_near.p, (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p,
-                       (int32_t*)nullptr, (int32_t*)nullptr, t.err.p);
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E,
+                       (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, (const int64_t*)t.slot_near.p,
+                       (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p, (int32_t*)nullptr, (int32_t*)nullptr,
+                       t.err.p);
 }
 
 }  // namespace vvhd
