@@ -71,6 +71,10 @@ def parse():
                     help="4M-vortex wake: FP32 pair sums plain vs Kahan against the FP64 direct sum, and the "
                          "2000-segment body SLAE filled and solved in FP64 (one JSON line)")
     ap.add_argument("--sweep-fcs", default="8,64,256")
+    ap.add_argument("--batched", action="store_true",
+                    help="64 independent cylinder simulations of uneven size in one context vs one at a time "
+                         "(one JSON line)")
+    ap.add_argument("--nsim", type=int, default=64)
     return ap.parse_args()
 
 
@@ -270,8 +274,85 @@ def precision_sweep(a):
     print(json.dumps(out), flush=True)
 
 
+def batched(a):
+    """Many independent simulations advanced together: nsim Karman-street
+    wakes of n_s vortices (n_s log-uniform in [10k, 400k], SplitMix64 seeds
+    0x5EED1000 + s) with Reynolds numbers log-uniform in [100, 3000].  One
+    step = tree build (far criterion 8) + epsilon (merging) + convective
+    (plain, FP32 Kahan) + diffusive, from a device-resident copy restored
+    each step.  Sequential: each simulation alone in the context, its mean
+    step time.  Batched: all simulations in one context (vvhd_set_batch), one
+    step for all."""
+    import vvflow_amd as V
+    rng = np.random.default_rng(0x5EED0040)
+    ns = np.exp(rng.uniform(np.log(10_000), np.log(400_000), a.nsim)).astype(np.int64)
+    res = np.exp(rng.uniform(np.log(100.0), np.log(3000.0), a.nsim))
+    wakes = [V.gen_wake(int(ns[s]), 0x5EED1000 + s) for s in range(a.nsim)]
+    ctx = V.Context(0)
+    ctx.set_params(re=600.0, dt=0.005)
+    ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
+    fc = 8
+
+    def step():
+        ctx.restore()
+        ctx.tree_build(fc)
+        ctx.epsilon(True)
+        ctx.convective((1.0, 0.0), kahan=True)
+        ctx.diffusive()
+
+    def timed(k, w):
+        for _ in range(w):
+            step()
+        ctx.phase_times()   # synchronises
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        ctx.phase_times()
+        return (time.perf_counter() - t0) / k
+
+    seq = []
+    for s in range(a.nsim):
+        ctx.set_params(re=float(res[s]), dt=0.005)
+        ctx.upload(wakes[s])
+        ctx.snapshot()
+        seq.append(timed(a.steps, 1))
+        if (s + 1) % 8 == 0:
+            print(f"sequential: {s + 1}/{a.nsim} simulations", file=sys.stderr, flush=True)
+    seq = np.array(seq)
+    ctx.set_params(re=600.0, dt=0.005)
+    ctx.upload(np.concatenate(wakes))
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    ctx.set_batch(off, np.zeros(a.nsim + 1, dtype=np.int32), res)
+    ctx.snapshot()
+    t_batch = timed(a.steps, a.warmup)
+    print("batched: done", file=sys.stderr, flush=True)
+    ph = ctx.phase_times()
+    ctx.close()
+    total = int(ns.sum())
+    out = {"metric": "aggregate particle-steps/s, independent simulations advanced together",
+           "value": total / t_batch, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": t_batch * 1e3, "higher_is_better": True, "dtype": "fp32",
+           "data": "synthetic",
+           "config": {"workload": f"{a.nsim} Karman-street wakes of {int(ns.min())}-{int(ns.max())} vortices "
+                                  f"({total} in all), Re {res.min():.0f}-{res.max():.0f}; one step = tree + "
+                                  "epsilon (merge) + convective (plain, Kahan FP32) + diffusive",
+                      "nsim": a.nsim, "far_criteria": fc},
+           "batched": {"ms_per_step": t_batch * 1e3, "per_simulation_step_ms_p50": t_batch * 1e3,
+                       "per_simulation_step_ms_p99": t_batch * 1e3},
+           "sequential": {"ms_all_simulations": float(seq.sum() * 1e3), "value": total / float(seq.sum()),
+                          "per_simulation_step_ms_p50": float(np.percentile(seq, 50) * 1e3),
+                          "per_simulation_step_ms_p99": float(np.percentile(seq, 99) * 1e3)},
+           "speedup_vs_sequential": float(seq.sum() / t_batch),
+           "phases_ms_batched": {"tree": float(ph[0]), "epsilon": float(ph[1]), "convective": float(ph[2]),
+                                 "diffusive": float(ph[3])}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.batched:
+        batched(a)
+        return
     if a.copy_calibration:
         copy_calibration()
         return
