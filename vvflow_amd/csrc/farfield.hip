@@ -86,15 +86,29 @@ constexpr int fA2 = fA1 + 2 * kSet;  // 8 sets: W2, (M block, K step)
 constexpr int fA3 = fA2 + 8 * kSet;  // 4 sets: W3 (rows 0,1 real), K steps
 constexpr int fA3T = fA3 + 4 * kSet; // 2 sets: W3^T, M blocks of hidden units, K = outputs
 constexpr int fA2T = fA3T + 2 * kSet;// 8 sets: W2^T, (M block of input units, K step over output units)
-constexpr int kFragElems = fA2T + 8 * kSet;
+// inference fragments for v_mfma_f32_16x16x32_bf16 (M blocks of 16 units, K steps of 32)
+constexpr int fB1 = fA2T + 8 * kSet; // 4 sets: W1, M blocks, K = 32 feature slots
+constexpr int fB2 = fB1 + 4 * kSet;  // 8 sets: W2, (M block b, K step st) = 2 b + st
+constexpr int fB3 = fB2 + 8 * kSet;  // 2 sets: W3 (rows 0, 1 real), K steps
+constexpr int kFragElems = fB3 + 2 * kSet;
 
 constexpr int kMom = 12;   // gp, gm, zpx, zpy, zmx, zmy, qpr, qpi, qmr, qmi, ep, em
 constexpr int kPartStride = kTrain + 2;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+__device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// 16x16x32 layout: A row l%16, K 8(l/16)+e; B column l%16, K 8(l/16)+e;
+// C column l%16, rows 4(l/16)+i.  Hidden unit held at (K step st, lane
+// group g, element e) when a layer's output blocks feed the next layer.
+__host__ __device__ inline int chain16(int st, int g, int e) { return 16 * (2 * st + (e >> 2)) + 4 * g + (e & 3); }
 
 __device__ inline f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
@@ -324,10 +338,30 @@ __global__ void k_pack_model(const float* P, uint16_t* frag) {
         decode(fA3);
     } else if (t < fA2T) {
         decode(fA3T);
-    } else {
+    } else if (t < fB1) {
         decode(fA2T);
+    } else if (t < fB2) {
+        decode(fB1);
+    } else if (t < fB3) {
+        decode(fB2);
+    } else {
+        decode(fB3);
     }
     const int lane = rem / 8, e = rem % 8, m = lane % 32, g = lane / 32;
+    if (t >= fB1) {   // 16x16x32 inference sets
+        const int m16 = lane % 16, g16 = lane / 16;
+        if (t < fB2) {   // W1 block set: rows 16 set + m16, K = feature 8 g16 + e
+            const int k = 8 * g16 + e;
+            if (k < kFeat) v = P[oW1 + (16 * set + m16) * kFeat + k];
+        } else if (t < fB3) {   // W2: block b = set / 2, K step st = set % 2
+            const int b = set / 2, st = set % 2;
+            v = P[oW2 + (16 * b + m16) * kHid + chain16(st, g16, e)];
+        } else {   // W3 rows 0, 1, K step set
+            if (m16 < kOut) v = P[oW3 + m16 * kHid + chain16(set, g16, e)];
+        }
+        frag[t] = bf16_bits(v);
+        return;
+    }
     if (t < fA2) {   // W1: A[m = unit][k = feature]
         const int k = 8 * g + e;
         if (k < kFeat) v = P[oW1 + (32 * set + m) * kFeat + k];
@@ -403,91 +437,61 @@ __global__ void k_far_nodes(TreeDev T, MomView M, int nnodes, FarNode* out) {
     out[F] = r;
 }
 
-// Inference forward of one tile; biases of the hidden layers come from LDS
-// (sb1, sb2: 64 floats each), the output biases from scalar arguments.
-// Returns the two outputs in lanes 0..31.
-__device__ inline void mlp_infer(const Weights& W, const float* sb1, const float* sb2, float b30, float b31,
-                                 bf16x8 x, float& y0, float& y1) {
-    const int g = lane_id() >> 5;
-    bf16x8 h[4];
-    for (int b = 0; b < 2; b++) {
-        f32x16 c;
-        for (int j = 0; j < 4; j++)
-            for (int i = 0; i < 4; i++) c[4 * j + i] = sb1[32 * b + 8 * j + 4 * g + i];
-        c = mfma32(W.a1[b], x, c);
-        for (int e = 0; e < 16; e++) h[2 * b + (e >> 3)][e & 7] = (__bf16)silu_f(c[e]);
+// Inference forward of one tile of 16 columns with two forward-mode
+// tangents, on v_mfma_f32_16x16x32_bf16: the tangent inputs tx, ty
+// (standardised by the deviations only) pass through the same weights, and
+// each SiLU scales them by its derivative s (1 + z (1 - s)), s = sigmoid(z);
+// tangents are rounded to bf16 where the activations are.  A layer's four
+// output blocks (16 units each) feed the next layer's two K steps without a
+// transpose (chain16).  Outputs (value, x tangent, y tangent) in lanes 0..15.
+// The weight fragments come from LDS (sw: W1, W2, W3 sets, 64 lanes each).
+__device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
+                                       bf16x8 x, bf16x8 tx, bf16x8 ty, float (&y)[2], float (&yx)[2],
+                                       float (&yy)[2]) {
+    const int ln = lane_id(), g = ln >> 4;
+    auto W = [&](int set) { return sw[set * kWave + ln]; };
+    bf16x8 h[2], hx[2], hy[2];
+    for (int b = 0; b < 4; b++) {
+        f32x4 c, cx = {}, cy = {};
+        for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i];
+        const bf16x8 a = W((fB1 - fB1) / kSet + b);
+        c = mfma16(a, x, c);
+        cx = mfma16(a, tx, cx);
+        cy = mfma16(a, ty, cy);
+        for (int i = 0; i < 4; i++) {
+            const float z = c[i], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
+            const int st = b >> 1, e = 4 * (b & 1) + i;
+            h[st][e] = (__bf16)(z * sg);
+            hx[st][e] = (__bf16)(d * cx[i]);
+            hy[st][e] = (__bf16)(d * cy[i]);
+        }
     }
-    bf16x8 h2[4];
-    for (int b = 0; b < 2; b++) {
-        f32x16 c;
-        for (int j = 0; j < 4; j++)
-            for (int i = 0; i < 4; i++) c[4 * j + i] = sb2[32 * b + 8 * j + 4 * g + i];
-        for (int st = 0; st < 4; st++) c = mfma32(W.a2[4 * b + st], h[st], c);
-        for (int e = 0; e < 16; e++) h2[2 * b + (e >> 3)][e & 7] = (__bf16)silu_f(c[e]);
+    bf16x8 h2[2], h2x[2], h2y[2];
+    for (int b = 0; b < 4; b++) {
+        f32x4 c, cx = {}, cy = {};
+        for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i];
+        for (int st = 0; st < 2; st++) {
+            const bf16x8 a = W((fB2 - fB1) / kSet + 2 * b + st);
+            c = mfma16(a, h[st], c);
+            cx = mfma16(a, hx[st], cx);
+            cy = mfma16(a, hy[st], cy);
+        }
+        for (int i = 0; i < 4; i++) {
+            const float z = c[i], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
+            const int st = b >> 1, e = 4 * (b & 1) + i;
+            h2[st][e] = (__bf16)(z * sg);
+            h2x[st][e] = (__bf16)(d * cx[i]);
+            h2y[st][e] = (__bf16)(d * cy[i]);
+        }
     }
-    f32x16 c = {};
+    f32x4 c = {}, cx = {}, cy = {};
     c[0] = g == 0 ? b30 : 0.f;
     c[1] = g == 0 ? b31 : 0.f;
-    for (int st = 0; st < 4; st++) c = mfma32(W.a3[st], h2[st], c);
-    y0 = c[0];
-    y1 = c[1];
-}
-
-// Inference forward of one tile with two forward-mode tangents: the tangent
-// inputs tx, ty (standardised by the deviations only) pass through the same
-// weights, and each SiLU scales them by its derivative s (1 + z (1 - s)),
-// s = sigmoid(z); tangents are rounded to bf16 where the activations are.
-// Outputs (value, x tangent, y tangent) in lanes 0..31.  The weight
-// fragments come from LDS (sw: the W1, W2, W3 sets, 64 lanes each).
-__device__ inline void mlp_infer_jvp(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
-                                     bf16x8 x, bf16x8 tx, bf16x8 ty, float (&y)[2], float (&yx)[2],
-                                     float (&yy)[2]) {
-    const int g = lane_id() >> 5, ln = lane_id();
-    auto A1 = [&](int b) { return sw[(fA1 / kSet + b) * kWave + ln]; };
-    auto A2 = [&](int k) { return sw[(fA2 / kSet + k) * kWave + ln]; };
-    auto A3 = [&](int k) { return sw[(fA3 / kSet + k) * kWave + ln]; };
-    bf16x8 h[4], hx[4], hy[4];
-    for (int b = 0; b < 2; b++) {
-        f32x16 c, cx = {}, cy = {};
-        for (int j = 0; j < 4; j++)
-            for (int i = 0; i < 4; i++) c[4 * j + i] = sb1[32 * b + 8 * j + 4 * g + i];
-        const bf16x8 a = A1(b);
-        c = mfma32(a, x, c);
-        cx = mfma32(a, tx, cx);
-        cy = mfma32(a, ty, cy);
-        for (int e = 0; e < 16; e++) {
-            const float z = c[e], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
-            h[2 * b + (e >> 3)][e & 7] = (__bf16)(z * sg);
-            hx[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cx[e]);
-            hy[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cy[e]);
-        }
-    }
-    bf16x8 h2[4], h2x[4], h2y[4];
-    for (int b = 0; b < 2; b++) {
-        f32x16 c, cx = {}, cy = {};
-        for (int j = 0; j < 4; j++)
-            for (int i = 0; i < 4; i++) c[4 * j + i] = sb2[32 * b + 8 * j + 4 * g + i];
-        for (int st = 0; st < 4; st++) {
-            const bf16x8 a = A2(4 * b + st);
-            c = mfma32(a, h[st], c);
-            cx = mfma32(a, hx[st], cx);
-            cy = mfma32(a, hy[st], cy);
-        }
-        for (int e = 0; e < 16; e++) {
-            const float z = c[e], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
-            h2[2 * b + (e >> 3)][e & 7] = (__bf16)(z * sg);
-            h2x[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cx[e]);
-            h2y[2 * b + (e >> 3)][e & 7] = (__bf16)(d * cy[e]);
-        }
-    }
-    f32x16 c = {}, cx = {}, cy = {};
-    c[0] = g == 0 ? b30 : 0.f;
-    c[1] = g == 0 ? b31 : 0.f;
-    for (int st = 0; st < 4; st++) {
-        const bf16x8 a = A3(st);
-        c = mfma32(a, h2[st], c);
-        cx = mfma32(a, h2x[st], cx);
-        cy = mfma32(a, h2y[st], cy);
+    for (int st = 0; st < 2; st++) {
+        const bf16x8 a = W((fB3 - fB1) / kSet + st);
+        c = mfma16(a, h2[st], c);
+        cx = mfma16(a, h2x[st], cx);
+        cy = mfma16(a, h2y[st], cy);
     }
     y[0] = c[0];
     y[1] = c[1];
@@ -509,16 +513,17 @@ __device__ inline void mlp_infer_jvp(const bf16x8* sw, const float* sb1, const f
 // Feature derivatives: with theta the angle of u, d theta / dp = (-uy, ux)/r;
 // the rotated moments turn with theta (d(f2 + i f3)/d theta = f3 - i f2, the
 // quadrupole twice as fast), d/r scales as 1/r, the rest is constant.
-// A tile holds 32 far nodes (column l%32); lane group g (lanes 32g..32g+31)
-// builds inputs 8g..8g+7 and their tangents.  Output per leaf: 6 doubles
+// A tile holds 16 far nodes (column l%16); lane groups 0 and 1 (lanes
+// 16g..16g+15) build inputs 8g..8g+7 and their tangents, groups 2 and 3 the
+// zero padding of the K step.  Output per leaf: 6 doubles
 // (u, v, du/dx, dv/dx, du/dy, dv/dy); zeros for a leaf without far nodes or
 // without a target of non-zero circulation.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) void k_far_learned(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_far_learned(
     TreeDev T, const FarNode* __restrict__ nodes, const double* __restrict__ g, const uint16_t* frag, const float* P,
     double* __restrict__ leaf_corr) {
-    constexpr int kFwdSets = fA3T / kSet;   // W1, W2, W3 fragment sets
+    constexpr int kInfSets = (kFragElems - fB1) / kSet;   // W1, W2, W3 inference sets
     __shared__ float sb1[kHid], sb2[kHid], smu[16], sisd[16];
-    __shared__ uint4 sw[kFwdSets * kWave];
+    __shared__ uint4 sw[kInfSets * kWave];
     for (int k = threadIdx.x; k < kHid; k += blockDim.x) {
         sb1[k] = P[oB1 + k];
         sb2[k] = P[oB2 + k];
@@ -527,12 +532,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
             sisd[k] = k < kFeat ? 1.f / P[oSd + k] : 0.f;
         }
     }
-    for (int k = threadIdx.x; k < kFwdSets * kWave; k += blockDim.x) sw[k] = reinterpret_cast<const uint4*>(frag)[k];
+    for (int k = threadIdx.x; k < kInfSets * kWave; k += blockDim.x)
+        sw[k] = reinterpret_cast<const uint4*>(frag + fB1)[k];
     __syncthreads();
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
-    const int lane = lane_id(), grp = lane >> 5, n = lane & 31;
+    const int lane = lane_id(), grp = lane >> 4, n = lane & 15;
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
@@ -550,15 +556,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, cmp[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // Kahan sums
     // the node record of the next tile is loaded while this tile computes
     FarNode nd = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
-    for (int64_t f0 = fb; f0 < fe; f0 += 32) {
+    for (int64_t f0 = fb; f0 < fe; f0 += 16) {
         const int64_t e = f0 + n;
-        const int64_t en = e + 32;
+        const int64_t en = e + 16;
         const FarNode nxt = nodes[(en < fe) ? T.far_list[en] : 0];
         const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
         const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
         const float ux = dx * rinv, uy = dy * rinv;
         const float thx = -uy * rinv, thy = ux * rinv;   // d theta / dp
-        const bool ok = e < fe && nd.ok != 0.f;
+        const bool ok = e < fe && nd.ok != 0.f && grp < 2;
         float f[8], fx[8], fy[8];
         if (grp == 0) {
             const float c2r = ux * ux - uy * uy, c2i = -2.f * ux * uy;
@@ -587,13 +593,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
         }
         bf16x8 xin, txin, tyin;
         for (int q = 0; q < 8; q++) {
-            const float is = sisd[8 * grp + q];
-            xin[q] = (__bf16)(ok ? (f[q] - smu[8 * grp + q]) * is : 0.f);
+            const int k = (8 * grp + q) & 15;
+            const float is = sisd[k];
+            xin[q] = (__bf16)(ok ? (f[q] - smu[k]) * is : 0.f);
             txin[q] = (__bf16)(ok ? fx[q] * is : 0.f);
             tyin[q] = (__bf16)(ok ? fy[q] * is : 0.f);
         }
         float y[2], yx[2], yy[2];
-        mlp_infer_jvp(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
+        mlp_infer_jvp16(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
         if (grp == 0 && ok) {
             const float sc = nd.scl * rinv * rinv * rinv;
             const float a = y[0] * ysd0, b = y[1] * ysd1;
@@ -615,10 +622,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) voi
         }
         nd = nxt;
     }
-    // sums over the 32 columns (lanes 0..31), FP64, fixed order
+    // sums over the 16 columns (lanes 0..15), FP64, fixed order
     for (int q = 0; q < 6; q++) {
         double v = grp == 0 ? (double)acc[q] - (double)cmp[q] : 0.;
-        for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
         if (lane == 0) out[q] = v;
     }
 }
