@@ -146,6 +146,7 @@ struct TreeArrays {
     DBuf<int32_t> cnt_near, cnt_far, err;               // list scratch
     DBuf<int64_t> slot_near, slot_far;                  // per (leaf, top entry) list offsets
     DBuf<uint8_t> entries;                              // top-of-tree entry tables of the list builder (per root)
+    DBuf<uint8_t> trav;                                 // packed node records of the list builder
     DBuf<int32_t> groups;                               // list builder leaf groups: {first leaf, count, root}
     int ngroups = 0;
     DBuf<int32_t> leaf_sim;                             // per leaf: its simulation (root)

@@ -197,6 +197,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     for (auto* b : {&t.near_off, &t.far_off, &t.seg_off, &t.slot_near, &t.slot_far}) b->release();
     for (auto* b : {&t.groups, &t.leaf_sim, &t.root_first}) b->release();
     t.entries.release();
+    t.trav.release();
     t.cmp_ie.release();
     t.cmm_ie.release();
     for (auto& e : c.ev)

@@ -607,7 +607,28 @@ __global__ void k_list_entries(NodeView nv, int nroots, ListEntries* Etab) {
     E->n = n;
 }
 
-__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries* __restrict__ Etab,
+// What the traversal reads of a node, in one 32-byte record: centre, h + w
+// (added first, as the far test does), depth and first child.
+struct TravNode {
+    double x, y, hw;
+    int32_t depth, ch1;
+};
+static_assert(sizeof(TravNode) == 32, "TravNode layout");
+
+__global__ void k_trav_nodes(NodeView nv, int nnodes, TravNode* out) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nnodes) return;
+    TravNode r;
+    r.x = nv.x[id];
+    r.y = nv.y[id];
+    r.hw = nv.h[id] + nv.w[id];
+    r.depth = nv.depth[id];
+    r.ch1 = nv.ch1[id];
+    out[id] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const TravNode* __restrict__ tn,
+                                                  const ListEntries* __restrict__ Etab,
                                                   const int32_t* __restrict__ groups, int ngroups,
                                                   const int32_t* leaf_node, double fc,
                                                   const int64_t* near_off, const int64_t* far_off,
@@ -628,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries
     if (__ballot(valid) == 0) return;
     int* stack = s_stack[w];
     const int L = valid ? leaf_node[l] : 0;
-    const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];
+    const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];   // (h_T + w_T) + lh + lw below
     const size_t slot = (size_t)l * kMaxEntries + q;
     int64_t pn = (valid && near_off) ? near_off[slot] : 0, pf = (valid && far_off) ? far_off[slot] : 0;
     int cn = 0, cf = 0;
@@ -637,8 +658,9 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries
     const int dE = E->depth[q];
     for (int k = 0; k < dE && valid; k++) {
         const int A = E->anc[q][k];
-        const double dx = nv.x[A] - lx, dy = nv.y[A] - ly;
-        const double hp = nv.h[A] + nv.w[A] + lh + lw;
+        const TravNode a = tn[A];
+        const double dx = a.x - lx, dy = a.y - ly;
+        const double hp = a.hw + lh + lw;
         if (dx * dx + dy * dy > fc * hp * hp) {
             if ((E->lead[q] >> k) & 1) {
                 if (far_list) far_list[pf++] = A;
@@ -654,18 +676,19 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const ListEntries
     while (sp > 0) {
         const int T = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
         sp--;
-        const int dT = nv.depth[T];
+        const TravNode r = tn[T];
+        const int dT = r.depth;
         if (skip_depth != INT32_MAX && dT <= skip_depth) skip_depth = INT32_MAX;
         const bool act = valid && skip_depth == INT32_MAX;
-        const double dx = nv.x[T] - lx, dy = nv.y[T] - ly;
-        const double hp = nv.h[T] + nv.w[T] + lh + lw;
+        const double dx = r.x - lx, dy = r.y - ly;
+        const double hp = r.hw + lh + lw;
         const bool far = act && (dx * dx + dy * dy > fc * hp * hp);
         if (far) {
             if (far_list) far_list[pf++] = T;
             cf++;
             skip_depth = dT;
         }
-        const int c = nv.ch1[T];
+        const int c = r.ch1;
         const bool need = act && !far;
         if (c >= 0) {
             if (__ballot(need) != 0) {
@@ -935,6 +958,9 @@ void tree_lists(Ctx& c) {
     t.near_off.grow(nl + 1);
     t.far_off.grow(nl + 1);
     const int nroots = c.nsim;
+    t.trav.grow((size_t)t.nnodes * sizeof(TravNode));
+    TravNode* tn = reinterpret_cast<TravNode*>(t.trav.p);
+    hipLaunchKernelGGL(k_trav_nodes, dim3(nblocks(t.nnodes)), dim3(kBlock), 0, s, nv, t.nnodes, tn);
     t.entries.grow(nroots * sizeof(ListEntries));
     ListEntries* E = reinterpret_cast<ListEntries*>(t.entries.p);
     hipLaunchKernelGGL(k_list_entries, dim3(nblocks(nroots, 64)), dim3(64), 0, s, nv, nroots, E);
@@ -945,7 +971,7 @@ void tree_lists(Ctx& c) {
     const int grid = (int)std::max<int64_t>(1, (waves + kBlock / kWave - 1) / (kBlock / kWave));
     t.err.grow(1);
     VVHD_HIP(hipMemsetAsync(t.err.p, 0, 4, s));
-    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E,
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const TravNode*)tn, (const ListEntries*)E,
                        (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, (const int64_t*)nullptr,
                        (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, cnt_n.p, cnt_f.p, t.err.p);
     scan_i32_to_i64(c, cnt_n.p, t.slot_near.p, (int)nslot);
@@ -963,7 +989,7 @@ void tree_lists(Ctx& c) {
     t.nfar = tot[1];
     t.near_list.grow(t.nnear + 1);
     t.far_list.grow(t.nfar + 1);
-    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const ListEntries*)E,
+    hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const TravNode*)tn, (const ListEntries*)E,
                        (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, (const int64_t*)t.slot_near.p,
                        (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p, (int32_t*)nullptr, (int32_t*)nullptr,
                        t.err.p);
