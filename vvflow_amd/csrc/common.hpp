@@ -147,6 +147,7 @@ struct TreeArrays {
     DBuf<int64_t> slot_near, slot_far;                  // per (leaf, top entry) list offsets
     DBuf<uint8_t> entries;                              // top-of-tree entry tables of the list builder (per root)
     DBuf<uint8_t> trav;                                 // packed node records of the list builder
+    DBuf<double4> pbox;                                 // packed node boxes (pack_boxes)
     DBuf<int32_t> groups;                               // list builder leaf groups: {first leaf, count, root}
     int ngroups = 0;
     DBuf<int32_t> leaf_sim;                             // per leaf: its simulation (root)
@@ -258,6 +259,7 @@ struct TreeDev {
     const double *cmp_x, *cmp_y, *cmp_g, *cmm_x, *cmm_y, *cmm_g;
     const float *cmp_ie, *cmm_ie;
     int nleaf;
+    const double4* pbox;   // per node {blx, bly, trx, try} as doubles (pack_boxes)
 };
 
 inline TreeDev tree_dev(const TreeArrays& t) {
@@ -265,7 +267,16 @@ inline TreeDev tree_dev(const TreeArrays& t) {
                    t.try_.p,    t.v0.p,     t.v1.p,      t.ch1.p,     t.leaf_rank.p, t.leaf_node.p, t.near_off.p,
                    t.far_off.p, t.seg_off.p, t.near_list.p, t.far_list.p, t.seg_list.p, t.vleaf.p,  t.cmp_x.p,
                    t.cmp_y.p,   t.cmp_g.p,  t.cmm_x.p,   t.cmm_y.p,   t.cmm_g.p,   t.cmp_ie.p,  t.cmm_ie.p,
-                   t.nleaves_h};
+                   t.nleaves_h, t.pbox.p};
+}
+
+// box_dist2 from the packed copy of the boxes: one 32-byte load per node.
+// Valid while the boxes are those pack_boxes last copied.
+__device__ inline double box_dist2p(const TreeDev& T, int a, int b) {
+    const double4 A = T.pbox[a], B = T.pbox[b];
+    double dx = fmax(0.0, fmax(B.x - A.z, A.x - B.z));
+    double dy = fmax(0.0, fmax(B.y - A.w, A.y - B.w));
+    return dx * dx + dy * dy;
 }
 
 // Box-to-box squared distance of two nodes (0 when the boxes touch or
@@ -286,6 +297,8 @@ void ensure_vortex_capacity(Ctx& c, int64_t n);
 // Refreshes the device copies of the simulation table (one simulation over
 // the whole context unless a batch is declared).
 void sim_sync(Ctx& c);
+// Copies the node boxes (ordered-integer images) into TreeArrays::pbox.
+void pack_boxes(Ctx& c);
 void reserve_vortices(Ctx& c, int64_t n, bool preserve);
 void sync(Ctx& c);
 

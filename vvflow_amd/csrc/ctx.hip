@@ -198,6 +198,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     for (auto* b : {&t.groups, &t.leaf_sim, &t.root_first}) b->release();
     t.entries.release();
     t.trav.release();
+    t.pbox.release();
     t.cmp_ie.release();
     t.cmm_ie.release();
     for (auto& e : c.ev)

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
             int cnt = 0, j0 = 0;
             if (e < e1) {
                 const int M = T.near_list[e];
-                if (box_dist2(T, L, M) <= R2) {
+                if (box_dist2p(T, L, M) <= R2) {
                     j0 = T.v0[M];
                     cnt = T.v1[M] - j0;
                 }
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
                 if (e < e1) {
                     M = T.near_list[e];
                     const int m = T.leaf_rank[M];
-                    q = T.seg_off[m + 1] > T.seg_off[m] && box_dist2(T, L, M) <= R2;
+                    q = T.seg_off[m + 1] > T.seg_off[m] && box_dist2p(T, L, M) <= R2;
                 }
                 uint64_t mask = __ballot(q);
                 while (mask) {

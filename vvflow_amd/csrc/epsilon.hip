@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_eps_top2(TreeDev T, const double* x,
                 int cnt = 0, j0 = 0;
                 if (e < e1) {
                     const int M = T.near_list[e];
-                    const double bd = box_dist2(T, L, M);
+                    const double bd = box_dist2p(T, L, M);
                     if (bd > lo && bd <= hi) {
                         j0 = T.v0[M];
                         cnt = T.v1[M] - j0;
@@ -779,6 +779,7 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
     if (nn2_out && n) VVHD_HIP(hipMemcpyAsync(nn2_out, c.nn2.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
     VVHD_CHECK(hc[2] == 0, VVHD_EUNSUPPORTED, "epsilon: too many targets to rescan after one merge");
+    if (hc[1] > 0) pack_boxes(c);   // merges grew leaf boxes
     if (merged_out) *merged_out = hc[1];
 }
 

@@ -607,6 +607,13 @@ __global__ void k_list_entries(NodeView nv, int nroots, ListEntries* Etab) {
     E->n = n;
 }
 
+__global__ void k_pack_boxes(NodeView nv, int nnodes, double4* out) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nnodes) return;
+    out[id] = make_double4(double_from_ord(nv.blx[id]), double_from_ord(nv.bly[id]), double_from_ord(nv.trx[id]),
+                           double_from_ord(nv.try_[id]));
+}
+
 // What the traversal reads of a node, in one 32-byte record: centre, h + w
 // (added first, as the far test does), depth and first child.
 struct TravNode {
@@ -941,6 +948,15 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     }
     t.built = true;
     tree_lists(c);
+    pack_boxes(c);
+}
+
+void pack_boxes(Ctx& c) {
+    TreeArrays& t = c.t;
+    t.pbox.grow(std::max(t.nnodes, 1));
+    if (t.nnodes)
+        hipLaunchKernelGGL(k_pack_boxes, dim3(nblocks(t.nnodes)), dim3(kBlock), 0, c.stream, view(t), t.nnodes,
+                           t.pbox.p);
 }
 
 void tree_lists(Ctx& c) {
