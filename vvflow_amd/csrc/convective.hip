@@ -203,14 +203,17 @@ __global__ __launch_bounds__(kBlock) void k_near(NearArgs A, const double* __res
     const int64_t e_beg = A.near_off[l], e_end = A.near_off[l + 1];
     for (int base = v0; base < v1; base += kTargets) {
         const int nt = min(kTargets, v1 - base);
-        if (nt <= 4)
-            near_chunk<4, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
-        else if (nt <= 8)
-            near_chunk<8, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
-        else if (nt <= 12)
-            near_chunk<12, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
-        else
-            near_chunk<16, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end);
+        // target slots in steps of two: few dummy targets per chunk
+        switch ((nt + 1) / 2) {
+            case 1: near_chunk<2, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 2: near_chunk<4, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 3: near_chunk<6, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 4: near_chunk<8, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 5: near_chunk<10, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 6: near_chunk<12, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 7: near_chunk<14, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            default: near_chunk<16, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+        }
     }
 }
 
