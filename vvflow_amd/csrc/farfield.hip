@@ -459,9 +459,10 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
         cx = mfma16(a, tx, cx);
         cy = mfma16(a, ty, cy);
         for (int i = 0; i < 4; i++) {
-            const float z = c[i], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
+            // SiLU and its derivative s (1 + z (1 - s)) = s + h (1 - s), h = z s
+            const float z = c[i], sg = sigmoid_f(z), hv = z * sg, d = __builtin_fmaf(hv, 1.f - sg, sg);
             const int st = b >> 1, e = 4 * (b & 1) + i;
-            h[st][e] = (__bf16)(z * sg);
+            h[st][e] = (__bf16)hv;
             hx[st][e] = (__bf16)(d * cx[i]);
             hy[st][e] = (__bf16)(d * cy[i]);
         }
@@ -477,9 +478,9 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
             cy = mfma16(a, hy[st], cy);
         }
         for (int i = 0; i < 4; i++) {
-            const float z = c[i], sg = sigmoid_f(z), d = sg * (1.f + z * (1.f - sg));
+            const float z = c[i], sg = sigmoid_f(z), hv = z * sg, d = __builtin_fmaf(hv, 1.f - sg, sg);
             const int st = b >> 1, e = 4 * (b & 1) + i;
-            h2[st][e] = (__bf16)(z * sg);
+            h2[st][e] = (__bf16)hv;
             h2x[st][e] = (__bf16)(d * cx[i]);
             h2y[st][e] = (__bf16)(d * cy[i]);
         }
