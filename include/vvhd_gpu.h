@@ -232,9 +232,9 @@ int vvhd_direct_sum(vvhd_ctx* ctx, const int64_t* targets, int64_t nt, double* v
 int vvhd_phase_times(vvhd_ctx* ctx, float* ms8);
 /* Algorithmic work counted on the interaction lists of the current tree:
  * {near convective pairs, far (leaf, node) terms, (vortex, far node) pairs,
- *  learned far-field network evaluations (three per far (leaf, node) term),
- *  -1, leaves, nodes}; far terms and evaluations count leaves with a target
- *  of non-zero circulation. */
+ *  learned far-field network passes (value and two tangents per far
+ *  (leaf, node) term), -1, leaves, nodes}; far terms and network passes
+ *  count leaves with a target of non-zero circulation. */
 int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7);
 
 /* ---- synthetic inputs (host) -------------------------------------------
