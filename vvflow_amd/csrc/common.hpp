@@ -243,6 +243,9 @@ struct Ctx {
 
     // timing
     hipEvent_t ev[16] = {};
+    // tree build: per-level counters copied asynchronously (pinned, two slots)
+    hipEvent_t lev_ev[2] = {};
+    int32_t* lev_counters = nullptr;
     bool phase_recorded[8] = {false, false, false, false, false, false, false, false};
 
     // work counts of the current tree

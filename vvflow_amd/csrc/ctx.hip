@@ -157,6 +157,15 @@ int vvhd_create(int device, vvhd_ctx** out) {
             delete ctx;
             return VVHD_EHIP;
         }
+    for (auto& e : ctx->c.lev_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            delete ctx;
+            return VVHD_EHIP;
+        }
+    if (hipHostMalloc((void**)&ctx->c.lev_counters, 2 * 8 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return VVHD_EHIP;
+    }
     *out = ctx;
     return VVHD_OK;
 }
@@ -203,6 +212,9 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     t.cmm_ie.release();
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c.lev_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c.lev_counters) (void)hipHostFree(c.lev_counters);
     (void)hipStreamDestroy(c.stream);
     delete ctx;
 }

@@ -857,8 +857,11 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     std::vector<std::pair<int, int>>& levels = t.levels;   // node id range of every level
     levels.clear();
     levels.push_back({0, nroots});
-    int32_t hc[8];
-    while (true) {
+    // One level per iteration.  The counters of level d are copied to pinned
+    // memory asynchronously and read only after level d+1 has been enqueued,
+    // so the device never waits for the host between levels.  When level d
+    // made no new nodes, level d+1 found an empty range and did nothing.
+    auto launch_level = [&](int d) {
         const int node_blocks = 1024;
         hipLaunchKernelGGL(k_decide, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, leaf_tmp.p, cap,
                            min_node, max_node);
@@ -869,7 +872,7 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         hipLaunchKernelGGL(k_ranks, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, left_pos,
                            right_pos);
         // the children of level d number at most nroots 2^(d+1)
-        const bool top = ((int64_t)nroots << std::min<int>(levels.size(), 30)) <= kTopNodes;
+        const bool top = ((int64_t)nroots << std::min<int>(d + 1, 30)) <= kTopNodes;
         hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
                            left_pos, right_pos, c.x.p, c.y.p, c.id.p, !top);
         if (top && n)
@@ -878,10 +881,18 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         if (nseg)
             hipLaunchKernelGGL(k_seg_step, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg, seg_node.p);
         hipLaunchKernelGGL(k_next_level, dim3(1), dim3(1), 0, s, t.counters.p);
-        VVHD_HIP(hipMemcpyAsync(hc, t.counters.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        VVHD_HIP(hipStreamSynchronize(s));
+        int32_t* slot = c.lev_counters + 8 * (d & 1);
+        VVHD_HIP(hipMemcpyAsync(slot, t.counters.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipEventRecord(c.lev_ev[d & 1], s));
+    };
+    int32_t hc[8];
+    launch_level(0);
+    for (int d = 0;; d++) {
+        launch_level(d + 1);
+        VVHD_HIP(hipEventSynchronize(c.lev_ev[d & 1]));
+        std::memcpy(hc, c.lev_counters + 8 * (d & 1), sizeof(hc));
         VVHD_CHECK(hc[4] == 0, VVHD_ENOMEM, "tree_build: node capacity exceeded");
-        if (hc[3] == hc[2]) break;   // no new nodes
+        if (hc[3] == hc[2]) break;   // level d made no new nodes; level d + 1 did nothing
         levels.push_back({hc[2], hc[3]});
         VVHD_CHECK((int)levels.size() < kMaxStack - 2, VVHD_EUNSUPPORTED, "tree_build: tree deeper than supported");
     }
