@@ -224,6 +224,8 @@ struct Ctx {
     DBuf<int64_t> rnear_off;                 // transposed near lists (repair)
     DBuf<int32_t> rnear_list, rnear_cnt, leaf_stamp;
     DBuf<uint64_t> leaf_maxd2;                       // per-leaf bound of the second-neighbour distance
+    DBuf<uint8_t> leaf_mark;                         // leaves whose reverse near lists are built
+    DBuf<int64_t> rep_state;                         // per simulation: resume target, commit count
 
     // learned far-field correction
     DBuf<double> fm;                         // per-node moments after epsilon: kMomArrays x node capacity

@@ -185,6 +185,8 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     c.sim_off.release();
     c.sim_re.release();
     c.leaf_maxd2.release();
+    c.leaf_mark.release();
+    c.rep_state.release();
     c.near_v.release();
     c.taylor.release();
     c.src_ge.release();

@@ -395,19 +395,34 @@ __device__ inline bool merge_flag(const double* g, int i, const Top2& st, double
 }
 
 // Transposed near lists: for every leaf M the leaves L with M in near(L).
-__global__ void k_rnear_count(TreeDev T, int32_t* cnt) {
-    const int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (l >= T.nleaf) return;
-    for (int64_t e = T.near_off[l] + lane_id(); e < T.near_off[l + 1]; e += kWave)
-        atomicAdd(&cnt[T.leaf_rank[T.near_list[e]]], 1);
-}
-
-__global__ void k_rnear_fill(TreeDev T, const int64_t* roff, int32_t* fillc, int32_t* rlist) {
+// With mark set, only the lists of marked leaves are built (the others stay
+// empty).
+__global__ void k_rnear_count(TreeDev T, int32_t* cnt, const uint8_t* mark) {
     const int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     if (l >= T.nleaf) return;
     for (int64_t e = T.near_off[l] + lane_id(); e < T.near_off[l + 1]; e += kWave) {
         const int m = T.leaf_rank[T.near_list[e]];
-        rlist[roff[m] + atomicAdd(&fillc[m], 1)] = l;
+        if (!mark || mark[m]) atomicAdd(&cnt[m], 1);
+    }
+}
+
+// Leaves of the flagged targets and of their first neighbours: the commits
+// of pass 1's flags need their reverse lists.
+__global__ void k_mark_flag_leaves(TreeDev T, const int32_t* flag_list, const int32_t* ecount, const int32_t* nn1,
+                                   uint8_t* mark) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ecount[0]) return;
+    const int q = flag_list[k];
+    mark[T.vleaf[q]] = 1;
+    mark[T.vleaf[nn1[q]]] = 1;
+}
+
+__global__ void k_rnear_fill(TreeDev T, const int64_t* roff, int32_t* fillc, int32_t* rlist, const uint8_t* mark) {
+    const int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    if (l >= T.nleaf) return;
+    for (int64_t e = T.near_off[l] + lane_id(); e < T.near_off[l + 1]; e += kWave) {
+        const int m = T.leaf_rank[T.near_list[e]];
+        if (!mark || mark[m]) rlist[roff[m] + atomicAdd(&fillc[m], 1)] = l;
     }
 }
 
@@ -415,6 +430,7 @@ constexpr int kRepairThreads = 1024;
 constexpr int kMaxHard = 2048;
 constexpr int kMaxPend = 8192;
 constexpr int kMaxLive = 2048;
+constexpr int kMarkedFlags = 256;   // up to this many flags, reverse lists of their leaves only
 
 struct RepairArgs {
     TreeDev T;
@@ -614,8 +630,15 @@ __device__ void commit_one(const RepairArgs& A, int q, int mark, CommitShared& s
 // flagged by later updates); the next commit is the smallest listed index at
 // or after the cursor whose flag is still set.  If the list overflows, the
 // remaining commits find their target by scanning the flags.
+// With mark set, only the reverse near lists of marked leaves exist: a commit
+// that needs another one stops the simulation's pass (state[2s] = its
+// target, state[2s+1] = the commit count, ecount[3] set); a second launch
+// with resume set continues from there once all lists are built, taking its
+// candidates from the flags at or after the cursor.  state[2s] = -1 marks a
+// finished simulation.
 __global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, const int64_t* sim_off,
-                                                               const int32_t* flag_list) {
+                                                               const int32_t* flag_list, const uint8_t* mark,
+                                                               int64_t* state, int resume) {
     __shared__ int s_q;
     __shared__ int s_np;
     __shared__ int s_pend[kMaxPend];
@@ -625,17 +648,30 @@ __global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, con
     const int64_t lo = sim_off[blockIdx.x], n = sim_off[blockIdx.x + 1];
     int64_t cursor = lo;
     int commit = 0;
+    if (resume) {
+        cursor = state[2 * blockIdx.x];
+        commit = (int)state[2 * blockIdx.x + 1];
+        if (cursor < 0) return;   // finished in the first launch
+    }
     if (tid == 0) {
         sh.overflow = 0;
         s_np = 0;
     }
     __syncthreads();
-    for (int k = tid; k < ecount[0]; k += kRepairThreads) {
-        const int i = flag_list[k];
-        if (i >= lo && i < n) {
-            const int idx = atomicAdd(&s_np, 1);
-            if (idx < kMaxPend) s_pend[idx] = i;
+    if (!resume) {
+        for (int k = tid; k < ecount[0]; k += kRepairThreads) {
+            const int i = flag_list[k];
+            if (i >= lo && i < n) {
+                const int idx = atomicAdd(&s_np, 1);
+                if (idx < kMaxPend) s_pend[idx] = i;
+            }
         }
+    } else {
+        for (int64_t i = cursor + tid; i < n; i += kRepairThreads)
+            if (A.eflag[i] & 2) {
+                const int idx = atomicAdd(&s_np, 1);
+                if (idx < kMaxPend) s_pend[idx] = (int)i;
+            }
     }
     __syncthreads();
     const bool scan_mode0 = s_np > kMaxPend;
@@ -661,6 +697,14 @@ __global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, con
         }
         const int q = s_q;
         if (q == INT32_MAX) break;
+        if (mark && !(mark[A.T.vleaf[q]] && mark[A.T.vleaf[A.nn1[q]]])) {
+            if (tid == 0) {
+                state[2 * blockIdx.x] = q;
+                state[2 * blockIdx.x + 1] = commit;
+                atomicExch(&ecount[3], 1);
+            }
+            return;   // uniform: q comes from LDS
+        }
         commit++;
         commit_one<kRepairThreads>(A, q, commit, sh, [&](int i) {
             const int k = atomicAdd(&s_np, 1);
@@ -668,16 +712,21 @@ __global__ __launch_bounds__(kRepairThreads) void k_eps_repair(RepairArgs A, con
         });
         cursor = q + 1;
     }
-    if (tid == 0 && sh.overflow) ecount[2] = 1;
+    if (tid == 0) {
+        state[2 * blockIdx.x] = -1;
+        if (sh.overflow) ecount[2] = 1;
+    }
 }
 
 // Applies _1_eps = 1/max(eps, floor) to every processed target (:49-54) and
-// packs the FP32 {g, eps^2} copy the pair kernels read.
+// packs the FP32 {g, eps^2} copy the pair kernels read.  Does nothing while
+// the merge pass still has to resume (ecount[3]): it must see the final
+// state only once.
 __global__ void k_eps_apply(const TreeDev T, const double* g, double* ie, int64_t n, const int32_t* nn1,
                             const int32_t* nn2, const double* d1a, const double* d2a, const uint8_t* eflag,
-                            const double* leaf_restr, float2* src_ge) {
+                            const double* leaf_restr, float2* src_ge, const int32_t* ecount) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || ecount[3]) return;
     double iv = ie[i];
     if (eflag[i] & 1) {
         double eps;
@@ -744,37 +793,68 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
     int32_t hc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 16, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
+    // reverse near lists: all of them, or with few flags only those of the
+    // flagged targets' leaves (a commit that needs another one resumes after
+    // the full build)
+    auto build_rnear = [&](const uint8_t* mark) {
+        VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
+        hipLaunchKernelGGL(k_rnear_count, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_cnt.p,
+                           mark);
+        size_t bytes = 0;
+        VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
+                                         rocprim::plus<int64_t>(), s));
+        c.tmp.grow(bytes + 16);
+        VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
+                                         rocprim::plus<int64_t>(), s));
+        VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
+        hipLaunchKernelGGL(k_rnear_fill, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_off.p,
+                           c.rnear_cnt.p, c.rnear_list.p, mark);
+    };
+    // (buffers grown first: the arguments hold their pointers)
+    auto repair_args = [&] {
+        return RepairArgs{T,         c.x.p,         c.y.p,       c.g.p,     c.leaf_crit.p, c.rnear_off.p,
+                          c.rnear_list.p, c.leaf_stamp.p, c.nn1.p, c.nn2.p, c.d1.p,        c.d2.p,
+                          c.eflag.p, c.ecount.p,    c.t.blx.p,   c.t.bly.p, c.t.trx.p,     c.t.try_.p,
+                          c.leaf_maxd2.p};
+    };
+    const bool marked = hc[0] > 0 && hc[0] <= kMarkedFlags;
     if (hc[0] > 0) {
         c.rnear_off.grow(nl + 1);
         c.rnear_cnt.grow(nl + 1);
         c.leaf_stamp.grow(nl + 1);
         c.rnear_list.grow(c.t.nnear + 1);
-        VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
-        VVHD_HIP(hipMemsetAsync(c.leaf_stamp.p, 0, (size_t)(nl + 1) * 4, s));
-        hipLaunchKernelGGL(k_rnear_count, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_cnt.p);
-        {
-            size_t bytes = 0;
-            VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
-                                             rocprim::plus<int64_t>(), s));
-            c.tmp.grow(bytes + 16);
-            VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
-                                             rocprim::plus<int64_t>(), s));
-        }
-        VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
-        hipLaunchKernelGGL(k_rnear_fill, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_off.p, c.rnear_cnt.p,
-                           c.rnear_list.p);
+        c.rep_state.grow(2 * (size_t)c.nsim);
+        c.leaf_mark.grow(nl + 1);
         c.leaf_maxd2.grow(nl + 1);
+        VVHD_HIP(hipMemsetAsync(c.leaf_stamp.p, 0, (size_t)(nl + 1) * 4, s));
+        if (marked) {
+            VVHD_HIP(hipMemsetAsync(c.leaf_mark.p, 0, (size_t)(nl + 1), s));
+            hipLaunchKernelGGL(k_mark_flag_leaves, dim3(nblocks(hc[0])), dim3(kBlock), 0, s, T,
+                               (const int32_t*)c.flag_list.p, (const int32_t*)c.ecount.p, (const int32_t*)c.nn1.p,
+                               c.leaf_mark.p);
+        }
+        build_rnear(marked ? c.leaf_mark.p : nullptr);
         hipLaunchKernelGGL(k_leaf_maxd2, dim3(nblocks(nl)), dim3(kBlock), 0, s, T, c.d2.p, c.eflag.p, c.leaf_maxd2.p);
-        const RepairArgs A{T,         c.x.p,         c.y.p,       c.g.p,     c.leaf_crit.p, c.rnear_off.p,
-                           c.rnear_list.p, c.leaf_stamp.p, c.nn1.p, c.nn2.p, c.d1.p,        c.d2.p,
-                           c.eflag.p, c.ecount.p,    c.t.blx.p,   c.t.bly.p, c.t.trx.p,     c.t.try_.p,
-                           c.leaf_maxd2.p};
-        hipLaunchKernelGGL(k_eps_repair, dim3(c.nsim), dim3(kRepairThreads), 0, s, A, (const int64_t*)c.sim_off.p,
-                           c.flag_list.p);
+        hipLaunchKernelGGL(k_eps_repair, dim3(c.nsim), dim3(kRepairThreads), 0, s, repair_args(),
+                           (const int64_t*)c.sim_off.p,
+                           c.flag_list.p, marked ? (const uint8_t*)c.leaf_mark.p : (const uint8_t*)nullptr,
+                           c.rep_state.p, 0);
     }
-    hipLaunchKernelGGL(k_eps_apply, dim3(nblocks(n)), dim3(kBlock), 0, s, T, c.g.p, c.ie.p, n, c.nn1.p, c.nn2.p,
-                       c.d1.p, c.d2.p, c.eflag.p, c.leaf_restr.p, c.src_ge.p);
-    VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 32, hipMemcpyDeviceToHost, s));
+    auto finish = [&] {
+        hipLaunchKernelGGL(k_eps_apply, dim3(nblocks(n)), dim3(kBlock), 0, s, T, c.g.p, c.ie.p, n, c.nn1.p, c.nn2.p,
+                           c.d1.p, c.d2.p, c.eflag.p, c.leaf_restr.p, c.src_ge.p, (const int32_t*)c.ecount.p);
+        VVHD_HIP(hipMemcpyAsync(hc, c.ecount.p, 32, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+    };
+    finish();
+    if (marked && hc[3]) {   // a merge reached a leaf without its reverse list
+        VVHD_HIP(hipMemsetAsync(c.ecount.p + 3, 0, 4, s));
+        build_rnear(nullptr);
+        hipLaunchKernelGGL(k_eps_repair, dim3(c.nsim), dim3(kRepairThreads), 0, s, repair_args(),
+                           (const int64_t*)c.sim_off.p,
+                           c.flag_list.p, (const uint8_t*)nullptr, c.rep_state.p, 1);
+        finish();
+    }
     if (nn1_out && n) VVHD_HIP(hipMemcpyAsync(nn1_out, c.nn1.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     if (nn2_out && n) VVHD_HIP(hipMemcpyAsync(nn2_out, c.nn2.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
