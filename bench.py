@@ -13,8 +13,8 @@ device-to-device copy inside the timed region).
                     [--no-cpu] [--error-sample S] [--model PATH]
 
 value: particle-steps/s of the learned path (far criterion --learned-fc,
-learned far-field correction, FP32 Kahan pair sums) = N * K / (time of K
-steps).  The plain tree path (the reference's method) is timed at the
+default 8 as in the reference, learned far-field correction, FP32 Kahan pair
+sums) = N * K / (time of K steps).  The plain tree path (the reference's method) is timed at the
 smallest far criterion of --plain-fcs whose convective error reaches 1e-5,
 and at the reference's far criterion 8.  Errors: relative L2 of the induced
 convective velocity against the FP64 direct sum on --error-sample targets.
@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--learned-fc", type=int, default=16)
+    ap.add_argument("--learned-fc", type=int, default=8)
     ap.add_argument("--plain-fcs", default="512,1024,2048,4096")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-plain", action="store_true")

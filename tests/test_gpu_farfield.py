@@ -219,16 +219,17 @@ MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained far-field model in the tree")
 def test_trained_network_gain_on_wake(ctx):
     """What the trained network gains: on a 200k-vortex wake (not a training
-    snapshot), convective velocity error against the FP64 direct sum for the
-    plain tree path (the reference's far field), the learned path without the
-    network (per-target monopoles only) and the learned path."""
+    snapshot) at the reference's far criterion 8, convective velocity error
+    against the FP64 direct sum for the plain tree path (the reference's far
+    field), the learned path without the network (per-target monopoles only)
+    and the learned path."""
     from vvflow_amd.train_farfield import load_model
-    print("case trained network gain, wake 200000 fc=16", flush=True)
+    print("case trained network gain, wake 200000 fc=8", flush=True)
     objs = V.gen_wake(200_000, 0x5EED0002)
     ctx.set_params(re=600.0, dt=0.005)
     ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
     ctx.upload(objs)
-    ctx.tree_build(16)
+    ctx.tree_build(8)
     ctx.epsilon(True)
     v0 = ctx.download()
     tg = np.random.default_rng(3).choice(np.nonzero(v0[:, 2] != 0)[0], 2000, replace=False).astype(np.int64)
@@ -248,6 +249,6 @@ def test_trained_network_gain_on_wake(ctx):
     ctx.far_model_set(p)
     e_learn = err_of(lambda: ctx.convective_learned((0.0, 0.0)))
     print(f"  rel L2 vs direct sum: plain {e_plain:.2e}  monopoles {e_mono:.2e}  learned {e_learn:.2e}", flush=True)
-    assert e_learn < 1.5e-5
-    assert e_learn < e_mono / 5
-    assert e_learn < e_plain / 10
+    assert e_learn < 1e-5
+    assert e_learn < e_mono / 10
+    assert e_learn < e_plain / 30
