@@ -131,6 +131,7 @@ struct TreeArrays {
     DBuf<double> cmp_x, cmp_y, cmp_g, cmm_x, cmm_y, cmm_g;
     DBuf<float> cmp_ie, cmm_ie;           // _1_eps of monopoles: +inf or 0
     DBuf<int32_t> nleaves, arrive, leaf_rank;
+    DBuf<int32_t> pre;                    // preorder (DFS) position of each node
     DBuf<int32_t> split;                  // per node: 0 undecided,1 leaf,2 split
     // levels
     DBuf<int32_t> counters;               // [0]=node count, [1]=leaf count, [2..] misc

@@ -206,7 +206,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
                     &t.node_L, &t.node_m, &t.leaf_tmp, &t.seg_node, &t.cnt_near, &t.cnt_far, &t.err})
         b->release();
     for (auto* b : {&t.near_off, &t.far_off, &t.seg_off, &t.slot_near, &t.slot_far}) b->release();
-    for (auto* b : {&t.groups, &t.leaf_sim, &t.root_first}) b->release();
+    for (auto* b : {&t.groups, &t.leaf_sim, &t.root_first, &t.pre}) b->release();
     t.entries.release();
     t.trav.release();
     t.pbox.release();

@@ -35,7 +35,7 @@ namespace vvhd {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxStack = 128;
+constexpr int kMaxLevels = 126;   // deepest tree accepted
 
 inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
 
@@ -237,12 +237,16 @@ __global__ void k_bbox_elems(NodeView nv, const int32_t* enode, const double* x,
 }
 
 // First leaf (DFS rank) of every root, from the subtree leaf counts.
-__global__ void k_root_first(const int32_t* nleaves, int nroots, int32_t* first, int32_t* root_first) {
+// First leaf of every root and its preorder position: a tree of k leaves
+// has 2k - 1 nodes, so root s starts at 2 first - s.
+__global__ void k_root_first(const int32_t* nleaves, int nroots, int32_t* first, int32_t* root_first,
+                             int32_t* pre) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     int f = 0;
     for (int s = 0; s < nroots; s++) {
         first[s] = f;
         root_first[s] = f;
+        pre[s] = 2 * f - s;
         f += nleaves[s];
     }
     root_first[nroots] = f;
@@ -527,7 +531,7 @@ __global__ void k_up_level(NodeView nv, CMView cm, int lb, int le, const double*
 
 // Top-down over one level: first leaf rank of every subtree (DFS order).
 __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves, int32_t* first, int32_t* leaf_rank,
-                             int32_t* leaf_node, int32_t* vleaf) {
+                             int32_t* leaf_node, int32_t* vleaf, int32_t* pre) {
     const int id = lb + blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= le) return;
     const int f = first[id];
@@ -535,6 +539,9 @@ __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves
         const int a = nv.ch1[id];
         first[a] = f;
         first[a + 1] = f + nleaves[a];
+        const int p = pre[id];   // the first child's subtree (2 nleaves - 1 nodes) follows the node
+        pre[a] = p + 1;
+        pre[a + 1] = p + 2 * nleaves[a];
     } else {
         leaf_rank[id] = f;
         leaf_node[f] = id;
@@ -546,11 +553,11 @@ __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves
 // the "entries" are the nodes at depth kSplitDepth plus the shallower
 // leaves, in DFS order (at most 2^kSplitDepth).  One wave handles one entry
 // for 64 DFS-consecutive leaves: each lane first tests the entry's
-// ancestors for its leaf, then the wave walks the entry's subtree once with
-// a shared stack (a node is expanded while any lane still needs it; a lane
-// that classifies a node as far ignores the rest of that subtree).  The
-// split gives 2^kSplitDepth times more independent waves than leaf groups,
-// which hides the latency of the dependent node loads.  Emissions follow the
+// ancestors for its leaf, then the wave walks the entry's subtree once (a
+// node is expanded while any lane still needs it; a lane that classifies a
+// node as far ignores the rest of that subtree).  The split gives
+// 2^kSplitDepth times more independent waves than leaf groups, which hides
+// the latency of the dependent node loads.  Emissions follow the
 // reference's preorder: a far ancestor is emitted by the first entry beneath
 // it only, and the per-(leaf, entry) segments are concatenated in entry
 // order.
@@ -562,14 +569,14 @@ constexpr int kMaxEntries = 1 << kSplitDepth;
 
 struct ListEntries {
     int n;                                 // number of entries
-    int node[kMaxEntries];                 // entry node
+    int node[kMaxEntries];                 // entry node (preorder position)
     int depth[kMaxEntries];                // its depth (<= kSplitDepth)
-    int anc[kMaxEntries][kSplitDepth];     // ancestors, root first
+    int anc[kMaxEntries][kSplitDepth];     // ancestors, root first (preorder positions)
     int lead[kMaxEntries];                 // bit d: first entry beneath anc[d]
 };
 
 // Entry table of every root (thread r: root r).
-__global__ void k_list_entries(NodeView nv, int nroots, ListEntries* Etab) {
+__global__ void k_list_entries(NodeView nv, const int32_t* pre, int nroots, ListEntries* Etab) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nroots) return;
     ListEntries* E = Etab + r;
@@ -580,15 +587,15 @@ __global__ void k_list_entries(NodeView nv, int nroots, ListEntries* Etab) {
         const int T = stack[--sp];
         const int d = nv.depth[T];
         if (d == kSplitDepth || nv.ch1[T] < 0) {
-            E->node[n] = T;
+            E->node[n] = pre[T];
             E->depth[n] = d;
             int a = T, lead = 0;
             for (int k = d - 1; k >= 0; k--) {
                 const int p = nv.parent[a];
-                E->anc[n][k] = p;
-                // T is the first entry beneath p when every step from p down is to a first child
+                E->anc[n][k] = pre[p];
                 a = p;
             }
+            // T is the first entry beneath an ancestor when every step from it down is to a first child
             int c = T;
             bool first = true;
             for (int k = d - 1; k >= 0; k--) {
@@ -614,34 +621,45 @@ __global__ void k_pack_boxes(NodeView nv, int nnodes, double4* out) {
                            double_from_ord(nv.try_[id]));
 }
 
-// What the traversal reads of a node, in one 32-byte record: centre, h + w
-// (added first, as the far test does), depth and first child.
+// What the list walk reads of a node, one 32-byte record per node stored at
+// the node's preorder position: centre, h + w (added first, as the far test
+// does), node id, and the position that follows its subtree (position + 1
+// for a leaf).  A walk then only moves forward through memory: into the
+// subtree at position + 1, or past it.
 struct TravNode {
     double x, y, hw;
-    int32_t depth, ch1;
+    int32_t id, skip;
 };
 static_assert(sizeof(TravNode) == 32, "TravNode layout");
 
-__global__ void k_trav_nodes(NodeView nv, int nnodes, TravNode* out) {
+__global__ void k_trav_nodes(NodeView nv, int nnodes, const int32_t* pre, const int32_t* nleaves, TravNode* out) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= nnodes) return;
     TravNode r;
     r.x = nv.x[id];
     r.y = nv.y[id];
     r.hw = nv.h[id] + nv.w[id];
-    r.depth = nv.depth[id];
-    r.ch1 = nv.ch1[id];
-    out[id] = r;
+    r.id = id;
+    const int p = pre[id];
+    r.skip = p + 2 * nleaves[id] - 1;
+    out[p] = r;
 }
 
+// Near and far lists of every leaf, two passes (counts, then entries at the
+// scanned offsets).  A wave takes one entry of one root for a group of up to
+// 64 DFS-consecutive leaves (one leaf per lane) and writes slot (leaf,
+// entry).  The wave walks the entry's subtree in preorder without a stack:
+// from a node it moves to the next position (the first child) when some lane
+// still needs the node's subtree, otherwise past the subtree.  A lane that
+// classifies a node as far (or reaches a near leaf) waits until the walk
+// leaves that subtree.
 __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const TravNode* __restrict__ tn,
                                                   const ListEntries* __restrict__ Etab,
                                                   const int32_t* __restrict__ groups, int ngroups,
-                                                  const int32_t* leaf_node, double fc,
+                                                  const int32_t* leaf_node, double fc, int nnodes,
                                                   const int64_t* near_off, const int64_t* far_off,
                                                   int32_t* near_list, int32_t* far_list, int32_t* near_cnt,
-                                                  int32_t* far_cnt, int32_t* err) {
-    __shared__ int s_stack[kBlock / kWave][kMaxStack];
+                                                  int32_t* far_cnt) {
     const int w = threadIdx.x / kWave;
     const int wave = blockIdx.x * (kBlock / kWave) + w;
     const int q = wave % kMaxEntries;
@@ -654,67 +672,60 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const TravNode* _
     if (q >= E->n) return;   // uniform over the wave
     const bool valid = lane < groups[3 * grp + 1];
     if (__ballot(valid) == 0) return;
-    int* stack = s_stack[w];
     const int L = valid ? leaf_node[l] : 0;
     const double lx = nv.x[L], ly = nv.y[L], lh = nv.h[L], lw = nv.w[L];   // (h_T + w_T) + lh + lw below
     const size_t slot = (size_t)l * kMaxEntries + q;
     int64_t pn = (valid && near_off) ? near_off[slot] : 0, pf = (valid && far_off) ? far_off[slot] : 0;
     int cn = 0, cf = 0;
+    const int P0 = E->node[q];
+    const int end = min(tn[P0].skip, nnodes);
+    // a lane takes part at positions >= resume
+    int resume = valid ? P0 : end;
     // ancestors of the entry: the first far one (if any) cuts the whole entry
-    int skip_depth = INT32_MAX;
     const int dE = E->depth[q];
     for (int k = 0; k < dE && valid; k++) {
-        const int A = E->anc[q][k];
-        const TravNode a = tn[A];
+        const TravNode a = tn[E->anc[q][k]];
         const double dx = a.x - lx, dy = a.y - ly;
         const double hp = a.hw + lh + lw;
         if (dx * dx + dy * dy > fc * hp * hp) {
             if ((E->lead[q] >> k) & 1) {
-                if (far_list) far_list[pf++] = A;
+                if (far_list) far_list[pf++] = a.id;
                 cf++;
             }
-            skip_depth = -1;   // never reset: nothing below the entry is needed
+            resume = end;
             break;
         }
     }
-    if (lane == 0) stack[0] = E->node[q];
-    int sp = 1;
-    __builtin_amdgcn_wave_barrier();
-    while (sp > 0) {
-        const int T = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
-        sp--;
-        const TravNode r = tn[T];
-        const int dT = r.depth;
-        if (skip_depth != INT32_MAX && dT <= skip_depth) skip_depth = INT32_MAX;
-        const bool act = valid && skip_depth == INT32_MAX;
+    // the position is uniform: records come through the scalar cache, all
+    // 32 bytes in one load
+    auto rec = [&](int p) {
+        const uint4* q4 = reinterpret_cast<const uint4*>(tn + p);
+        const uint4 u0 = q4[0], u1 = q4[1];
+        TravNode r;
+        __builtin_memcpy(&r, &u0, 16);
+        __builtin_memcpy(reinterpret_cast<char*>(&r) + 16, &u1, 16);
+        return r;
+    };
+    int P = __builtin_amdgcn_readfirstlane(__ballot(resume < end) != 0 ? P0 : end);
+    while (P < end) {
+        const TravNode r = rec(P);
+        const int skip = __builtin_amdgcn_readfirstlane(r.skip);
+        const int id = __builtin_amdgcn_readfirstlane(r.id);
+        const bool act = P >= resume;
+        const bool leaf = skip == P + 1;
         const double dx = r.x - lx, dy = r.y - ly;
         const double hp = r.hw + lh + lw;
         const bool far = act && (dx * dx + dy * dy > fc * hp * hp);
         if (far) {
-            if (far_list) far_list[pf++] = T;
+            if (far_list) far_list[pf++] = id;
             cf++;
-            skip_depth = dT;
-        }
-        const int c = r.ch1;
-        const bool need = act && !far;
-        if (c >= 0) {
-            if (__ballot(need) != 0) {
-                if (sp + 2 > kMaxStack) {
-                    if (lane == 0) *err = 1;
-                    break;
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) {
-                    stack[sp] = c + 1;
-                    stack[sp + 1] = c;
-                }
-                __builtin_amdgcn_wave_barrier();
-                sp += 2;
-            }
-        } else if (need) {
-            if (near_list) near_list[pn++] = T;
+            resume = skip;
+        } else if (act && leaf) {
+            if (near_list) near_list[pn++] = id;
             cn++;
         }
+        const bool need = act && !far && !leaf;
+        P = __builtin_amdgcn_readfirstlane(__ballot(need) != 0 ? P + 1 : max(skip, P + 1));
     }
     if (valid && near_cnt) near_cnt[slot] = cn;
     if (valid && far_cnt) far_cnt[slot] = cf;
@@ -808,7 +819,8 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     const int cap = 2 * (n + nseg) + 64 + nroots;
     for (auto* b : {&t.x, &t.y, &t.h, &t.w}) b->grow(cap);
     for (auto* b : {&t.blx, &t.bly, &t.trx, &t.try_, &t.path}) b->grow(cap);
-    for (auto* b : {&t.v0, &t.v1, &t.ch1, &t.parent, &t.depth, &t.nseg, &t.split, &t.nleaves, &t.arrive, &t.leaf_rank})
+    for (auto* b : {&t.v0, &t.v1, &t.ch1, &t.parent, &t.depth, &t.nseg, &t.split, &t.nleaves, &t.arrive, &t.leaf_rank,
+                    &t.pre})
         b->grow(cap);
     for (auto* b : {&t.cmp_x, &t.cmp_y, &t.cmp_g, &t.cmm_x, &t.cmm_y, &t.cmm_g}) b->grow(cap);
     t.cmp_ie.grow(cap);
@@ -894,7 +906,7 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         VVHD_CHECK(hc[4] == 0, VVHD_ENOMEM, "tree_build: node capacity exceeded");
         if (hc[3] == hc[2]) break;   // level d made no new nodes; level d + 1 did nothing
         levels.push_back({hc[2], hc[3]});
-        VVHD_CHECK((int)levels.size() < kMaxStack - 2, VVHD_EUNSUPPORTED, "tree_build: tree deeper than supported");
+        VVHD_CHECK((int)levels.size() < kMaxLevels, VVHD_EUNSUPPORTED, "tree_build: tree deeper than supported");
     }
     t.nnodes = hc[0];
     t.nleaves_h = hc[1];
@@ -919,11 +931,11 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     }
     int32_t* first = t.arrive.p;
     t.root_first.grow(nroots + 1);
-    hipLaunchKernelGGL(k_root_first, dim3(1), dim3(1), 0, s, t.nleaves.p, nroots, first, t.root_first.p);
+    hipLaunchKernelGGL(k_root_first, dim3(1), dim3(1), 0, s, t.nleaves.p, nroots, first, t.root_first.p, t.pre.p);
     for (int d = 0; d < (int)levels.size(); d++) {
         const int lb = levels[d].first, le = levels[d].second;
         hipLaunchKernelGGL(k_down_level, dim3(nblocks(le - lb)), dim3(kBlock), 0, s, nv, lb, le, t.nleaves.p, first,
-                           t.leaf_rank.p, t.leaf_node.p, t.vleaf.p);
+                           t.leaf_rank.p, t.leaf_node.p, t.vleaf.p, t.pre.p);
     }
     // leaf -> simulation and the list builder's leaf groups (64 leaves of one root)
     t.leaf_sim.grow(nl + 1);
@@ -987,39 +999,34 @@ void tree_lists(Ctx& c) {
     const int nroots = c.nsim;
     t.trav.grow((size_t)t.nnodes * sizeof(TravNode));
     TravNode* tn = reinterpret_cast<TravNode*>(t.trav.p);
-    hipLaunchKernelGGL(k_trav_nodes, dim3(nblocks(t.nnodes)), dim3(kBlock), 0, s, nv, t.nnodes, tn);
+    hipLaunchKernelGGL(k_trav_nodes, dim3(nblocks(t.nnodes)), dim3(kBlock), 0, s, nv, t.nnodes, t.pre.p, t.nleaves.p,
+                       tn);
     t.entries.grow(nroots * sizeof(ListEntries));
     ListEntries* E = reinterpret_cast<ListEntries*>(t.entries.p);
-    hipLaunchKernelGGL(k_list_entries, dim3(nblocks(nroots, 64)), dim3(64), 0, s, nv, nroots, E);
+    hipLaunchKernelGGL(k_list_entries, dim3(nblocks(nroots, 64)), dim3(64), 0, s, nv, t.pre.p, nroots, E);
     VVHD_HIP(hipMemsetAsync(cnt_n.p, 0, (nslot + 1) * 4, s));
     VVHD_HIP(hipMemsetAsync(cnt_f.p, 0, (nslot + 1) * 4, s));
     const double fc = (double)t.far_criteria;
     const int64_t waves = (int64_t)t.ngroups * kMaxEntries;
     const int grid = (int)std::max<int64_t>(1, (waves + kBlock / kWave - 1) / (kBlock / kWave));
-    t.err.grow(1);
-    VVHD_HIP(hipMemsetAsync(t.err.p, 0, 4, s));
     hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const TravNode*)tn, (const ListEntries*)E,
-                       (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, (const int64_t*)nullptr,
-                       (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, cnt_n.p, cnt_f.p, t.err.p);
+                       (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, t.nnodes, (const int64_t*)nullptr,
+                       (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, cnt_n.p, cnt_f.p);
     scan_i32_to_i64(c, cnt_n.p, t.slot_near.p, (int)nslot);
     scan_i32_to_i64(c, cnt_f.p, t.slot_far.p, (int)nslot);
     hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_near.p, nl, t.near_off.p);
     hipLaunchKernelGGL(k_leaf_offsets, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.slot_far.p, nl, t.far_off.p);
     int64_t tot[2];
-    int32_t herr = 0;
     VVHD_HIP(hipMemcpyAsync(&tot[0], t.slot_near.p + nslot, 8, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipMemcpyAsync(&tot[1], t.slot_far.p + nslot, 8, hipMemcpyDeviceToHost, s));
-    VVHD_HIP(hipMemcpyAsync(&herr, t.err.p, 4, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
-    VVHD_CHECK(herr == 0, VVHD_EUNSUPPORTED, "tree lists: traversal stack exceeded");
     t.nnear = tot[0];
     t.nfar = tot[1];
     t.near_list.grow(t.nnear + 1);
     t.far_list.grow(t.nfar + 1);
     hipLaunchKernelGGL(k_lists, dim3(grid), dim3(kBlock), 0, s, nv, (const TravNode*)tn, (const ListEntries*)E,
-                       (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, (const int64_t*)t.slot_near.p,
-                       (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p, (int32_t*)nullptr, (int32_t*)nullptr,
-                       t.err.p);
+                       (const int32_t*)t.groups.p, t.ngroups, t.leaf_node.p, fc, t.nnodes, (const int64_t*)t.slot_near.p,
+                       (const int64_t*)t.slot_far.p, t.near_list.p, t.far_list.p, (int32_t*)nullptr, (int32_t*)nullptr);
 }
 
 }  // namespace vvhd
