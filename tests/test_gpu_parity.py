@@ -231,6 +231,29 @@ def test_repeatable(ctx):
 
 # ------------------------------------------------------------ full size
 
+def test_full_size_tree_repeatable(ctx):
+    """Three builds of the 1M-vortex wake give the same tree: permutation,
+    every node's geometry (exact bounding boxes) and element range, keyed by
+    (depth, path) since device node ids follow allocation order, and the leaf
+    table.  Races inside a level (e.g. reading coordinates that a concurrent
+    swap rewrites) show up here long before they do on small cases."""
+    print("case wake 1M tree repeatability", flush=True)
+    objs = V.gen_wake(1_000_000)
+    ref = None
+    for _ in range(3):
+        load_case(ctx, objs)
+        perm = ctx.tree_build(8, perm=True)
+        nodes = ctx.tree_nodes()
+        order = np.lexsort((nodes["path"], nodes["ints"][:, 3]))
+        out = (perm, nodes["geom"][order], nodes["ints"][order][:, [0, 1, 3]], nodes["path"][order],
+               ctx.tree_leaves()["geom"])
+        if ref is None:
+            ref = out
+        else:
+            for a, b in zip(ref, out):
+                assert np.array_equal(a, b, equal_nan=True)
+
+
 def test_full_size_wake_properties(ctx):
     """1M-vortex wake: permutation, leaf partition, circulation conservation
     through merges, agreement with the FP64 direct sum on sampled targets."""

@@ -247,8 +247,9 @@ struct Ctx {
     // timing
     hipEvent_t ev[16] = {};
     // tree build: per-level counters copied asynchronously (pinned, two slots)
-    hipEvent_t lev_ev[2] = {};
-    int32_t* lev_counters = nullptr;
+    static constexpr int kLevSlots = 2;   // tree levels in flight (pinned counter slots and events)
+    hipEvent_t lev_ev[kLevSlots] = {};
+    int32_t* lev_counters = nullptr;      // kLevSlots x 8 pinned counters
     bool phase_recorded[8] = {false, false, false, false, false, false, false, false};
 
     // work counts of the current tree

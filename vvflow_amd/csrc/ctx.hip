@@ -162,7 +162,8 @@ int vvhd_create(int device, vvhd_ctx** out) {
             delete ctx;
             return VVHD_EHIP;
         }
-    if (hipHostMalloc((void**)&ctx->c.lev_counters, 2 * 8 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&ctx->c.lev_counters, Ctx::kLevSlots * 8 * sizeof(int32_t), hipHostMallocDefault) !=
+        hipSuccess) {
         delete ctx;
         return VVHD_EHIP;
     }

@@ -25,6 +25,8 @@
 
 #include <cstring>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -313,50 +315,56 @@ __device__ inline bool goes_left(const NodeView& nv, int node, double px, double
     return (nv.h[node] < nv.w[node]) ? (px < nv.x[node]) : (py < nv.y[node]);
 }
 
-// Per element: "goes left" flag of elements in split nodes (else 0).
-__global__ void k_flags(NodeView nv, const int32_t* enode, const double* x, const double* y, int n, int32_t* flag) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > n) return;
-    int f = 0;
-    if (k < n) {
+// Per element: "goes left" flag of elements in split nodes (else 0), read
+// by the scan through a transform iterator; 0 past the last element.
+struct GoesLeftFlag {
+    NodeView nv;
+    const int32_t* enode;
+    const double* x;
+    const double* y;
+    int n;
+    __device__ int32_t operator()(int k) const {
+        if (k >= n) return 0;
         const int node = enode[k];
-        if (nv.split[node] == 2) f = goes_left(nv, node, x[k], y[k]) ? 1 : 0;
+        return (nv.split[node] == 2 && goes_left(nv, node, x[k], y[k])) ? 1 : 0;
     }
-    flag[k] = f;
-}
+};
 
-// Per split node: split position and number of misplaced pairs.
-// P is the exclusive prefix of the flags over all elements (n+1 entries).
-__global__ void k_node_counts(NodeView nv, const int32_t* counters, const int32_t* P, int32_t* node_L, int32_t* node_m) {
-    const int lb = counters[2], le = counters[3];
-    for (int id = lb + blockIdx.x * blockDim.x + threadIdx.x; id < le; id += gridDim.x * blockDim.x) {
-        if (nv.split[id] != 2) continue;
-        const int v0 = nv.v0[id], v1 = nv.v1[id];
-        const int Lc = P[v1] - P[v0];
-        const int Labs = v0 + Lc;
-        const int m = Lc - (P[Labs] - P[v0]);   // left-goers sitting right of Labs
-        node_L[id] = Labs;
-        node_m[id] = m;
-        const int c = nv.ch1[id];
+// Ranks of misplaced elements: left_pos[v0 + r] / right_pos[v0 + q].  P is
+// the exclusive prefix of the flags over all elements (n + 1 entries).  The
+// first element of a split node also records its split position and number
+// of misplaced pairs and sets the children's ranges.  Thread 0 advances the
+// level range (the nodes of the next level were allocated by k_decide).
+// The flags are kept for k_swap: it must not derive them from coordinates
+// that the swaps may be rewriting.
+__global__ void k_ranks(NodeView nv, const int32_t* enode, const double* x, const double* y, const int32_t* P,
+                        int n, int32_t* node_L, int32_t* node_m, int32_t* left_pos, int32_t* right_pos,
+                        int32_t* counters, int32_t* flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        counters[2] = counters[3];
+        counters[3] = counters[0];
+    }
+    if (k >= n) return;
+    const int node = enode[k];
+    if (nv.split[node] != 2) return;
+    const int v0 = nv.v0[node], v1 = nv.v1[node];   // a split node's own range is unchanged
+    const int P0 = P[v0];
+    const int Lc = P[v1] - P0;
+    const int Labs = v0 + Lc;
+    if (k == v0) {
+        node_L[node] = Labs;
+        node_m[node] = Lc - (P[Labs] - P0);   // left-goers sitting right of Labs
+        const int c = nv.ch1[node];
         nv.v0[c] = v0;
         nv.v1[c] = Labs;
         nv.v0[c + 1] = Labs;
         nv.v1[c + 1] = v1;
     }
-}
-
-// Ranks of misplaced elements: left_pos[v0 + r] / right_pos[v0 + q].
-__global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* flag, const int32_t* P, int n,
-                        const int32_t* node_L, int32_t* left_pos, int32_t* right_pos) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int node = enode[k];
-    if (nv.split[node] != 2) return;
-    const int v0 = nv.v0[node];   // a split node's own range is unchanged
-    const int Labs = node_L[node];
-    const int f = flag[k];
+    const bool f = goes_left(nv, node, x[k], y[k]);
+    flag[k] = f ? 1 : 0;
     if (k < Labs && !f) {
-        const int r = (k - v0) - (P[k] - P[v0]);
+        const int r = (k - v0) - (P[k] - P0);
         left_pos[v0 + r] = k;
     } else if (k >= Labs && f) {
         const int q = P[k] - P[Labs];
@@ -385,7 +393,8 @@ __global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const i
     const bool left_mis = active && k < Labs && !f;
     const bool right_mis = active && k >= Labs && f;
     // Bounding boxes by element identity: each element's own flag decides its
-    // child.  A misplaced pair is accounted by its left thread.
+    // child.  A misplaced pair is accounted by its left thread (the right
+    // thread's coordinates may already be swapped and are not used).
     const double px = active ? x[k] : 0, py = active ? y[k] : 0;
     double qx = 0, qy = 0;
     int partner = -1;
@@ -425,10 +434,6 @@ __global__ void k_seg_step(NodeView nv, const vvhd_seg* segs, int nseg, int32_t*
     bbox_add(nv, c, segs[s].x, segs[s].y);
 }
 
-__global__ void k_next_level(int32_t* counters) {
-    counters[2] = counters[3];
-    counters[3] = counters[0];
-}
 
 // Gathers the non-key vortex fields into tree order.
 __global__ void k_gather(const int32_t* id, int n, const double* g, const double* vx, const double* vy,
@@ -794,7 +799,9 @@ void scan_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t n) {
                                      c.stream));
 }
 
-void scan_i32(Ctx& c, const int32_t* in, int32_t* out, int64_t n) {
+// Exclusive scan of the goes-left flags, computed on the fly.
+void scan_flags(Ctx& c, const GoesLeftFlag& f, int32_t* out, int64_t n) {
+    const auto in = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), f);
     size_t bytes = 0;
     VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int32_t)0, (size_t)n, rocprim::plus<int32_t>(),
                                      c.stream));
@@ -869,42 +876,41 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     std::vector<std::pair<int, int>>& levels = t.levels;   // node id range of every level
     levels.clear();
     levels.push_back({0, nroots});
-    // One level per iteration.  The counters of level d are copied to pinned
-    // memory asynchronously and read only after level d+1 has been enqueued,
-    // so the device never waits for the host between levels.  When level d
-    // made no new nodes, level d+1 found an empty range and did nothing.
+    // One level per iteration.  The kernels take their node range from the
+    // device counters; the host reads a level's counters (copied to a pinned
+    // slot) only to record the range and to stop, and it keeps kLevSlots - 1
+    // further levels enqueued meanwhile, so the device does not wait for the
+    // host between levels.  Levels after the last one find an empty range and
+    // do nothing.
     auto launch_level = [&](int d) {
         const int node_blocks = 1024;
         hipLaunchKernelGGL(k_decide, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, leaf_tmp.p, cap,
                            min_node, max_node);
-        hipLaunchKernelGGL(k_flags, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, c.x.p, c.y.p, n, flag);
-        scan_i32(c, flag, P, n + 1);
-        hipLaunchKernelGGL(k_node_counts, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, P, node_L.p,
-                           node_m.p);
-        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, left_pos,
-                           right_pos);
+        scan_flags(c, GoesLeftFlag{nv, enode.p, c.x.p, c.y.p, n}, P, n + 1);
+        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, c.x.p, c.y.p, P, n,
+                           node_L.p, node_m.p, left_pos, right_pos, t.counters.p, flag);
         // the children of level d number at most nroots 2^(d+1)
         const bool top = ((int64_t)nroots << std::min<int>(d + 1, 30)) <= kTopNodes;
         hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
                            left_pos, right_pos, c.x.p, c.y.p, c.id.p, !top);
         if (top && n)
             hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv,
-                               (const int32_t*)(t.counters.p + 3), (const int32_t*)enode.p, c.x.p, c.y.p, n);
+                               (const int32_t*)(t.counters.p + 2), (const int32_t*)enode.p, c.x.p, c.y.p, n);
         if (nseg)
             hipLaunchKernelGGL(k_seg_step, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg, seg_node.p);
-        hipLaunchKernelGGL(k_next_level, dim3(1), dim3(1), 0, s, t.counters.p);
-        int32_t* slot = c.lev_counters + 8 * (d & 1);
+        int32_t* slot = c.lev_counters + 8 * (d % Ctx::kLevSlots);
         VVHD_HIP(hipMemcpyAsync(slot, t.counters.p, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        VVHD_HIP(hipEventRecord(c.lev_ev[d & 1], s));
+        VVHD_HIP(hipEventRecord(c.lev_ev[d % Ctx::kLevSlots], s));
     };
     int32_t hc[8];
-    launch_level(0);
+    constexpr int ahead = Ctx::kLevSlots - 1;
+    for (int d = 0; d < ahead; d++) launch_level(d);
     for (int d = 0;; d++) {
-        launch_level(d + 1);
-        VVHD_HIP(hipEventSynchronize(c.lev_ev[d & 1]));
-        std::memcpy(hc, c.lev_counters + 8 * (d & 1), sizeof(hc));
+        launch_level(d + ahead);
+        VVHD_HIP(hipEventSynchronize(c.lev_ev[d % Ctx::kLevSlots]));
+        std::memcpy(hc, c.lev_counters + 8 * (d % Ctx::kLevSlots), sizeof(hc));
         VVHD_CHECK(hc[4] == 0, VVHD_ENOMEM, "tree_build: node capacity exceeded");
-        if (hc[3] == hc[2]) break;   // level d made no new nodes; level d + 1 did nothing
+        if (hc[3] == hc[2]) break;   // level d made no new nodes; the levels after it did nothing
         levels.push_back({hc[2], hc[3]});
         VVHD_CHECK((int)levels.size() < kMaxLevels, VVHD_EUNSUPPORTED, "tree_build: tree deeper than supported");
     }
