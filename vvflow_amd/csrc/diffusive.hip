@@ -66,13 +66,16 @@ constexpr int kStage = 512;   // staged sources per wave
 // s = l / ntc) runs over every nslot-th staged source: all 64 lanes work
 // whatever the leaf's occupancy.  Per pair: same sign, |dr|_1 >= 1e-10,
 // |dr| / eps_i <= 8; |dr| and 1/|dr| from one reciprocal square root.
+// kSeg: the FP64 segment terms are compiled in (their library exp and sqrt
+// take registers that would cost the vortex-only launches occupancy).
+template <bool kSeg>
 __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ g,
                                                       const double* __restrict__ ie, const vvhd_seg* segs, int nseg,
                                                       double* vx, double* vy, double re_one, const double* sim_re,
                                                       const int32_t* leaf_sim) {
     __shared__ float s_x[kWavesPerBlock][kStage], s_y[kWavesPerBlock][kStage], s_g[kWavesPerBlock][kStage];
-    __shared__ int s_off[kWavesPerBlock][kWave], s_j0[kWavesPerBlock][kWave];
+    __shared__ int s_idx[kWavesPerBlock][kStage];   // vortex index of each slot of the chunk being staged
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= T.nleaf) return;
@@ -86,8 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
     float* sx = s_x[w];
     float* sy = s_y[w];
     float* sg = s_g[w];
-    int* soff = s_off[w];
-    int* sj0 = s_j0[w];
+    int* sidx = s_idx[w];
 
     for (int base = v0; base < v1; base += 16) {
         const int ntc = min(16, v1 - base);
@@ -147,24 +149,15 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
             const int off = incl - cnt;
             const int tot = __shfl(incl, kWave - 1, kWave);
             if (tot == 0) continue;
-            soff[lane] = off;
-            sj0[lane] = j0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (int done = 0; done < tot;) {
                 const int take = min(tot - done, kStage - fill);
+                // each lane lists the vortices of its leaf that fall in this chunk
+                for (int k = max(off, done); k < min(off + cnt, done + take); k++) sidx[k - done] = j0 + (k - off);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int q = lane; q < take; q += kWave) {
-                    const int gidx = done + q;
-                    int lo = 0, hi = kWave - 1;   // last lane whose offset <= gidx
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (soff[mid] <= gidx)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
-                    }
-                    const int j = sj0[lo] + (gidx - soff[lo]);
+                    const int j = sidx[q];
                     sx[fill + q] = (float)(x[j] - cx);
                     sy[fill + q] = (float)(y[j] - cy);
                     sg[fill + q] = (float)g[j];
@@ -175,16 +168,16 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
                     compute(fill);
                     fill = 0;
                 }
+                // the next chunk rewrites the index list
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            // the next chunk rewrites the lane table
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         if (fill) compute(fill);
         // body segments of the near leaves within the cutoff (FP64)
         double s3x = 0, s3y = 0, s0 = 0;
-        if (nseg) {
+        if (kSeg && nseg) {
             for (int64_t eb = e0; eb < e1; eb += kWave) {
                 const int64_t e = eb + lane;
                 int M = -1;
@@ -313,9 +306,13 @@ void diffusive(Ctx& c) {
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nseg = (int)c.segs_h.size();
     const bool batch = c.nsim > 1;
-    hipLaunchKernelGGL(k_diffusive, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p,
-                       nseg, c.vx.p, c.vy.p, c.re, batch ? (const double*)c.sim_re.p : (const double*)nullptr,
-                       (const int32_t*)c.t.leaf_sim.p);
+    const double* sim_re = batch ? (const double*)c.sim_re.p : (const double*)nullptr;
+    if (nseg)
+        hipLaunchKernelGGL(k_diffusive<true>, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p,
+                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p);
+    else
+        hipLaunchKernelGGL(k_diffusive<false>, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p,
+                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p);
     if (nseg) {
         hipStream_t s = c.stream;
         c.iscratch0.grow(nl + 1);
