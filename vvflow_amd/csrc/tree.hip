@@ -580,43 +580,58 @@ struct ListEntries {
     int lead[kMaxEntries];                 // bit d: first entry beneath anc[d]
 };
 
-// Entry table of every root (thread r: root r).
+// Entry table of every root, one wave per root: the frontier starts at the
+// root and is expanded kSplitDepth times (an internal node above the split
+// depth is replaced by its two children in place), which keeps it in DFS
+// order; at most 2^kSplitDepth = 64 entries, one per lane.
 __global__ void k_list_entries(NodeView nv, const int32_t* pre, int nroots, ListEntries* Etab) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nroots) return;
+    __shared__ int fr[2][kMaxEntries];
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
     ListEntries* E = Etab + r;
-    int stack[2 * kMaxEntries + 2];
-    int sp = 0, n = 0;
-    stack[sp++] = r;
-    while (sp > 0) {
-        const int T = stack[--sp];
-        const int d = nv.depth[T];
-        if (d == kSplitDepth || nv.ch1[T] < 0) {
-            E->node[n] = pre[T];
-            E->depth[n] = d;
-            int a = T, lead = 0;
-            for (int k = d - 1; k >= 0; k--) {
-                const int p = nv.parent[a];
-                E->anc[n][k] = pre[p];
-                a = p;
-            }
-            // T is the first entry beneath an ancestor when every step from it down is to a first child
-            int c = T;
-            bool first = true;
-            for (int k = d - 1; k >= 0; k--) {
-                const int p = nv.parent[c];
-                first = first && (nv.ch1[p] == c);
-                if (first) lead |= 1 << k;
-                c = p;
-            }
-            E->lead[n] = lead;
-            n++;
-        } else {
-            stack[sp++] = nv.ch1[T] + 1;
-            stack[sp++] = nv.ch1[T];
+    if (lane == 0) fr[0][0] = r;
+    int n = 1, cur = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int step = 0; step < kSplitDepth; step++) {
+        const int T = lane < n ? fr[cur][lane] : -1;
+        int c = -1;
+        if (T >= 0 && nv.depth[T] < kSplitDepth) c = nv.ch1[T];
+        const int out = T < 0 ? 0 : (c >= 0 ? 2 : 1);
+        const int incl = wave_incl_scan(out);
+        const int pos = incl - out;
+        if (c >= 0) {
+            fr[cur ^ 1][pos] = c;
+            fr[cur ^ 1][pos + 1] = c + 1;
+        } else if (T >= 0) {
+            fr[cur ^ 1][pos] = T;
         }
+        n = __shfl(incl, kWave - 1, kWave);
+        cur ^= 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    E->n = n;
+    if (lane < n) {
+        const int T = fr[cur][lane];
+        const int d = nv.depth[T];
+        E->node[lane] = pre[T];
+        E->depth[lane] = d;
+        // ancestors, root first; T is the first entry beneath an ancestor when
+        // every step from it down is to a first child
+        int c = T, lead = 0;
+        bool first = true;
+        for (int k = d - 1; k >= 0; k--) {
+            const int p = nv.parent[c];
+            E->anc[lane][k] = pre[p];
+            first = first && (nv.ch1[p] == c);
+            if (first) lead |= 1 << k;
+            c = p;
+        }
+        E->lead[lane] = lead;
+    }
+    if (lane == 0) E->n = n;
 }
 
 __global__ void k_pack_boxes(NodeView nv, int nnodes, double4* out) {
@@ -1009,7 +1024,7 @@ void tree_lists(Ctx& c) {
                        tn);
     t.entries.grow(nroots * sizeof(ListEntries));
     ListEntries* E = reinterpret_cast<ListEntries*>(t.entries.p);
-    hipLaunchKernelGGL(k_list_entries, dim3(nblocks(nroots, 64)), dim3(64), 0, s, nv, t.pre.p, nroots, E);
+    hipLaunchKernelGGL(k_list_entries, dim3(nroots), dim3(kWave), 0, s, nv, t.pre.p, nroots, E);
     VVHD_HIP(hipMemsetAsync(cnt_n.p, 0, (nslot + 1) * 4, s));
     VVHD_HIP(hipMemsetAsync(cnt_f.p, 0, (nslot + 1) * 4, s));
     const double fc = (double)t.far_criteria;
