@@ -9,6 +9,8 @@ Diffusive velocity: FP32 pair sums, relative L2 within 1e-5 and pointwise
 within 1e-3 (floor 1e-3 rms) of the FP64 reference (exp-cutoff decisions at
 the |dr|/eps = 8 boundary may flip in FP32).  Segment friction: FP64, 1e-12.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -227,6 +229,33 @@ def test_repeatable(ctx):
         ctx.diffusive()
         outs.append(ctx.download())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_diffusive_beside_convective(ctx, learned):
+    """A diffusive call right after a convective one runs on a second stream
+    beside the convective kernels and adds its increments afterwards; any
+    call in between keeps the serial path.  Both give identical velocities."""
+    print(f"case diffusive beside convective learned={learned}", flush=True)
+    from vvflow_amd.train_farfield import MODEL_PATH, load_model
+    if learned and not os.path.exists(MODEL_PATH):
+        pytest.skip("no trained far-field model in the tree")
+    objs = V.gen_wake(200_000, 0x5EED0002)
+    load_case(ctx, objs)
+    if learned:
+        ctx.far_model_set(load_model(MODEL_PATH))
+    conv = (lambda: ctx.convective_learned((1.0, 0.0))) if learned else (lambda: ctx.convective((1.0, 0.0)))
+    out = []
+    for serial in (False, True):
+        load_case(ctx, objs)
+        ctx.tree_build(8)
+        ctx.epsilon(True)
+        conv()
+        if serial:
+            ctx.phase_times()
+        ctx.diffusive()
+        out.append(ctx.download())
+    assert np.array_equal(out[0], out[1], equal_nan=True)
 
 
 # ------------------------------------------------------------ full size

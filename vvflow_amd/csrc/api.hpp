@@ -21,8 +21,9 @@ struct vvhd_ctx {
     }                                 \
     return VVHD_OK;
 
-#define REQUIRE_CTX(ctx) \
-    if (!(ctx)) return VVHD_EINVAL
+#define REQUIRE_CTX(ctx)      \
+    if (!(ctx)) return VVHD_EINVAL; \
+    ++(ctx)->c.api_calls
 
 // Entry points whose reference counterpart couples the whole Space refuse to
 // run while several simulations share the context.

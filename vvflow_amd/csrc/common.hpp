@@ -165,6 +165,13 @@ struct TreeArrays {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // A diffusive call right after a convective one runs on stream2 (default
+    // priority), from the state the convective call started from, into
+    // dvx/dvy; the main stream adds them after its convective kernels.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_pre_conv = nullptr;
+    uint64_t api_calls = 0, conv_call = 0;   // entry-point calls so far; the last convective one
+    DBuf<double> dvx, dvy;
     std::string err;
     double re = INFINITY, dt = 1, inf_g = 0, time = 0;
 
@@ -316,7 +323,7 @@ void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 void convective(Ctx& c, double inf_x, double inf_y, int kahan);
 void convective_near(Ctx& c, int kahan);
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out);
-void diffusive(Ctx& c);
+void diffusive(Ctx& c, bool overlap);
 void bodies_prepare(Ctx& c);
 void bodies_release(Ctx& c);
 void farfield_release(Ctx& c);

@@ -162,6 +162,11 @@ int vvhd_create(int device, vvhd_ctx** out) {
             delete ctx;
             return VVHD_EHIP;
         }
+    if (hipStreamCreateWithFlags(&ctx->c.stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->c.ev_pre_conv, hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return VVHD_EHIP;
+    }
     if (hipHostMalloc((void**)&ctx->c.lev_counters, Ctx::kLevSlots * 8 * sizeof(int32_t), hipHostMallocDefault) !=
         hipSuccess) {
         delete ctx;
@@ -215,6 +220,13 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     t.cmm_ie.release();
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);
+    if (c.ev_pre_conv) (void)hipEventDestroy(c.ev_pre_conv);
+    if (c.stream2) {
+        (void)hipStreamSynchronize(c.stream2);
+        (void)hipStreamDestroy(c.stream2);
+    }
+    c.dvx.release();
+    c.dvy.release();
     for (auto& e : c.lev_ev)
         if (e) (void)hipEventDestroy(e);
     if (c.lev_counters) (void)hipHostFree(c.lev_counters);
@@ -426,6 +438,8 @@ int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan) {
     API_BEGIN
     Ctx& c = ctx->c;
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "MConvectiveFast::process_all_lists(): tree is not built");
+    VVHD_HIP(hipEventRecord(c.ev_pre_conv, c.stream));
+    c.conv_call = c.api_calls;
     VVHD_HIP(hipEventRecord(c.ev[4], c.stream));
     convective(c, inf_x, inf_y, kahan);
     VVHD_HIP(hipEventRecord(c.ev[5], c.stream));
@@ -449,9 +463,16 @@ int vvhd_diffusive(vvhd_ctx* ctx) {
     API_BEGIN
     Ctx& c = ctx->c;
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "MDiffusiveFast::process_vort_list(): tree is not built");
-    VVHD_HIP(hipEventRecord(c.ev[6], c.stream));
-    diffusive(c);
-    VVHD_HIP(hipEventRecord(c.ev[7], c.stream));
+    // right after a convective call the state it started from is the
+    // diffusive's input too: run beside the convective kernels
+    const bool overlap = c.conv_call + 1 == c.api_calls && c.segs_h.empty() && c.t.nleaves_h > 0;
+    if (overlap) {
+        VVHD_HIP(hipStreamWaitEvent(c.stream2, c.ev_pre_conv, 0));
+        VVHD_HIP(hipEventRecord(c.ev[6], c.stream2));
+    } else {
+        VVHD_HIP(hipEventRecord(c.ev[6], c.stream));
+    }
+    diffusive(c, overlap);
     c.phase_recorded[3] = true;
     API_END(ctx)
 }

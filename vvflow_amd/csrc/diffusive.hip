@@ -68,12 +68,14 @@ constexpr int kStage = 512;   // staged sources per wave
 // |dr| / eps_i <= 8; |dr| and 1/|dr| from one reciprocal square root.
 // kSeg: the FP64 segment terms are compiled in (their library exp and sqrt
 // take registers that would cost the vortex-only launches occupancy).
+// dvx/dvy non-null (no segments only): the increment is stored there for
+// k_add_diffusive instead of being added to vx/vy.
 template <bool kSeg>
 __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ g,
                                                       const double* __restrict__ ie, const vvhd_seg* segs, int nseg,
                                                       double* vx, double* vy, double re_one, const double* sim_re,
-                                                      const int32_t* leaf_sim) {
+                                                      const int32_t* leaf_sim, double* dvx, double* dvy) {
     __shared__ float s_x[kWavesPerBlock][kStage], s_y[kWavesPerBlock][kStage], s_g[kWavesPerBlock][kStage];
     __shared__ int s_idx[kWavesPerBlock][kStage];   // vortex index of each slot of the chunk being staged
     const int w = threadIdx.x / kWave;
@@ -216,8 +218,13 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
         if (lane < ntc && gi != 0) {
             const double si = sgn(gi);
             if ((sgn(S1) != si) || (fabs(S1) < fabs(0.1 * gi))) S1 = 0.1 * gi;
-            double ax = vx[i], ay = vy[i];
             const double c1 = iei / (re * S1);
+            if (!kSeg && dvx) {
+                dvx[i] = S2x * c1;
+                dvy[i] = S2y * c1;
+                continue;
+            }
+            double ax = vx[i], ay = vy[i];
             ax += S2x * c1;
             ay += S2y * c1;
             if (S0 > kPi) S0 = kPi;
@@ -299,20 +306,51 @@ __global__ __launch_bounds__(kBlock) void k_friction(TreeDev T, const double* __
 
 }  // namespace
 
-void diffusive(Ctx& c) {
+// Adds the increments of a diffusive pass that ran beside the convective
+// kernels, in the order the in-place pass adds them (its segment term is +0).
+__global__ void k_add_diffusive(const double* g, const double* dvx, const double* dvy, double* vx, double* vy,
+                                int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n || g[i] == 0) return;
+    double ax = vx[i] + dvx[i], ay = vy[i] + dvy[i];
+    ax += 0.0;
+    ay += 0.0;
+    vx[i] = ax;
+    vy[i] = ay;
+}
+
+void diffusive(Ctx& c, bool overlap) {
     const TreeDev T = tree_dev(c.t);
-    if (T.nleaf == 0) return;
+    if (T.nleaf == 0) {
+        VVHD_HIP(hipEventRecord(c.ev[7], c.stream));
+        return;
+    }
     const int nl = T.nleaf;
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nseg = (int)c.segs_h.size();
     const bool batch = c.nsim > 1;
     const double* sim_re = batch ? (const double*)c.sim_re.p : (const double*)nullptr;
+    if (overlap && nseg == 0) {
+        c.dvx.grow(c.n + 1);
+        c.dvy.grow(c.n + 1);
+        hipLaunchKernelGGL(k_diffusive<false>, dim3(grid), dim3(kBlock), 0, c.stream2, T, c.x.p, c.y.p, c.g.p, c.ie.p,
+                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p, c.dvx.p,
+                           c.dvy.p);
+        VVHD_HIP(hipEventRecord(c.ev[7], c.stream2));
+        VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev[7], 0));
+        hipLaunchKernelGGL(k_add_diffusive, dim3((unsigned)std::max<int64_t>(1, (c.n + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, c.stream, c.g.p, c.dvx.p, c.dvy.p,
+                           c.vx.p, c.vy.p, c.n);
+        return;
+    }
     if (nseg)
         hipLaunchKernelGGL(k_diffusive<true>, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p,
-                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p);
+                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p,
+                           (double*)nullptr, (double*)nullptr);
     else
         hipLaunchKernelGGL(k_diffusive<false>, dim3(grid), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p, c.ie.p,
-                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p);
+                           c.segs.p, nseg, c.vx.p, c.vy.p, c.re, sim_re, (const int32_t*)c.t.leaf_sim.p,
+                           (double*)nullptr, (double*)nullptr);
     if (nseg) {
         hipStream_t s = c.stream;
         c.iscratch0.grow(nl + 1);
@@ -349,6 +387,7 @@ void diffusive(Ctx& c) {
         hipLaunchKernelGGL(k_friction, dim3((nseg + kWavesPerBlock - 1) / kWavesPerBlock), dim3(kBlock), 0, s, T,
                            c.x.p, c.y.p, c.g.p, c.ie.p, c.segs.p, c.t.seg_leaf.p, off, c.iscratch3.p, nseg, c.fric.p);
     }
+    VVHD_HIP(hipEventRecord(c.ev[7], c.stream));
     VVHD_HIP(hipGetLastError());
 }
 

@@ -1273,6 +1273,8 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "convective_learned: tree is not built");
     require_model(c);
     VVHD_HIP(hipSetDevice(c.device));
+    VVHD_HIP(hipEventRecord(c.ev_pre_conv, c.stream));
+    c.conv_call = c.api_calls;
     VVHD_HIP(hipEventRecord(c.ev[4], c.stream));
     convective_near(c, kahan);
     const TreeDev T = tree_dev(c.t);
