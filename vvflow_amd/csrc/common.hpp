@@ -169,6 +169,7 @@ struct Ctx {
     // priority), from the state the convective call started from, into
     // dvx/dvy; the main stream adds them after its convective kernels.
     hipStream_t stream2 = nullptr;
+    hipStream_t stream3 = nullptr;   // the learned path's near field, beside its far field
     hipEvent_t ev_pre_conv = nullptr;
     uint64_t api_calls = 0, conv_call = 0;   // entry-point calls so far; the last convective one
     DBuf<double> dvx, dvy;
@@ -321,7 +322,7 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
 void tree_lists(Ctx& c);
 void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 void convective(Ctx& c, double inf_x, double inf_y, int kahan);
-void convective_near(Ctx& c, int kahan);
+void convective_near(Ctx& c, int kahan, hipStream_t s);
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out);
 void diffusive(Ctx& c, bool overlap);
 void bodies_prepare(Ctx& c);

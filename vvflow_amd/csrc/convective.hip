@@ -360,21 +360,19 @@ __global__ __launch_bounds__(kDsBlock) void k_direct_sum(const double* x, const 
 
 // Near-field sums of every vortex into c.near_v (shared by the plain and the
 // learned path).
-void convective_near(Ctx& c, int kahan) {
+void convective_near(Ctx& c, int kahan, hipStream_t s) {
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf == 0) return;
     const int nl = T.nleaf;
     c.near_v.grow(c.n + 1);
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
-    VVHD_HIP(hipEventRecord(c.ev[12], c.stream));
+    VVHD_HIP(hipEventRecord(c.ev[12], s));
     if (kahan)
-        hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.x.p, c.y.p, c.src_ge.p,
-                           c.near_v.p);
+        hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, c.near_v.p);
     else
-        hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.x.p, c.y.p, c.src_ge.p,
-                           c.near_v.p);
-    VVHD_HIP(hipEventRecord(c.ev[13], c.stream));
+        hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, c.near_v.p);
+    VVHD_HIP(hipEventRecord(c.ev[13], s));
     c.phase_recorded[6] = true;
 }
 
@@ -385,7 +383,7 @@ void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
     c.taylor.grow(nl + 1);
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, c.taylor.p);
-    convective_near(c, kahan);
+    convective_near(c, kahan, c.stream);
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
                        c.taylor.p, c.x.p, c.y.p, c.g.p, c.near_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     VVHD_HIP(hipGetLastError());

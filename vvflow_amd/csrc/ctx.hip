@@ -163,6 +163,7 @@ int vvhd_create(int device, vvhd_ctx** out) {
             return VVHD_EHIP;
         }
     if (hipStreamCreateWithFlags(&ctx->c.stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->c.stream3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->c.ev_pre_conv, hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return VVHD_EHIP;
@@ -221,10 +222,11 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);
     if (c.ev_pre_conv) (void)hipEventDestroy(c.ev_pre_conv);
-    if (c.stream2) {
-        (void)hipStreamSynchronize(c.stream2);
-        (void)hipStreamDestroy(c.stream2);
-    }
+    for (hipStream_t* st : {&c.stream2, &c.stream3})
+        if (*st) {
+            (void)hipStreamSynchronize(*st);
+            (void)hipStreamDestroy(*st);
+        }
     c.dvx.release();
     c.dvy.release();
     for (auto& e : c.lev_ev)
