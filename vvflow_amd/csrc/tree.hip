@@ -225,6 +225,49 @@ __global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t*
     }
 }
 
+// Bounding boxes of a new level's nodes once k_swap has put every element in
+// place: a node's elements are then the range [v0, v1), so one wave per node
+// reduces it (order-preserving integer images, NaN coordinates skipped, as
+// the atomics do) without contending with the other nodes; segments add
+// their atomics afterwards.
+__global__ void k_child_boxes(NodeView nv, const int32_t* counters, const double* x, const double* y) {
+    const int lb = counters[2], le = counters[3];
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    const uint64_t lo0 = ord_from_double(DBL_MAX), hi0 = ord_from_double(-DBL_MAX);
+    for (int id = lb + blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; id < le; id += waves) {
+        const int v0 = nv.v0[id], v1 = nv.v1[id];
+        if (v0 >= v1) continue;   // uniform over the wave
+        uint64_t lx = lo0, hx = hi0, ly = lo0, hy = hi0;
+        for (int k = v0 + lane_id(); k < v1; k += kWave) {
+            const double px = x[k], py = y[k];
+            if (!isnan(px)) {
+                const uint64_t o = ord_from_double(px);
+                lx = o < lx ? o : lx;
+                hx = o > hx ? o : hx;
+            }
+            if (!isnan(py)) {
+                const uint64_t o = ord_from_double(py);
+                ly = o < ly ? o : ly;
+                hy = o > hy ? o : hy;
+            }
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const uint64_t a = __shfl_xor(lx, o, kWave), b = __shfl_xor(hx, o, kWave);
+            const uint64_t c = __shfl_xor(ly, o, kWave), d = __shfl_xor(hy, o, kWave);
+            lx = a < lx ? a : lx;
+            hx = b > hx ? b : hx;
+            ly = c < ly ? c : ly;
+            hy = d > hy ? d : hy;
+        }
+        if (lane_id() == 0) {
+            if (lx != lo0) atomicMin((unsigned long long*)&nv.blx[id], (unsigned long long)lx);
+            if (hx != hi0) atomicMax((unsigned long long*)&nv.trx[id], (unsigned long long)hx);
+            if (ly != lo0) atomicMin((unsigned long long*)&nv.bly[id], (unsigned long long)ly);
+            if (hy != hi0) atomicMax((unsigned long long*)&nv.try_[id], (unsigned long long)hy);
+        }
+    }
+}
+
 // Segments of the root bounding boxes.
 __global__ void k_root_segs(NodeView nv, const vvhd_seg* segs, const int32_t* seg_root, int nseg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -907,10 +950,16 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         // the children of level d number at most nroots 2^(d+1)
         const bool top = ((int64_t)nroots << std::min<int>(d + 1, 30)) <= kTopNodes;
         hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
-                           left_pos, right_pos, c.x.p, c.y.p, c.id.p, !top);
+                           left_pos, right_pos, c.x.p, c.y.p, c.id.p, false);
         if (top && n)
             hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv,
                                (const int32_t*)(t.counters.p + 2), (const int32_t*)enode.p, c.x.p, c.y.p, n);
+        else if (n) {
+            const int64_t kids = std::min<int64_t>((int64_t)nroots << std::min<int>(d + 1, 30), cap);
+            const int grid = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (kids + 3) / 4));
+            hipLaunchKernelGGL(k_child_boxes, dim3(grid), dim3(kBlock), 0, s, nv, (const int32_t*)t.counters.p,
+                               c.x.p, c.y.p);
+        }
         if (nseg)
             hipLaunchKernelGGL(k_seg_step, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg, seg_node.p);
         int32_t* slot = c.lev_counters + 8 * (d % Ctx::kLevSlots);
