@@ -161,7 +161,7 @@ __device__ inline void bbox_add_wave(NodeView nv, int node, double px, double py
 // reduce per node in the wave, then in LDS per block, and one lane per block
 // and node issues the global atomics.  Each block covers kTopChunk
 // consecutive elements.
-constexpr int kTopNodes = 64;
+constexpr int kTopNodes = 512;
 constexpr int kTopChunk = 4096;
 
 __global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t* base_ptr, const int32_t* enode,
@@ -226,19 +226,24 @@ __global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t*
 }
 
 // Bounding boxes of a new level's nodes once k_swap has put every element in
-// place: a node's elements are then the range [v0, v1), so one wave per node
-// reduces it (order-preserving integer images, NaN coordinates skipped, as
-// the atomics do) without contending with the other nodes; segments add
-// their atomics afterwards.
+// place: a node's elements are then the range [v0, v1), so LPN lanes per
+// node (64, or 16 when nodes are small) reduce it (order-preserving integer
+// images, NaN coordinates skipped, as the atomics do) without contending
+// with the other nodes; segments add their atomics afterwards.
+template <int LPN>
 __global__ void k_child_boxes(NodeView nv, const int32_t* counters, const double* x, const double* y) {
     const int lb = counters[2], le = counters[3];
-    const int waves = gridDim.x * (blockDim.x / kWave);
+    constexpr int kPer = kWave / LPN;   // nodes per wave
+    const int slots = gridDim.x * (blockDim.x / kWave) * kPer;
+    const int sub = lane_id() % LPN;
     const uint64_t lo0 = ord_from_double(DBL_MAX), hi0 = ord_from_double(-DBL_MAX);
-    for (int id = lb + blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; id < le; id += waves) {
-        const int v0 = nv.v0[id], v1 = nv.v1[id];
-        if (v0 >= v1) continue;   // uniform over the wave
+    const int first = lb + (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kPer + lane_id() / LPN;
+    for (int base = lb; base < le; base += slots) {   // uniform trip count: shuffles stay converged
+        const int id = first + (base - lb);
+        const bool ok = id < le;
+        const int v0 = ok ? nv.v0[id] : 0, v1 = ok ? nv.v1[id] : 0;
         uint64_t lx = lo0, hx = hi0, ly = lo0, hy = hi0;
-        for (int k = v0 + lane_id(); k < v1; k += kWave) {
+        for (int k = v0 + sub; k < v1; k += LPN) {
             const double px = x[k], py = y[k];
             if (!isnan(px)) {
                 const uint64_t o = ord_from_double(px);
@@ -251,7 +256,7 @@ __global__ void k_child_boxes(NodeView nv, const int32_t* counters, const double
                 hy = o > hy ? o : hy;
             }
         }
-        for (int o = kWave / 2; o > 0; o >>= 1) {
+        for (int o = LPN / 2; o > 0; o >>= 1) {
             const uint64_t a = __shfl_xor(lx, o, kWave), b = __shfl_xor(hx, o, kWave);
             const uint64_t c = __shfl_xor(ly, o, kWave), d = __shfl_xor(hy, o, kWave);
             lx = a < lx ? a : lx;
@@ -259,7 +264,7 @@ __global__ void k_child_boxes(NodeView nv, const int32_t* counters, const double
             ly = c < ly ? c : ly;
             hy = d > hy ? d : hy;
         }
-        if (lane_id() == 0) {
+        if (ok && sub == 0) {
             if (lx != lo0) atomicMin((unsigned long long*)&nv.blx[id], (unsigned long long)lx);
             if (hx != hi0) atomicMax((unsigned long long*)&nv.trx[id], (unsigned long long)hx);
             if (ly != lo0) atomicMin((unsigned long long*)&nv.bly[id], (unsigned long long)ly);
@@ -955,10 +960,17 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
             hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv,
                                (const int32_t*)(t.counters.p + 2), (const int32_t*)enode.p, c.x.p, c.y.p, n);
         else if (n) {
+            // at most nroots 2^(d+1) children; 16 lanes each once they average under 256 elements
             const int64_t kids = std::min<int64_t>((int64_t)nroots << std::min<int>(d + 1, 30), cap);
-            const int grid = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (kids + 3) / 4));
-            hipLaunchKernelGGL(k_child_boxes, dim3(grid), dim3(kBlock), 0, s, nv, (const int32_t*)t.counters.p,
-                               c.x.p, c.y.p);
+            const bool small = (int64_t)n < 256 * kids;
+            const int64_t waves = small ? (kids + 3) / 4 : kids;
+            const int grid = (int)std::min<int64_t>(2048, std::max<int64_t>(1, (waves + 3) / 4));
+            if (small)
+                hipLaunchKernelGGL(k_child_boxes<16>, dim3(grid), dim3(kBlock), 0, s, nv,
+                                   (const int32_t*)t.counters.p, c.x.p, c.y.p);
+            else
+                hipLaunchKernelGGL(k_child_boxes<64>, dim3(grid), dim3(kBlock), 0, s, nv,
+                                   (const int32_t*)t.counters.p, c.x.p, c.y.p);
         }
         if (nseg)
             hipLaunchKernelGGL(k_seg_step, dim3(nblocks(nseg)), dim3(kBlock), 0, s, nv, c.segs.p, nseg, seg_node.p);
