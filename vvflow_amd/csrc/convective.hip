@@ -145,35 +145,65 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                 syr = (float)(y[j] - cy);
             }
         };
-        float sxr, syr, gj, e2;
-        if (total > 0) fetch(0, sxr, syr, gj, e2);
-        for (int sb = 0; sb < total; sb += kWave) {
-            float nsxr = 0.f, nsyr = 0.f, ngj = 0.f, ne2 = 1.f;
-            if (sb + kWave < total) fetch(sb + kWave, nsxr, nsyr, ngj, ne2);
+        if (KAHAN) {
+            // two sources per lane and step: their terms are added (FMAs
+            // folding in the compensation) before one compensated add, whose
+            // extra rounding is relative to a term, not to the running sum
+            float sa, ta, ga, ea, sbx, tb, gb, eb2;
+            if (total > 0) {
+                fetch(0, sa, ta, ga, ea);
+                fetch(kWave, sbx, tb, gb, eb2);
+            }
+            for (int sb = 0; sb < total; sb += 2 * kWave) {
+                float na = 0.f, nta = 0.f, nga = 0.f, nea = 1.f, nb = 0.f, ntb = 0.f, ngb = 0.f, neb = 1.f;
+                if (sb + 2 * kWave < total) {
+                    fetch(sb + 2 * kWave, na, nta, nga, nea);
+                    fetch(sb + 3 * kWave, nb, ntb, ngb, neb);
+                }
 #pragma unroll
-            for (int k = 0; k < NT; k++) {
-                const float dx = tx[k] - sxr;
-                const float dy = ty[k] - syr;
-                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, e2));
-                const float f = gj * __builtin_amdgcn_rcpf(r2);
-                if (KAHAN) {
-                    const float ux = __builtin_fmaf(-dy, f, -qx[k]);
+                for (int k = 0; k < NT; k++) {
+                    const float dxa = tx[k] - sa, dya = ty[k] - ta;
+                    const float fa = ga * __builtin_amdgcn_rcpf(__builtin_fmaf(dxa, dxa, __builtin_fmaf(dya, dya, ea)));
+                    const float dxb = tx[k] - sbx, dyb = ty[k] - tb;
+                    const float fb = gb * __builtin_amdgcn_rcpf(__builtin_fmaf(dxb, dxb, __builtin_fmaf(dyb, dyb, eb2)));
+                    const float ux = __builtin_fmaf(-dyb, fb, __builtin_fmaf(-dya, fa, -qx[k]));
                     const float tsx = sx[k] + ux;
                     qx[k] = (tsx - sx[k]) - ux;
                     sx[k] = tsx;
-                    const float uy = __builtin_fmaf(dx, f, -qy[k]);
+                    const float uy = __builtin_fmaf(dxb, fb, __builtin_fmaf(dxa, fa, -qy[k]));
                     const float tsy = sy[k] + uy;
                     qy[k] = (tsy - sy[k]) - uy;
                     sy[k] = tsy;
-                } else {
+                }
+                sa = na;
+                ta = nta;
+                ga = nga;
+                ea = nea;
+                sbx = nb;
+                tb = ntb;
+                gb = ngb;
+                eb2 = neb;
+            }
+        } else {
+            float sxr, syr, gj, e2;
+            if (total > 0) fetch(0, sxr, syr, gj, e2);
+            for (int sb = 0; sb < total; sb += kWave) {
+                float nsxr = 0.f, nsyr = 0.f, ngj = 0.f, ne2 = 1.f;
+                if (sb + kWave < total) fetch(sb + kWave, nsxr, nsyr, ngj, ne2);
+#pragma unroll
+                for (int k = 0; k < NT; k++) {
+                    const float dx = tx[k] - sxr;
+                    const float dy = ty[k] - syr;
+                    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, e2));
+                    const float f = gj * __builtin_amdgcn_rcpf(r2);
                     sx[k] = __builtin_fmaf(-dy, f, sx[k]);
                     sy[k] = __builtin_fmaf(dx, f, sy[k]);
                 }
+                sxr = nsxr;
+                syr = nsyr;
+                gj = ngj;
+                e2 = ne2;
             }
-            sxr = nsxr;
-            syr = nsyr;
-            gj = ngj;
-            e2 = ne2;
         }
     }
     double nearx = 0, neary = 0;
