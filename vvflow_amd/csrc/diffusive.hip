@@ -110,14 +110,18 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
         R = wave_max(R);
         const double R2 = R * R * (1.0 + 1e-6);
         Kahan s2x, s2y, s1;
-        auto compute = [&](int nsrc) {
+        // staged sources: positive ones from the front of the buffer, negative
+        // ones from the back, each in staging order; a target runs over its
+        // own sign's part only
+        int npos = 0, nneg = 0;
+        auto compute = [&]() {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (active) {
-                for (int k = slot; k < nsrc; k += nslot) {
+                const int k0 = pos ? 0 : kStage - nneg, k1 = pos ? npos : kStage;
+                for (int k = k0 + slot; k < k1; k += nslot) {
                     const float gj = sg[k];
-                    if (gj == 0.f || (gj > 0.f) != pos) continue;
                     const float dx = tx - sx[k];
                     const float dy = ty - sy[k];
                     if (fabsf(dx) + fabsf(dy) < 1e-10f) continue;
@@ -158,17 +162,33 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                for (int q = lane; q < take; q += kWave) {
-                    const int j = sidx[q];
-                    sx[fill + q] = (float)(x[j] - cx);
-                    sy[fill + q] = (float)(y[j] - cy);
-                    sg[fill + q] = (float)g[j];
+                for (int q0 = 0; q0 < take; q0 += kWave) {
+                    const int q = q0 + lane;
+                    float gv = 0.f, xv = 0.f, yv = 0.f;
+                    if (q < take) {
+                        const int j = sidx[q];
+                        gv = (float)g[j];
+                        xv = (float)(x[j] - cx);
+                        yv = (float)(y[j] - cy);
+                    }
+                    const uint64_t pm = __ballot(gv > 0.f), nm = __ballot(gv < 0.f);
+                    const uint64_t below = (1ull << lane) - 1;
+                    int k = -1;
+                    if (gv > 0.f) k = npos + __popcll(pm & below);
+                    if (gv < 0.f) k = kStage - 1 - (nneg + __popcll(nm & below));
+                    if (k >= 0) {
+                        sx[k] = xv;
+                        sy[k] = yv;
+                        sg[k] = gv;
+                    }
+                    npos += __popcll(pm);
+                    nneg += __popcll(nm);
                 }
-                fill += take;
+                fill = npos + nneg;
                 done += take;
                 if (fill == kStage) {
-                    compute(fill);
-                    fill = 0;
+                    compute();
+                    npos = nneg = fill = 0;
                 }
                 // the next chunk rewrites the index list
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -176,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
-        if (fill) compute(fill);
+        if (fill) compute();
         // body segments of the near leaves within the cutoff (FP64)
         double s3x = 0, s3y = 0, s0 = 0;
         if (kSeg && nseg) {
