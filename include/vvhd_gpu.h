@@ -236,6 +236,12 @@ int vvhd_phase_times(vvhd_ctx* ctx, float* ms8);
  *  (leaf, node) term), -1, leaves, nodes}; far terms and network passes
  *  count leaves with a target of non-zero circulation. */
 int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7);
+/* Concurrency of the step's kernels (default on): the learned near field
+ * runs beside its far field, and a diffusive call made right after a
+ * convective call runs beside the convective kernels; results are the same
+ * bit for bit either way.  Off, every kernel runs alone on the context
+ * stream, which is how a single kernel's own rate is measured. */
+int vvhd_set_overlap(vvhd_ctx* ctx, int enable);
 
 /* ---- synthetic inputs (host) -------------------------------------------
  * Karman-street wake: 400 Gaussian clusters (sigma 0.05) in two staggered

@@ -234,8 +234,10 @@ def test_repeatable(ctx):
 @pytest.mark.parametrize("learned", [False, True])
 def test_diffusive_beside_convective(ctx, learned):
     """A diffusive call right after a convective one runs on a second stream
-    beside the convective kernels and adds its increments afterwards; any
-    call in between keeps the serial path.  Both give identical velocities."""
+    beside the convective kernels and adds its increments afterwards (and the
+    learned near field beside its far field); any call in between keeps the
+    diffusive serial, and vvhd_set_overlap(0) runs every kernel alone.  All
+    three give identical velocities."""
     print(f"case diffusive beside convective learned={learned}", flush=True)
     from vvflow_amd.train_farfield import MODEL_PATH, load_model
     if learned and not os.path.exists(MODEL_PATH):
@@ -246,16 +248,19 @@ def test_diffusive_beside_convective(ctx, learned):
         ctx.far_model_set(load_model(MODEL_PATH))
     conv = (lambda: ctx.convective_learned((1.0, 0.0))) if learned else (lambda: ctx.convective((1.0, 0.0)))
     out = []
-    for serial in (False, True):
+    for mode in ("overlap", "serial diffusive", "no overlap"):
         load_case(ctx, objs)
+        ctx.set_overlap(mode != "no overlap")
         ctx.tree_build(8)
         ctx.epsilon(True)
         conv()
-        if serial:
+        if mode == "serial diffusive":
             ctx.phase_times()
         ctx.diffusive()
         out.append(ctx.download())
+    ctx.set_overlap(True)
     assert np.array_equal(out[0], out[1], equal_nan=True)
+    assert np.array_equal(out[0], out[2], equal_nan=True)
 
 
 # ------------------------------------------------------------ full size

@@ -91,6 +91,7 @@ ABI = {
     "vvhd_integrals": (_I, [_P, _VP]),
     "vvhd_direct_sum": (_I, [_P, _VP, _I64, _VP]),
     "vvhd_phase_times": (_I, [_P, _VP]),
+    "vvhd_set_overlap": (_I, [_P, _I32]),
     "vvhd_work_counts": (_I, [_P, _VP]),
     "vvhd_gen_wake": (_I, [_I64, _U64, _VP]),
     "vvhd_gen_cylinder": (_I, [_D, _I32, _VP]),
@@ -192,6 +193,11 @@ class Context:
     def upload(self, objs):
         objs = np.ascontiguousarray(objs, dtype=np.float64).reshape(-1, 6)
         self._check(self.L.vvhd_upload_vortices(self.h, _ptr(objs), objs.shape[0]))
+
+    def set_overlap(self, enable):
+        """Concurrent kernels within a step (default on); off, each kernel
+        runs alone (bit-identical results), for per-kernel measurement."""
+        self._check(self.L.vvhd_set_overlap(self.h, int(bool(enable))))
 
     def set_batch(self, vortex_off, body_off, re):
         """Declares the uploaded vortices and bodies as len(re) independent

@@ -467,7 +467,7 @@ int vvhd_diffusive(vvhd_ctx* ctx) {
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "MDiffusiveFast::process_vort_list(): tree is not built");
     // right after a convective call the state it started from is the
     // diffusive's input too: run beside the convective kernels
-    const bool overlap = c.conv_call + 1 == c.api_calls && c.segs_h.empty() && c.t.nleaves_h > 0;
+    const bool overlap = c.overlap && c.conv_call + 1 == c.api_calls && c.segs_h.empty() && c.t.nleaves_h > 0;
     if (overlap) {
         VVHD_HIP(hipStreamWaitEvent(c.stream2, c.ev_pre_conv, 0));
         VVHD_HIP(hipEventRecord(c.ev[6], c.stream2));
@@ -501,6 +501,13 @@ int vvhd_direct_sum(vvhd_ctx* ctx, const int64_t* targets, int64_t nt, double* v
     Ctx& c = ctx->c;
     VVHD_CHECK(nt >= 0 && (nt == 0 || v_out), VVHD_EINVAL, "direct_sum: bad arguments");
     direct_sum(c, targets, nt, v_out);
+    API_END(ctx)
+}
+
+int vvhd_set_overlap(vvhd_ctx* ctx, int enable) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    ctx->c.overlap = enable != 0;
     API_END(ctx)
 }
 

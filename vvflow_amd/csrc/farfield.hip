@@ -1277,8 +1277,9 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
     c.conv_call = c.api_calls;
     VVHD_HIP(hipEventRecord(c.ev[4], c.stream));
     // the near field runs on its own stream beside the far field
-    VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev_pre_conv, 0));
-    convective_near(c, kahan, c.stream3);
+    const hipStream_t ns = c.overlap ? c.stream3 : c.stream;
+    if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev_pre_conv, 0));
+    convective_near(c, kahan, ns);
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf > 0) {
         MomView M = moments(c);
@@ -1295,7 +1296,7 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
         c.phase_recorded[7] = true;
         hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.x.p, c.y.p,
                            c.far_leaf.p, c.far_v.p);
-        VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev[13], 0));
+        if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev[13], 0));
         hipLaunchKernelGGL(k_finish_learned, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, c.g.p, c.near_v.p,
                            c.far_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     }
