@@ -369,7 +369,8 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the hot path runs on a single MI355X (asked for --gpus {a.gpus}); "
-              "see DESIGN.md for how it would shard", file=sys.stderr)
+              "across GPUs it would shard by slabs of tree leaves with a halo exchange of the near region, "
+              "which is not built here", file=sys.stderr)
         sys.exit(2)
     import vvflow_amd as V
 
