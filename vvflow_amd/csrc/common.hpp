@@ -171,6 +171,7 @@ struct Ctx {
     hipStream_t stream2 = nullptr;
     hipStream_t stream3 = nullptr;   // the learned path's near field, beside its far field
     hipEvent_t ev_pre_conv = nullptr;
+    hipEvent_t ev_side = nullptr;            // end of the learned path's side-stream work
     uint64_t api_calls = 0, conv_call = 0;   // entry-point calls so far; the last convective one
     bool overlap = true;                     // vvhd_set_overlap
     DBuf<double> dvx, dvy;

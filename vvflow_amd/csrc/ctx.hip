@@ -164,7 +164,8 @@ int vvhd_create(int device, vvhd_ctx** out) {
         }
     if (hipStreamCreateWithFlags(&ctx->c.stream2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->c.stream3, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->c.ev_pre_conv, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->c.ev_pre_conv, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->c.ev_side, hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return VVHD_EHIP;
     }
@@ -222,6 +223,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);
     if (c.ev_pre_conv) (void)hipEventDestroy(c.ev_pre_conv);
+    if (c.ev_side) (void)hipEventDestroy(c.ev_side);
     for (hipStream_t* st : {&c.stream2, &c.stream3})
         if (*st) {
             (void)hipStreamSynchronize(*st);

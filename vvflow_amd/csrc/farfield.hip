@@ -16,7 +16,7 @@
 //     matrix cores (v_mfma_f32_32x32x16_bf16).  The network runs per
 //     (leaf, far node) at the leaf centre with forward-mode tangents, and
 //     each target takes the first-order expansion of the correction about
-//     the centre (k_far_learned, k_far_mono).
+//     the centre (k_far_learned, k_finish_learned).
 // The MLP's target is the residual between the direct Lamb-kernel sum over
 // the node's vortices (FP64) and the monopole field, expressed in the frame
 // of the target direction and scaled by S d^2 / (2 pi r^3) (S = |G+| + |G-|,
@@ -507,7 +507,7 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
 // with respect to the target position (forward-mode tangents through the
 // network, and the derivatives of the features, of the rotation back to
 // the target frame and of the scale S d^2 / (2 pi r^3)), summed over the far
-// list.  k_far_mono gives each target t the first-order expansion
+// list.  k_finish_learned gives each target t the first-order expansion
 // C + (t - c) . grad C, so the network runs once per (leaf, far node)
 // instead of once per (target, far node); the correction varies slowly over
 // a leaf, which lies far from the node.
@@ -631,16 +631,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
 }
 
-// Far field of every vortex by the learned path: the exact field at the
-// vortex of the far nodes' per-sign monopoles (FP32 terms from FP64
-// differences, Kahan sums, lane partials combined in FP64), plus the leaf's
-// network correction expanded to first order about the leaf centre.  One wave
-// per leaf; targets in chunks of up to 16 padded to a power of two, the far
+// Monopole part of the learned far field of every vortex: the exact field at
+// the vortex of the far nodes' per-sign monopoles (FP32 terms from FP64
+// differences, Kahan sums, lane partials combined in FP64); the leaf's network
+// correction is added by k_finish_learned.  One wave per leaf; targets in chunks of up to 16 padded to a power of two, the far
 // list split over 64 / npad lanes per target; the monopoles of 64 far nodes
 // at a time are staged in LDS.
 __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* __restrict__ nodes,
                                                      const double* __restrict__ x, const double* __restrict__ y,
-                                                     const double* __restrict__ leaf_corr, double2* far_out) {
+                                                     double2* far_out) {
     // far-node monopoles of the current batch of 64 far-list entries
     __shared__ double s_cx[kWavesPerBlock][kWave], s_cy[kWavesPerBlock][kWave];
     __shared__ float s_m[kWavesPerBlock][6][kWave];   // gpk, gmk, zpx, zpy, zmx, zmy
@@ -651,9 +650,6 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
     const int L = T.leaf_node[l];
     const int v0 = T.v0[L], v1 = T.v1[L];
     const int64_t fb = T.far_off[l], fe = T.far_off[l + 1];
-    const double cx = T.nx[L], cy = T.ny[L];
-    const double* C = leaf_corr + 6 * (size_t)l;
-    const double c0u = C[0], c0v = C[1], cxu = C[2], cxv = C[3], cyu = C[4], cyv = C[5];
     double* scx = s_cx[w];
     double* scy = s_cy[w];
     float(*sm)[kWave] = s_m[w];
@@ -716,26 +712,31 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
             du += __shfl_xor(du, o, kWave);
             dv += __shfl_xor(dv, o, kWave);
         }
-        if (fo == 0 && tvalid) {
-            const double ox = tx - cx, oy = ty - cy;
-            du += c0u + ox * cxu + oy * cyu;
-            dv += c0v + ox * cxv + oy * cyv;
-            far_out[base + k] = make_double2(du, dv);
-        }
+        if (fo == 0 && tvalid) far_out[base + k] = make_double2(du, dv);
     }
 }
 
-// v += U_inf; += near / (2 pi); += far (learned path).
-__global__ void k_finish_learned(const double* g, const double2* near_in, const double2* far_in, double* vx, double* vy,
-                                 int64_t n, double inf_x, double inf_y) {
+// v += U_inf; += near / (2 pi); += far (learned path): the far monopoles at
+// the vortex plus its leaf's network correction expanded to first order
+// about the leaf centre, C + (t - c) . grad C.
+__global__ void k_finish_learned(TreeDev T, const double* x, const double* y, const double* g,
+                                 const double2* near_in, const double2* far_in, const double* leaf_corr, double* vx,
+                                 double* vy, int64_t n, double inf_x, double inf_y) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || g[i] == 0) return;
+    const int l = T.vleaf[i];
+    const int L = T.leaf_node[l];
+    const double* C = leaf_corr + 6 * (size_t)l;
+    const double ox = x[i] - T.nx[L], oy = y[i] - T.ny[L];
+    double du = far_in[i].x, dv = far_in[i].y;
+    du += C[0] + ox * C[2] + oy * C[4];
+    dv += C[1] + ox * C[3] + oy * C[5];
     double ax = vx[i] + inf_x, ay = vy[i] + inf_y;
     ax += near_in[i].x * k1_2Pi;
     ay += near_in[i].y * k1_2Pi;
-    ax += far_in[i].x;
-    ay += far_in[i].y;
+    ax += du;
+    ay += dv;
     vx[i] = ax;
     vy[i] = ay;
 }
@@ -1290,15 +1291,19 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
         hipLaunchKernelGGL(k_far_nodes, dim3(nblocks(c.t.nnodes)), dim3(kBlock), 0, c.stream, T, M, c.t.nnodes, nodes);
         VVHD_HIP(hipEventRecord(c.ev[14], c.stream));
         const int grid = (T.nleaf + kWavesPerBlock - 1) / kWavesPerBlock;
+        // the far monopoles at the vortices need only the node records: after
+        // the near field on the side stream, beside the network
+        if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev[14], 0));
+        hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, ns, T, (const FarNode*)nodes, c.x.p, c.y.p,
+                           c.far_v.p);
+        if (c.overlap) VVHD_HIP(hipEventRecord(c.ev_side, c.stream3));
         hipLaunchKernelGGL(k_far_learned, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.g.p,
                            c.mlp_frags.p, c.mlp_params.p, c.far_leaf.p);
         VVHD_HIP(hipEventRecord(c.ev[15], c.stream));
         c.phase_recorded[7] = true;
-        hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.x.p, c.y.p,
-                           c.far_leaf.p, c.far_v.p);
-        if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev[13], 0));
-        hipLaunchKernelGGL(k_finish_learned, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, c.g.p, c.near_v.p,
-                           c.far_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
+        if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev_side, 0));
+        hipLaunchKernelGGL(k_finish_learned, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p,
+                           c.near_v.p, c.far_v.p, c.far_leaf.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     }
     VVHD_HIP(hipEventRecord(c.ev[5], c.stream));
     c.phase_recorded[2] = true;
