@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time the plain path at --learned-fc instead")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--far-gradient", choices=("network", "geometric"), default="network",
+                    help="gradient of the learned correction: the network's own forward-mode derivative "
+                         "(default) or the geometric terms only, with the network output held fixed")
     ap.add_argument("--error-sample", type=int, default=4000)
     ap.add_argument("--cpu-fraction", type=float, default=0.05)
     ap.add_argument("--model", default=os.path.join(ROOT, "vvflow_amd", "farfield_mlp.npy"))
@@ -386,6 +389,7 @@ def main():
     if learned:
         from vvflow_amd.train_farfield import load_model
         ctx.far_model_set(load_model(a.model))
+        ctx.set_far_gradient(a.far_gradient == "network")
     fc = a.learned_fc
 
     elapsed, ph = B.time(fc, learned, a.steps, a.warmup)
@@ -458,6 +462,7 @@ def main():
                         f"diffusive, {'learned far field' if learned else 'plain tree path'} at far criterion {fc}",
             "n": n, "far_criteria": fc, "re": 600.0, "accumulation": "kahan", "pair_dtype": "fp32",
             "tree_epsilon_dtype": "fp64", "far_field": "learned (bf16 MLP)" if learned else "plain",
+            "far_gradient": a.far_gradient if learned else None,
         },
         "phases_ms": {"tree": float(ph[0]), "epsilon": float(ph[1]), "convective": float(ph[2]),
                       "diffusive": float(ph[3]), "near_kernel": near_ms, "far_learned_kernel": far_ms},

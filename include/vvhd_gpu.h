@@ -242,6 +242,12 @@ int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7);
  * bit for bit either way.  Off, every kernel runs alone on the context
  * stream, which is how a single kernel's own rate is measured. */
 int vvhd_set_overlap(vvhd_ctx* ctx, int enable);
+/* Gradient of the learned far-field correction used for its first-order
+ * expansion over a leaf (no counterpart in the reference): network = 1
+ * (default) differentiates through the network (forward-mode tangents, three
+ * MFMA chains); network = 0 holds the network output fixed over the leaf and
+ * keeps the rotation and 1/r^3 terms (one chain, faster, larger error). */
+int vvhd_set_far_gradient(vvhd_ctx* ctx, int network);
 
 /* ---- synthetic inputs (host) -------------------------------------------
  * Karman-street wake: 400 Gaussian clusters (sigma 0.05) in two staggered

@@ -141,11 +141,13 @@ def test_training_reduces_loss(ctx, case):
     assert l1 < 0.2 * l0
 
 
-def _oracle_learned(o, targets, p=None):
+def _oracle_learned(o, targets, p=None, network_gradient=True):
     """Induced velocity of the learned path for the given targets, on the
     oracle: exact near field + per far node monopoles at the target + the
     network correction of the far nodes at the leaf centre, expanded to
-    first order in the target's offset from it (forward-mode gradient)."""
+    first order in the target's offset from it (forward-mode gradient; with
+    network_gradient False the feature tangents are zero, which holds the
+    network output fixed and keeps the rotation and 1/r^3 terms)."""
     T = o.tree()
     v = o.vortices()
     noff, nlst = T["near"]
@@ -172,6 +174,8 @@ def _oracle_learned(o, targets, p=None):
             D = np.array([cx, cy]) - geom[far, :2]
             rinv = 1.0 / np.hypot(D[:, 0], D[:, 1])
             TX, TY = M.feature_tangents(X, u[:, 0], u[:, 1], rinv)
+            if not network_gradient:
+                TX, TY = np.zeros_like(TX), np.zeros_like(TY)
             C = M.correction_jvp(p, X[ok], TX[ok], TY[ok], sc[ok], u[ok, 0], u[ok, 1], rinv[ok]).sum(0)
             out[q] += C[:2] + (v[t, 0] - cx) * C[2:4] + (v[t, 1] - cy) * C[4:6]
     return out
@@ -195,7 +199,8 @@ def test_learned_path_monopoles(ctx, case):
     assert err < 5e-6
 
 
-def test_learned_path_with_network(ctx, case):
+@pytest.mark.parametrize("network_gradient", [True, False])
+def test_learned_path_with_network(ctx, case, network_gradient):
     o, _ = case
     s = ctx.far_sample(0.05, seed=19)
     p = sample_model(s["X"], s["Y"], seed=9)
@@ -204,12 +209,14 @@ def test_learned_path_with_network(ctx, case):
     ctx.far_train(np.full(300, 2e-3, np.float32), batch=4096, seed=5)
     p = ctx.far_model_get()
     v0 = ctx.download()   # convective adds into v: compare the increment
+    ctx.set_far_gradient(network_gradient)
     ctx.convective_learned((0.0, 0.0))
+    ctx.set_far_gradient(True)
     vg = ctx.download()[:, 3:5] - v0[:, 3:5]
     tg = np.random.default_rng(2).choice(np.nonzero(v0[:, 2] != 0)[0], 150, replace=False)
-    want = _oracle_learned(o, tg, p)
+    want = _oracle_learned(o, tg, p, network_gradient)
     err = rel_l2(vg[tg], want)
-    print(f"  learned path rel L2 {err:.2e}", flush=True)
+    print(f"  learned path (network gradient {network_gradient}) rel L2 {err:.2e}", flush=True)
     assert err < 1e-5
 
 

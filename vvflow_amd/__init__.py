@@ -92,6 +92,7 @@ ABI = {
     "vvhd_direct_sum": (_I, [_P, _VP, _I64, _VP]),
     "vvhd_phase_times": (_I, [_P, _VP]),
     "vvhd_set_overlap": (_I, [_P, _I32]),
+    "vvhd_set_far_gradient": (_I, [_P, _I32]),
     "vvhd_work_counts": (_I, [_P, _VP]),
     "vvhd_gen_wake": (_I, [_I64, _U64, _VP]),
     "vvhd_gen_cylinder": (_I, [_D, _I32, _VP]),
@@ -193,6 +194,11 @@ class Context:
     def upload(self, objs):
         objs = np.ascontiguousarray(objs, dtype=np.float64).reshape(-1, 6)
         self._check(self.L.vvhd_upload_vortices(self.h, _ptr(objs), objs.shape[0]))
+
+    def set_far_gradient(self, network):
+        """Learned far field: differentiate the correction through the network
+        (True, default) or hold its output fixed over a leaf (False)."""
+        self._check(self.L.vvhd_set_far_gradient(self.h, int(bool(network))))
 
     def set_overlap(self, enable):
         """Concurrent kernels within a step (default on); off, each kernel

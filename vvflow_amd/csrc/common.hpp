@@ -174,6 +174,7 @@ struct Ctx {
     hipEvent_t ev_side = nullptr;            // end of the learned path's side-stream work
     uint64_t api_calls = 0, conv_call = 0;   // entry-point calls so far; the last convective one
     bool overlap = true;                     // vvhd_set_overlap
+    bool far_net_grad = true;                // vvhd_set_far_gradient
     DBuf<double> dvx, dvy;
     std::string err;
     double re = INFINITY, dt = 1, inf_g = 0, time = 0;

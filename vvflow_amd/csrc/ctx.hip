@@ -506,6 +506,13 @@ int vvhd_direct_sum(vvhd_ctx* ctx, const int64_t* targets, int64_t nt, double* v
     API_END(ctx)
 }
 
+int vvhd_set_far_gradient(vvhd_ctx* ctx, int network) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    ctx->c.far_net_grad = network != 0;
+    API_END(ctx)
+}
+
 int vvhd_set_overlap(vvhd_ctx* ctx, int enable) {
     REQUIRE_CTX(ctx);
     API_BEGIN

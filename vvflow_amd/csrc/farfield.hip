@@ -502,6 +502,33 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
     yy[1] = cy[1];
 }
 
+// The value chain alone (no tangents), same tiles and fragments.
+__device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
+                                       bf16x8 x, float (&y)[2]) {
+    const int ln = lane_id(), g = ln >> 4;
+    auto W = [&](int set) { return sw[set * kWave + ln]; };
+    bf16x8 h[2];
+    for (int b = 0; b < 4; b++) {
+        f32x4 c;
+        for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i];
+        c = mfma16(W((fB1 - fB1) / kSet + b), x, c);
+        for (int i = 0; i < 4; i++) h[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * sigmoid_f(c[i]));
+    }
+    bf16x8 h2[2];
+    for (int b = 0; b < 4; b++) {
+        f32x4 c;
+        for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i];
+        for (int st = 0; st < 2; st++) c = mfma16(W((fB2 - fB1) / kSet + 2 * b + st), h[st], c);
+        for (int i = 0; i < 4; i++) h2[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * sigmoid_f(c[i]));
+    }
+    f32x4 c = {};
+    c[0] = g == 0 ? b30 : 0.f;
+    c[1] = g == 0 ? b31 : 0.f;
+    for (int st = 0; st < 2; st++) c = mfma16(W((fB3 - fB1) / kSet + st), h2[st], c);
+    y[0] = c[0];
+    y[1] = c[1];
+}
+
 // Network part of the learned far field, one wave per leaf: for every far
 // node of the leaf, the correction at the leaf centre c and its gradient
 // with respect to the target position (forward-mode tangents through the
@@ -519,6 +546,10 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
 // zero padding of the K step.  Output per leaf: 6 doubles
 // (u, v, du/dx, dv/dx, du/dy, dv/dy); zeros for a leaf without far nodes or
 // without a target of non-zero circulation.
+// kJvp: the gradient takes the network's own derivative (forward-mode
+// tangents, three chains); otherwise the network output is held fixed over
+// the leaf and the gradient keeps the rotation and 1/r^3 terms (one chain).
+template <bool kJvp>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_far_learned(
     TreeDev T, const FarNode* __restrict__ nodes, const double* __restrict__ g, const uint16_t* frag, const float* P,
     double* __restrict__ leaf_corr) {
@@ -555,17 +586,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
     const double px = T.nx[L], py = T.ny[L];
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, cmp[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // Kahan sums
-    // the node record of the next tile is loaded while this tile computes
-    FarNode nd = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
-    for (int64_t f0 = fb; f0 < fe; f0 += 16) {
-        const int64_t e = f0 + n;
-        const int64_t en = e + 16;
-        const FarNode nxt = nodes[(en < fe) ? T.far_list[en] : 0];
+    // one tile: 16 far nodes (column n)
+    auto tile = [&](const FarNode& nd, bool valid) {
         const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
         const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
         const float ux = dx * rinv, uy = dy * rinv;
         const float thx = -uy * rinv, thy = ux * rinv;   // d theta / dp
-        const bool ok = e < fe && nd.ok != 0.f && grp < 2;
+        const bool ok = valid && nd.ok != 0.f && grp < 2;
         float f[8], fx[8], fy[8];
         if (grp == 0) {
             const float c2r = ux * ux - uy * uy, c2i = -2.f * ux * uy;
@@ -577,10 +604,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             f[5] = nd.dmy * ux - nd.dmx * uy;
             f[6] = nd.qqx * c2r - nd.qqy * c2i;
             f[7] = nd.qqx * c2i + nd.qqy * c2r;
-            const float rot[8] = {0.f, 0.f, f[3], -f[2], f[5], -f[4], 2.f * f[7], -2.f * f[6]};
-            for (int q = 0; q < 8; q++) {
-                fx[q] = rot[q] * thx;
-                fy[q] = rot[q] * thy;
+            if constexpr (kJvp) {
+                const float rot[8] = {0.f, 0.f, f[3], -f[2], f[5], -f[4], 2.f * f[7], -2.f * f[6]};
+                for (int q = 0; q < 8; q++) {
+                    fx[q] = rot[q] * thx;
+                    fy[q] = rot[q] * thy;
+                }
             }
         } else {
             f[0] = nd.e8;
@@ -588,39 +617,59 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             f[2] = nd.sd * rinv;
             f[3] = nd.asp;
             f[4] = f[5] = f[6] = f[7] = 0.f;
-            for (int q = 0; q < 8; q++) fx[q] = fy[q] = 0.f;
-            fx[2] = -f[2] * (ux * rinv);
-            fy[2] = -f[2] * (uy * rinv);
+            if constexpr (kJvp) {
+                for (int q = 0; q < 8; q++) fx[q] = fy[q] = 0.f;
+                fx[2] = -f[2] * (ux * rinv);
+                fy[2] = -f[2] * (uy * rinv);
+            }
         }
         bf16x8 xin, txin, tyin;
         for (int q = 0; q < 8; q++) {
             const int k = (8 * grp + q) & 15;
             const float is = sisd[k];
             xin[q] = (__bf16)(ok ? (f[q] - smu[k]) * is : 0.f);
-            txin[q] = (__bf16)(ok ? fx[q] * is : 0.f);
-            tyin[q] = (__bf16)(ok ? fy[q] * is : 0.f);
+            if constexpr (kJvp) {
+                txin[q] = (__bf16)(ok ? fx[q] * is : 0.f);
+                tyin[q] = (__bf16)(ok ? fy[q] * is : 0.f);
+            }
         }
-        float y[2], yx[2], yy[2];
-        mlp_infer_jvp16(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
+        float y[2], yx[2] = {0.f, 0.f}, yy[2] = {0.f, 0.f};
+        if constexpr (kJvp)
+            mlp_infer_jvp16(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
+        else
+            mlp_infer_val16(swf, sb1, sb2, b30, b31, xin, y);
         if (grp == 0 && ok) {
             const float sc = nd.scl * rinv * rinv * rinv;
             const float a = y[0] * ysd0, b = y[1] * ysd1;
             const float U = (a * ux - b * uy) * sc, V = (a * uy + b * ux) * sc;
-            const float ax = yx[0] * ysd0, bx = yx[1] * ysd1, ay = yy[0] * ysd0, by = yy[1] * ysd1;
             const float kx = -3.f * ux * rinv, ky = -3.f * uy * rinv;
             float t[6];
             t[0] = U;
             t[1] = V;
-            t[2] = kx * U - thx * V + (ax * ux - bx * uy) * sc;
-            t[3] = kx * V + thx * U + (ax * uy + bx * ux) * sc;
-            t[4] = ky * U - thy * V + (ay * ux - by * uy) * sc;
-            t[5] = ky * V + thy * U + (ay * uy + by * ux) * sc;
+            t[2] = kx * U - thx * V;
+            t[3] = kx * V + thx * U;
+            t[4] = ky * U - thy * V;
+            t[5] = ky * V + thy * U;
+            if constexpr (kJvp) {
+                const float ax = yx[0] * ysd0, bx = yx[1] * ysd1, ay = yy[0] * ysd0, by = yy[1] * ysd1;
+                t[2] += (ax * ux - bx * uy) * sc;
+                t[3] += (ax * uy + bx * ux) * sc;
+                t[4] += (ay * ux - by * uy) * sc;
+                t[5] += (ay * uy + by * ux) * sc;
+            }
             for (int q = 0; q < 6; q++) {
                 const float yk = t[q] - cmp[q], tk = acc[q] + yk;
                 cmp[q] = (tk - acc[q]) - yk;
                 acc[q] = tk;
             }
         }
+    };
+    // tiles of 16 consecutive far-list entries; the next tile's records load meanwhile
+    FarNode nd = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
+    for (int64_t f0 = fb; f0 < fe; f0 += 16) {
+        const int64_t en = f0 + 16 + n;
+        const FarNode nxt = nodes[(en < fe) ? T.far_list[en] : 0];
+        tile(nd, f0 + n < fe);
         nd = nxt;
     }
     // sums over the 16 columns (lanes 0..15), FP64, fixed order
@@ -1289,16 +1338,26 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
         FarNode* nodes = reinterpret_cast<FarNode*>(c.far_nodes.p);
         c.far_leaf.grow(6 * (size_t)T.nleaf);
         hipLaunchKernelGGL(k_far_nodes, dim3(nblocks(c.t.nnodes)), dim3(kBlock), 0, c.stream, T, M, c.t.nnodes, nodes);
-        VVHD_HIP(hipEventRecord(c.ev[14], c.stream));
         const int grid = (T.nleaf + kWavesPerBlock - 1) / kWavesPerBlock;
         // the far monopoles at the vortices need only the node records: after
-        // the near field on the side stream, beside the network
-        if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev[14], 0));
-        hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, ns, T, (const FarNode*)nodes, c.x.p, c.y.p,
-                           c.far_v.p);
-        if (c.overlap) VVHD_HIP(hipEventRecord(c.ev_side, c.stream3));
-        hipLaunchKernelGGL(k_far_learned, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.g.p,
-                           c.mlp_frags.p, c.mlp_params.p, c.far_leaf.p);
+        // the near field on the side stream, beside the network (one stream:
+        // before the network, outside the network's own phase events)
+        if (!c.overlap)
+            hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes, c.x.p,
+                               c.y.p, c.far_v.p);
+        VVHD_HIP(hipEventRecord(c.ev[14], c.stream));
+        if (c.overlap) {
+            VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev[14], 0));
+            hipLaunchKernelGGL(k_far_mono, dim3(grid), dim3(kBlock), 0, c.stream3, T, (const FarNode*)nodes, c.x.p,
+                               c.y.p, c.far_v.p);
+            VVHD_HIP(hipEventRecord(c.ev_side, c.stream3));
+        }
+        if (c.far_net_grad)
+            hipLaunchKernelGGL(k_far_learned<true>, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes,
+                               c.g.p, c.mlp_frags.p, c.mlp_params.p, c.far_leaf.p);
+        else
+            hipLaunchKernelGGL(k_far_learned<false>, dim3(grid), dim3(kBlock), 0, c.stream, T, (const FarNode*)nodes,
+                               c.g.p, c.mlp_frags.p, c.mlp_params.p, c.far_leaf.p);
         VVHD_HIP(hipEventRecord(c.ev[15], c.stream));
         c.phase_recorded[7] = true;
         if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev_side, 0));
