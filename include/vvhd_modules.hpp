@@ -201,6 +201,9 @@ class MConvectiveFast {
     // set, process_all_lists uses it; clear_learned() restores the plain path.
     void set_learned(const std::vector<float>& params);
     void clear_learned() { learned_ = false; }
+    // Gradient of the learned correction over a leaf: the network's own
+    // forward-mode derivative (true, default) or the geometric terms only.
+    void set_learned_gradient(bool network);
     // Pair sums: Kahan-compensated FP32 (default) or plain FP32.
     void set_kahan(bool k) { kahan_ = k; }
 

@@ -292,6 +292,10 @@ void MConvectiveFast::set_learned(const std::vector<float>& params) {
     learned_ = true;
 }
 
+void MConvectiveFast::set_learned_gradient(bool network) {
+    S->check(vvhd_set_far_gradient(S->ctx_, network ? 1 : 0));
+}
+
 // ---------------------------------------------------------- MEpsilonFast --
 
 void MEpsilonFast::CalcEpsilonFast(bool merge) {
