@@ -98,6 +98,21 @@ constexpr int kPartStride = kTrain + 2;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Eight floats to bf16 (round to nearest even), converted in pairs, and the
+// whole vector zeroed when keep is false (one select per packed word).
+__device__ inline bf16x8 pack_bf16x8(const float (&v)[8], bool keep) {
+    u32x4 w;
+    for (int q = 0; q < 4; q++) {
+        const bf16x2 p = __builtin_convertvector((f32x2){v[2 * q], v[2 * q + 1]}, bf16x2);
+        const uint32_t u = __builtin_bit_cast(uint32_t, p);
+        w[q] = keep ? u : 0u;
+    }
+    return __builtin_bit_cast(bf16x8, w);
+}
 
 inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
 
@@ -623,15 +638,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 fy[2] = -f[2] * (uy * rinv);
             }
         }
-        bf16x8 xin, txin, tyin;
+        float xs[8], txs[8], tys[8];
         for (int q = 0; q < 8; q++) {
             const int k = (8 * grp + q) & 15;
             const float is = sisd[k];
-            xin[q] = (__bf16)(ok ? (f[q] - smu[k]) * is : 0.f);
+            xs[q] = (f[q] - smu[k]) * is;
             if constexpr (kJvp) {
-                txin[q] = (__bf16)(ok ? fx[q] * is : 0.f);
-                tyin[q] = (__bf16)(ok ? fy[q] * is : 0.f);
+                txs[q] = fx[q] * is;
+                tys[q] = fy[q] * is;
             }
+        }
+        const bf16x8 xin = pack_bf16x8(xs, ok);
+        bf16x8 txin, tyin;
+        if constexpr (kJvp) {
+            txin = pack_bf16x8(txs, ok);
+            tyin = pack_bf16x8(tys, ok);
         }
         float y[2], yx[2] = {0.f, 0.f}, yy[2] = {0.f, 0.f};
         if constexpr (kJvp)
