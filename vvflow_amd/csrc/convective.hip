@@ -206,18 +206,39 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
             }
         }
     }
-    double nearx = 0, neary = 0;
+    // Totals over the 64 lanes of the P values of each component (P = NT
+    // rounded up to a power of two) by recursive halving: at lane offset o a
+    // lane keeps the half of its values picked by (lane & o) and adds the
+    // partner's copy of that half, so each exchange moves half the remaining
+    // values.  Every value is summed in the same order as a butterfly over
+    // offsets 32, 16, ..., 1.  Lane l ends with the total of target slot
+    // idx, shared by the lanes that differ only in the low bits.
+    constexpr int P = NT <= 2 ? 2 : NT <= 4 ? 4 : NT <= 8 ? 8 : 16;
+    constexpr int kLow = kWave / P - 1;   // lanes sharing a total: the lowest writes it
 #pragma unroll
-    for (int k = 0; k < NT; k++) {
-        const double px = KAHAN ? (double)sx[k] - (double)qx[k] : (double)sx[k];
-        const double py = KAHAN ? (double)sy[k] - (double)qy[k] : (double)sy[k];
-        const double rx = wave_sum(px), ry = wave_sum(py);
-        if (lane == k) {
-            nearx = rx;
-            neary = ry;
+    for (int comp = 0; comp < 2; comp++) {
+        double v[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+            const float s_ = comp == 0 ? sx[k < NT ? k : 0] : sy[k < NT ? k : 0];
+            const float q_ = comp == 0 ? qx[k < NT ? k : 0] : qy[k < NT ? k : 0];
+            v[k] = k < NT ? (KAHAN ? (double)s_ - (double)q_ : (double)s_) : 0.;
         }
+        int idx = 0, o = kWave / 2;
+#pragma unroll
+        for (int h = P / 2; h >= 1; h >>= 1, o >>= 1) {
+            const bool hi = (lane & o) != 0;
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                const double keep = hi ? v[j + h] : v[j];
+                const double send = hi ? v[j] : v[j + h];
+                v[j] = keep + __shfl_xor(send, o, kWave);
+            }
+            if (hi) idx += h;
+        }
+        for (; o > 0; o >>= 1) v[0] += __shfl_xor(v[0], o, kWave);
+        if ((lane & kLow) == 0 && idx < nt) reinterpret_cast<double*>(near_out)[2 * (base + idx) + comp] = v[0];
     }
-    if (lane < nt) near_out[base + lane] = make_double2(nearx, neary);
 }
 
 template <bool KAHAN>
