@@ -34,6 +34,10 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+// Sources of one batch of 64 near-list entries whose global indices fit the
+// per-wave LDS table of the near kernel (a larger batch searches the entry
+// prefix sums instead).
+constexpr int kNearTab = 1024;
 constexpr int kTargets = 16;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double k1_2Pi = 1. / (2. * kPi);
@@ -88,10 +92,11 @@ struct NearArgs {
 // Near-field sums of one chunk of NT targets (base .. base+nt-1, nt <= NT;
 // slots beyond nt hold far-away dummies whose sums are discarded), with no
 // per-target branches in the pair loop.
+// tab: this wave's LDS table of source indices (kNearTab entries).
 template <int NT, bool KAHAN>
 __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ x, const double* __restrict__ y,
                                   const float2* __restrict__ src_ge, double2* __restrict__ near_out, int base, int nt,
-                                  double cx, double cy, int64_t e_beg, int64_t e_end) {
+                                  double cx, double cy, int64_t e_beg, int64_t e_end, int* tab) {
     const int lane = lane_id();
     float my_tx = 1e15f, my_ty = 1e15f;
     if (lane < nt) {
@@ -121,18 +126,35 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
         }
         const int incl = wave_incl_scan(cnt);
         const int total = __shfl(incl, kWave - 1, kWave);
+        // source position -> global index: each lane writes its entry's run
+        // into the table (the previous batch's reads are done: one wave
+        // issues its LDS operations in order)
+        const bool use_tab = total <= kNearTab;   // uniform
+        if (use_tab) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int t = 0; t < cnt; t++) tab[incl - cnt + t] = s0 + t;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         auto fetch = [&](int sb, float& sxr, float& syr, float& gj, float& e2) {
             const int sp = sb + lane;
-            // entry index: number of entries whose inclusive prefix <= sp
-            int pos = 0;
+            int j;
+            if (use_tab) {
+                j = sp < total ? tab[sp] : 0;
+            } else {
+                // entry index: number of entries whose inclusive prefix <= sp
+                int pos = 0;
 #pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int v = __shfl(incl, pos + step - 1, kWave);
-                if (v <= sp) pos += step;
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int v = __shfl(incl, pos + step - 1, kWave);
+                    if (v <= sp) pos += step;
+                }
+                pos = min(pos, kWave - 1);
+                const int ex = __shfl(incl, pos, kWave) - __shfl(cnt, pos, kWave);
+                j = __shfl(s0, pos, kWave) + (sp - ex);
             }
-            pos = min(pos, kWave - 1);
-            const int ex = __shfl(incl, pos, kWave) - __shfl(cnt, pos, kWave);
-            const int j = __shfl(s0, pos, kWave) + (sp - ex);
             sxr = 0.f;
             syr = 0.f;
             gj = 0.f;
@@ -245,6 +267,7 @@ template <bool KAHAN>
 __global__ __launch_bounds__(kBlock) void k_near(NearArgs A, const double* __restrict__ x,
                                                  const double* __restrict__ y, const float2* __restrict__ src_ge,
                                                  double2* __restrict__ near_out) {
+    __shared__ int s_tab[kWavesPerBlock][kNearTab];
     const int w = threadIdx.x / kWave;
     const int l = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + w;
     if (l >= A.nleaf) return;
@@ -256,14 +279,14 @@ __global__ __launch_bounds__(kBlock) void k_near(NearArgs A, const double* __res
         const int nt = min(kTargets, v1 - base);
         // target slots in steps of two: few dummy targets per chunk
         switch ((nt + 1) / 2) {
-            case 1: near_chunk<2, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            case 2: near_chunk<4, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            case 3: near_chunk<6, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            case 4: near_chunk<8, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            case 5: near_chunk<10, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            case 6: near_chunk<12, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            case 7: near_chunk<14, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
-            default: near_chunk<16, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end); break;
+            case 1: near_chunk<2, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 2: near_chunk<4, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 3: near_chunk<6, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 4: near_chunk<8, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 5: near_chunk<10, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 6: near_chunk<12, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 7: near_chunk<14, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            default: near_chunk<16, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
         }
     }
 }
