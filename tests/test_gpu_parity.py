@@ -140,6 +140,18 @@ def test_random_clusters_vs_oracle(ctx, seed, signs, n):
     assert max_rel(dv_g, dv_o) < DIFF_TOL_MAX
 
 
+def test_large_leaves_vs_oracle(ctx):
+    """Min node 0.02 stops the splits inside the clusters: leaves of up to 63
+    vortices, and 115 of the 958 batches of 64 near-list entries hold more
+    than 1024 sources, so the near kernel locates sources both through its
+    LDS index table and through the prefix-sum search."""
+    print("case large leaves", flush=True)
+    conv_err, dv_g, dv_o, _ = run_both(ctx, random_cluster_case(8000, 41, "cluster"), mn=0.02)
+    assert conv_err < CONV_TOL_KAHAN
+    assert rel_l2(dv_g, dv_o) < DIFF_TOL_L2
+    assert max_rel(dv_g, dv_o) < DIFF_TOL_MAX
+
+
 def test_plain_fp32_accumulation(ctx):
     print("case plain fp32", flush=True)
     conv_err, _, _, _ = run_both(ctx, random_cluster_case(5000, 7, "cluster"), kahan=False)
