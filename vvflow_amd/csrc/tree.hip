@@ -25,8 +25,6 @@
 
 #include <cstring>
 #include <rocprim/device/device_scan.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-#include <rocprim/iterator/transform_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -363,8 +361,8 @@ __device__ inline bool goes_left(const NodeView& nv, int node, double px, double
     return (nv.h[node] < nv.w[node]) ? (px < nv.x[node]) : (py < nv.y[node]);
 }
 
-// Per element: "goes left" flag of elements in split nodes (else 0), read
-// by the scan through a transform iterator; 0 past the last element.
+// Per element: "goes left" flag of elements in split nodes (else 0); 0 past
+// the last element.
 struct GoesLeftFlag {
     NodeView nv;
     const int32_t* enode;
@@ -863,14 +861,69 @@ void scan_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t n) {
 }
 
 // Exclusive scan of the goes-left flags, computed on the fly.
-void scan_flags(Ctx& c, const GoesLeftFlag& f, int32_t* out, int64_t n) {
-    const auto in = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), f);
-    size_t bytes = 0;
-    VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int32_t)0, (size_t)n, rocprim::plus<int32_t>(),
-                                     c.stream));
-    c.tmp.grow(bytes + 16);
-    VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, in, out, (int32_t)0, (size_t)n, rocprim::plus<int32_t>(),
-                                     c.stream));
+// Exclusive prefix of the flags in two passes over blocks of kScanSpan
+// elements: k_flag_count stores every flag and each block's count,
+// k_flag_prefix adds the counts of the blocks before it to a block scan.
+// The flag and prefix arrays are padded to whole blocks (zero flags).
+constexpr int kScanSpan = 2048;   // 256 threads x 8 elements
+
+__device__ inline int block_sum_256(int v, int* s_w) {
+    v = wave_sum_i(v);
+    const int w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane_id() == 0) s_w[w] = v;
+    __syncthreads();
+    return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(256) void k_flag_count(GoesLeftFlag f, int64_t m, int32_t* flag, int32_t* bcount) {
+    __shared__ int s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanSpan;
+    int cnt = 0;
+    for (int i = 0; i < kScanSpan / 256; i++) {
+        const int64_t k = base + i * 256 + threadIdx.x;
+        const int v = k < m ? f((int)k) : 0;
+        flag[k] = v;
+        cnt += v;
+    }
+    cnt = block_sum_256(cnt, s_w);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_flag_prefix(const int32_t* flag, const int32_t* bcount, int32_t* P) {
+    __shared__ int s_w[4];
+    const int b = blockIdx.x, t = threadIdx.x, w = t / kWave;
+    int pre = 0;
+    for (int j = t; j < b; j += 256) pre += bcount[j];
+    pre = block_sum_256(pre, s_w);
+    const int64_t k0 = (int64_t)b * kScanSpan + 8 * t;
+    const int4 lo = *reinterpret_cast<const int4*>(flag + k0), hi = *reinterpret_cast<const int4*>(flag + k0 + 4);
+    const int v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    int ex[8], tot = 0;
+    for (int i = 0; i < 8; i++) {
+        ex[i] = tot;
+        tot += v[i];
+    }
+    const int incl = wave_incl_scan(tot);
+    __syncthreads();   // s_w is reused
+    if (lane_id() == kWave - 1) s_w[w] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int q = 0; q < w; q++) woff += s_w[q];
+    const int start = pre + woff + incl - tot;
+    *reinterpret_cast<int4*>(P + k0) = make_int4(start + ex[0], start + ex[1], start + ex[2], start + ex[3]);
+    *reinterpret_cast<int4*>(P + k0 + 4) = make_int4(start + ex[4], start + ex[5], start + ex[6], start + ex[7]);
+}
+
+// P[k] = number of flags before k, for k < m (flag is written too); the
+// arrays hold scan_blocks(m) * kScanSpan entries, bcount scan_blocks(m).
+int64_t scan_blocks(int64_t m) { return (m + kScanSpan - 1) / kScanSpan; }
+
+void scan_flags(Ctx& c, const GoesLeftFlag& f, int32_t* flag, int32_t* P, int32_t* bcount, int64_t m) {
+    const int64_t nb = scan_blocks(m);
+    hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nb), dim3(256), 0, c.stream, f, m, flag, bcount);
+    hipLaunchKernelGGL(k_flag_prefix, dim3((unsigned)nb), dim3(256), 0, c.stream, (const int32_t*)flag,
+                       (const int32_t*)bcount, P);
 }
 
 }  // namespace
@@ -899,8 +952,10 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     t.leaf_node.grow(cap);
     t.vleaf.grow(n + 1);
     c.iscratch0.grow(n + 1);   // element node
-    c.iscratch1.grow(n + 2);   // flags
-    c.iscratch2.grow(n + 2);   // prefix
+    const int64_t scan_pad = scan_blocks((int64_t)n + 1) * kScanSpan;
+    c.iscratch1.grow(std::max<int64_t>(n + 2, scan_pad));   // flags
+    c.iscratch2.grow(std::max<int64_t>(n + 2, scan_pad));   // prefix
+    c.tmp.grow(scan_blocks((int64_t)n + 1) * sizeof(int32_t) + 16);   // per-block flag counts
     c.iscratch3.grow(2 * (size_t)n + 2);   // left/right positions
     DBuf<int32_t>& enode = c.iscratch0;
     int32_t* flag = c.iscratch1.p;
@@ -949,7 +1004,7 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         const int node_blocks = 1024;
         hipLaunchKernelGGL(k_decide, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, leaf_tmp.p, cap,
                            min_node, max_node);
-        scan_flags(c, GoesLeftFlag{nv, enode.p, c.x.p, c.y.p, n}, P, n + 1);
+        scan_flags(c, GoesLeftFlag{nv, enode.p, c.x.p, c.y.p, n}, flag, P, reinterpret_cast<int32_t*>(c.tmp.p), n + 1);
         hipLaunchKernelGGL(k_ranks, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, c.x.p, c.y.p, P, n,
                            node_L.p, node_m.p, left_pos, right_pos, t.counters.p, flag);
         // the children of level d number at most nroots 2^(d+1)
