@@ -381,11 +381,11 @@ struct GoesLeftFlag {
 // first element of a split node also records its split position and number
 // of misplaced pairs and sets the children's ranges.  Thread 0 advances the
 // level range (the nodes of the next level were allocated by k_decide).
-// The flags are kept for k_swap: it must not derive them from coordinates
-// that the swaps may be rewriting.
-__global__ void k_ranks(NodeView nv, const int32_t* enode, const double* x, const double* y, const int32_t* P,
-                        int n, int32_t* node_L, int32_t* node_m, int32_t* left_pos, int32_t* right_pos,
-                        int32_t* counters, int32_t* flag) {
+// The flags come from k_flag_count, before any swap of this level: k_swap
+// reads them too and must not derive them from coordinates that the swaps
+// may be rewriting.
+__global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* P, int n, int32_t* node_L, int32_t* node_m,
+                        int32_t* left_pos, int32_t* right_pos, int32_t* counters, const int32_t* flag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         counters[2] = counters[3];
@@ -407,8 +407,7 @@ __global__ void k_ranks(NodeView nv, const int32_t* enode, const double* x, cons
         nv.v0[c + 1] = Labs;
         nv.v1[c + 1] = v1;
     }
-    const bool f = goes_left(nv, node, x[k], y[k]);
-    flag[k] = f ? 1 : 0;
+    const bool f = flag[k] != 0;
     if (k < Labs && !f) {
         const int r = (k - v0) - (P[k] - P0);
         left_pos[v0 + r] = k;
@@ -1005,8 +1004,8 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         hipLaunchKernelGGL(k_decide, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, leaf_tmp.p, cap,
                            min_node, max_node);
         scan_flags(c, GoesLeftFlag{nv, enode.p, c.x.p, c.y.p, n}, flag, P, reinterpret_cast<int32_t*>(c.tmp.p), n + 1);
-        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, c.x.p, c.y.p, P, n,
-                           node_L.p, node_m.p, left_pos, right_pos, t.counters.p, flag);
+        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, P, n, node_L.p, node_m.p,
+                           left_pos, right_pos, t.counters.p, (const int32_t*)flag);
         // the children of level d number at most nroots 2^(d+1)
         const bool top = ((int64_t)nroots << std::min<int>(d + 1, 30)) <= kTopNodes;
         hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
