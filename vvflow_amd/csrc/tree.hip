@@ -859,8 +859,7 @@ void scan_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t n) {
                                      c.stream));
 }
 
-// Exclusive scan of the goes-left flags, computed on the fly.
-// Exclusive prefix of the flags in two passes over blocks of kScanSpan
+// Exclusive prefix of the goes-left flags in two passes over blocks of kScanSpan
 // elements: k_flag_count stores every flag and each block's count,
 // k_flag_prefix adds the counts of the blocks before it to a block scan.
 // The flag and prefix arrays are padded to whole blocks (zero flags).
