@@ -171,17 +171,11 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
             // two sources per lane and step: their terms are added (FMAs
             // folding in the compensation) before one compensated add, whose
             // extra rounding is relative to a term, not to the running sum
-            float sa, ta, ga, ea, sbx, tb, gb, eb2;
-            if (total > 0) {
-                fetch(0, sa, ta, ga, ea);
-                fetch(kWave, sbx, tb, gb, eb2);
-            }
-            for (int sb = 0; sb < total; sb += 2 * kWave) {
-                float na = 0.f, nta = 0.f, nga = 0.f, nea = 1.f, nb = 0.f, ntb = 0.f, ngb = 0.f, neb = 1.f;
-                if (sb + 2 * kWave < total) {
-                    fetch(sb + 2 * kWave, na, nta, nga, nea);
-                    fetch(sb + 3 * kWave, nb, ntb, ngb, neb);
-                }
+            // two sets of prefetched sources used in turn (no copies between
+            // steps); past the batch the fetch returns zero-circulation sources
+            float sa0, ta0, ga0, ea0, sb0, tb0, gb0, eb0;
+            float sa1, ta1, ga1, ea1, sb1, tb1, gb1, eb1;
+            auto pairs = [&](float sa, float ta, float ga, float ea, float sbx, float tb, float gb, float eb2) {
 #pragma unroll
                 for (int k = 0; k < NT; k++) {
                     const float dxa = tx[k] - sa, dya = ty[k] - ta;
@@ -197,14 +191,19 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                     qy[k] = (tsy - sy[k]) - uy;
                     sy[k] = tsy;
                 }
-                sa = na;
-                ta = nta;
-                ga = nga;
-                ea = nea;
-                sbx = nb;
-                tb = ntb;
-                gb = ngb;
-                eb2 = neb;
+            };
+            if (total > 0) {
+                fetch(0, sa0, ta0, ga0, ea0);
+                fetch(kWave, sb0, tb0, gb0, eb0);
+            }
+            for (int sb = 0; sb < total; sb += 4 * kWave) {
+                fetch(sb + 2 * kWave, sa1, ta1, ga1, ea1);
+                fetch(sb + 3 * kWave, sb1, tb1, gb1, eb1);
+                pairs(sa0, ta0, ga0, ea0, sb0, tb0, gb0, eb0);
+                if (sb + 2 * kWave >= total) break;
+                fetch(sb + 4 * kWave, sa0, ta0, ga0, ea0);
+                fetch(sb + 5 * kWave, sb0, tb0, gb0, eb0);
+                pairs(sa1, ta1, ga1, ea1, sb1, tb1, gb1, eb1);
             }
         } else {
             float sxr, syr, gj, e2;
