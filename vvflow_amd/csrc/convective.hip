@@ -206,11 +206,8 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                 pairs(sa1, ta1, ga1, ea1, sb1, tb1, gb1, eb1);
             }
         } else {
-            float sxr, syr, gj, e2;
-            if (total > 0) fetch(0, sxr, syr, gj, e2);
-            for (int sb = 0; sb < total; sb += kWave) {
-                float nsxr = 0.f, nsyr = 0.f, ngj = 0.f, ne2 = 1.f;
-                if (sb + kWave < total) fetch(sb + kWave, nsxr, nsyr, ngj, ne2);
+            float s0x, s0y, g0, e0, s1x, s1y, g1, e1;
+            auto terms = [&](float sxr, float syr, float gj, float e2) {
 #pragma unroll
                 for (int k = 0; k < NT; k++) {
                     const float dx = tx[k] - sxr;
@@ -220,10 +217,14 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                     sx[k] = __builtin_fmaf(-dy, f, sx[k]);
                     sy[k] = __builtin_fmaf(dx, f, sy[k]);
                 }
-                sxr = nsxr;
-                syr = nsyr;
-                gj = ngj;
-                e2 = ne2;
+            };
+            if (total > 0) fetch(0, s0x, s0y, g0, e0);
+            for (int sb = 0; sb < total; sb += 2 * kWave) {
+                fetch(sb + kWave, s1x, s1y, g1, e1);
+                terms(s0x, s0y, g0, e0);
+                if (sb + kWave >= total) break;
+                fetch(sb + 2 * kWave, s0x, s0y, g0, e0);
+                terms(s1x, s1y, g1, e1);
             }
         }
     }
