@@ -685,13 +685,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             }
         }
     };
-    // tiles of 16 consecutive far-list entries; the next tile's records load meanwhile
-    FarNode nd = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
-    for (int64_t f0 = fb; f0 < fe; f0 += 16) {
-        const int64_t en = f0 + 16 + n;
-        const FarNode nxt = nodes[(en < fe) ? T.far_list[en] : 0];
-        tile(nd, f0 + n < fe);
-        nd = nxt;
+    // tiles of 16 consecutive far-list entries; the next tile's records load
+    // meanwhile, into two record sets used in turn
+    FarNode nd0 = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
+    for (int64_t f0 = fb; f0 < fe; f0 += 32) {
+        const int64_t e1 = f0 + 16 + n;
+        const FarNode nd1 = nodes[(e1 < fe) ? T.far_list[e1] : 0];
+        tile(nd0, f0 + n < fe);
+        if (f0 + 16 >= fe) break;
+        const int64_t e2 = f0 + 32 + n;
+        nd0 = nodes[(e2 < fe) ? T.far_list[e2] : 0];
+        tile(nd1, e1 < fe);
     }
     // sums over the 16 columns (lanes 0..15), FP64, fixed order
     for (int q = 0; q < 6; q++) {
