@@ -27,10 +27,17 @@ node) term, against the dense bf16 peak 2.5 PFLOP/s) -- measured with HIP events
 the library's stream, in an untimed pass with the step's kernels run one at a
 time (the timed step overlaps the near field, the far field and the
 diffusive on separate streams; phases_ms then overlap too).
-cpu_baseline: the FP64 oracle (the reference's algorithm, plain path at far
-criterion 8) on this host: full tree build and epsilon (serial, as the
-reference), convective and diffusive on a subset of leaves spread over the
-host cores, scaled to the full step.
+cpu_baseline: the FP64 oracle (the reference's algorithm and OpenMP structure,
+plain path) on this host's cores: at far criterion 8 the whole step (tree and
+epsilon serial as in the reference, convective and diffusive over the host
+threads); at the plain path's matched criterion the full tree build, and
+epsilon, convective and diffusive on a contiguous block of leaves, scaled by
+the vortices of the block.  Threads: the CPUs this process may run on,
+limited by OMP_NUM_THREADS when that is set (the host's share for one GPU).
+roofline_other also carries the tree build and flow-move against the HBM
+bandwidth (algorithmic bytes: 72 B per vortex and tree level -- an 8-byte key
+and two 32-byte records moved per level; 96 B per vortex for flow-move -- six
+FP64 fields read and written once).
 """
 import argparse
 import json
@@ -48,8 +55,13 @@ BF16_PEAK_TFLOPS = 2500.0
 FLOPS_PER_NEAR_PAIR = 11
 FLOPS_PER_NET_PASS = 2 * (12 * 64 + 64 * 64 + 64 * 2)
 TARGET_ERR = 1e-5
+HBM_PEAK_GBS = 8000.0
+TREE_BYTES_PER_VORTEX_LEVEL = 8 + 2 * 32
+FLOWMOVE_BYTES_PER_VORTEX = 6 * 8 * 2
 TRAFFIC_FILES = {"k_near": os.path.join(ROOT, "profiles", "near_kernel_traffic_wake1M.txt"),
-                 "k_far_learned": os.path.join(ROOT, "profiles", "far_learned_traffic_wake1M.txt")}
+                 "k_far_learned": os.path.join(ROOT, "profiles", "far_learned_traffic_wake1M.txt"),
+                 "tree": os.path.join(ROOT, "profiles", "tree_traffic_wake1M.txt"),
+                 "flowmove": os.path.join(ROOT, "profiles", "flowmove_traffic_wake1M.txt")}
 
 
 def parse():
@@ -68,7 +80,8 @@ def parse():
                     help="gradient of the learned correction: the network's own forward-mode derivative "
                          "(default) or the geometric terms only, with the network output held fixed")
     ap.add_argument("--error-sample", type=int, default=4000)
-    ap.add_argument("--cpu-fraction", type=float, default=0.05)
+    ap.add_argument("--cpu-fraction", type=float, default=0.1,
+                    help="leaf block sampled by the CPU baseline at the plain path's matched criterion")
     ap.add_argument("--model", default=os.path.join(ROOT, "vvflow_amd", "farfield_mlp.npy"))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
@@ -108,37 +121,88 @@ def traffic_bytes(kernel):
     return None
 
 
-def cpu_baseline(objs, fc, fraction):
-    """FP64 oracle on the host: returns (particle-steps/s, threads, sample text)."""
+def host_cpu():
+    """(model, nproc, os.cpu_count(), threads used by the CPU baseline)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    threads = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return model, nproc, os.cpu_count() or 1, threads
+
+
+def _leaf_vortex_ranges(o):
+    """Vortex range [v0, v1) of every leaf of the oracle's tree, DFS order."""
+    nn = o.L.vvo_tree_num_nodes(o.h)
+    nl = o.L.vvo_tree_num_leaves(o.h)
+    geom = np.zeros((nn, 4))
+    ints = np.zeros((nn, 6), dtype=np.int32)
+    cm = np.zeros((nn, 6))
+    cme = np.zeros((nn, 2))
+    o.L.vvo_tree_nodes(o.h, geom, ints, cm, cme)
+    leaves = np.zeros(nl, dtype=np.int32)
+    o.L.vvo_tree_leaves(o.h, leaves)
+    return ints[leaves, 0], ints[leaves, 1]
+
+
+def cpu_baseline(objs, fc, fraction, threads):
+    """FP64 oracle on the host, the plain path at far criterion fc: returns
+    (particle-steps/s, seconds of CPU work, sample text).  fraction 1 runs
+    the whole step; below 1, epsilon, convective and diffusive run on a
+    contiguous block of that share of the leaves (middle of the DFS order)
+    and their time is scaled by the vortex count over the block's."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import Oracle
 
-    threads = min(16, os.cpu_count() or 1)
     n = len(objs)
     o = Oracle()
     o.set_params(re=600.0, dt=0.005, inf=(1.0, 0.0))
     o.set_vortices(objs)
     t0 = time.perf_counter()
     o.tree_build(fc)
-    t1 = time.perf_counter()
-    o.epsilon(True)
-    t2 = time.perf_counter()
-    nl = o.L.vvo_tree_num_leaves(o.h)
-    cnt = max(1, int(nl * fraction))   # a contiguous block of leaves in the middle of the DFS order
+    t_tree = time.perf_counter() - t0
+    if fraction >= 1:
+        t0 = time.perf_counter()
+        o.epsilon(True)
+        t_eps = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.convective(threads=threads)
+        o.diffusive(threads=threads)
+        t_cd = time.perf_counter() - t0
+        t_step = t_tree + t_eps + t_cd
+        sample = (f"whole step at far criterion {fc}: tree build {t_tree:.2f} s and epsilon {t_eps:.2f} s "
+                  f"(serial, as the reference), convective + diffusive {t_cd:.2f} s on {threads} threads")
+        return n / t_step, t_tree + t_eps + t_cd, sample
+    v0, v1 = _leaf_vortex_ranges(o)
+    nl = len(v0)
+    cnt = max(1, int(nl * fraction))
     lb = (nl - cnt) // 2
     le = lb + cnt
-    before = o.vortices()
-    t3 = time.perf_counter()
+    block = int(v1[le - 1] - v0[lb])
+    t0 = time.perf_counter()
+    o.epsilon_leaves(True, lb, le)
+    t_eps = time.perf_counter() - t0
+    t0 = time.perf_counter()
     o.convective(threads=threads, leaf_begin=lb, leaf_end=le)
     o.diffusive(threads=threads, leaf_begin=lb, leaf_end=le)
-    t4 = time.perf_counter()
-    after = o.vortices()
-    touched = max(int(np.sum(np.any(after[:, 3:5] != before[:, 3:5], axis=1))), 1)
-    t_step = (t1 - t0) + (t2 - t1) + (t4 - t3) * (n / touched)
-    sample = (f"full tree build ({t1 - t0:.2f} s) and epsilon ({t2 - t1:.2f} s), serial; convective + diffusive "
-              f"on {cnt} of {nl} leaves ({touched} targets, {t4 - t3:.2f} s on {threads} threads) scaled by "
-              f"{n / touched:.1f}")
-    return n / t_step, threads, sample
+    t_cd = time.perf_counter() - t0
+    scale = n / max(block, 1)
+    t_step = t_tree + (t_eps + t_cd) * scale
+    sample = (f"far criterion {fc}: full tree build {t_tree:.2f} s (serial); epsilon (serial, {t_eps:.2f} s) and "
+              f"convective + diffusive ({t_cd:.2f} s on {threads} threads) on {cnt} of {nl} leaves "
+              f"({block} vortices), scaled by {scale:.2f}")
+    return n / t_step, t_tree + t_eps + t_cd, sample
 
 
 class Bench:
@@ -189,7 +253,21 @@ class Bench:
         err_max = float(np.max(np.linalg.norm(v[tg] - ref, axis=1) / np.maximum(nr, 1e-3 * np.sqrt(np.mean(nr ** 2)))))
         work = c.work_counts()
         return dict(rel_l2=err, max_rel=err_max, merged=int(merged), near_pairs=int(work[0]),
-                    far_pairs=int(work[2]), net_passes=int(work[3]))
+                    far_pairs=int(work[2]), net_passes=int(work[3]), levels=int(work[4]))
+
+    def flowmove(self, fc, learned, reps):
+        """MFlowmove::move_and_clean after a whole step (velocities set):
+        mean device time (HIP events) over reps and the vortex count."""
+        c = self.ctx
+        ms = 0.0
+        n0 = 0
+        for _ in range(reps):
+            self.step(fc, learned)
+            c.tree_destroy()
+            n0 = c.n
+            c.move_and_clean(True)
+            ms += float(c.phase_times()[4])
+        return ms / reps, n0
 
 
 ROOFLINE_STEPS = 3
@@ -198,8 +276,8 @@ ALONE_NOTE = ("kernel_ms: HIP events around the kernel, mean of %d steps run wit
               "the GPU with the step's concurrent kernels" % ROOFLINE_STEPS)
 
 
-def roofline(name, achieved, peak, traffic, extra):
-    out = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+def roofline(name, bound, achieved, peak, unit, traffic, extra):
+    out = {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
            "frac": achieved / peak, "traffic": traffic}
     out.update(extra)
     return out
@@ -403,7 +481,8 @@ def main():
     acc = B.error(fc, learned, a.error_sample)
     near_ms, far_ms = float(ph_alone[6]), float(ph_alone[7]) if learned else 0.0
     near_tf = acc["near_pairs"] * FLOPS_PER_NEAR_PAIR / (near_ms * 1e-3) / 1e12
-    rl_near = roofline("k_near (convective near field, FP32 VALU)", near_tf, FP32_PEAK_TFLOPS, traffic_bytes("k_near"),
+    rl_near = roofline("k_near (convective near field, FP32 VALU)", "fp32_valu", near_tf, FP32_PEAK_TFLOPS, "TFLOP/s",
+                       traffic_bytes("k_near"),
                        {"compute_unit": "FP32 VALU (no matrix instructions); peak is the published FP32 figure",
                         "algorithmic_flops": acc["near_pairs"] * FLOPS_PER_NEAR_PAIR, "near_pairs": acc["near_pairs"],
                         "kernel_ms": near_ms, "kernel_ms_in_timed_step": float(ph[6]),
@@ -411,38 +490,84 @@ def main():
     rls = [rl_near]
     if learned:
         far_tf = acc["net_passes"] * FLOPS_PER_NET_PASS / (far_ms * 1e-3) / 1e12
-        rls.append(roofline("k_far_learned (learned far-field network, bf16 MFMA)", far_tf, BF16_PEAK_TFLOPS,
-                            traffic_bytes("k_far_learned"),
+        rls.append(roofline("k_far_learned (learned far-field network, bf16 MFMA)", "mfma", far_tf, BF16_PEAK_TFLOPS,
+                            "TFLOP/s", traffic_bytes("k_far_learned"),
                             {"compute_unit": "bf16 matrix cores, dense peak",
                              "algorithmic_flops": acc["net_passes"] * FLOPS_PER_NET_PASS,
                              "network_passes": acc["net_passes"], "far_pairs": acc["far_pairs"],
                              "kernel_ms": far_ms, "kernel_ms_in_timed_step": float(ph[7]),
                              "timing": ALONE_NOTE}))
     rls.sort(key=lambda r: -r["kernel_ms"])
+    # tree build and flow-move against the HBM bandwidth
+    tree_ms = float(ph_alone[0])
+    tree_bytes = n * acc["levels"] * TREE_BYTES_PER_VORTEX_LEVEL
+    hbm = [roofline("tree build (level kernels, monopoles, interaction lists)", "hbm",
+                    tree_bytes / (tree_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", traffic_bytes("tree"),
+                    {"algorithmic_bytes": tree_bytes, "levels": acc["levels"], "phase_ms": tree_ms,
+                     "timing": "HIP events around the tree build, steps run with kernels one at a time"})]
+    fm_ms, fm_n = B.flowmove(fc, learned, ROOFLINE_STEPS)
+    fm_bytes = fm_n * FLOWMOVE_BYTES_PER_VORTEX
+    hbm.append(roofline("flow-move (advect, remove, compact)", "hbm", fm_bytes / (fm_ms * 1e-3) / 1e9,
+                        HBM_PEAK_GBS, "GB/s", traffic_bytes("flowmove"),
+                        {"algorithmic_bytes": fm_bytes, "vortices": fm_n, "phase_ms": fm_ms,
+                         "timing": "HIP events around MFlowmove::move_and_clean after a step, mean of %d" %
+                                   ROOFLINE_STEPS}))
 
     plain = None
+    cpu_fc = [8]
     if learned and not a.no_plain:
         plain = {"reference_fc8": None, "at_target": None, "sweep": []}
         e8 = B.error(8, False, a.error_sample)
         el8, _ = B.time(8, False, a.plain_steps, 1)
         plain["reference_fc8"] = {"far_criteria": 8, "value": n * a.plain_steps / el8, "rel_l2": e8["rel_l2"],
                                   "ms_per_step": el8 / a.plain_steps * 1e3}
+        errs = {}
+
+        def err_at(f):
+            if f not in errs:
+                e = B.error(f, False, a.error_sample)
+                errs[f] = e
+                plain["sweep"].append({"far_criteria": f, "rel_l2": e["rel_l2"], "near_pairs": e["near_pairs"]})
+                print(f"plain fc={f}: rel L2 {e['rel_l2']:.4e}", file=sys.stderr, flush=True)
+            return errs[f]["rel_l2"]
+
+        lo, hi = None, None
         for f in [int(x) for x in a.plain_fcs.split(",")]:
-            e = B.error(f, False, a.error_sample)
-            rec = {"far_criteria": f, "rel_l2": e["rel_l2"], "near_pairs": e["near_pairs"]}
-            plain["sweep"].append(rec)
-            print(f"plain fc={f}: rel L2 {e['rel_l2']:.3e}", file=sys.stderr, flush=True)
-            if e["rel_l2"] <= TARGET_ERR:
-                elp, _ = B.time(f, False, a.plain_steps, 1)
-                rec["value"] = n * a.plain_steps / elp
-                rec["ms_per_step"] = elp / a.plain_steps * 1e3
-                plain["at_target"] = rec
+            if err_at(f) <= TARGET_ERR:
+                hi = f
                 break
+            lo = f
+        if hi is not None:
+            # smallest integer criterion in (lo, hi] that reaches the target
+            lo = lo if lo is not None else 0
+            while hi - lo > 1:
+                mid = (lo + hi) // 2
+                if err_at(mid) <= TARGET_ERR:
+                    hi = mid
+                else:
+                    lo = mid
+            elp, _ = B.time(hi, False, a.plain_steps, 1)
+            rec = {"far_criteria": hi, "rel_l2": errs[hi]["rel_l2"], "near_pairs": errs[hi]["near_pairs"],
+                   "value": n * a.plain_steps / elp, "ms_per_step": elp / a.plain_steps * 1e3,
+                   "search": "smallest integer far criterion with rel L2 <= 1e-5: sweep %s, then bisection "
+                             "(error taken as non-increasing in the criterion between the bracketing points)" %
+                             a.plain_fcs}
+            plain["at_target"] = rec
+            cpu_fc.append(hi)
+        plain["sweep"].sort(key=lambda r: r["far_criteria"])
 
     cpu = None
     if not a.no_cpu:
-        cv, cores, sample = cpu_baseline(objs, 8, a.cpu_fraction)
-        cpu = {"value": cv, "unit": "particle-steps/s", "cores": cores, "kind": "port", "sample": sample}
+        model, nproc, ncpu, threads = host_cpu()
+        cv, cs, sample = cpu_baseline(objs, 8, 1.0, threads)
+        cpu = {"value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port", "sample": sample,
+               "cpu_seconds": cs, "far_criteria": 8, "cpu_model": model, "nproc": nproc, "os_cpu_count": ncpu,
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+        if len(cpu_fc) > 1:
+            cv2, cs2, sample2 = cpu_baseline(objs, cpu_fc[1], a.cpu_fraction, threads)
+            cpu["at_plain_matched_criterion"] = {"far_criteria": cpu_fc[1], "value": cv2, "cpu_seconds": cs2,
+                                                 "sample": sample2}
+        print(f"cpu baseline: {cv:.4g} particle-steps/s at fc 8 on {threads} threads", file=sys.stderr, flush=True)
 
     out = {
         "metric": "particle-steps/s at N=1M (conv+eps+diff), learned vs plain at <=1e-5 rel err",
@@ -469,7 +594,7 @@ def main():
         "accuracy": {"rel_l2_vs_fp64_direct_sum": acc["rel_l2"], "max_rel": acc["max_rel"],
                      "sample_targets": min(a.error_sample, n), "merged": acc["merged"]},
         "roofline": rls[0],
-        "roofline_other": rls[1:],
+        "roofline_other": rls[1:] + hbm,
         "plain": plain,
         "speedup_vs_plain_at_target": (value / plain["at_target"]["value"]
                                        if plain and plain["at_target"] and acc["rel_l2"] <= TARGET_ERR else None),

@@ -233,8 +233,9 @@ int vvhd_phase_times(vvhd_ctx* ctx, float* ms8);
 /* Algorithmic work counted on the interaction lists of the current tree:
  * {near convective pairs, far (leaf, node) terms, (vortex, far node) pairs,
  *  learned far-field network passes (value and two tangents per far
- *  (leaf, node) term), -1, leaves, nodes}; far terms and network passes
- *  count leaves with a target of non-zero circulation. */
+ *  (leaf, node) term), tree levels (depths the build split at), leaves,
+ *  nodes}; far terms and network passes count leaves with a target of
+ *  non-zero circulation. */
 int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7);
 /* Concurrency of the step's kernels (default on): the learned near field
  * runs beside its far field, and a diffusive call made right after a

@@ -1191,19 +1191,11 @@ void vvo_tree_leaf_segs(vvo_ctx* c, int64_t* off, int32_t* segs) {
 int32_t vvo_tree_find(vvo_ctx* c, double x, double y) { return find_node(c, V2{x, y}); }
 
 // MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:11-63)
-int vvo_epsilon(vvo_ctx* c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merges, int32_t max_merges,
-                int32_t* merged) {
-    if (!c->built) {
-        c->err = "epsilon: tree is not built";
-        return 1;
-    }
-    EpsCtx e{c, 0, merges, max_merges};
-    const int64_t N = (int64_t)c->vort.size();
-    if (nn1)
-        for (int64_t i = 0; i < N; i++) nn1[i] = -1;
-    if (nn2)
-        for (int64_t i = 0; i < N; i++) nn2[i] = -1;
-    for (size_t l = 0; l < c->leaves.size(); l++) {
+// Per-leaf body of MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:17-62)
+// over the leaves [leaf_begin, leaf_end) in DFS order.
+static void epsilon_leaves(vvo_ctx* c, EpsCtx& e, int merge, size_t leaf_begin, size_t leaf_end, int32_t* nn1,
+                           int32_t* nn2) {
+    for (size_t l = leaf_begin; l < leaf_end; l++) {
         const int leaf = c->leaves[l];
         const Node& L = c->nodes[leaf];
         V2 centre{L.x, L.y};
@@ -1223,6 +1215,35 @@ int vvo_epsilon(vvo_ctx* c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merg
             if (nn2) nn2[i] = e.last_n2;
         }
     }
+}
+
+int vvo_epsilon(vvo_ctx* c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merges, int32_t max_merges,
+                int32_t* merged) {
+    if (!c->built) {
+        c->err = "epsilon: tree is not built";
+        return 1;
+    }
+    EpsCtx e{c, 0, merges, max_merges};
+    const int64_t N = (int64_t)c->vort.size();
+    if (nn1)
+        for (int64_t i = 0; i < N; i++) nn1[i] = -1;
+    if (nn2)
+        for (int64_t i = 0; i < N; i++) nn2[i] = -1;
+    epsilon_leaves(c, e, merge, 0, c->leaves.size(), nn1, nn2);
+    if (merged) *merged = e.merged;
+    return 0;
+}
+
+int vvo_epsilon_leaves(vvo_ctx* c, int merge, int64_t leaf_begin, int64_t leaf_end, int32_t* merged) {
+    if (!c->built) {
+        c->err = "epsilon: tree is not built";
+        return 1;
+    }
+    const int64_t nl = (int64_t)c->leaves.size();
+    if (leaf_end < 0 || leaf_end > nl) leaf_end = nl;
+    if (leaf_begin < 0) leaf_begin = 0;
+    EpsCtx e{c, 0, nullptr, 0};
+    if (leaf_begin < leaf_end) epsilon_leaves(c, e, merge, (size_t)leaf_begin, (size_t)leaf_end, nullptr, nullptr);
     if (merged) *merged = e.merged;
     return 0;
 }

@@ -99,6 +99,10 @@ int32_t vvo_tree_find(vvo_ctx*, double x, double y);
  * merges (optional, 2*max_merges) receives (target, partner) pairs in order. */
 int     vvo_epsilon(vvo_ctx*, int merge, int32_t* nn1, int32_t* nn2,
                     int32_t* merges, int32_t max_merges, int32_t* merged);
+/* The same per-leaf pass over the leaves [leaf_begin, leaf_end) only (DFS
+ * order; leaf_end < 0 = all), merging as the full pass does there.  For
+ * timing a sample of the step on the host. */
+int     vvo_epsilon_leaves(vvo_ctx*, int merge, int64_t leaf_begin, int64_t leaf_end, int32_t* merged);
 
 /* ---- Convective (MConvectiveFast.cpp:36-151) -------------------------- */
 /* threads: 0 = the reference's own structure (serial leaf loop);

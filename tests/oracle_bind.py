@@ -85,6 +85,7 @@ class Oracle:
             _sig(L, "vvo_tree_find", I32, [P, F64, F64])
             _sig(L, "vvo_epsilon", C.c_int, [P, C.c_int, C.c_void_p, C.c_void_p,
                                              C.c_void_p, I32, C.POINTER(I32)])
+            _sig(L, "vvo_epsilon_leaves", C.c_int, [P, C.c_int, I64, I64, C.c_void_p])
             _sig(L, "vvo_convective", C.c_int, [P, C.c_int, I64, I64])
             _sig(L, "vvo_velocity", C.c_int, [P, dp, I64, dp])
             _sig(L, "vvo_diffusive", C.c_int, [P, C.c_int, I64, I64])
@@ -214,6 +215,11 @@ class Oracle:
                                        mg.ctypes.data, max_merges, C.byref(merged)))
         m = merged.value
         return nn1, nn2, mg[:2 * min(m, max_merges)].reshape(-1, 2), m
+
+    def epsilon_leaves(self, merge=True, leaf_begin=0, leaf_end=-1):
+        merged = I32(0)
+        self._check(self.L.vvo_epsilon_leaves(self.h, int(merge), leaf_begin, leaf_end, C.byref(merged)))
+        return merged.value
 
     def convective(self, threads=0, leaf_begin=0, leaf_end=-1):
         self._check(self.L.vvo_convective(self.h, threads, leaf_begin, leaf_end))

@@ -296,3 +296,20 @@ def test_dead_terms_of_slip_and_vanishing_segments():
     assert body["fdt_dead"][2] == pytest.approx(imp[2], rel=1e-10)
     assert body["g_dead"] == pytest.approx(probe[:, 2].sum(), rel=1e-10)
     assert np.allclose(o.segments()["gsum"], np.where(dead, -segs["g"], 0.0), rtol=0, atol=0)
+
+
+# ------------------------------------------------------ cylinder run fixture
+
+def test_cylinder_run_fixture_reproduces():
+    """tests/golden/cylinder400_oracle.npz (the oracle's 400-step cylinder
+    run, the yardstick of the GPU's free-running run) follows from the
+    oracle's code: its first steps are reproduced exactly."""
+    import os
+    import sys
+    from cases import GOLDEN
+    sys.path.insert(0, GOLDEN)
+    from make_cylinder_run import run
+    ref = np.load(os.path.join(GOLDEN, "cylinder400_oracle.npz"), allow_pickle=False)
+    got = run(12)
+    for k in ("n", "merged", "gsum", "gabs", "force"):
+        assert np.array_equal(got[k], ref[k][:12]), k

@@ -202,7 +202,7 @@ int vvhd_work_counts(vvhd_ctx* ctx, int64_t* out7) {
     out7[1] = acc[1];
     out7[2] = acc[2];
     out7[3] = (c.far_net_grad ? 3 : 1) * acc[1];
-    out7[4] = -1;
+    out7[4] = (int64_t)c.t.levels.size();
     out7[5] = c.t.nleaves_h;
     out7[6] = c.t.nnodes;
     API_END(ctx)
