@@ -149,6 +149,20 @@ int vvhd_epsilon(vvhd_ctx* ctx, int merge, int32_t* nn1, int32_t* nn2, int32_t* 
  * when kahan != 0, in leaf-local coordinates) + far-field Taylor terms
  * (FP64).  inf_x/inf_y: Space::inf_speed() evaluated by the caller. */
 int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan);
+/* Allocation-free variant of vvhd_convective (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * kernels and results (bit for bit) as vvhd_convective, but its scratch (the
+ * per-leaf Taylor terms and the per-vortex near-field sums) lives in one
+ * fixed device array of VVHD_NOALLOC_SCRATCH_BYTES per device instead of
+ * buffers grown on demand, and it calls no hipMalloc/hipFree.  Needs
+ * 32 (leaves + 1) + 16 (vortices + 1) bytes, each part rounded up to 256
+ * (about 20 MB for a 1M-vortex wake); a larger input returns VVHD_EINVAL.  All
+ * contexts on a device share that array: calls that overlap on one device
+ * race.  The tree must be built (and the vortex list uploaded) beforehand, as
+ * for vvhd_convective. */
+#define VVHD_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int vvhd_convective_noalloc(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan);
 /* MConvectiveFast::velocity(TVec) (:20-34) for n points (host xy, host out). */
 int vvhd_velocity_points(vvhd_ctx* ctx, const double* xy, int64_t n,
                          double inf_x, double inf_y, double* v_out);

@@ -31,6 +31,7 @@ LIB_PATH = os.path.join(_HERE, "libvvhd_gpu.so")
 
 VVHD_OK, VVHD_EINVAL, VVHD_ESTATE, VVHD_EHIP, VVHD_ENOMEM, VVHD_ESINGULAR, VVHD_EUNSUPPORTED = range(7)
 FAR_TRAINABLE, FAR_PARAMS = 5122, 5148   # VVHD_FAR_TRAINABLE, VVHD_FAR_PARAMS
+NOALLOC_SCRATCH_BYTES = 64 << 20          # VVHD_NOALLOC_SCRATCH_BYTES
 
 SEG_DTYPE = np.dtype([(n, np.float64) for n in
                       ("x", "y", "g", "inv_eps", "cx", "cy", "dlx", "dly",
@@ -71,6 +72,7 @@ ABI = {
     "vvhd_tree_nodes": (_I, [_P, _VP, _VP, _VP, _VP]),
     "vvhd_epsilon": (_I, [_P, _I, _VP, _VP, _VP]),
     "vvhd_convective": (_I, [_P, _D, _D, _I]),
+    "vvhd_convective_noalloc": (_I, [_P, _D, _D, _I]),
     "vvhd_velocity_points": (_I, [_P, _VP, _I64, _D, _D, _VP]),
     "vvhd_diffusive": (_I, [_P]),
     "vvhd_move_and_clean": (_I, [_P, _I, _VP]),
@@ -304,6 +306,11 @@ class Context:
 
     def convective(self, inf=(0.0, 0.0), kahan=True):
         self._check(self.L.vvhd_convective(self.h, inf[0], inf[1], int(kahan)))
+
+    def convective_noalloc(self, inf=(0.0, 0.0), kahan=True):
+        """vvhd_convective with its scratch in the library's fixed device
+        array (no device allocation)."""
+        self._check(self.L.vvhd_convective_noalloc(self.h, inf[0], inf[1], int(kahan)))
 
     def velocity(self, pts, inf=(0.0, 0.0)):
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)

@@ -325,7 +325,14 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
 void tree_lists(Ctx& c);
 void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 void convective(Ctx& c, double inf_x, double inf_y, int kahan);
-void convective_near(Ctx& c, int kahan, hipStream_t s);
+void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out);
+// Allocation-free convective (vvhd_convective_noalloc): the scratch lives in a
+// fixed device array of kNoallocScratchBytes; noalloc_init looks its address
+// up once per device (the device must be current).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+void noalloc_init(int device);
+size_t convective_noalloc_bytes(int64_t n, int64_t nleaf);
+void convective_noalloc(Ctx& c, double inf_x, double inf_y, int kahan);
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out);
 void diffusive(Ctx& c, bool overlap);
 void bodies_prepare(Ctx& c);

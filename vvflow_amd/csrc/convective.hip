@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -432,35 +434,97 @@ __global__ __launch_bounds__(kDsBlock) void k_direct_sum(const double* x, const 
 
 }  // namespace
 
-// Near-field sums of every vortex into c.near_v (shared by the plain and the
-// learned path).
-void convective_near(Ctx& c, int kahan, hipStream_t s) {
+// Near-field sums of every vortex into near_out (n + 1 entries; the plain and
+// the learned path share it).
+void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out) {
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf == 0) return;
     const int nl = T.nleaf;
-    c.near_v.grow(c.n + 1);
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
     VVHD_HIP(hipEventRecord(c.ev[12], s));
     if (kahan)
-        hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, c.near_v.p);
+        hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, near_out);
     else
-        hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, c.near_v.p);
+        hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, near_out);
     VVHD_HIP(hipEventRecord(c.ev[13], s));
     c.phase_recorded[6] = true;
 }
 
-void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
+// process_all_lists with the per-leaf Taylor terms in taylor (nleaf + 1) and
+// the per-vortex near sums in near_out (n + 1): every entry is written before
+// it is read, so neither needs clearing.
+static void convective_into(Ctx& c, double inf_x, double inf_y, int kahan, double4* taylor, double2* near_out) {
     const TreeDev T = tree_dev(c.t);
-    if (T.nleaf == 0) return;
     const int nl = T.nleaf;
-    c.taylor.grow(nl + 1);
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, c.taylor.p);
-    convective_near(c, kahan, c.stream);
+    hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, taylor);
+    convective_near(c, kahan, c.stream, near_out);
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
-                       c.taylor.p, c.x.p, c.y.p, c.g.p, c.near_v.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
+                       taylor, c.x.p, c.y.p, c.g.p, near_out, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     VVHD_HIP(hipGetLastError());
+}
+
+void convective(Ctx& c, double inf_x, double inf_y, int kahan) {
+    if (c.t.nleaves_h == 0) return;
+    c.taylor.grow(c.t.nleaves_h + 1);
+    c.near_v.grow(c.n + 1);
+    convective_into(c, inf_x, inf_y, kahan, c.taylor.p, c.near_v.p);
+}
+
+// ---------------------------------------------- allocation-free convective --
+//
+// Scratch of convective_noalloc: the per-leaf Taylor terms followed by the
+// per-vortex near-field sums, in a device array of fixed size instead of
+// buffers grown on demand.  In host code `scratch` names a host-side stand-in,
+// not device memory: its device address is looked up once per device
+// (noalloc_init, called by vvhd_create with the device current) and only that
+// address is handed to kernels.  Every context on a device shares the array,
+// so calls that overlap on one device race; each call writes every entry it
+// reads, so nothing carries over from the previous call.
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+void noalloc_init(int device) {
+    VVHD_CHECK(device >= 0 && device < kMaxDevices, VVHD_EUNSUPPORTED,
+               "device index " + std::to_string(device) + " beyond the " + std::to_string(kMaxDevices) +
+                   " devices the allocation-free scratch table holds");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    VVHD_CHECK(e == hipSuccess && p, VVHD_EHIP,
+               std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+}
+
+size_t convective_noalloc_bytes(int64_t n, int64_t nleaf) {
+    return align256((size_t)(nleaf + 1) * sizeof(double4)) + align256((size_t)(n + 1) * sizeof(double2));
+}
+
+void convective_noalloc(Ctx& c, double inf_x, double inf_y, int kahan) {
+    if (c.t.nleaves_h == 0) return;
+    const size_t need = convective_noalloc_bytes(c.n, c.t.nleaves_h);
+    VVHD_CHECK(need <= kNoallocScratchBytes, VVHD_EINVAL,
+               "convective_noalloc: " + std::to_string(c.n) + " vortices in " + std::to_string(c.t.nleaves_h) +
+                   " leaves need " + std::to_string(need) + " bytes of scratch, more than its " +
+                   std::to_string(kNoallocScratchBytes) + " bytes");
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = (unsigned char*)g_scratch_ptr[c.device];
+    }
+    VVHD_CHECK(base, VVHD_ESTATE, "convective_noalloc: scratch of this device not set up");
+    double4* taylor = (double4*)base;
+    double2* near_out = (double2*)(base + align256((size_t)(c.t.nleaves_h + 1) * sizeof(double4)));
+    convective_into(c, inf_x, inf_y, kahan, taylor, near_out);
 }
 
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out) {

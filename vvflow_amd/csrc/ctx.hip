@@ -174,6 +174,14 @@ int vvhd_create(int device, vvhd_ctx** out) {
         delete ctx;
         return VVHD_EHIP;
     }
+    // device address of the allocation-free entry point's scratch array,
+    // looked up once per device while the device is current
+    try {
+        noalloc_init(device);
+    } catch (const Error& e) {
+        vvhd_destroy(ctx);
+        return e.code;
+    }
     *out = ctx;
     return VVHD_OK;
 }
@@ -446,6 +454,21 @@ int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan) {
     c.conv_call = c.api_calls;
     VVHD_HIP(hipEventRecord(c.ev[4], c.stream));
     convective(c, inf_x, inf_y, kahan);
+    VVHD_HIP(hipEventRecord(c.ev[5], c.stream));
+    c.phase_recorded[2] = true;
+    API_END(ctx)
+}
+
+int vvhd_convective_noalloc(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_CHECK(c.t.built, VVHD_ESTATE, "MConvectiveFast::process_all_lists(): tree is not built");
+    VVHD_HIP(hipSetDevice(c.device));
+    VVHD_HIP(hipEventRecord(c.ev_pre_conv, c.stream));
+    c.conv_call = c.api_calls;
+    VVHD_HIP(hipEventRecord(c.ev[4], c.stream));
+    convective_noalloc(c, inf_x, inf_y, kahan);
     VVHD_HIP(hipEventRecord(c.ev[5], c.stream));
     c.phase_recorded[2] = true;
     API_END(ctx)

@@ -1354,7 +1354,8 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
     // the near field runs on its own stream beside the far field
     const hipStream_t ns = c.overlap ? c.stream3 : c.stream;
     if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev_pre_conv, 0));
-    convective_near(c, kahan, ns);
+    c.near_v.grow(c.n + 1);
+    convective_near(c, kahan, ns, c.near_v.p);
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf > 0) {
         MomView M = moments(c);
