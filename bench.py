@@ -338,7 +338,7 @@ def precision_sweep(a):
     ctx.restore()
     ctx.set_params(re=600.0, dt=0.005, time=0.0)
     ctx.upload_bodies(np.ascontiguousarray(segs, dtype=V.SEG_DTYPE), body)
-    dl = ctx.download_bodies()[1][0]["slen"] / len(segs)
+    dl = ctx.download_bodies()[1][0]["slen"] / (len(segs) - 1)   # Space::average_segment_length
     ctx.tree_build(8, 5 * dl, 100 * dl)
     ctx.calc_circulation((1.0, 0.0))
     first_ms = float(ctx.phase_times()[5])   # includes the solver library's first-use set-up

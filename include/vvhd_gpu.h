@@ -133,8 +133,10 @@ int vvhd_tree_lists(vvhd_ctx* ctx, int far, int64_t* count, int64_t* offsets,
 /* stree::findNode for n points: leaf index (DFS order) or -1. */
 int vvhd_tree_find(vvhd_ctx* ctx, const double* xy, int64_t n, int32_t* leaf_out);
 /* Every node of the built tree in device numbering (inspection):
- * geom {x, y, h, w}, ints {v0, v1, first child or -1, depth}, path code. */
-int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints, uint64_t* path);
+ * geom {x, y, h, w}, ints {v0, v1, first child or -1, depth}, path code, and
+ * the per-sign monopoles cm = {CMp.x, CMp.y, CMp.g, CMm.x, CMm.y, CMm.g}
+ * (snode::CMp/CMm, TSortedTree.hpp:30).  Any pointer may be NULL. */
+int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints, uint64_t* path, double* cm);
 
 /* ---- epsilon: MEpsilonFast::CalcEpsilonFast (MEpsilonFast.cpp:11-63) --
  * Sets _1_eps of every vortex with g != 0 from its two nearest neighbours
@@ -234,6 +236,49 @@ int vvhd_slae_system(vvhd_ctx* ctx, int32_t* n, double* A, double* rhs);
 int vvhd_calc_forces(vvhd_ctx* ctx);
 int vvhd_zero_forces(vvhd_ctx* ctx);
 int vvhd_integrals(vvhd_ctx* ctx, double* out5);
+
+/* ---- point queries and grid fields -----------------------------------
+ * MEpsilonFast::eps2h / h2 (MEpsilonFast.cpp:66-107, MEpsilonFast.hpp:15-18)
+ * for n points (host xy; out n doubles) over the near list of the leaf given
+ * by leaf[k] (DFS index) or, with leaf == NULL, of the leaf stree::findNode
+ * gives; with xy == NULL and n == vortex count, for every vortex (tree order)
+ * with its own leaf.  eps2h: the second smallest non-zero squared distance to
+ * the vortices of the near leaves (the smallest when only one, the lowest
+ * double when none); h2: the smallest squared distance to the body segments
+ * of the near leaves (+inf when none). */
+int vvhd_eps2h(vvhd_ctx* ctx, const double* xy, const int32_t* leaf, int64_t n, double* out);
+int vvhd_h2(vvhd_ctx* ctx, const double* xy, const int32_t* leaf, int64_t n, double* out);
+
+/* A field on the grid p(xi, yj) = (xmin + dxdy xi, ymin + dxdy yj), xi < xres,
+ * yj < yres, written row by row (yj major) as floats, as XField::map
+ * (XField.hpp:56-63). */
+typedef struct vvhd_grid {
+    double xmin, ymin, dxdy;
+    int32_t xres, yres;
+} vvhd_grid;
+enum {
+    VVHD_FIELD_VX = 0,             /* XVelocity mode 'x' (XVelocity.cpp:24-73)          */
+    VVHD_FIELD_VY = 1,             /* XVelocity mode 'y'                                */
+    VVHD_FIELD_VORTICITY = 2,      /* XVorticity (XVorticity.cpp:26-94)                 */
+    VVHD_FIELD_STREAMFUNCTION = 3, /* XStreamfunction (XStreamfunction.cpp:47-118, 160-205) */
+    VVHD_FIELD_PRESSURE = 4        /* XPressure (XPressure.cpp:32-146)                  */
+};
+/* Evaluates one field over the built tree of this context.  The reference's
+ * evaluators prepare a copy of the Space first; the caller does the same here
+ * in a context of its own (the evaluator classes of vvhd_modules.hpp do):
+ *  - VX / VY: tree (8, 20 dl, 0.1), epsilon without merging; map = velocity(p)
+ *    minus (ref_x, ref_y); inf = U.
+ *  - VORTICITY: vortex_shed, tree (8, 20 dl); per-vortex weights from eps2h
+ *    and eps_mult are computed here.
+ *  - STREAMFUNCTION: tree (8, 10 dl, 20 dl) (0 and +inf without bodies);
+ *    weights from eps2h here; inf - ref enters as p . rotl(U - U_ref).
+ *  - PRESSURE: vortex_shed, tree (8, 20 dl, 0.1), epsilon without merging,
+ *    convective and diffusive (the vortex velocities); with_ref = 0 is
+ *    ref_frame 's'.
+ * dl: Space::average_segment_length().  The context's source list is empty
+ * (no sources), so the streamfunction has no gaps. */
+int vvhd_field(vvhd_ctx* ctx, int kind, const vvhd_grid* grid, double eps_mult, double dl, double inf_x,
+               double inf_y, int with_ref, double ref_x, double ref_y, float* map);
 
 /* ---- measurement (no counterpart in the reference) -------------------
  * FP64 direct sum of the induced Lamb-kernel velocity over all vortices for

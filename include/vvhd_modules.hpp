@@ -11,6 +11,10 @@
  *   MFlowmove             libvvhd/headers/MFlowmove.hpp:5-19
  *   Space (the members the hot path reads and writes), TObj, TAtt, TBody,
  *   TVec                  libvvhd/headers/TSpace.hpp, TObj.hpp, TBody.hpp
+ *   XField, XVelocity, XVorticity, XStreamfunction, XPressure
+ *                         libvvhd/headers/XField.hpp:18-63, XVelocity.hpp,
+ *                         XVorticity.hpp, XStreamfunction.hpp, XPressure.hpp
+ *   sensors               utils/vvflow/sensors.cpp:8-70
  *
  * The vortex list and the bodies live on the device between calls; the host
  * containers of Space are a mirror that is synchronised explicitly:
@@ -32,7 +36,9 @@
 #include <cstdio>
 #include <limits>
 #include <memory>
+#include <ostream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "vvhd_gpu.h"
@@ -44,6 +50,7 @@ struct TVec {
     TVec operator+(TVec o) const { return {x + o.x, y + o.y}; }
     TVec operator-(TVec o) const { return {x - o.x, y - o.y}; }
     TVec operator*(double c) const { return {x * c, y * c}; }
+    double operator*(TVec o) const { return x * o.x + y * o.y; }   // scalar product, as TVec.hpp
     TVec& operator+=(TVec o) {
         x += o.x;
         y += o.y;
@@ -127,6 +134,8 @@ class Space {
     // sums over the vortex list on the device (TSpace.hpp:85-117)
     double gsum();
     double gmax();
+    double integral();
+    TVec hydrodynamic_momentum();
 
     // device synchronisation (no counterpart in the reference)
     void push();
@@ -140,6 +149,12 @@ class Space {
     friend class MEpsilonFast;
     friend class MDiffusiveFast;
     friend class MFlowmove;
+    friend class XField;
+    friend class XVelocity;
+    friend class XVorticity;
+    friend class XStreamfunction;
+    friend class XPressure;
+    friend class sensors;
     vvhd_ctx* ctx_ = nullptr;
     bool pushed_ = false;
     int32_t nseg_ = 0;
@@ -149,14 +164,31 @@ class Space {
     void check(int rc, bool state_error = false) const;
 };
 
-// Host view of one leaf of the device tree (the fields the reference's
-// callers read from TSortedNode: centre, size, depth i, path code j and the
-// vortex range as indices into Space::VortexList after Space::pull()).
+// Host view of one node of the device tree (TSortedTree.hpp:20-57): centre
+// and size, depth i, path code j, per-sign monopoles CMp/CMm, the vortex
+// range as indices [v0, v1) into Space::VortexList (and as TObj pointers in
+// vRange while the host mirror holds the device's list), and for a leaf its
+// near (leaves) and far (any level) interaction lists in the reference's
+// order.  Views of every node and the lists are loaded from the device when
+// the tree is first asked for its leaves.
+class stree;
 class TSortedNode {
    public:
+    struct range {
+        TObj *first = nullptr, *last = nullptr;
+        size_t size() const { return (size_t)(last - first); }
+    };
     double x = 0, y = 0, h = 0, w = 0;
     int i = 0, j = 0;
+    TObj CMp, CMm;
     int64_t v0 = 0, v1 = 0;
+    range vRange;
+    std::vector<TSortedNode*>* NearNodes = nullptr;
+    std::vector<TSortedNode*>* FarNodes = nullptr;
+    TSortedNode *ch1 = nullptr, *ch2 = nullptr;
+    // leaf index in DFS order (-1 for an internal node) and the owning tree
+    int leaf = -1;
+    const stree* tree = nullptr;
 };
 
 class stree {
@@ -175,13 +207,16 @@ class stree {
 
     int farCriteria() const { return farCriteria_; }
 
+    Space* space() const { return S; }
+
    private:
     Space* S;
     int farCriteria_;
     double minNodeSize, maxNodeSize;
     bool built_ = false;
-    mutable std::vector<TSortedNode> leaves_;
-    mutable std::vector<TSortedNode*> leaf_ptrs_;
+    mutable std::vector<TSortedNode> nodes_;              // every node, device numbering
+    mutable std::vector<TSortedNode*> leaf_ptrs_;         // leaves in DFS order
+    mutable std::vector<std::vector<TSortedNode*>> near_, far_;
     mutable bool leaves_valid_ = false;
     void load_leaves() const;
 };
@@ -218,6 +253,13 @@ class MEpsilonFast {
     MEpsilonFast(Space* S_, const TSortedTree* Tree_) : S(S_), Tree(Tree_), merged_(0) {}
     void CalcEpsilonFast(bool merge);
     int Merged() { return merged_; }
+    // Squared distances over the near list of a leaf (MEpsilonFast.cpp:66-107),
+    // evaluated on the device of the node's tree.
+    static double eps2h(const TSortedNode& node, TVec p);
+    static double h2(const TSortedNode& node, TVec p);
+
+   private:
+    static double near_query(const TSortedNode& node, TVec p, bool h2);
 
    private:
     Space* S;
@@ -247,6 +289,96 @@ class MFlowmove {
 
    private:
     Space* S;
+};
+
+// ------------------------------------------------------------ fields --
+// Grid evaluators (XField.hpp:18-63 and its subclasses): each works on a copy
+// of the Space in a device context of its own, prepared as the reference's
+// evaluate() prepares its copy, then fills map on the device (vvhd_field).
+class XField {
+   public:
+    XField() = delete;
+    XField(const XField&) = delete;
+    XField& operator=(const XField&) = delete;
+    XField(float xmin, float ymin, float dxdy, int xres, int yres);
+    virtual ~XField() = default;
+    const float& at(int xi, int yj) const { return map_[(size_t)yj * xres + xi]; }
+    float min() const;
+    float max() const;
+    float percentile(float p) const;
+    friend std::ostream& operator<<(std::ostream& os, const XField& field);
+
+   protected:
+    float xmin, ymin, dxdy;
+    int xres, yres;
+    std::vector<float> map_;
+    bool evaluated = false;
+    // the copy: device context, host lists and the average segment length
+    void copy_space(const Space& S, Space& copy) const;
+    vvhd_grid grid() const;
+};
+
+class XVelocity : public XField {
+   public:
+    XVelocity(const Space& S, double xmin, double ymin, double dxdy, int xres, int yres);
+    char mode = 0;        // 'x' or 'y'
+    char ref_frame = 0;   // 'o', 'f' or 'b'
+    void evaluate();
+
+   private:
+    const Space& S;
+};
+
+class XVorticity : public XField {
+   public:
+    XVorticity(const Space& S, double xmin, double ymin, double dxdy, int xres, int yres);
+    double eps_mult = 0;
+    void evaluate();
+
+   private:
+    const Space& S;
+};
+
+class XStreamfunction : public XField {
+   public:
+    XStreamfunction(const Space& S, double xmin, double ymin, double dxdy, int xres, int yres);
+    double eps_mult = 0;
+    char ref_frame = 0;   // 'o', 'f' or 'b'
+    void evaluate();
+
+   private:
+    const Space& S;
+};
+
+class XPressure : public XField {
+   public:
+    XPressure(const Space& S, double xmin, double ymin, double dxdy, int xres, int yres);
+    double eps_mult = 0;
+    char ref_frame = 0;   // 's', 'o', 'f' or 'b'
+    void evaluate();
+
+   private:
+    const Space& S;
+};
+
+// Velocity probes (utils/vvflow/sensors.cpp:8-70): points read from a file
+// ("x y" per line); output() appends "time \tu \tv ..." with the
+// convective velocity at every point (one device call for all points).
+class sensors {
+   public:
+    sensors() = delete;
+    sensors(Space* S, MConvectiveFast* conv, const char* sensors_file, const char* output);
+    sensors(const sensors&) = delete;
+    sensors& operator=(const sensors&) = delete;
+    ~sensors();
+    void loadFile(const char* file);
+    void output();
+
+   private:
+    Space* S;
+    MConvectiveFast* conv;
+    FILE* fout;
+    std::vector<TVec> slist;
 };
 
 #endif

@@ -1299,6 +1299,188 @@ int vvo_velocity(vvo_ctx* c, const double* xy, int64_t n, double* vout) {
     return 0;
 }
 
+// MEpsilonFast::eps2h (MEpsilonFast.cpp:66-91) over the near leaves of leaf.
+static double eps2h_at(vvo_ctx* c, int leaf, V2 p) {
+    double res1 = kInf, res2 = kInf;
+    for (int m : c->near[c->nodes[leaf].leaf_index]) {
+        const Node& M = c->nodes[m];
+        for (int j = M.v0; j < M.v1; j++) {
+            const double d = abs2(p - V2{c->vort[j].x, c->vort[j].y});
+            if (!d) {
+                continue;
+            } else if (d < res1) {
+                res2 = res1;
+                res1 = d;
+            } else if (d < res2) {
+                res2 = d;
+            }
+        }
+    }
+    if (std::isfinite(res2)) return res2;
+    if (std::isfinite(res1)) return res1;
+    return std::numeric_limits<double>::lowest();
+}
+
+// MEpsilonFast::h2 (MEpsilonFast.cpp:93-107): the segments of the near leaves.
+static double h2_at(vvo_ctx* c, int leaf, V2 p) {
+    double res = kInf;
+    for (int m : c->near[c->nodes[leaf].leaf_index])
+        for (int s : c->nodes[m].segs) res = std::min(res, abs2(p - c->segs[s].r()));
+    return res;
+}
+
+// Leaf of every vortex (its own range).
+static std::vector<int> vortex_leaves(vvo_ctx* c) {
+    std::vector<int> lv(c->vort.size(), -1);
+    for (int leaf : c->leaves)
+        for (int k = c->nodes[leaf].v0; k < c->nodes[leaf].v1; k++) lv[k] = leaf;
+    return lv;
+}
+
+int vvo_eps2h_h2(vvo_ctx* c, int h2, const double* xy, int64_t n, double* out) {
+    if (!c->built) {
+        c->err = "eps2h: tree is not built";
+        return 1;
+    }
+    std::vector<int> lv;
+    if (!xy) lv = vortex_leaves(c);
+    for (int64_t i = 0; i < n; i++) {
+        V2 p = xy ? V2{xy[2 * i], xy[2 * i + 1]} : V2{c->vort[i].x, c->vort[i].y};
+        const int leaf = xy ? find_node(c, p) : lv[i];
+        out[i] = h2 ? h2_at(c, leaf, p) : eps2h_at(c, leaf, p);
+    }
+    return 0;
+}
+
+static bool point_is_in_body(vvo_ctx* c, V2 p) {   // Space::point_is_in_body (TSpace.hpp:140-145)
+    for (const Body& b : c->bodies)
+        if (point_invalid(c, b, p) >= 0) return true;
+    return false;
+}
+
+// Grid fields of XVelocity (XVelocity.cpp:24-73), XVorticity (XVorticity.cpp:
+// 26-94), XStreamfunction (XStreamfunction.cpp:47-118, 160-205) and XPressure
+// (XPressure.cpp:32-146) over the built tree; kinds as VVHD_FIELD_*.  The
+// caller prepares the Space as the evaluator does (shed, tree parameters,
+// epsilon, velocities).  The per-vortex weights go to v.x / v.y as in the
+// reference.
+int vvo_field(vvo_ctx* c, int kind, double xmin, double ymin, double dxdy, int xres, int yres, double m, double dl,
+              int with_ref, double ref_x, double ref_y, float* map) {
+    if (!c->built) {
+        c->err = "field: tree is not built";
+        return 1;
+    }
+    const V2 U{c->inf_vx, c->inf_vy}, ref{ref_x, ref_y};
+    std::vector<int> lv = vortex_leaves(c);
+    if (kind == 2) {
+        for (size_t i = 0; i < c->vort.size(); i++) {
+            Obj& o = c->vort[i];
+            o.vx = 1. / (sq(m) * std::max(eps2h_at(c, lv[i], V2{o.x, o.y}), sq(0.6 * dl)));
+            o.vy = o.vx * o.g;
+        }
+    } else if (kind == 3) {
+        for (size_t i = 0; i < c->vort.size(); i++) {
+            Obj& o = c->vort[i];
+            o.vx = sq(m) * std::max(eps2h_at(c, lv[i], V2{o.x, o.y}) * 0.25, sq(0.2 * dl));
+        }
+    }
+    std::vector<double> gtmp(c->segs.size());
+    for (const Body& b : c->bodies) {
+        double t = 0;
+        for (int k = 0; k < b.n; k++) gtmp[b.first + k] = (t += c->segs[b.first + k].gsum);
+    }
+    const double rd2 = sq(0.2 * dl);
+    for (int yj = 0; yj < yres; yj++)
+        for (int xi = 0; xi < xres; xi++) {
+            const V2 p = V2{xmin, ymin} + dxdy * V2{(double)xi, (double)yj};
+            double res = 0;
+            if (kind == 0 || kind == 1) {
+                double v[2];
+                vvo_velocity(c, &p.x, 1, v);
+                res = (kind == 0) ? v[0] - ref.x : v[1] - ref.y;
+            } else if (kind == 2) {
+                if (point_is_in_body(c, p)) {
+                    map[yj * xres + xi] = 0;
+                    continue;
+                }
+                const int leaf = find_node(c, p);
+                for (int mm : c->near[c->nodes[leaf].leaf_index])
+                    for (int j = c->nodes[mm].v0; j < c->nodes[mm].v1; j++) {
+                        const Obj& o = c->vort[j];
+                        const double exparg = -abs2(p - V2{o.x, o.y}) * o.vx;
+                        res += (exparg > -6) ? o.vy * std::exp(exparg) : 0;
+                    }
+                res *= k1_Pi;
+                const double erfarg = h2_at(c, leaf, p) / sq(dl * m);
+                res += (erfarg < 3) ? 0.5 * (1 - std::erf(erfarg)) : 0;
+            } else if (kind == 3) {
+                double psi_g = 0, psi_q = 0;
+                for (const Body& b : c->bodies) {
+                    for (int k = 0; k < b.n; k++) {
+                        const Seg& sg = c->segs[b.first + k];
+                        psi_g += -sg.g * std::log(abs2(p - sg.r()) + rd2);
+                    }
+                    if (std::fabs(b.speed_slae[0]) + std::fabs(b.speed_slae[1]) + std::fabs(b.speed_slae[2]) < 1e-10)
+                        continue;
+                    for (int k = 0; k < b.n; k++) {
+                        const Seg& sg = c->segs[b.first + k];
+                        const V2 Vs = V2{b.speed_slae[0], b.speed_slae[1]} + b.speed_slae[2] * rotl(sg.r() - b.axis());
+                        const double g = -dot(Vs, sg.dl()), q = -dot(rotl(Vs), sg.dl());
+                        psi_g += -g * std::log(abs2(p - sg.r()) + rd2);
+                        const V2 v1 = b.cofm - p, v2 = sg.r() - p;
+                        psi_q += -q * std::atan2(dot(rotl(v2), v1), dot(v2, v1));
+                    }
+                }
+                const int leaf = find_node(c, p);
+                for (int f : c->far[c->nodes[leaf].leaf_index]) {
+                    const Node& F = c->nodes[f];
+                    psi_g += -F.cmp.g * std::log(abs2(p - V2{F.cmp.x, F.cmp.y}) + rd2);
+                    psi_g += -F.cmm.g * std::log(abs2(p - V2{F.cmm.x, F.cmm.y}) + rd2);
+                }
+                for (int mm : c->near[c->nodes[leaf].leaf_index])
+                    for (int j = c->nodes[mm].v0; j < c->nodes[mm].v1; j++) {
+                        const Obj& o = c->vort[j];
+                        psi_g += -o.g * std::log(abs2(p - V2{o.x, o.y}) + o.vx);
+                    }
+                res = psi_g / (4 * kPi) + psi_q * k1_2Pi + dot(p, rotl(U - ref));
+            } else if (kind == 4) {
+                if (point_is_in_body(c, p)) {
+                    map[yj * xres + xi] = 0;
+                    continue;
+                }
+                double cp = 0;
+                for (const Body& b : c->bodies) {
+                    for (int k = 0; k < b.n; k++) {
+                        const Seg& sg = c->segs[b.first + k];
+                        const V2 Vs = V2{b.speed_slae[0], b.speed_slae[1]} + b.speed_slae[2] * rotl(sg.r() - b.axis());
+                        const double g = -dot(Vs, sg.dl()), q = -dot(rotl(Vs), sg.dl());
+                        const V2 K = (p - sg.r()) / abs2(p - sg.r());
+                        cp += dot(rotl(K) * g + K * q, Vs);
+                    }
+                    for (int k = 0; k < b.n; k++) {
+                        const Seg& sg = c->segs[b.first + k];
+                        const V2 K = (p - sg.r()) / abs2(p - sg.r());
+                        cp -= dot(sg.dl() / c->dt, rotl(K)) * gtmp[b.first + k];
+                    }
+                }
+                for (const Obj& o : c->vort) {
+                    const V2 K = (p - V2{o.x, o.y}) / abs2(p - V2{o.x, o.y});
+                    cp += dot(V2{o.vx, o.vy}, rotl(K)) * o.g;
+                }
+                double v[2];
+                vvo_velocity(c, &p.x, 1, v);
+                const V2 u{v[0], v[1]};
+                res = k1_2Pi * cp + 0.5 * (abs2(U) - abs2(u));
+                if (with_ref) res += 0.5 * abs2(u - ref);
+            } else {
+                c->err = "field: unknown kind";
+                return 1;
+            }
+            map[yj * xres + xi] = (float)res;
+        }
+    return 0;
+}
+
 // MDiffusiveFast::process_vort_list (MDiffusiveFast.cpp:8-48)
 int vvo_diffusive(vvo_ctx* c, int threads, int64_t leaf_begin, int64_t leaf_end) {
     if (!c->built) {

@@ -111,6 +111,15 @@ int     vvo_convective(vvo_ctx*, int threads, int64_t leaf_begin, int64_t leaf_e
 /* MConvectiveFast::velocity(p) (:20-34) for n points. */
 int     vvo_velocity(vvo_ctx*, const double* xy, int64_t n, double* vout);
 
+/* ---- Point queries and grid fields ------------------------------------ */
+/* MEpsilonFast::eps2h (h2 == 0) or h2 (MEpsilonFast.cpp:66-107) at n points
+ * (leaf from findNode) or, with xy == NULL, at every vortex with its own leaf. */
+int     vvo_eps2h_h2(vvo_ctx*, int h2, const double* xy, int64_t n, double* out);
+/* XVelocity x/y (kind 0/1), XVorticity (2), XStreamfunction (3), XPressure
+ * (4) on the grid (xmin + dxdy xi, ymin + dxdy yj), row by row (yj major). */
+int     vvo_field(vvo_ctx*, int kind, double xmin, double ymin, double dxdy, int xres, int yres,
+                  double eps_mult, double dl, int with_ref, double ref_x, double ref_y, float* map);
+
 /* ---- Diffusive (MDiffusiveFast.cpp:8-48, 93-123) ---------------------- */
 int     vvo_diffusive(vvo_ctx*, int threads, int64_t leaf_begin, int64_t leaf_end);
 

@@ -22,7 +22,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 from cases import cylinder_corners, oracle_for  # noqa: E402
 
-DL = 2 * 0.5 * np.sin(np.pi / 500)
+DL = 500 * np.sin(np.pi / 500) / 499       # Space::average_segment_length: slen / (N - 1)
 MIN_NODE, MAX_NODE = 5 * DL, 100 * DL   # utils/vvflow/vvflow.cpp:200-202
 DT = 0.005
 

@@ -86,6 +86,9 @@ class Oracle:
             _sig(L, "vvo_epsilon", C.c_int, [P, C.c_int, C.c_void_p, C.c_void_p,
                                              C.c_void_p, I32, C.POINTER(I32)])
             _sig(L, "vvo_epsilon_leaves", C.c_int, [P, C.c_int, I64, I64, C.c_void_p])
+            _sig(L, "vvo_eps2h_h2", C.c_int, [P, C.c_int, C.c_void_p, I64, dp])
+            _sig(L, "vvo_field", C.c_int, [P, C.c_int, F64, F64, F64, C.c_int, C.c_int, F64, F64, C.c_int, F64, F64,
+                                           C.c_void_p])
             _sig(L, "vvo_convective", C.c_int, [P, C.c_int, I64, I64])
             _sig(L, "vvo_velocity", C.c_int, [P, dp, I64, dp])
             _sig(L, "vvo_diffusive", C.c_int, [P, C.c_int, I64, I64])
@@ -220,6 +223,20 @@ class Oracle:
         merged = I32(0)
         self._check(self.L.vvo_epsilon_leaves(self.h, int(merge), leaf_begin, leaf_end, C.byref(merged)))
         return merged.value
+
+    def eps2h(self, pts=None, h2=False):
+        n = self.L.vvo_num_vortices(self.h) if pts is None else len(pts)
+        out = np.zeros(n)
+        xy = None if pts is None else np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+        self._check(self.L.vvo_eps2h_h2(self.h, int(h2), None if xy is None else xy.ctypes.data, n, out))
+        return out
+
+    def field(self, kind, xmin, ymin, dxdy, xres, yres, eps_mult=1.0, dl=0.0, ref=None):
+        out = np.zeros((yres, xres), dtype=np.float32)
+        r = (0.0, 0.0) if ref is None else ref
+        self._check(self.L.vvo_field(self.h, kind, xmin, ymin, dxdy, xres, yres, eps_mult, dl,
+                                     0 if ref is None else 1, r[0], r[1], out.ctypes.data))
+        return out
 
     def convective(self, threads=0, leaf_begin=0, leaf_end=-1):
         self._check(self.L.vvo_convective(self.h, threads, leaf_begin, leaf_end))

@@ -19,7 +19,7 @@ from cases import cylinder_corners, layer_case, oracle_for, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-DL = 2 * 0.5 * np.sin(np.pi / 500)          # segment length of the 500-corner cylinder
+DL = 500 * np.sin(np.pi / 500) / 499       # Space::average_segment_length: slen / (N - 1)
 MIN_NODE, MAX_NODE = 5 * DL, 100 * DL       # utils/vvflow/vvflow.cpp:200-202
 SEG_EXACT = ["x", "y", "g", "inv_eps", "cx", "cy", "dlx", "dly", "gsum", "hsum", "fric", "Cp", "Fr", "Nu"]
 BODY_EXACT = ["holder", "dpos", "speed_slae", "speed_slae_prev", "friction", "friction_prev", "fdt_dead", "g_dead",

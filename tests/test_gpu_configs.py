@@ -197,7 +197,7 @@ def test_slae_2000_segments_vs_oracle(ctx):
     vort[:, 0], vort[:, 1] = rr * np.cos(th), rr * np.sin(th)
     vort[:, 2] = rng.normal(size=m) * 1e-3
     o = oracle_for(vort, corners=corners, inf=(1.0, 0.0))
-    dl = o.body(0)["slen"] / nseg
+    dl = o.body(0)["slen"] / (nseg - 1)   # Space::average_segment_length
     o.tree_build(8, 5 * dl, 100 * dl)
     A_o, b_o = o.calc_circulation(want_matrix=True)
     rec = np.zeros(1, dtype=V.BODY_DTYPE)
@@ -236,7 +236,7 @@ def cylinder_sims():
 def test_batched_cylinders_vs_oracle(ctx):
     sims = cylinder_sims()
     print(f"case batch of {len(sims)} cylinder simulations: sizes {[len(s[0]) for s in sims]}", flush=True)
-    dl = 2 * 0.5 * np.sin(np.pi / 500)
+    dl = 500 * np.sin(np.pi / 500) / 499   # slen / (N - 1)
     mn, mx = 5 * dl, 100 * dl
     objs = np.concatenate([s[0] for s in sims])
     segs, bodies = [], []
@@ -291,7 +291,7 @@ def test_batched_cylinders_vs_oracle(ctx):
 
 # ------------------------------------------------- 400-step cylinder (config 0)
 
-DL500 = 2 * 0.5 * np.sin(np.pi / 500)
+DL500 = 500 * np.sin(np.pi / 500) / 499   # Space::average_segment_length: slen / (N - 1)
 MIN_NODE, MAX_NODE = 5 * DL500, 100 * DL500
 DT = 0.005
 

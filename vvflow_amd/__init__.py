@@ -32,6 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libvvhd_gpu.so")
 VVHD_OK, VVHD_EINVAL, VVHD_ESTATE, VVHD_EHIP, VVHD_ENOMEM, VVHD_ESINGULAR, VVHD_EUNSUPPORTED = range(7)
 FAR_TRAINABLE, FAR_PARAMS = 5122, 5148   # VVHD_FAR_TRAINABLE, VVHD_FAR_PARAMS
 NOALLOC_SCRATCH_BYTES = 64 << 20          # VVHD_NOALLOC_SCRATCH_BYTES
+FIELD_VX, FIELD_VY, FIELD_VORTICITY, FIELD_STREAMFUNCTION, FIELD_PRESSURE = range(5)   # VVHD_FIELD_*
 
 SEG_DTYPE = np.dtype([(n, np.float64) for n in
                       ("x", "y", "g", "inv_eps", "cx", "cy", "dlx", "dly",
@@ -69,7 +70,7 @@ ABI = {
     "vvhd_tree_leaves": (_I, [_P, _VP, _VP, _VP, _VP]),
     "vvhd_tree_lists": (_I, [_P, _I, _VP, _VP, _VP, _VP, _VP]),
     "vvhd_tree_find": (_I, [_P, _VP, _I64, _VP]),
-    "vvhd_tree_nodes": (_I, [_P, _VP, _VP, _VP, _VP]),
+    "vvhd_tree_nodes": (_I, [_P, _VP, _VP, _VP, _VP, _VP]),
     "vvhd_epsilon": (_I, [_P, _I, _VP, _VP, _VP]),
     "vvhd_convective": (_I, [_P, _D, _D, _I]),
     "vvhd_convective_noalloc": (_I, [_P, _D, _D, _I]),
@@ -90,6 +91,9 @@ ABI = {
     "vvhd_far_gradients": (_I, [_P, _VP, _VP, _VP, _I64, _VP, _VP]),
     "vvhd_far_train_data": (_I, [_P, _VP, _VP, _VP, _I64]),
     "vvhd_far_train": (_I, [_P, _I32, _I32, _VP, _U64, _VP]),
+    "vvhd_eps2h": (_I, [_P, _VP, _VP, _I64, _VP]),
+    "vvhd_h2": (_I, [_P, _VP, _VP, _I64, _VP]),
+    "vvhd_field": (_I, [_P, _I, _VP, _D, _D, _D, _D, _I, _D, _D, _VP]),
     "vvhd_integrals": (_I, [_P, _VP]),
     "vvhd_direct_sum": (_I, [_P, _VP, _I64, _VP]),
     "vvhd_phase_times": (_I, [_P, _VP]),
@@ -281,13 +285,14 @@ class Context:
 
     def tree_nodes(self):
         nn = C.c_int32(0)
-        self._check(self.L.vvhd_tree_nodes(self.h, C.addressof(nn), None, None, None))
+        self._check(self.L.vvhd_tree_nodes(self.h, C.addressof(nn), None, None, None, None))
         n = nn.value
         geom = np.zeros((n, 4))
         ints = np.zeros((n, 4), dtype=np.int32)
         path = np.zeros(n, dtype=np.uint64)
-        self._check(self.L.vvhd_tree_nodes(self.h, C.addressof(nn), _ptr(geom), _ptr(ints), _ptr(path)))
-        return dict(geom=geom, ints=ints, path=path)
+        cm = np.zeros((n, 6))
+        self._check(self.L.vvhd_tree_nodes(self.h, C.addressof(nn), _ptr(geom), _ptr(ints), _ptr(path), _ptr(cm)))
+        return dict(geom=geom, ints=ints, path=path, cm=cm)
 
     def tree_find(self, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
@@ -416,6 +421,40 @@ class Context:
         loss = np.zeros(len(lr), dtype=np.float32)
         self._check(self.L.vvhd_far_train(self.h, len(lr), batch, _ptr(lr), seed, _ptr(loss)))
         return loss
+
+    def eps2h(self, pts=None, leaf=None):
+        """MEpsilonFast::eps2h at points (over the near list of leaf, or of
+        the leaf findNode gives) or, with pts None, at every vortex with its
+        own leaf."""
+        return self._eps2h_h2(self.L.vvhd_eps2h, pts, leaf)
+
+    def h2(self, pts=None, leaf=None):
+        """MEpsilonFast::h2 at points or at every vortex."""
+        return self._eps2h_h2(self.L.vvhd_h2, pts, leaf)
+
+    def _eps2h_h2(self, fn, pts, leaf):
+        if pts is None:
+            out = np.zeros(self.n)
+            self._check(fn(self.h, None, None, len(out), _ptr(out)))
+            return out
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+        lf = None if leaf is None else np.ascontiguousarray(leaf, dtype=np.int32).reshape(-1)
+        out = np.zeros(len(pts))
+        self._check(fn(self.h, _ptr(pts), _ptr(lf), len(pts), _ptr(out)))
+        return out
+
+    def field(self, kind, xmin, ymin, dxdy, xres, yres, eps_mult=1.0, dl=0.0, inf=(0.0, 0.0), ref=None):
+        """vvhd_field: kind one of FIELD_*; ref None is ref_frame 's' (pressure)
+        or a zero reference; returns the (yres, xres) float map."""
+        g = (C.c_double * 3)(xmin, ymin, dxdy)
+        grid = (C.c_byte * 32)()
+        C.memmove(grid, g, 24)
+        C.memmove(C.addressof(grid) + 24, (C.c_int32 * 2)(xres, yres), 8)
+        out = np.zeros((yres, xres), dtype=np.float32)
+        r = (0.0, 0.0) if ref is None else ref
+        self._check(self.L.vvhd_field(self.h, int(kind), C.addressof(grid), eps_mult, dl, inf[0], inf[1],
+                                      0 if ref is None else 1, r[0], r[1], _ptr(out)))
+        return out
 
     def integrals(self):
         out = np.zeros(5)

@@ -5,10 +5,10 @@
 //    and TBody::move (:179-198): one thread per body, sequential in segment
 //    order, so every FP64 quantity rounds as in the reference.
 //  * MFlowmove::move_and_clean (MFlowmove.cpp:11-215) for rigid bodies
-//    without collisions: advection (thread per vortex), stable removal of
+//    without collisions: two passes (k_fm_count, k_fm_scatter) advect, flag
 //    |g| < 1e-10 and in-body vortices (TBody::isPointInvalid, TBody.cpp:
-//    217-281) by an exclusive scan and a scatter, the bodies' dead force,
-//    moment and circulation accumulated in vortex order, then the zero/slip
+//    217-281) and compact the survivors stably; then the bodies' dead force,
+//    moment and circulation accumulated in vortex order and the zero/slip
 //    segment loop.
 //  * MFlowmove::vortex_shed (MFlowmove.cpp:217-235): appended behind the
 //    list in segment order.
@@ -23,6 +23,7 @@
 //    (TMatrix.cpp:225-259).
 
 #include "api.hpp"
+#include "body_geom.hpp"
 
 #include <cstring>
 #include <rocprim/device/device_scan.hpp>
@@ -149,34 +150,6 @@ __global__ void k_body_move(vvhd_body* bodies, int nbody, vvhd_seg* segs, double
     body_properties(B, segs, geo + 5 * b);
 }
 
-// TBody::isPointInContour over the corners (TBody.cpp:238-281): index of the
-// nearest segment when p is inside the body, else -1.
-__device__ int point_invalid(const vvhd_body& B, const vvhd_seg* segs, const double* geo, double px, double py) {
-    bool in = B.area <= 0;
-    if (!in && (px < geo[1] || py < geo[2] || px > geo[3] || py > geo[4] ||
-                (px - B.cofm[0]) * (px - B.cofm[0]) + (py - B.cofm[1]) * (py - B.cofm[1]) > geo[0]))
-        return -1;
-    const int f = B.first_seg, n = B.nseg;
-    for (int i = 0, j = n - 1; i < n; j = i++) {
-        const double vix = segs[f + i].cx, viy = segs[f + i].cy, vjx = segs[f + j].cx, vjy = segs[f + j].cy;
-        if (((viy < vjy) && (viy < py) && (py <= vjy) && ((vjy - viy) * (px - vix) > (vjx - vix) * (py - viy))) ||
-            ((viy > vjy) && (viy > py) && (py >= vjy) && ((vjy - viy) * (px - vix) < (vjx - vix) * (py - viy))))
-            in = !in;
-    }
-    if (!in) return -1;
-    int best = -1;
-    double bd = DBL_MAX;
-    for (int k = 0; k < n; k++) {
-        const double dx = segs[f + k].x - px, dy = segs[f + k].y - py;
-        const double d2 = dx * dx + dy * dy;
-        if (d2 < bd) {
-            best = f + k;
-            bd = d2;
-        }
-    }
-    return best;
-}
-
 __device__ inline int body_of_seg(const vvhd_body* bodies, int nbody, int s) {
     int b = 0;
     while (b + 1 < nbody && bodies[b + 1].first_seg <= s) b++;
@@ -185,62 +158,174 @@ __device__ inline int body_of_seg(const vvhd_body* bodies, int nbody, int s) {
 
 // ---------------------------------------------------------------- flow-move
 
-__global__ void k_advect_flags(double* x, double* y, const double* vx, const double* vy, const double* g, int64_t n,
-                               double dt, int remove, const vvhd_body* bodies, int nbody, const vvhd_seg* segs,
-                               const double* geo, int32_t* keep, int32_t* inbody, int32_t* badseg) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double nx = x[i] + vx[i] * dt, ny = y[i] + vy[i] * dt;
-    x[i] = nx;
-    y[i] = ny;
-    int k = 1, ib = 0, bs = -1;
-    if (fabs(g[i]) < kRemoveEps) {
-        k = 0;
-    } else if (remove) {
-        for (int b = 0; b < nbody; b++) {
-            const int s = point_invalid(bodies[b], segs, geo + 5 * b, nx, ny);
-            if (s >= 0) {
-                k = 0;
-                ib = 1;
-                bs = s;
-                break;
+// Flow-move over the vortex list (MFlowmove.cpp:104-145) in two passes with
+// no waits between blocks: k_fm_count advects each tile of kFmTile vortices
+// and counts its survivors and in-body removals (wave ballots); k_fm_scatter
+// recomputes the same positions and flags, adds the counts of the tiles
+// before its own (a block reduction over the tile counts) and writes the
+// survivors, velocity zeroed, stably into the double buffers, and the in-body
+// removals in vortex order as dead records {x, y, g, segment} for the dead
+// force.  The last tile writes both totals to counts[0..1].
+constexpr int kFmBlock = 256;
+constexpr int kFmItems = 8;
+constexpr int kFmTile = kFmBlock * kFmItems;
+constexpr int kFmWaves = kFmBlock / 64;
+
+struct FmArgs {
+    const double *x, *y, *vx, *vy, *g, *ie;
+    int64_t n;
+    double dt;
+    int remove;
+    const vvhd_body* bodies;
+    int nbody;
+    const vvhd_seg* segs;
+    const double* geo;
+};
+
+// Moved position and removal class of vortex i: 1 kept, 2 removed in a body
+// (segment in *bs), 0 removed for |g| < remove_eps or past the list.
+__device__ inline int fm_classify(const FmArgs& A, int64_t i, double& px, double& py, int& bs) {
+    bs = -1;
+    if (i >= A.n) return 0;
+    px = A.x[i] + A.vx[i] * A.dt;
+    py = A.y[i] + A.vy[i] * A.dt;
+    if (fabs(A.g[i]) < kRemoveEps) return 0;
+    if (A.remove)
+        for (int b = 0; b < A.nbody; b++) {
+            const int sgi = point_invalid(A.bodies[b], A.segs, A.geo + 5 * b, px, py);
+            if (sgi >= 0) {
+                bs = sgi;
+                return 2;
             }
         }
-    }
-    keep[i] = k;
-    inbody[i] = ib;
-    badseg[i] = bs;
+    return 1;
 }
 
-__global__ void k_compact(const int32_t* keep, const int32_t* kpos, const int32_t* inbody, const int32_t* ipos,
-                          int64_t n, const double* x, const double* y, const double* g, const double* ie, double* x2,
-                          double* y2, double* g2, double* vx2, double* vy2, double* ie2, int32_t* dead_list) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (keep[i]) {
-        const int p = kpos[i];
-        x2[p] = x[i];
-        y2[p] = y[i];
-        g2[p] = g[i];
-        vx2[p] = 0.;
-        vy2[p] = 0.;
-        ie2[p] = ie[i];
-    } else if (inbody[i]) {
-        dead_list[ipos[i]] = (int)i;
+__global__ __launch_bounds__(kFmBlock) void k_fm_count(FmArgs A, int32_t* tile_cnt) {
+    __shared__ uint32_t s_cnt[2][kFmWaves];
+    const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kFmTile;
+    uint32_t ck = 0, cd = 0;
+#pragma unroll
+    for (int k = 0; k < kFmItems; k++) {
+        double px, py;
+        int bs;
+        const int cls = fm_classify(A, base + k * kFmBlock + t, px, py, bs);
+        ck += (uint32_t)__popcll(__ballot(cls == 1));
+        cd += (uint32_t)__popcll(__ballot(cls == 2));
     }
+    if (lane == 0) {
+        s_cnt[0][w] = ck;
+        s_cnt[1][w] = cd;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t a = 0, b = 0;
+        for (int v = 0; v < kFmWaves; v++) {
+            a += s_cnt[0][v];
+            b += s_cnt[1][v];
+        }
+        tile_cnt[2 * blockIdx.x] = (int32_t)a;
+        tile_cnt[2 * blockIdx.x + 1] = (int32_t)b;
+    }
+}
+
+__global__ __launch_bounds__(kFmBlock) void k_fm_scatter(FmArgs A, const int32_t* tile_cnt, int ntiles,
+                                                         double* __restrict__ x2, double* __restrict__ y2,
+                                                         double* __restrict__ g2, double* __restrict__ vx2,
+                                                         double* __restrict__ vy2, double* __restrict__ ie2,
+                                                         double4* __restrict__ dead, int32_t* counts) {
+    __shared__ uint32_t s_cnt[2][kFmItems][kFmWaves];
+    __shared__ uint32_t s_red[2][kFmWaves];
+    const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    const int tile = blockIdx.x;
+    const int64_t base = (int64_t)tile * kFmTile;
+    // counts of the tiles before this one
+    uint32_t pk = 0, pd = 0;
+    for (int q = t; q < tile; q += kFmBlock) {
+        pk += (uint32_t)tile_cnt[2 * q];
+        pd += (uint32_t)tile_cnt[2 * q + 1];
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        pk += __shfl_xor(pk, o, kWave);
+        pd += __shfl_xor(pd, o, kWave);
+    }
+    double px[kFmItems], py[kFmItems];
+    int bs[kFmItems];
+    uint64_t mk[kFmItems], md[kFmItems];
+#pragma unroll
+    for (int k = 0; k < kFmItems; k++) {
+        const int cls = fm_classify(A, base + k * kFmBlock + t, px[k], py[k], bs[k]);
+        mk[k] = __ballot(cls == 1);
+        md[k] = __ballot(cls == 2);
+        if (lane == 0) {
+            s_cnt[0][k][w] = (uint32_t)__popcll(mk[k]);
+            s_cnt[1][k][w] = (uint32_t)__popcll(md[k]);
+        }
+    }
+    if (lane == 0) {
+        s_red[0][w] = pk;
+        s_red[1][w] = pd;
+    }
+    __syncthreads();
+    uint32_t ok = 0, od = 0;
+    for (int v = 0; v < kFmWaves; v++) {
+        ok += s_red[0][v];
+        od += s_red[1][v];
+    }
+    if (t == 0 && tile == ntiles - 1) {
+        uint32_t ak = ok, ad = od;
+        for (int k = 0; k < kFmItems; k++)
+            for (int v = 0; v < kFmWaves; v++) {
+                ak += s_cnt[0][k][v];
+                ad += s_cnt[1][k][v];
+            }
+        counts[0] = (int32_t)ak;
+        counts[1] = (int32_t)ad;
+    }
+    const uint64_t lt = (1ull << lane) - 1;
+#pragma unroll
+    for (int k = 0; k < kFmItems; k++) {
+        uint32_t rk = ok, rd = od;
+        for (int v = 0; v < w; v++) {
+            rk += s_cnt[0][k][v];
+            rd += s_cnt[1][k][v];
+        }
+        const int64_t i = base + k * kFmBlock + t;
+        if (mk[k] >> lane & 1) {
+            const uint32_t q = rk + (uint32_t)__popcll(mk[k] & lt);
+            x2[q] = px[k];
+            y2[q] = py[k];
+            g2[q] = A.g[i];
+            vx2[q] = 0.;
+            vy2[q] = 0.;
+            ie2[q] = A.ie[i];
+        } else if (md[k] >> lane & 1) {
+            const uint32_t q = rd + (uint32_t)__popcll(md[k] & lt);
+            dead[q] = make_double4(px[k], py[k], A.g[i], (double)bs[k]);
+        }
+        for (int v = 0; v < kFmWaves; v++) {
+            ok += s_cnt[0][k][v];
+            od += s_cnt[1][k][v];
+        }
+    }
+}
+
+__global__ void k_fill_u64(uint64_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Dead force, moment and circulation of the removed vortices in vortex order
-// (MFlowmove.cpp:265-285), then the zero/slip segment loop (:342-355).
-__global__ void k_dead(vvhd_body* bodies, int nbody, vvhd_seg* segs, const int32_t* dead_list, const int32_t* ndead,
-                       const int32_t* badseg, const double* x, const double* y, const double* g) {
+// (MFlowmove.cpp:124-144), then the zero/slip segment loop (:205-214).
+__global__ void k_dead(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double4* dead, const int32_t* counts) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int nd = *ndead;
+    const int nd = counts[1];
     for (int q = 0; q < nd; q++) {
-        const int i = dead_list[q];
-        const int s = badseg[i];
+        const double4 r = dead[q];
+        const int s = (int)r.w;
         vvhd_body& B = bodies[body_of_seg(bodies, nbody, s)];
-        const double gi = g[i], xi = x[i], yi = y[i];
+        const double gi = r.z, xi = r.x, yi = r.y;
         B.fdt_dead[0] += (-yi) * gi;
         B.fdt_dead[1] += xi * gi;
         const double ax = B.holder[0] + B.dpos[0], ay = B.holder[1] + B.dpos[1];
@@ -686,31 +771,27 @@ void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
         hipLaunchKernelGGL(k_body_move, dim3(nblocks(nb, 64)), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p,
                            c.body_geo.p, c.dt);
     const int64_t n = c.n;
-    c.iscratch0.grow(n + 1);
-    c.iscratch1.grow(n + 1);
-    c.iscratch2.grow(n + 1);
-    c.iscratch3.grow(2 * (size_t)n + 2);
-    c.iscratch4.grow(n + 1);
-    int32_t* keep = c.iscratch0.p;
-    int32_t* inbody = c.iscratch1.p;
-    int32_t* badseg = c.iscratch2.p;
-    int32_t* kpos = c.iscratch3.p;
-    int32_t* ipos = c.iscratch3.p + n + 1;
-    int32_t* dead_list = c.iscratch4.p;
-    VVHD_HIP(hipMemsetAsync(keep + n, 0, 4, s));
-    VVHD_HIP(hipMemsetAsync(inbody + n, 0, 4, s));
-    hipLaunchKernelGGL(k_advect_flags, dim3(nblocks(n)), dim3(kBlock), 0, s, c.x.p, c.y.p, c.vx.p, c.vy.p, c.g.p, n,
-                       c.dt, remove, c.bodies_d.p, nb, c.segs.p, c.body_geo.p, keep, inbody, badseg);
-    exclusive_scan_i32(c, keep, kpos, (size_t)n + 1);
-    exclusive_scan_i32(c, inbody, ipos, (size_t)n + 1);
-    hipLaunchKernelGGL(k_compact, dim3(nblocks(n)), dim3(kBlock), 0, s, keep, kpos, inbody, ipos, n, c.x.p, c.y.p,
-                       c.g.p, c.ie.p, c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p, dead_list);
-    if (nb)
-        hipLaunchKernelGGL(k_dead, dim3(1), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p, dead_list, ipos + n, badseg,
-                           c.x.p, c.y.p, c.g.p);
-    int32_t hn[2] = {0, 0};
-    VVHD_HIP(hipMemcpyAsync(&hn[0], kpos + n, 4, hipMemcpyDeviceToHost, s));
-    VVHD_HIP(hipMemcpyAsync(&hn[1], ipos + n, 4, hipMemcpyDeviceToHost, s));
+    VVHD_CHECK(n < (int64_t(1) << 31), VVHD_EUNSUPPORTED, "move_and_clean: more than 2^31 - 1 vortices");
+    const int ntiles = (int)std::max<int64_t>(1, (n + kFmTile - 1) / kFmTile);
+    c.iscratch0.grow(4);
+    int32_t* counts = c.iscratch0.p;
+    c.iscratch1.grow(2 * (size_t)ntiles + 2);
+    int32_t* tile_cnt = c.iscratch1.p;
+    c.tmp.grow((size_t)std::max<int64_t>(n, 1) * sizeof(double4));
+    double4* dead = (double4*)c.tmp.p;
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, (uint64_t*)counts, (int64_t)1);
+    } else {
+        const FmArgs A{c.x.p, c.y.p, c.vx.p, c.vy.p, c.g.p, c.ie.p, n, c.dt, remove, c.bodies_d.p, nb, c.segs.p,
+                       c.body_geo.p};
+        hipLaunchKernelGGL(k_fm_count, dim3(ntiles), dim3(kFmBlock), 0, s, A, tile_cnt);
+        hipLaunchKernelGGL(k_fm_scatter, dim3(ntiles), dim3(kFmBlock), 0, s, A, (const int32_t*)tile_cnt, ntiles,
+                           c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p, dead, counts);
+    }
+    if (nb) hipLaunchKernelGGL(k_dead, dim3(1), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p, dead, counts);
+    VVHD_HIP(hipGetLastError());
+    VVHD_HIP(hipEventRecord(c.ev[9], s));   // end of the device work of the phase
+    VVHD_HIP(hipMemcpyAsync(c.fm_counts, counts, 8, hipMemcpyDeviceToHost, s));
     VVHD_HIP(hipStreamSynchronize(s));
     if (n) {
         std::swap(c.x.p, c.x2.p);
@@ -720,8 +801,8 @@ void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
         std::swap(c.vy.p, c.vy2.p);
         std::swap(c.ie.p, c.ie2.p);
     }
-    c.n = hn[0];
-    if (cleaned) *cleaned = hn[1];
+    c.n = c.fm_counts[0];
+    if (cleaned) *cleaned = c.fm_counts[1];
     c.t.built = false;
     if (nb) sync_bodies_to_host(c);
 }
@@ -845,8 +926,7 @@ int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned) {
     Ctx& c = ctx->c;
     VVHD_HIP(hipSetDevice(c.device));
     VVHD_HIP(hipEventRecord(c.ev[8], c.stream));
-    move_and_clean(c, remove, cleaned);
-    VVHD_HIP(hipEventRecord(c.ev[9], c.stream));
+    move_and_clean(c, remove, cleaned);   // records ev[9] when its device work is queued
     c.phase_recorded[4] = true;
     API_END(ctx)
 }

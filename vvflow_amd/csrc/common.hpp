@@ -255,12 +255,17 @@ struct Ctx {
     DBuf<int64_t> pair_off;
     DBuf<int32_t> pair_cnt, pair_t, pair_f;
 
+    // field evaluators: per-vortex weights, the grid map, running gsum
+    DBuf<double> fld_a, fld_b, fld_gtmp;
+    DBuf<float> fld_map;
+
     // timing
     hipEvent_t ev[16] = {};
     // tree build: per-level counters copied asynchronously (pinned, two slots)
     static constexpr int kLevSlots = 2;   // tree levels in flight (pinned counter slots and events)
     hipEvent_t lev_ev[kLevSlots] = {};
     int32_t* lev_counters = nullptr;      // kLevSlots x 8 pinned counters
+    int32_t* fm_counts = nullptr;         // pinned: flow-move survivors, removals in bodies, scan error
     bool phase_recorded[8] = {false, false, false, false, false, false, false, false};
 
     // work counts of the current tree
@@ -340,5 +345,8 @@ void bodies_release(Ctx& c);
 void farfield_release(Ctx& c);
 void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out);
 void pack_sources(Ctx& c);
+void eps2h_h2(Ctx& c, bool h2, const double* xy, const int32_t* leaf, int64_t n, double* out);
+void grid_field(Ctx& c, int kind, const vvhd_grid& gr, double m, double dl, double inf_x, double inf_y, int with_ref,
+                double ref_x, double ref_y, float* map);
 
 }  // namespace vvhd

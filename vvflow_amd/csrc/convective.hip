@@ -170,11 +170,14 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
             }
         };
         if (KAHAN) {
-            // two sources per lane and step: their terms are added (FMAs
-            // folding in the compensation) before one compensated add, whose
-            // extra rounding is relative to a term, not to the running sum
-            // two sets of prefetched sources used in turn (no copies between
-            // steps); past the batch the fetch returns zero-circulation sources
+            // block-compensated sums: the terms go by FMA into the low-order
+            // accumulators qx/qy, which are folded into the running sums
+            // sx/sy with a compensated (Fast2Sum) step every eight terms per
+            // lane; qx/qy keep the rounding error of each fold, so every
+            // rounding is relative to at most eight terms, never to the
+            // running sum.  Two sets of prefetched sources are used in turn
+            // (no copies between steps); past the batch the fetch returns
+            // zero-circulation sources.
             float sa0, ta0, ga0, ea0, sb0, tb0, gb0, eb0;
             float sa1, ta1, ga1, ea1, sb1, tb1, gb1, eb1;
             auto pairs = [&](float sa, float ta, float ga, float ea, float sbx, float tb, float gb, float eb2) {
@@ -184,13 +187,18 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                     const float fa = ga * __builtin_amdgcn_rcpf(__builtin_fmaf(dxa, dxa, __builtin_fmaf(dya, dya, ea)));
                     const float dxb = tx[k] - sbx, dyb = ty[k] - tb;
                     const float fb = gb * __builtin_amdgcn_rcpf(__builtin_fmaf(dxb, dxb, __builtin_fmaf(dyb, dyb, eb2)));
-                    const float ux = __builtin_fmaf(-dyb, fb, __builtin_fmaf(-dya, fa, -qx[k]));
-                    const float tsx = sx[k] + ux;
-                    qx[k] = (tsx - sx[k]) - ux;
+                    qx[k] = __builtin_fmaf(-dyb, fb, __builtin_fmaf(-dya, fa, qx[k]));
+                    qy[k] = __builtin_fmaf(dxb, fb, __builtin_fmaf(dxa, fa, qy[k]));
+                }
+            };
+            auto fold = [&]() {
+#pragma unroll
+                for (int k = 0; k < NT; k++) {
+                    const float tsx = sx[k] + qx[k];
+                    qx[k] = qx[k] - (tsx - sx[k]);
                     sx[k] = tsx;
-                    const float uy = __builtin_fmaf(dxb, fb, __builtin_fmaf(dxa, fa, -qy[k]));
-                    const float tsy = sy[k] + uy;
-                    qy[k] = (tsy - sy[k]) - uy;
+                    const float tsy = sy[k] + qy[k];
+                    qy[k] = qy[k] - (tsy - sy[k]);
                     sy[k] = tsy;
                 }
             };
@@ -206,7 +214,9 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
                 fetch(sb + 4 * kWave, sa0, ta0, ga0, ea0);
                 fetch(sb + 5 * kWave, sb0, tb0, gb0, eb0);
                 pairs(sa1, ta1, ga1, ea1, sb1, tb1, gb1, eb1);
+                if (sb & (4 * kWave)) fold();   // every other step: eight terms per lane
             }
+            fold();
         } else {
             float s0x, s0y, g0, e0, s1x, s1y, g1, e1;
             auto terms = [&](float sxr, float syr, float gj, float e2) {
@@ -246,7 +256,7 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
         for (int k = 0; k < P; k++) {
             const float s_ = comp == 0 ? sx[k < NT ? k : 0] : sy[k < NT ? k : 0];
             const float q_ = comp == 0 ? qx[k < NT ? k : 0] : qy[k < NT ? k : 0];
-            v[k] = k < NT ? (KAHAN ? (double)s_ - (double)q_ : (double)s_) : 0.;
+            v[k] = k < NT ? (KAHAN ? (double)s_ + (double)q_ : (double)s_) : 0.;
         }
         int idx = 0, o = kWave / 2;
 #pragma unroll

@@ -170,7 +170,8 @@ int vvhd_create(int device, vvhd_ctx** out) {
         return VVHD_EHIP;
     }
     if (hipHostMalloc((void**)&ctx->c.lev_counters, Ctx::kLevSlots * 8 * sizeof(int32_t), hipHostMallocDefault) !=
-        hipSuccess) {
+            hipSuccess ||
+        hipHostMalloc((void**)&ctx->c.fm_counts, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return VVHD_EHIP;
     }
@@ -239,9 +240,12 @@ void vvhd_destroy(vvhd_ctx* ctx) {
         }
     c.dvx.release();
     c.dvy.release();
+    for (auto* b : {&c.fld_a, &c.fld_b, &c.fld_gtmp}) b->release();
+    c.fld_map.release();
     for (auto& e : c.lev_ev)
         if (e) (void)hipEventDestroy(e);
     if (c.lev_counters) (void)hipHostFree(c.lev_counters);
+    if (c.fm_counts) (void)hipHostFree(c.fm_counts);
     (void)hipStreamDestroy(c.stream);
     delete ctx;
 }

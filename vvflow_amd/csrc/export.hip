@@ -93,7 +93,7 @@ int vvhd_tree_leaves(vvhd_ctx* ctx, int32_t* nleaves, double* geom, int32_t* int
     API_END(ctx)
 }
 
-int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints, uint64_t* path) {
+int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints, uint64_t* path, double* cm) {
     if (!ctx) return VVHD_EINVAL;
     API_BEGIN
     Ctx& c = ctx->c;
@@ -101,11 +101,16 @@ int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints,
     VVHD_CHECK(t.built, VVHD_ESTATE, "tree_nodes: tree is not built");
     const int nn = t.nnodes;
     if (nnodes) *nnodes = nn;
-    if (!geom && !ints && !path) return VVHD_OK;
+    if (!geom && !ints && !path && !cm) return VVHD_OK;
     auto x = d2h(c, t.x.p, nn), y = d2h(c, t.y.p, nn), h = d2h(c, t.h.p, nn), w = d2h(c, t.w.p, nn);
     auto v0 = d2h(c, t.v0.p, nn), v1 = d2h(c, t.v1.p, nn), ch = d2h(c, t.ch1.p, nn), dp = d2h(c, t.depth.p, nn);
     auto sp = d2h(c, t.split.p, nn);
     auto pa = d2h(c, t.path.p, nn);
+    std::vector<double> cmv[6];
+    if (cm) {
+        const double* src[6] = {t.cmp_x.p, t.cmp_y.p, t.cmp_g.p, t.cmm_x.p, t.cmm_y.p, t.cmm_g.p};
+        for (int q = 0; q < 6; q++) cmv[q] = d2h(c, src[q], nn);
+    }
     sync(c);
     for (int k = 0; k < nn; k++) {
         if (geom) {
@@ -121,6 +126,8 @@ int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints,
             ints[4 * k + 3] = dp[k];
         }
         if (path) path[k] = pa[k];
+        if (cm)
+            for (int q = 0; q < 6; q++) cm[6 * k + q] = cmv[q][k];
     }
     API_END(ctx)
 }

@@ -163,57 +163,77 @@ struct MomView {
     __device__ double& operator()(int k, int id) const { return m[(size_t)k * st + id]; }
 };
 
-// One tree level bottom-up: leaves from their vortices, internal nodes from
-// their two children (parallel-axis combination of the second moments).
-__global__ void k_moments_level(const int32_t* v0, const int32_t* v1, const int32_t* ch1, const int32_t* split,
-                                const double* nx, const double* ny, const double* x, const double* y, const double* g,
-                                const double* ie, MomView M, int lb, int le) {
+// Per-sign moments of one node from its vortices (a leaf).
+__device__ inline void leaf_moments(int id, const int32_t* v0, const int32_t* v1, const double* nx, const double* ny,
+                                    const double* x, const double* y, const double* g, const double* ie, MomView M) {
 #pragma clang fp contract(off)
-    const int id = lb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= le) return;
     for (int s = 0; s < 2; s++) {
         double G = 0, zx, zy, qr = 0, qi = 0, E = 0;
-        if (split[id] != 2) {
-            double sx = 0, sy = 0;
+        double sx = 0, sy = 0;
+        for (int k = v0[id]; k < v1[id]; k++) {
+            const double gk = g[k];
+            if (s == 0 ? !(gk > 0) : !(gk < 0)) continue;
+            G += gk;
+            sx += gk * x[k];
+            sy += gk * y[k];
+            const double ek = 1. / ie[k];
+            E += gk * (ek * ek);
+        }
+        if (G != 0) {
+            zx = sx / G;
+            zy = sy / G;
             for (int k = v0[id]; k < v1[id]; k++) {
                 const double gk = g[k];
                 if (s == 0 ? !(gk > 0) : !(gk < 0)) continue;
-                G += gk;
-                sx += gk * x[k];
-                sy += gk * y[k];
-                const double ek = 1. / ie[k];
-                E += gk * (ek * ek);
-            }
-            if (G != 0) {
-                zx = sx / G;
-                zy = sy / G;
-                for (int k = v0[id]; k < v1[id]; k++) {
-                    const double gk = g[k];
-                    if (s == 0 ? !(gk > 0) : !(gk < 0)) continue;
-                    const double dx = x[k] - zx, dy = y[k] - zy;
-                    qr += gk * (dx * dx - dy * dy);
-                    qi += gk * (2 * dx * dy);
-                }
-            } else {
-                zx = nx[id];
-                zy = ny[id];
+                const double dx = x[k] - zx, dy = y[k] - zy;
+                qr += gk * (dx * dx - dy * dy);
+                qi += gk * (2 * dx * dy);
             }
         } else {
-            const int a = ch1[id], b = a + 1;
-            const double Ga = M(s, a), Gb = M(s, b);
-            G = Ga + Gb;
-            E = M(10 + s, a) + M(10 + s, b);
-            if (G != 0) {
-                const double ax = M(2 + 2 * s, a), ay = M(3 + 2 * s, a), bx = M(2 + 2 * s, b), by = M(3 + 2 * s, b);
-                zx = (Ga * ax + Gb * bx) / G;
-                zy = (Ga * ay + Gb * by) / G;
-                const double dax = ax - zx, day = ay - zy, dbx = bx - zx, dby = by - zy;
-                qr = M(6 + 2 * s, a) + M(6 + 2 * s, b) + Ga * (dax * dax - day * day) + Gb * (dbx * dbx - dby * dby);
-                qi = M(7 + 2 * s, a) + M(7 + 2 * s, b) + Ga * (2 * dax * day) + Gb * (2 * dbx * dby);
-            } else {
-                zx = nx[id];
-                zy = ny[id];
-            }
+            zx = nx[id];
+            zy = ny[id];
+        }
+        M(s, id) = G;
+        M(2 + 2 * s, id) = zx;
+        M(3 + 2 * s, id) = zy;
+        M(6 + 2 * s, id) = qr;
+        M(7 + 2 * s, id) = qi;
+        M(10 + s, id) = E;
+    }
+}
+
+// Every leaf from its vortices, in one launch.
+__global__ void k_moments_leaves(const int32_t* leaf_node, int nleaf, const int32_t* v0, const int32_t* v1,
+                                 const double* nx, const double* ny, const double* x, const double* y, const double* g,
+                                 const double* ie, MomView M) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf) return;
+    leaf_moments(leaf_node[l], v0, v1, nx, ny, x, y, g, ie, M);
+}
+
+// The internal nodes of one tree level from their two children (parallel-axis
+// combination of the second moments); leaves are done beforehand.
+__global__ void k_moments_level(const int32_t* ch1, const int32_t* split, const double* nx, const double* ny,
+                                MomView M, int lb, int le) {
+#pragma clang fp contract(off)
+    const int id = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= le || split[id] != 2) return;
+    for (int s = 0; s < 2; s++) {
+        double G, zx, zy, qr = 0, qi = 0, E;
+        const int a = ch1[id], b = a + 1;
+        const double Ga = M(s, a), Gb = M(s, b);
+        G = Ga + Gb;
+        E = M(10 + s, a) + M(10 + s, b);
+        if (G != 0) {
+            const double ax = M(2 + 2 * s, a), ay = M(3 + 2 * s, a), bx = M(2 + 2 * s, b), by = M(3 + 2 * s, b);
+            zx = (Ga * ax + Gb * bx) / G;
+            zy = (Ga * ay + Gb * by) / G;
+            const double dax = ax - zx, day = ay - zy, dbx = bx - zx, dby = by - zy;
+            qr = M(6 + 2 * s, a) + M(6 + 2 * s, b) + Ga * (dax * dax - day * day) + Gb * (dbx * dbx - dby * dby);
+            qi = M(7 + 2 * s, a) + M(7 + 2 * s, b) + Ga * (2 * dax * day) + Gb * (2 * dbx * dby);
+        } else {
+            zx = nx[id];
+            zy = ny[id];
         }
         M(s, id) = G;
         M(2 + 2 * s, id) = zx;
@@ -1090,10 +1110,15 @@ MomView moments(Ctx& c) {
         c.fm_stride = st;
     }
     MomView M{c.fm.p, c.fm_stride};
-    for (int d = (int)t.levels.size() - 1; d >= 0; d--) {
+    if (t.nleaves_h > 0)
+        hipLaunchKernelGGL(k_moments_leaves, dim3(nblocks(t.nleaves_h)), dim3(kBlock), 0, c.stream, t.leaf_node.p,
+                           t.nleaves_h, t.v0.p, t.v1.p, t.x.p, t.y.p, c.x.p, c.y.p, c.g.p, c.ie.p, M);
+    // internal nodes level by level from the deepest one that has any (the
+    // last level holds leaves only)
+    for (int d = (int)t.levels.size() - 2; d >= 0; d--) {
         const int lb = t.levels[d].first, le = t.levels[d].second;
-        hipLaunchKernelGGL(k_moments_level, dim3(nblocks(le - lb)), dim3(kBlock), 0, c.stream, t.v0.p, t.v1.p, t.ch1.p,
-                           t.split.p, t.x.p, t.y.p, c.x.p, c.y.p, c.g.p, c.ie.p, M, lb, le);
+        hipLaunchKernelGGL(k_moments_level, dim3(nblocks(le - lb)), dim3(kBlock), 0, c.stream, t.ch1.p, t.split.p,
+                           t.x.p, t.y.p, M, lb, le);
     }
     return M;
 }

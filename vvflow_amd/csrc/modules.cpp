@@ -4,6 +4,7 @@
 
 #include "vvhd_modules.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 
@@ -158,15 +159,19 @@ void Space::zero_forces() {
     pull_bodies();
 }
 
+// Space::average_segment_length (TSpace.hpp:119-129): the first body's
+// contour length over its segment count minus one.
 double Space::average_segment_length() const {
-    double len = 0;
-    size_t n = 0;
-    for (const auto& b : BodyList) {
-        const size_t m = b->alist.size();
-        for (size_t k = 0; k < m; k++) len += (b->alist[(k + 1) % m].corner - b->alist[k].corner).abs();
-        n += m;
+    if (BodyList.empty()) return 0;
+    const TBody& b = *BodyList.front();
+    const size_t m = b.alist.size();
+    if (m <= 1) return 0;
+    double len = b.get_slen();
+    if (!(len > 0)) {   // before the first push: from the corners
+        len = 0;
+        for (size_t k = 0; k < m; k++) len += (b.alist[(k + 1) % m].corner - b.alist[k].corner).abs();
     }
-    return n ? len / (double)n : 0;
+    return len / (double)(m - 1);
 }
 
 double Space::gsum() {
@@ -181,6 +186,20 @@ double Space::gmax() {
     double out[5];
     check(vvhd_integrals(ctx_, out));
     return out[4];
+}
+
+double Space::integral() {
+    ensure_pushed();
+    double out[5];
+    check(vvhd_integrals(ctx_, out));
+    return out[1];
+}
+
+TVec Space::hydrodynamic_momentum() {
+    ensure_pushed();
+    double out[5];
+    check(vvhd_integrals(ctx_, out));
+    return TVec(out[2], out[3]);
 }
 
 // ---------------------------------------------------------------- stree --
@@ -209,28 +228,71 @@ void stree::destroy() {
     leaves_valid_ = false;
 }
 
+// Views of every node with its monopoles, the leaves in DFS order and their
+// near/far lists, the nodes of a list found by (depth, path code).
 void stree::load_leaves() const {
     if (leaves_valid_) return;
     if (!built_) throw std::invalid_argument("stree::getBottomNodes(): tree is not built");
-    int32_t nl = 0;
-    S->check(vvhd_tree_leaves(S->ctx_, &nl, nullptr, nullptr, nullptr));
-    std::vector<double> geom(4 * (size_t)nl);
-    std::vector<int32_t> ints(4 * (size_t)nl);
-    std::vector<uint64_t> path(nl);
-    S->check(vvhd_tree_leaves(S->ctx_, &nl, geom.data(), ints.data(), path.data()));
-    leaves_.assign(nl, TSortedNode());
+    vvhd_ctx* ctx = S->ctx_;
+    int32_t nn = 0, nl = 0;
+    S->check(vvhd_tree_nodes(ctx, &nn, nullptr, nullptr, nullptr, nullptr));
+    std::vector<double> geom(4 * (size_t)nn), cm(6 * (size_t)nn);
+    std::vector<int32_t> ints(4 * (size_t)nn);
+    std::vector<uint64_t> path(nn);
+    S->check(vvhd_tree_nodes(ctx, &nn, geom.data(), ints.data(), path.data(), cm.data()));
+    const bool mirror = (int64_t)S->VortexList.size() == vvhd_num_vortices(ctx);
+    nodes_.assign(nn, TSortedNode());
+    std::unordered_map<uint64_t, int32_t> by_key;   // path code with a leading bit at its depth
+    by_key.reserve((size_t)nn * 2);
+    auto key = [](int depth, uint64_t p) { return (depth < 64 ? (1ull << depth) : 0ull) | p; };
+    for (int32_t k = 0; k < nn; k++) {
+        TSortedNode& t = nodes_[k];
+        t.x = geom[4 * k];
+        t.y = geom[4 * k + 1];
+        t.h = geom[4 * k + 2];
+        t.w = geom[4 * k + 3];
+        t.v0 = ints[4 * k];
+        t.v1 = ints[4 * k + 1];
+        t.i = ints[4 * k + 3];
+        t.j = (int)(uint32_t)path[k];   // snode::j wraps at 32 bits
+        t.CMp = TObj(cm[6 * k], cm[6 * k + 1], cm[6 * k + 2]);
+        t.CMm = TObj(cm[6 * k + 3], cm[6 * k + 4], cm[6 * k + 5]);
+        const int c = ints[4 * k + 2];
+        t.ch1 = c >= 0 ? &nodes_[c] : nullptr;
+        t.ch2 = c >= 0 ? &nodes_[c + 1] : nullptr;
+        t.tree = this;
+        if (mirror && t.v1 > t.v0) {
+            t.vRange.first = S->VortexList.data() + t.v0;
+            t.vRange.last = S->VortexList.data() + t.v1;
+        }
+        by_key[key(t.i, path[k])] = k;
+    }
+    S->check(vvhd_tree_leaves(ctx, &nl, nullptr, nullptr, nullptr));
+    std::vector<int32_t> lints(4 * (size_t)nl);
+    std::vector<uint64_t> lpath(nl);
+    S->check(vvhd_tree_leaves(ctx, &nl, nullptr, lints.data(), lpath.data()));
     leaf_ptrs_.resize(nl);
+    near_.assign(nl, {});
+    far_.assign(nl, {});
     for (int32_t l = 0; l < nl; l++) {
-        TSortedNode& t = leaves_[l];
-        t.x = geom[4 * l];
-        t.y = geom[4 * l + 1];
-        t.h = geom[4 * l + 2];
-        t.w = geom[4 * l + 3];
-        t.v0 = ints[4 * l];
-        t.v1 = ints[4 * l + 1];
-        t.i = ints[4 * l + 2];
-        t.j = (int)(uint32_t)path[l];   // snode::j wraps at 32 bits
-        leaf_ptrs_[l] = &t;
+        TSortedNode* t = &nodes_[by_key.at(key(lints[4 * l + 2], lpath[l]))];
+        t->leaf = l;
+        t->NearNodes = &near_[l];
+        t->FarNodes = &far_[l];
+        leaf_ptrs_[l] = t;
+    }
+    for (int far = 0; far < 2; far++) {
+        int64_t cnt = 0;
+        S->check(vvhd_tree_lists(ctx, far, &cnt, nullptr, nullptr, nullptr, nullptr));
+        std::vector<int64_t> off((size_t)nl + 1);
+        std::vector<int32_t> dep((size_t)std::max<int64_t>(cnt, 1));
+        std::vector<uint64_t> pth((size_t)std::max<int64_t>(cnt, 1));
+        S->check(vvhd_tree_lists(ctx, far, &cnt, off.data(), dep.data(), pth.data(), nullptr));
+        auto& lists = far ? far_ : near_;
+        for (int32_t l = 0; l < nl; l++) {
+            lists[l].reserve((size_t)(off[l + 1] - off[l]));
+            for (int64_t e = off[l]; e < off[l + 1]; e++) lists[l].push_back(&nodes_[by_key.at(key(dep[e], pth[e]))]);
+        }
     }
     leaves_valid_ = true;
 }
@@ -246,12 +308,13 @@ const TSortedNode* stree::findNode(TVec p) const {
     int32_t leaf = -1;
     const double xy[2] = {p.x, p.y};
     S->check(vvhd_tree_find(S->ctx_, xy, 1, &leaf));
-    return leaf >= 0 ? &leaves_[leaf] : nullptr;
+    return leaf >= 0 ? leaf_ptrs_[leaf] : nullptr;
 }
 
 void stree::printBottomNodes(FILE* f, bool PrintDepth) const {
     load_leaves();
-    for (const TSortedNode& t : leaves_) {
+    for (const TSortedNode* tp : leaf_ptrs_) {
+        const TSortedNode& t = *tp;
         if (PrintDepth)
             fprintf(f, "%g\t%g\t%g\t%g\t%d\n", t.x, t.y, t.w, t.h, t.i);
         else
@@ -305,6 +368,20 @@ void MEpsilonFast::CalcEpsilonFast(bool merge) {
     merged_ = m;
 }
 
+double MEpsilonFast::near_query(const TSortedNode& node, TVec p, bool h2) {
+    if (!node.tree || node.leaf < 0)
+        throw std::invalid_argument("MEpsilonFast::eps2h/h2: the node is not a leaf of a built tree");
+    Space* S = node.tree->space();
+    const double xy[2] = {p.x, p.y};
+    const int32_t leaf = node.leaf;
+    double out = 0;
+    S->check(h2 ? vvhd_h2(S->ctx_, xy, &leaf, 1, &out) : vvhd_eps2h(S->ctx_, xy, &leaf, 1, &out));
+    return out;
+}
+
+double MEpsilonFast::eps2h(const TSortedNode& node, TVec p) { return near_query(node, p, false); }
+double MEpsilonFast::h2(const TSortedNode& node, TVec p) { return near_query(node, p, true); }
+
 // -------------------------------------------------------- MDiffusiveFast --
 
 void MDiffusiveFast::process_vort_list() {
@@ -331,4 +408,241 @@ void MFlowmove::move_and_clean(bool remove, const void** collision, size_t* clea
 void MFlowmove::vortex_shed() {
     S->check(vvhd_vortex_shed(S->ctx_));
     S->pull_bodies();
+}
+
+// ---------------------------------------------------------------- XField --
+
+XField::XField(float xmin_, float ymin_, float dxdy_, int xres_, int yres_)
+    : xmin(xmin_), ymin(ymin_), dxdy(dxdy_), xres(xres_), yres(yres_) {
+    if (xres <= 1) throw std::invalid_argument("XField(): xres must be > 1");
+    if (yres <= 1) throw std::invalid_argument("XField(): yres must be > 1");
+    if (!std::isfinite(dxdy)) throw std::invalid_argument("XField(): dxdy must be finite");
+    if (dxdy <= 0) throw std::invalid_argument("XField(): dxdy must be positive");
+    map_.assign((size_t)xres * yres, 0.f);
+}
+
+float XField::min() const {
+    if (!evaluated) throw std::invalid_argument("XField::min(): not evaluated");
+    return *std::min_element(map_.begin(), map_.end());
+}
+
+float XField::max() const {
+    if (!evaluated) throw std::invalid_argument("XField::max(): not evaluated");
+    return *std::max_element(map_.begin(), map_.end());
+}
+
+// XField::percentile (XField.cpp:95-121): over the non-zero values.
+float XField::percentile(float p) const {
+    if (!evaluated) throw std::invalid_argument("XField::percentile(): not evaluated");
+    if (!(p > 0 && p < 1)) throw std::invalid_argument("XField::percentile(p): p must be in range (0, 1)");
+    const bool low = p < 0.5f;
+    if (!low) p = 1 - p;
+    std::vector<float> v;
+    for (float x : map_)
+        if (x != 0) v.push_back(x);
+    const size_t N = (size_t)std::floor(v.size() * p);
+    if (low)
+        std::partial_sort(v.begin(), v.begin() + N, v.end(), [](float a, float b) { return a < b; });
+    else
+        std::partial_sort(v.begin(), v.begin() + N, v.end(), [](float a, float b) { return a > b; });
+    return v[N - 1];
+}
+
+// XField's binary matrix (XField.cpp:124-152): N = xres, the x coordinates,
+// then per row its y and the row's values, all float.
+std::ostream& operator<<(std::ostream& os, const XField& field) {
+    if (!field.evaluated) throw std::invalid_argument("XField::operator<<: not evaluated");
+    const float N = (float)field.xres;
+    os.write(reinterpret_cast<const char*>(&N), sizeof(float));
+    for (int xi = 0; xi < field.xres; xi++) {
+        const float x = field.xmin + field.dxdy * xi;
+        os.write(reinterpret_cast<const char*>(&x), sizeof(float));
+    }
+    for (int yj = 0; yj < field.yres; yj++) {
+        const float y = field.ymin + field.dxdy * yj;
+        os.write(reinterpret_cast<const char*>(&y), sizeof(float));
+        os.write(reinterpret_cast<const char*>(field.map_.data() + (size_t)yj * field.xres), sizeof(float) * field.xres);
+    }
+    return os;
+}
+
+vvhd_grid XField::grid() const {
+    vvhd_grid g;
+    g.xmin = xmin;
+    g.ymin = ymin;
+    g.dxdy = dxdy;
+    g.xres = xres;
+    g.yres = yres;
+    return g;
+}
+
+// The evaluators' copy of the Space (the reference's evaluators hold a Space
+// by value): parameters, bodies and the vortex list as the device has them.
+void XField::copy_space(const Space& S, Space& copy) const {
+    Space& src = const_cast<Space&>(S);
+    if (src.pushed_) src.pull();
+    copy.re = S.re;
+    copy.pr = S.pr;
+    copy.inf_g = S.inf_g;
+    copy.inf_vel = S.inf_vel;
+    copy.time = S.time;
+    copy.dt = S.dt;
+    copy.finish = S.finish;
+    copy.BodyList.clear();
+    for (const auto& b : S.BodyList) copy.BodyList.push_back(std::make_shared<TBody>(*b));
+    copy.VortexList = S.VortexList;
+    copy.push();
+}
+
+XVelocity::XVelocity(const Space& S_, double xmin, double ymin, double dxdy, int xres, int yres)
+    : XField((float)xmin, (float)ymin, (float)dxdy, xres, yres), S(S_) {}
+
+// XVelocity::evaluate (XVelocity.cpp:24-73)
+void XVelocity::evaluate() {
+    TVec ref(0, 0);
+    switch (ref_frame) {
+        case 'o': break;
+        case 'f': ref = S.inf_speed(); break;
+        case 'b': ref = S.BodyList.at(0)->speed_slae.r; break;
+        default: throw std::invalid_argument("XVelocity(): bad ref_frame");
+    }
+    if (mode != 'x' && mode != 'y') throw std::invalid_argument("XPressure(): bad mode");
+    if (evaluated) return;
+    Space copy(0);
+    copy_space(S, copy);
+    const double dl = copy.average_segment_length();
+    TSortedTree tree(&copy, 8, dl * 20, 0.1);
+    tree.build();
+    MEpsilonFast eps(&copy, &tree);
+    eps.CalcEpsilonFast(false);
+    const vvhd_grid g = grid();
+    const TVec u = copy.inf_speed();
+    copy.check(vvhd_field(copy.ctx_, mode == 'x' ? VVHD_FIELD_VX : VVHD_FIELD_VY, &g, 1.0, dl, u.x, u.y, 1, ref.x,
+                          ref.y, map_.data()));
+    evaluated = true;
+}
+
+XVorticity::XVorticity(const Space& S_, double xmin, double ymin, double dxdy, int xres, int yres)
+    : XField((float)xmin, (float)ymin, (float)dxdy, xres, yres), S(S_) {}
+
+// XVorticity::evaluate (XVorticity.cpp:26-72)
+void XVorticity::evaluate() {
+    if (eps_mult <= 0) throw std::invalid_argument("XVorticity(): eps_mult must be positive");
+    if (evaluated) return;
+    Space copy(0);
+    copy_space(S, copy);
+    const double dl = copy.average_segment_length();
+    MFlowmove flowmove(&copy);
+    flowmove.vortex_shed();
+    TSortedTree tree(&copy, 8, dl * 20);
+    tree.build();
+    const vvhd_grid g = grid();
+    copy.check(vvhd_field(copy.ctx_, VVHD_FIELD_VORTICITY, &g, eps_mult, dl, 0, 0, 0, 0, 0, map_.data()));
+    evaluated = true;
+}
+
+XStreamfunction::XStreamfunction(const Space& S_, double xmin, double ymin, double dxdy, int xres, int yres)
+    : XField((float)xmin, (float)ymin, (float)dxdy, xres, yres), S(S_) {}
+
+// XStreamfunction::evaluate (XStreamfunction.cpp:47-118)
+void XStreamfunction::evaluate() {
+    if (eps_mult <= 0) throw std::invalid_argument("XStreamfunction(): eps_mult must be positive");
+    TVec ref(0, 0);
+    switch (ref_frame) {
+        case 'o': break;
+        case 'f': ref = S.inf_speed(); break;
+        case 'b': ref = S.BodyList.at(0)->speed_slae.r; break;
+        default: throw std::invalid_argument("XStreamfunction(): bad ref_frame");
+    }
+    if (evaluated) return;
+    Space copy(0);
+    copy_space(S, copy);
+    const double dl = copy.average_segment_length();
+    const double min_node = dl > 0 ? dl * 10 : 0;
+    const double max_node = dl > 0 ? dl * 20 : std::numeric_limits<double>::infinity();
+    TSortedTree tree(&copy, 8, min_node, max_node);
+    tree.build();
+    const vvhd_grid g = grid();
+    const TVec u = copy.inf_speed();
+    copy.check(vvhd_field(copy.ctx_, VVHD_FIELD_STREAMFUNCTION, &g, eps_mult, dl, u.x, u.y, 1, ref.x, ref.y,
+                          map_.data()));
+    evaluated = true;
+}
+
+XPressure::XPressure(const Space& S_, double xmin, double ymin, double dxdy, int xres, int yres)
+    : XField((float)xmin, (float)ymin, (float)dxdy, xres, yres), S(S_) {}
+
+// XPressure::evaluate (XPressure.cpp:32-100)
+void XPressure::evaluate() {
+    if (eps_mult <= 0) throw std::invalid_argument("XPressure(): eps_mult must be positive");
+    TVec ref(0, 0);
+    switch (ref_frame) {
+        case 's': break;
+        case 'o': break;
+        case 'f': ref = S.inf_speed(); break;
+        case 'b': ref = S.BodyList.at(0)->speed_slae.r; break;
+        default: throw std::invalid_argument("XPressure(): bad ref_frame");
+    }
+    if (evaluated) return;
+    Space copy(0);
+    copy_space(S, copy);
+    const double dl = copy.average_segment_length();
+    MFlowmove flowmove(&copy);
+    flowmove.vortex_shed();
+    TSortedTree tree(&copy, 8, dl * 20, 0.1);
+    tree.build();
+    MEpsilonFast eps(&copy, &tree);
+    eps.CalcEpsilonFast(false);
+    MConvectiveFast convective(&copy, &tree);
+    convective.process_all_lists();
+    MDiffusiveFast diffusive(&copy, &tree);
+    diffusive.process_vort_list();
+    const vvhd_grid g = grid();
+    const TVec u = copy.inf_speed();
+    copy.check(vvhd_field(copy.ctx_, VVHD_FIELD_PRESSURE, &g, eps_mult, dl, u.x, u.y, ref_frame != 's', ref.x,
+                          ref.y, map_.data()));
+    evaluated = true;
+}
+
+// --------------------------------------------------------------- sensors --
+
+sensors::sensors(Space* S_, MConvectiveFast* conv_, const char* sensors_file, const char* output)
+    : S(S_), conv(conv_), fout(nullptr), slist() {
+    loadFile(sensors_file);
+    if (slist.size()) fout = fopen(output, "a");
+}
+
+sensors::~sensors() {
+    if (fout) fclose(fout);
+    fout = nullptr;
+}
+
+void sensors::loadFile(const char* file) {
+    if (!file) return;
+    FILE* fin = fopen(file, "r");
+    if (!fin) {
+        fprintf(stderr, "No file called '%s'\n", file);
+        return;
+    }
+    TVec vec(0, 0);
+    while (fscanf(fin, "%lf %lf", &vec.x, &vec.y) == 2) slist.push_back(vec);
+    fclose(fin);
+}
+
+// sensors::output (sensors.cpp:58-69): MConvectiveFast::velocity at every
+// probe, all probes in one device call.
+void sensors::output() {
+    if (!fout) return;
+    (void)conv;
+    std::vector<double> xy(2 * slist.size()), v(2 * slist.size());
+    for (size_t k = 0; k < slist.size(); k++) {
+        xy[2 * k] = slist[k].x;
+        xy[2 * k + 1] = slist[k].y;
+    }
+    const TVec u = S->inf_speed();
+    S->check(vvhd_velocity_points(S->ctx_, xy.data(), (int64_t)slist.size(), u.x, u.y, v.data()));
+    fprintf(fout, "%lg", double(S->time));
+    for (size_t k = 0; k < slist.size(); k++) fprintf(fout, " \t%lg \t%lg", v[2 * k], v[2 * k + 1]);
+    fprintf(fout, "\n");
+    fflush(fout);
 }

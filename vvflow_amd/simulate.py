@@ -37,7 +37,7 @@ class CylinderRun:
         self.ctx.set_params(re=self.re, dt=self.dt, time=0.0)
         self.ctx.upload_bodies(np.ascontiguousarray(segs, dtype=SEG_DTYPE), body)
         self.ctx.upload(np.zeros((0, 6)))
-        dl = self.ctx.download_bodies()[1][0]["slen"] / nseg   # Space::average_segment_length
+        dl = self.ctx.download_bodies()[1][0]["slen"] / (nseg - 1)   # Space::average_segment_length
         self.min_node, self.max_node = 5 * dl, 100 * dl
         self.step_no = 0
         self.time = 0.0
