@@ -39,3 +39,138 @@ def test_cpp_driver_matches_python_driver():
     assert np.array_equal(rows[:, 2].astype(int), np.array(counts))
     assert np.allclose(rows[:, 3:9], f, rtol=1e-9, atol=1e-12 * np.abs(f).max())
     print(f"  n after {steps} steps: {counts[-1]}; force_hydro x {f[-1, 0]:.6f}", flush=True)
+
+
+def read_xfield(path):
+    """XField's binary matrix (XField.cpp:124-152) -> (xs, ys, map)."""
+    a = np.fromfile(path, dtype=np.float32)
+    n = int(a[0])
+    xs = a[1:1 + n]
+    rows = a[1 + n:].reshape(-1, n + 1)
+    return xs, rows[:, 0], rows[:, 1:]
+
+
+def read_state(path):
+    """The state vvflow_cylinder --fields writes before evaluating: vortex
+    records, segment records (16 doubles in vvhd_seg order) and the body's
+    kinematic triples and g_dead."""
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw, np.int64, 1, 0)[0])
+    o = 8
+    vort = np.frombuffer(raw, np.float64, 6 * n, o).reshape(n, 6).copy()
+    o += 48 * n
+    ns = int(np.frombuffer(raw, np.int64, 1, o)[0])
+    o += 8
+    rec = np.frombuffer(raw, np.float64, 16 * ns, o).reshape(ns, 16)
+    o += 128 * ns
+    segs = np.zeros(ns, dtype=V.SEG_DTYPE)
+    for k, name in enumerate(V.SEG_DTYPE.names):
+        segs[name] = rec[:, k]
+    body = np.zeros(1, dtype=V.BODY_DTYPE)
+    body["nseg"] = ns
+    tri = np.frombuffer(raw, np.float64, 18, o).reshape(6, 3)
+    for k, name in enumerate(("holder", "dpos", "speed_slae", "speed_slae_prev", "friction_prev", "fdt_dead")):
+        body[name] = tri[k]
+    body["g_dead"] = np.frombuffer(raw, np.float64, 1, o + 144)[0]
+    return vort, segs, body
+
+
+def field_copy(state, kind, run):
+    """The preparation of the reference's evaluators on a copy of the state
+    (XVelocity.cpp:47-52, XVorticity.cpp:35-44, XStreamfunction.cpp:69-78,
+    XPressure.cpp:55-66), then vvhd_field on the driver's grid (its float
+    origin and step, as XField keeps them)."""
+    vort, segs, body = state
+    c = V.Context(0)
+    try:
+        c.set_params(re=run.re, dt=run.dt, time=run.time)
+        c.upload_bodies(segs, body)
+        c.upload(vort)
+        dl = c.download_bodies()[1][0]["slen"] / (len(segs) - 1)
+        f = float(np.float32(-1.0)), float(np.float32(0.05))
+        grid = (f[0], f[0], f[1], 80, 40)
+        if kind == "velocity":
+            c.tree_build(8, dl * 20, 0.1)
+            c.epsilon(False)
+            return c.field(V.FIELD_VX, *grid, inf=run.inf, ref=(0.0, 0.0))
+        if kind == "vorticity":
+            c.vortex_shed()
+            c.tree_build(8, dl * 20)
+            return c.field(V.FIELD_VORTICITY, *grid, eps_mult=1.0, dl=dl)
+        if kind == "streamfunction":
+            c.tree_build(8, dl * 10, dl * 20)
+            return c.field(V.FIELD_STREAMFUNCTION, *grid, eps_mult=1.0, dl=dl, inf=run.inf, ref=(0.0, 0.0))
+        c.vortex_shed()
+        c.tree_build(8, dl * 20, 0.1)
+        c.epsilon(False)
+        c.convective(run.inf)
+        c.diffusive()
+        return c.field(V.FIELD_PRESSURE, *grid, eps_mult=1.0, dl=dl, inf=run.inf, ref=None)
+    finally:
+        c.close()
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="vvflow_cylinder not built")
+def test_cpp_sensors_fields_and_node_views(tmp_path):
+    """The module classes' sensors, XField evaluators, node views and
+    MEpsilonFast::eps2h/h2 against the same computations through the C ABI
+    on the Python driver's state."""
+    steps = 12
+    print(f"case C++ sensors and fields after {steps} steps", flush=True)
+    pts = np.array([[0.8, 0.05], [1.2, -0.3], [2.0, 0.4], [0.6, -0.6]])
+    pfile = tmp_path / "points.txt"
+    pfile.write_text("".join(f"{x} {y}\n" for x, y in pts))
+    sout = tmp_path / "sensors.txt"
+    prefix = str(tmp_path / "f")
+    r = subprocess.run([EXE, "--steps", str(steps), "--sensors", str(pfile), "--sensors-out", str(sout),
+                        "--fields", prefix], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    ctx = V.Context(0)
+    try:
+        run = CylinderRun(ctx)
+        run.sensor_points = pts
+        for _ in range(steps):
+            run.step()
+        # sensors: time then (u, v) per point, every step, printed with %lg
+        # (six significant digits) as sensors::output does
+        rows = np.array([[float(x) for x in ln.split()] for ln in sout.read_text().splitlines()])
+        assert rows.shape == (steps, 1 + 2 * len(pts))
+        want = np.array([s.reshape(-1) for s in run.sensor_rows])
+        assert np.allclose(rows[:, 1:], want, rtol=1e-5, atol=1e-6)
+        assert np.allclose(rows[:, 0], 0.005 * np.arange(steps), rtol=1e-5, atol=1e-9)
+        # fields through the classes == fields through the C ABI from the
+        # same state (the driver's, written before its evaluators ran)
+        state = read_state(f"{prefix}_state.bin")
+        assert len(state[0]) == ctx.n
+        for kind in ("velocity", "vorticity", "streamfunction", "pressure"):
+            xs, ys, m = read_xfield(f"{prefix}_{kind}.bin")
+            assert m.shape == (40, 80)
+            assert np.allclose(xs, np.float32(-1.0) + np.float32(0.05) * np.arange(80, dtype=np.float32))
+            g = field_copy(state, kind, run)
+            scale = float(np.abs(g).max())
+            err = float(np.abs(m - g).max()) / scale
+            print(f"  {kind}: max |class - abi| / max = {err:.2e}", flush=True)
+            assert scale > 0 and err < 1e-6
+        # node views and the static eps2h / h2 at the driver's probe points
+        ctx.tree_build(8, run.min_node, run.max_node)
+        lv = ctx.tree_leaves()
+        nl = ctx.tree_lists(False)
+        fl = ctx.tree_lists(True)
+        probe = np.array([[0.7, 0.1], [1.5, -0.2], [0.0, 0.55]])
+        leaf = ctx.tree_find(probe)
+        e2 = ctx.eps2h(probe, leaf=leaf)
+        h2 = ctx.h2(probe, leaf=leaf)
+        lines = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("# node")]
+        assert len(lines) == 3 and len(lv["geom"]) > 0
+        for k, ln in enumerate(lines):
+            assert float(ln[5]) == pytest.approx(e2[k], rel=1e-9)
+            assert float(ln[7]) == pytest.approx(h2[k], rel=1e-9)
+            assert int(ln[9]) == nl["off"][leaf[k] + 1] - nl["off"][leaf[k]]
+            assert int(ln[11]) == fl["off"][leaf[k] + 1] - fl["off"][leaf[k]]
+        gs, integ, mx, my, _ = ctx.integrals()
+        ln = [x for x in r.stdout.splitlines() if x.startswith("# integrals")][0].split()
+        assert float(ln[3]) == pytest.approx(gs, rel=1e-9, abs=1e-12)
+        assert float(ln[5]) == pytest.approx(integ, rel=1e-9)
+        assert float(ln[7]) == pytest.approx(mx, rel=1e-9) and float(ln[8]) == pytest.approx(my, rel=1e-9)
+    finally:
+        ctx.close()

@@ -5,9 +5,20 @@
 //
 //   vvflow_cylinder [--steps 400] [--re 600] [--dt 0.005] [--nseg 500]
 //                   [--learned model.npy] [--far-criteria 8] [--every 1]
+//                   [--sensors points.txt --sensors-out out.txt]
+//                   [--fields PREFIX]
 //
 // Prints one line per `every` steps: step, time, vortex count, force_hydro
 // (x, y, o), friction (x, y, o), step wall time in ms; then a summary line.
+// --sensors appends the velocity at the listed points every step, after the
+// convective velocities (utils/vvflow/vvflow.cpp:248, sensors::output).  --fields evaluates, after
+// the last step, XVelocity ('x', frame 'o'), XVorticity, XStreamfunction
+// (frame 'o') and XPressure (frame 's') with eps_mult 1 on the grid
+// x in [-1, 3), y in [-1, 1) step 0.05, written with XField's operator<< to
+// PREFIX_{velocity,vorticity,streamfunction,pressure}.bin (and the state
+// they start from to PREFIX_state.bin), and prints
+// MEpsilonFast::eps2h / h2 at three points with the near and far list sizes
+// of their leaves.
 
 #include <algorithm>
 #include <chrono>
@@ -16,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -60,7 +72,7 @@ std::vector<float> load_npy_f32(const std::string& path) {
 int main(int argc, char** argv) {
     int steps = 400, nseg = 500, fc = 8, every = 1;
     double re = 600, dt = 0.005, radius = 0.5;
-    std::string learned;
+    std::string learned, sensors_in, sensors_out, fields;
     for (int k = 1; k < argc; k++) {
         const std::string a = argv[k];
         auto next = [&]() -> const char* {
@@ -74,6 +86,9 @@ int main(int argc, char** argv) {
         else if (a == "--far-criteria") fc = std::atoi(next());
         else if (a == "--learned") learned = next();
         else if (a == "--every") every = std::atoi(next());
+        else if (a == "--sensors") sensors_in = next();
+        else if (a == "--sensors-out") sensors_out = next();
+        else if (a == "--fields") fields = next();
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -101,6 +116,10 @@ int main(int argc, char** argv) {
         MFlowmove flowmove(&S);
         if (!learned.empty()) convective.set_learned(load_npy_f32(learned));
         const bool is_viscous = std::isfinite(S.re);
+        std::unique_ptr<sensors> probes;
+        if (!sensors_in.empty())
+            probes.reset(new sensors(&S, &convective, sensors_in.c_str(),
+                                     sensors_out.empty() ? "sensors.txt" : sensors_out.c_str()));
 
         const void* collision = nullptr;
         double total_ms = 0;
@@ -121,6 +140,7 @@ int main(int argc, char** argv) {
             tr.build();
             epsilon.CalcEpsilonFast(is_viscous);
             convective.process_all_lists();
+            if (probes) probes->output();   // vvflow.cpp:248, on this step's tree
             if (is_viscous) {
                 diffusive.process_vort_list();
                 diffusive.process_heat_list();
@@ -138,6 +158,67 @@ int main(int argc, char** argv) {
         }
         printf("# steps %d vortices %lld total %.1f ms mean %.3f ms/step\n", steps, (long long)S.vortex_count(),
                total_ms, total_ms / std::max(1, steps));
+        if (!fields.empty()) {
+            // the state the evaluators copy: vortices (6 doubles each), then
+            // per segment its vvhd_seg fields as 16 doubles, then the body's
+            // holder, dpos, speed_slae, speed_slae_prev, friction_prev and
+            // fdt_dead triples and g_dead
+            S.pull();
+            std::ofstream st(fields + "_state.bin", std::ios::binary);
+            const int64_t nv = (int64_t)S.VortexList.size();
+            st.write(reinterpret_cast<const char*>(&nv), 8);
+            st.write(reinterpret_cast<const char*>(S.VortexList.data()), nv * (int64_t)sizeof(TObj));
+            const TBody& bd = *S.BodyList[0];
+            const int64_t ns = (int64_t)bd.alist.size();
+            st.write(reinterpret_cast<const char*>(&ns), 8);
+            for (const TAtt& a : bd.alist) {
+                const double rec[16] = {a.r.x,  a.r.y,  a.g,  a._1_eps, a.corner.x, a.corner.y,     a.dl.x,
+                                        a.dl.y, a.gsum, a.hsum, a.fric, a.Cp,      a.Fr,       a.Nu,
+                                        (double)a.slip, (double)a.eq_no};
+                st.write(reinterpret_cast<const char*>(rec), sizeof(rec));
+            }
+            const TVec3D* v3[6] = {&bd.holder, &bd.dpos, &bd.speed_slae, &bd.speed_slae_prev, &bd.friction_prev,
+                                   &bd.fdt_dead};
+            for (const TVec3D* v : v3) {
+                const double t[3] = {v->r.x, v->r.y, v->o};
+                st.write(reinterpret_cast<const char*>(t), sizeof(t));
+            }
+            st.write(reinterpret_cast<const char*>(&bd.g_dead), 8);
+            st.close();
+            XVelocity fv(S, -1, -1, 0.05, 80, 40);
+            fv.mode = 'x';
+            fv.ref_frame = 'o';
+            XVorticity fw(S, -1, -1, 0.05, 80, 40);
+            fw.eps_mult = 1;
+            XStreamfunction fs(S, -1, -1, 0.05, 80, 40);
+            fs.eps_mult = 1;
+            fs.ref_frame = 'o';
+            XPressure fp(S, -1, -1, 0.05, 80, 40);
+            fp.eps_mult = 1;
+            fp.ref_frame = 's';
+            XField* all[4] = {&fv, &fw, &fs, &fp};
+            const char* names[4] = {"velocity", "vorticity", "streamfunction", "pressure"};
+            for (int k = 0; k < 4; k++) {
+                if (k == 0) fv.evaluate();
+                if (k == 1) fw.evaluate();
+                if (k == 2) fs.evaluate();
+                if (k == 3) fp.evaluate();
+                std::ofstream out(fields + "_" + names[k] + ".bin", std::ios::binary);
+                out << *all[k];
+                printf("# field %s min %.9g max %.9g\n", names[k], all[k]->min(), all[k]->max());
+            }
+            tr.build();
+            const TVec probe[3] = {TVec(0.7, 0.1), TVec(1.5, -0.2), TVec(0.0, 0.55)};
+            for (const TVec& p : probe) {
+                const TSortedNode* nd = tr.findNode(p);
+                printf("# node %.17g %.17g eps2h %.17g h2 %.17g near %zu far %zu\n", p.x, p.y,
+                       MEpsilonFast::eps2h(*nd, p), MEpsilonFast::h2(*nd, p), nd->NearNodes->size(),
+                       nd->FarNodes->size());
+            }
+            printf("# integrals gsum %.17g integral %.17g momentum %.17g %.17g\n", S.gsum(), S.integral(),
+                   S.hydrodynamic_momentum().x, S.hydrodynamic_momentum().y);
+            tr.destroy();
+        }
     } catch (const std::exception& e) {
         fprintf(stderr, "vvflow_cylinder: %s\n", e.what());
         return 1;

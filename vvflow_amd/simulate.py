@@ -43,6 +43,10 @@ class CylinderRun:
         self.time = 0.0
         self.forces = []
         self.phase_ms = np.zeros(8)
+        # velocity probes (utils/vvflow/sensors.cpp): points whose convective
+        # velocity is recorded every step right after the convective call
+        self.sensor_points = None
+        self.sensor_rows = []
 
     def step(self):
         c = self.ctx
@@ -61,6 +65,8 @@ class CylinderRun:
             c.convective_learned(self.inf, kahan=self.kahan)
         else:
             c.convective(self.inf, kahan=self.kahan)
+        if self.sensor_points is not None:
+            self.sensor_rows.append(c.velocity(self.sensor_points, self.inf))
         c.diffusive()
         c.tree_destroy()
         c.move_and_clean(True)
