@@ -49,6 +49,7 @@ void init_space(Harness* h) {
     new (&S->StreakSourceList) std::vector<TObj>();
     new (&S->StreakList) std::vector<TObj>();
     S->re = 0;
+    S->pr = 0;
 }
 
 }  // namespace
@@ -80,6 +81,50 @@ void vvr_destroy(void* p) {
 }
 
 void vvr_set_re(void* p, double re) { static_cast<Harness*>(p)->S->re = re; }
+void vvr_set_pr(void* p, double pr) { static_cast<Harness*>(p)->S->pr = pr; }
+
+static std::vector<TObj>* list_of(Space* S, int list) {
+    switch (list) {
+        case 0: return &S->VortexList;
+        case 1: return &S->HeatList;
+        case 2: return &S->StreakList;
+        default: return nullptr;
+    }
+}
+
+// list: 0 VortexList, 1 HeatList, 2 StreakList; records of 6 doubles.
+int vvr_set_list(void* p, int list, const double* o6, int64_t n) {
+    std::vector<TObj>* L = list_of(static_cast<Harness*>(p)->S, list);
+    if (!L) return 1;
+    L->clear();
+    for (int64_t i = 0; i < n; i++) {
+        TObj o(o6[6 * i], o6[6 * i + 1], o6[6 * i + 2]);
+        o.v = TVec(o6[6 * i + 3], o6[6 * i + 4]);
+        o._1_eps = o6[6 * i + 5];
+        L->push_back(o);
+    }
+    return 0;
+}
+
+int64_t vvr_list_size(void* p, int list) {
+    std::vector<TObj>* L = list_of(static_cast<Harness*>(p)->S, list);
+    return L ? (int64_t)L->size() : -1;
+}
+
+int vvr_get_list(void* p, int list, double* o6) {
+    std::vector<TObj>* L = list_of(static_cast<Harness*>(p)->S, list);
+    if (!L) return 1;
+    for (size_t i = 0; i < L->size(); i++) {
+        const TObj& o = (*L)[i];
+        o6[6 * i] = o.r.x;
+        o6[6 * i + 1] = o.r.y;
+        o6[6 * i + 2] = o.g;
+        o6[6 * i + 3] = o.v.x;
+        o6[6 * i + 4] = o.v.y;
+        o6[6 * i + 5] = o._1_eps;
+    }
+    return 0;
+}
 
 void vvr_set_vortices(void* p, const double* o6, int64_t n) {
     Space* S = static_cast<Harness*>(p)->S;
@@ -182,6 +227,21 @@ void vvr_leaves(void* p, double* geom, int32_t* ints) {
     }
 }
 
+// Per leaf (DFS order): hRange and sRange as list indices {h0, h1, s0, s1}.
+void vvr_leaves_hs(void* p, int32_t* out) {
+    Harness* h = static_cast<Harness*>(p);
+    const TObj* hb = h->S->HeatList.data();
+    const TObj* sb = h->S->StreakList.data();
+    const auto& bn = h->tree->getBottomNodes();
+    for (size_t l = 0; l < bn.size(); l++) {
+        const snode* n = bn[l];
+        out[4 * l + 0] = n->hRange.first ? (int32_t)(n->hRange.first - hb) : 0;
+        out[4 * l + 1] = n->hRange.last ? (int32_t)(n->hRange.last - hb) : 0;
+        out[4 * l + 2] = n->sRange.first ? (int32_t)(n->sRange.first - sb) : 0;
+        out[4 * l + 3] = n->sRange.last ? (int32_t)(n->sRange.last - sb) : 0;
+    }
+}
+
 // Flattened lists in leaf DFS order.  near/far entries: {depth, path_code};
 // segs: global segment index.
 void vvr_lists(void* p, int32_t* near_ij, int32_t* far_ij, int32_t* segs) {
@@ -233,6 +293,12 @@ void vvr_diffusive(void* p) {
     Harness* h = static_cast<Harness*>(p);
     MDiffusiveFast d(h->S, h->tree);
     d.process_vort_list();
+}
+
+void vvr_diffusive_heat(void* p) {
+    Harness* h = static_cast<Harness*>(p);
+    MDiffusiveFast d(h->S, h->tree);
+    d.process_heat_list();
 }
 
 }  // extern "C"

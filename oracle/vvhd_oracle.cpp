@@ -92,6 +92,8 @@ struct Body {         // TBody (TBody.hpp:67-158), rigid and static
     bool slip = false;
     double slen = 0, area = 0, moi_cofm = 0, min_disc_r2 = 0;
     V2 cofm{0, 0}, rect_bl{0, 0}, rect_tr{0, 0};
+    int heat_cond = 0;             // hc_t: neglect, isolate, const_t, const_w (TBody.hpp:15)
+    std::vector<V2> heat_layer;    // r + rotl(dl) per segment, made on first use (TBody.cpp:222-233)
     V2 axis() const { return V2{holder[0], holder[1]} + V2{dpos[0], dpos[1]}; }
     double moi_axis() const { return moi_cofm + abs2(axis() - cofm) * area; }
     static bool isrigid(double k) { return !(k >= 0 && std::isfinite(k)); }
@@ -105,6 +107,7 @@ struct Node {
     double x = 0, y = 0, h = 0, w = 0;
     int depth = 0;
     int v0 = 0, v1 = 0;
+    int h0 = 0, h1 = 0, s0 = 0, s1 = 0;   // hRange, sRange (TSortedTree.hpp:28-29)
     std::vector<int> segs;
     CM cmp{0, 0, 0, kInf}, cmm{0, 0, 0, kInf};   // snode ctor, TSortedTree.cpp:25
     int ch1 = -1, parent = -1;
@@ -117,7 +120,11 @@ struct vvo_ctx {
     std::string err;
     double re = kInf, dt = 1, inf_vx = 0, inf_vy = 0, inf_g = 0, time = 0;
     std::vector<Obj> vort;
+    std::vector<Obj> heat, streak, streak_src;   // HeatList, StreakList, StreakSourceList (TSpace.hpp:40-43)
+    double pr = 0;                               // Space::pr
     std::vector<Seg> segs;
+    std::vector<float> heat_const;               // TAtt::heat_const per segment
+    std::vector<int64_t> heat_layer_obj;         // TAtt::heat_layer_obj_no per segment
     std::vector<Body> bodies;
     // tree
     bool built = false;
@@ -200,14 +207,17 @@ void fill_properties(vvo_ctx* c, Body& b) {
     }
 }
 
-// TBody::isPointInContour over the corner list (TBody.cpp:238-281)
-int point_invalid(vvo_ctx* c, const Body& b, V2 p) {
+// TBody::isPointInContour (TBody.cpp:238-281) over the segment corners or,
+// with poly, over another contour of b.n points (the heat layer); the
+// pre-test uses the body's own box and disc in both cases.
+int point_in_contour(vvo_ctx* c, const Body& b, V2 p, const std::vector<V2>* poly) {
     bool in = b.area <= 0;   // isInsideValid
     if (!in && (p.x < b.rect_bl.x || p.y < b.rect_bl.y || p.x > b.rect_tr.x ||
                 p.y > b.rect_tr.y || abs2(p - b.cofm) > b.min_disc_r2))
         return -1;
     for (int i = 0, j = b.n - 1; i < b.n; j = i++) {
-        V2 vi = c->segs[b.first + i].corner(), vj = c->segs[b.first + j].corner();
+        V2 vi = poly ? (*poly)[i] : c->segs[b.first + i].corner();
+        V2 vj = poly ? (*poly)[j] : c->segs[b.first + j].corner();
         if (((vi.y < vj.y) && (vi.y < p.y) && (p.y <= vj.y) &&
              ((vj.y - vi.y) * (p.x - vi.x) > (vj.x - vi.x) * (p.y - vi.y))) ||
             ((vi.y > vj.y) && (vi.y > p.y) && (p.y >= vj.y) &&
@@ -225,6 +235,19 @@ int point_invalid(vvo_ctx* c, const Body& b, V2 p) {
         }
     }
     return best;
+}
+
+int point_invalid(vvo_ctx* c, const Body& b, V2 p) { return point_in_contour(c, b, p, nullptr); }
+
+// TBody::isPointInHeatLayer (TBody.cpp:222-233): the layer is made once, from
+// the segments as they are then, and kept.
+int point_in_heat_layer(vvo_ctx* c, Body& b, V2 p) {
+    if (b.heat_layer.empty())
+        for (int k = 0; k < b.n; k++) {
+            const Seg& s = c->segs[b.first + k];
+            b.heat_layer.push_back(s.r() + rotl(s.dl()));
+        }
+    return point_in_contour(c, b, p, &b.heat_layer);
 }
 
 // ------------------------------------------------------------------ tree --
@@ -258,13 +281,18 @@ void stretch(vvo_ctx* c, Node& nd) {
         bl.x = std::min(bl.x, q.x);
         bl.y = std::min(bl.y, q.y);
     }
-    for (int k = nd.v0; k < nd.v1; k++) {
-        const Obj& o = c->vort[k];
-        tr.x = std::max(tr.x, o.x);
-        tr.y = std::max(tr.y, o.y);
-        bl.x = std::min(bl.x, o.x);
-        bl.y = std::min(bl.y, o.y);
-    }
+    auto range = [&](const std::vector<Obj>& list, int k0, int k1) {
+        for (int k = k0; k < k1; k++) {
+            const Obj& o = list[k];
+            tr.x = std::max(tr.x, o.x);
+            tr.y = std::max(tr.y, o.y);
+            bl.x = std::min(bl.x, o.x);
+            bl.y = std::min(bl.y, o.y);
+        }
+    };
+    range(c->vort, nd.v0, nd.v1);
+    range(c->heat, nd.h0, nd.h1);
+    range(c->streak, nd.s0, nd.s1);
     nd.x = (bl + tr).x * 0.5;
     nd.y = (bl + tr).y * 0.5;
     nd.h = (tr - bl).y;
@@ -281,7 +309,8 @@ void divide(vvo_ctx* c, int id) {
             c->leaves.push_back(id);
             return;
         }
-        long maxlist = std::max<long>((long)nd.segs.size(), (long)(nd.v1 - nd.v0));
+        long maxlist = std::max<long>({(long)nd.segs.size(), (long)(nd.v1 - nd.v0), (long)(nd.h1 - nd.h0),
+                                       (long)(nd.s1 - nd.s0)});
         if (std::max(nd.h, nd.w) < c->max_node && maxlist < kLeafMaxList) {
             nd.leaf_index = (int)c->leaves.size();
             c->leaves.push_back(id);
@@ -303,6 +332,16 @@ void divide(vvo_ctx* c, int id) {
     a.v1 = split;
     b.v0 = split;
     b.v1 = nd.v1;
+    split = hoare_split(c->heat.data(), nd.h0, nd.h1, by_x, nd.x, nd.y);
+    a.h0 = nd.h0;
+    a.h1 = split;
+    b.h0 = split;
+    b.h1 = nd.h1;
+    split = hoare_split(c->streak.data(), nd.s0, nd.s1, by_x, nd.x, nd.y);
+    a.s0 = nd.s0;
+    a.s1 = split;
+    b.s0 = split;
+    b.s1 = nd.s1;
     for (int s : nd.segs) {   // TSortedTree.cpp:139-148 (stable)
         if (goes_left(by_x, c->segs[s].x, c->segs[s].y, nd.x, nd.y))
             a.segs.push_back(s);
@@ -468,6 +507,58 @@ double epsv(EpsCtx& e, int leaf_node, int i, double crit_sq) {
     if ((r1 < crit_sq) || ((s1 == s2) && (s1 != st))) {
         merge_pair(e, i, n1);
         return epsv(e, leaf_node, i, kNaN);
+    }
+    return std::sqrt(r2);
+}
+
+// MergeVortexes (MEpsilonFast.cpp:111-126) on two heat particles.
+void merge_heat(EpsCtx& e, int i, int j) {
+    Obj* a = &e.c->heat[i];
+    Obj* b = &e.c->heat[j];
+    e.merged++;
+    if (sgn(a->g) == sgn(b->g)) {
+        double inv = 1. / (a->g + b->g);
+        double nx = (a->x * a->g + b->x * b->g) * inv;
+        double ny = (a->y * a->g + b->y * b->g) * inv;
+        a->x = nx;
+        a->y = ny;
+    } else if (std::fabs(a->g) < std::fabs(b->g)) {
+        a->x = b->x;
+        a->y = b->y;
+    }
+    a->g += b->g;
+    b->g = 0;
+}
+
+// MEpsilonFast::epsh (MEpsilonFast.cpp:175-212): the heat particles of the
+// near leaves; one candidate returns sqrt(d1) before the merge test, which
+// is d1 < crit only (no sign rule).
+double epsh(EpsCtx& e, int leaf_node, int i, double crit_sq) {
+    vvo_ctx* c = e.c;
+    double r1 = std::numeric_limits<double>::max(), r2 = r1;
+    int n1 = -1;
+    const Obj& t = c->heat[i];
+    const Node& L = c->nodes[leaf_node];
+    for (int m : c->near[L.leaf_index]) {
+        const Node& M = c->nodes[m];
+        for (int j = M.h0; j < M.h1; j++) {
+            const Obj& s = c->heat[j];
+            if (!s.g || j == i) continue;
+            double d2 = abs2(V2{t.x, t.y} - V2{s.x, s.y});
+            if (r1 > d2) {
+                r2 = r1;
+                r1 = d2;
+                n1 = j;
+            } else if (r2 > d2) {
+                r2 = d2;
+            }
+        }
+    }
+    if (r1 == std::numeric_limits<double>::max()) return std::numeric_limits<double>::min();
+    if (r2 == std::numeric_limits<double>::max()) return std::sqrt(r1);
+    if (r1 < crit_sq) {
+        merge_heat(e, i, n1);
+        return epsh(e, leaf_node, i, kNaN);
     }
     return std::sqrt(r2);
 }
@@ -638,6 +729,21 @@ void convective_leaf(vvo_ctx* c, int leaf_idx) {
         addto(v, V2{T1, T2});
         addto(v, V2{dot(V2{T3, T4}, drl), dot(V2{T4, -T3}, drl)});
     }
+    // the heat (g != 0) and streak ranges get the same terms (:87-110)
+    auto passive = [&](Obj& o) {
+        V2 p{o.x, o.y};
+        V2 drl = p - centre;
+        double* v = &o.vx;
+        addto(v, inf);
+        addto(v, near_influence(c, L, p));
+        addto(v, V2{0, 0} * k1_2Pi);
+        addto(v, body_influence(c, p));
+        addto(v, V2{T1, T2});
+        addto(v, V2{dot(V2{T3, T4}, drl), dot(V2{T4, -T3}, drl)});
+    };
+    for (int i = L.h0; i < L.h1; i++)
+        if (c->heat[i].g) passive(c->heat[i]);
+    for (int i = L.s0; i < L.s1; i++) passive(c->streak[i]);
 }
 
 // ------------------------------------------------------------- diffusive --
@@ -697,6 +803,34 @@ void diffusive_leaf(vvo_ctx* c, int leaf_idx, double* fric) {
         o.vy += t.y;
         if (S0 > kPi) S0 = kPi;
         t = S3 * (sq(o.inv_eps) / (re * (k2Pi - S0)));
+        o.vx += t.x;
+        o.vy += t.y;
+    }
+}
+
+// Per-leaf body of MDiffusiveFast::process_heat_list (MDiffusiveFast.cpp:
+// 50-89): heat particles of the near leaves and their segments, no friction.
+void heat_diffusive_leaf(vvo_ctx* c, int leaf_idx) {
+    const Node& L = c->nodes[c->leaves[leaf_idx]];
+    const double re = c->re, pr = c->pr;
+    for (int i = L.h0; i < L.h1; i++) {
+        Obj& o = c->heat[i];
+        if (!o.g) continue;
+        V2 S2{0, 0}, S3{0, 0};
+        double S1 = 0, S0 = 0;
+        double unused = 0;
+        for (int m : c->near[leaf_idx]) {
+            const Node& M = c->nodes[m];
+            for (int j = M.h0; j < M.h1; j++) {
+                const Obj& s = c->heat[j];
+                if (!s.g) continue;
+                vortex_influence(o, s, S2, S1);
+            }
+            for (int sidx : M.segs) segment_influence(o, c->segs[sidx], S3, S0, &unused);
+        }
+        if ((sgn(S1) != sgn(o.g)) || (std::fabs(S1) < std::fabs(0.1 * o.g))) S1 = 0.1 * o.g;
+        V2 num = V2{S2.x * o.inv_eps, S2.y * o.inv_eps} + S3;
+        V2 t = num / (S1 + S0 / sq(o.inv_eps)) / (pr * re);
         o.vx += t.x;
         o.vy += t.y;
     }
@@ -1021,6 +1155,8 @@ int vvo_add_body(vvo_ctx* c, const vvo_corner* corners, int32_t n) {
         s.cy = corners[k].y;
         s.slip = corners[k].slip;
         c->segs.push_back(s);
+        c->heat_const.push_back(0.f);
+        c->heat_layer_obj.push_back(-1);   // TAtt constructor (TBody.hpp:38-48)
     }
     c->bodies.push_back(b);
     Body& bb = c->bodies.back();
@@ -1106,6 +1242,10 @@ int vvo_tree_build(vvo_ctx* c, int far_criteria, double min_node, double max_nod
         for (int k = 0; k < b.n; k++) root.segs.push_back(b.first + k);
     root.v0 = 0;
     root.v1 = (int)c->vort.size();
+    root.h0 = 0;
+    root.h1 = (int)c->heat.size();
+    root.s0 = 0;
+    root.s1 = (int)c->streak.size();
     stretch(c, c->nodes[0]);
     divide(c, 0);
     calc_cmass(c, 0);
@@ -1213,6 +1353,10 @@ static void epsilon_leaves(vvo_ctx* c, EpsCtx& e, int merge, size_t leaf_begin, 
             c->vort[i].inv_eps = 1.0 / std::max(eps, restriction);
             if (nn1) nn1[i] = e.last_n1;
             if (nn2) nn2[i] = e.last_n2;
+        }
+        for (int i = L.h0; i < L.h1; i++) {
+            if (!c->heat[i].g) continue;
+            c->heat[i].inv_eps = 1.0 / std::max(epsh(e, leaf, i, crit), restriction);
         }
     }
 }
@@ -1570,14 +1714,21 @@ int vvo_move_and_clean(vvo_ctx* c, int remove, int64_t* cleaned) {
         update_segments(c, b);
         fill_properties(c, b);
     }
-    for (Obj& o : c->vort) {
-        o.x += o.vx * dt;
-        o.y += o.vy * dt;
-    }
+    for (std::vector<Obj>* list : {&c->vort, &c->heat, &c->streak})
+        for (Obj& o : *list) {
+            o.x += o.vx * dt;
+            o.y += o.vy * dt;
+        }
     std::vector<Obj> keep;
     keep.reserve(c->vort.size());
     for (const Obj& o : c->vort)
         if (!(std::fabs(o.g) < kRemoveEps)) keep.push_back(o);
+    {
+        std::vector<Obj> hk;
+        for (const Obj& o : c->heat)
+            if (!(std::fabs(o.g) < kRemoveEps)) hk.push_back(o);
+        c->heat.swap(hk);
+    }
     if (remove) {
         std::vector<Obj> keep2;
         keep2.reserve(keep.size());
@@ -1606,10 +1757,57 @@ int vvo_move_and_clean(vvo_ctx* c, int remove, int64_t* cleaned) {
         keep.swap(keep2);
     }
     c->vort.swap(keep);
-    for (Obj& o : c->vort) {
-        o.vx = 0.;
-        o.vy = 0.;
+    // heat particles (MFlowmove.cpp:144-186): inside a body -> the nearest
+    // segment's hsum gives their heat back; in the heat layer of a const_t
+    // body -> the first one of a segment stays as its layer particle, later
+    // ones are absorbed
+    {
+        std::vector<Obj> hk;
+        hk.reserve(c->heat.size());
+        for (const Obj& o : c->heat) {
+            const V2 p{o.x, o.y};
+            int bad = -1;
+            for (int b = 0; b < (int)c->bodies.size() && bad < 0; b++) bad = point_invalid(c, c->bodies[b], p);
+            if (bad >= 0) {
+                c->segs[bad].hsum -= o.g;
+                continue;
+            }
+            int lay = -1, lb = -1;
+            for (int b = 0; b < (int)c->bodies.size(); b++) {
+                lay = point_in_heat_layer(c, c->bodies[b], p);
+                if (lay >= 0) {
+                    lb = b;
+                    break;
+                }
+            }
+            if (lb < 0 || c->bodies[lb].heat_cond != 2) {
+                hk.push_back(o);
+                continue;
+            }
+            if (c->heat_layer_obj[lay] >= 0) {
+                c->segs[lay].hsum -= o.g;
+                continue;
+            }
+            c->heat_layer_obj[lay] = (int64_t)hk.size();
+            hk.push_back(o);
+        }
+        c->heat.swap(hk);
     }
+    {   // streak particles inside a body (MFlowmove.cpp:188-192)
+        std::vector<Obj> sk;
+        sk.reserve(c->streak.size());
+        for (const Obj& o : c->streak) {
+            bool in = false;
+            for (int b = 0; b < (int)c->bodies.size() && !in; b++) in = point_invalid(c, c->bodies[b], V2{o.x, o.y}) >= 0;
+            if (!in) sk.push_back(o);
+        }
+        c->streak.swap(sk);
+    }
+    for (std::vector<Obj>* list : {&c->vort, &c->heat, &c->streak})
+        for (Obj& o : *list) {
+            o.vx = 0.;
+            o.vy = 0.;
+        }
     for (Body& b : c->bodies) {
         V2 axis = b.axis();
         for (int k = 0; k < b.n; k++) {
@@ -1643,6 +1841,134 @@ int vvo_vortex_shed(vvo_ctx* c) {
         }
     }
     return 0;
+}
+
+// MFlowmove::heat_shed (MFlowmove.cpp:237-273)
+int vvo_heat_shed(vvo_ctx* c) {
+    c->built = false;
+    for (Body& b : c->bodies) {
+        for (int k = 0; k < b.n; k++) {
+            Seg& s = c->segs[b.first + k];
+            const V2 p = s.r() + rotl(s.dl()) * 0.5;
+            if (b.heat_cond == 1) {   // isolate
+                if (!s.hsum) continue;
+                c->heat.push_back(Obj{p.x, p.y, -s.hsum, 0, 0, 0});
+                s.hsum = 0;
+            } else if (b.heat_cond == 2) {   // const_t
+                const double tg = abs2(s.dl()) * c->heat_const[b.first + k];
+                const int64_t no = c->heat_layer_obj[b.first + k];
+                if (no >= 0) {
+                    if (no >= (int64_t)c->heat.size()) {
+                        c->err = "heat_shed: heat layer particle index beyond the heat list";
+                        return 1;
+                    }
+                    s.hsum += tg - c->heat[no].g;
+                    c->heat[no].g = tg;
+                } else {
+                    s.hsum += tg;
+                    c->heat.push_back(Obj{p.x, p.y, tg, 0, 0, 0});
+                }
+            } else if (b.heat_cond == 3) {   // const_w
+                const double tg = abs2(s.dl()) * c->heat_const[b.first + k];
+                s.hsum += tg;
+                c->heat.push_back(Obj{p.x, p.y, tg, 0, 0, 0});
+            }
+        }
+    }
+    return 0;
+}
+
+// MFlowmove::heat_crop (MFlowmove.cpp:275-282)
+int vvo_heat_crop(vvo_ctx* c, double scale) {
+    c->built = false;
+    std::vector<Obj> hk;
+    for (const Obj& o : c->heat)
+        if (!(o.x > scale)) hk.push_back(o);
+    c->heat.swap(hk);
+    return 0;
+}
+
+// MFlowmove::streak_shed (MFlowmove.cpp:284-291) once the caller has found
+// the time divisible by dt_streak.
+int vvo_streak_shed(vvo_ctx* c) {
+    c->built = false;
+    for (const Obj& o : c->streak_src) c->streak.push_back(o);
+    return 0;
+}
+
+int vvo_diffusive_heat(vvo_ctx* c, int threads) {
+    if (!c->built) {
+        c->err = "diffusive: tree is not built";
+        return 1;
+    }
+    const int64_t nl = (int64_t)c->leaves.size();
+    if (threads <= 0) {
+        for (int64_t l = 0; l < nl; l++) heat_diffusive_leaf(c, (int)l);
+    } else {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+        for (int64_t l = 0; l < nl; l++) heat_diffusive_leaf(c, (int)l);
+    }
+    return 0;
+}
+
+static std::vector<Obj>* list_of(vvo_ctx* c, int list) {
+    switch (list) {
+        case 0: return &c->vort;
+        case 1: return &c->heat;
+        case 2: return &c->streak;
+        case 3: return &c->streak_src;
+        default: return nullptr;
+    }
+}
+
+int vvo_set_list(vvo_ctx* c, int list, const double* objs6, int64_t n) {
+    std::vector<Obj>* l = list_of(c, list);
+    if (!l) return 1;
+    l->assign(reinterpret_cast<const Obj*>(objs6), reinterpret_cast<const Obj*>(objs6) + n);
+    c->built = false;
+    return 0;
+}
+
+int64_t vvo_list_size(vvo_ctx* c, int list) {
+    std::vector<Obj>* l = list_of(c, list);
+    return l ? (int64_t)l->size() : -1;
+}
+
+int vvo_get_list(vvo_ctx* c, int list, double* objs6) {
+    std::vector<Obj>* l = list_of(c, list);
+    if (!l) return 1;
+    if (!l->empty()) std::memcpy(objs6, l->data(), l->size() * sizeof(Obj));
+    return 0;
+}
+
+void vvo_set_pr(vvo_ctx* c, double pr) { c->pr = pr; }
+
+int vvo_set_heat(vvo_ctx* c, const int32_t* heat_condition, const float* heat_const) {
+    for (size_t b = 0; b < c->bodies.size(); b++) {
+        if (heat_condition[b] < 0 || heat_condition[b] > 3) return 1;
+        c->bodies[b].heat_cond = heat_condition[b];
+    }
+    c->heat_const.assign(heat_const, heat_const + c->segs.size());
+    return 0;
+}
+
+void vvo_get_heat_layer(vvo_ctx* c, int64_t* out) {
+    for (size_t k = 0; k < c->segs.size(); k++) out[k] = c->heat_layer_obj[k];
+}
+
+void vvo_set_heat_layer(vvo_ctx* c, const int64_t* in) {
+    for (size_t k = 0; k < c->segs.size(); k++) c->heat_layer_obj[k] = in[k];
+}
+
+// Per leaf (DFS order): {h0, h1, s0, s1}.
+void vvo_tree_leaf_hs(vvo_ctx* c, int32_t* out) {
+    for (size_t l = 0; l < c->leaves.size(); l++) {
+        const Node& L = c->nodes[c->leaves[l]];
+        out[4 * l] = L.h0;
+        out[4 * l + 1] = L.h1;
+        out[4 * l + 2] = L.s0;
+        out[4 * l + 3] = L.s1;
+    }
 }
 
 int vvo_slae_size(vvo_ctx* c) { return (int)c->segs.size() + 9 * (int)c->bodies.size(); }
@@ -1721,7 +2047,7 @@ int vvo_calc_circulation(vvo_ctx* c, double* A_out, double* rhs_out) {
 
 // Space::calc_forces (TSpace.cpp:94-120)
 void vvo_calc_forces(vvo_ctx* c) {
-    const double re = c->re, dt = c->dt, pr = 0;
+    const double re = c->re, dt = c->dt, pr = c->pr;
     const double C_NyuDt_Pi = dt / (M_PI * re);
     const double C_Nyu_Pi = 1. / (M_PI * re);
     for (Body& b : c->bodies) {
@@ -1749,6 +2075,7 @@ void vvo_zero_forces(vvo_ctx* c) {
         for (int k = 0; k < b.n; k++) {
             Seg& s = c->segs[b.first + k];
             s.gsum = s.fric = s.hsum = 0;
+            c->heat_layer_obj[b.first + k] = -1;
         }
         for (int i = 0; i < 3; i++) {
             b.force_hydro[i] = 0;

@@ -73,11 +73,33 @@ void    vvo_set_body(vvo_ctx*, int32_t b, const vvo_body_state* in);
 /* TBody::isPointInvalid (TBody.cpp:217-281): returns segment index or -1. */
 int32_t vvo_point_invalid(vvo_ctx*, int32_t body, double x, double y);
 
+/* ---- heat and streak lists (TSpace.hpp:40-43) ---------------------- */
+/* list: 0 VortexList, 1 HeatList, 2 StreakList, 3 StreakSourceList. */
+int     vvo_set_list(vvo_ctx*, int list, const double* objs6, int64_t n);
+int64_t vvo_list_size(vvo_ctx*, int list);
+int     vvo_get_list(vvo_ctx*, int list, double* objs6);
+void    vvo_set_pr(vvo_ctx*, double pr);
+/* TBody::heat_condition per body (0 neglect, 1 isolate, 2 const_t,
+ * 3 const_w; TBody.hpp:15) and TAtt::heat_const per segment. */
+int     vvo_set_heat(vvo_ctx*, const int32_t* heat_condition, const float* heat_const);
+/* TAtt::heat_layer_obj_no per segment. */
+void    vvo_get_heat_layer(vvo_ctx*, int64_t* out);
+void    vvo_set_heat_layer(vvo_ctx*, const int64_t* in);
+/* MFlowmove::heat_shed, heat_crop, streak_shed (MFlowmove.cpp:237-291);
+ * streak_shed appends unconditionally (the caller tests the time). */
+int     vvo_heat_shed(vvo_ctx*);
+int     vvo_heat_crop(vvo_ctx*, double scale);
+int     vvo_streak_shed(vvo_ctx*);
+/* MDiffusiveFast::process_heat_list (MDiffusiveFast.cpp:50-89). */
+int     vvo_diffusive_heat(vvo_ctx*, int threads);
+
 /* ---- Tree (TSortedTree.cpp) ----------------------------------------- */
 int     vvo_tree_build(vvo_ctx*, int far_criteria, double min_node, double max_node);
 void    vvo_tree_destroy(vvo_ctx*);
 int32_t vvo_tree_num_nodes(vvo_ctx*);
 int32_t vvo_tree_num_leaves(vvo_ctx*);
+/* hRange and sRange of every leaf (DFS order): {h0, h1, s0, s1}. */
+void    vvo_tree_leaf_hs(vvo_ctx*, int32_t* out);
 /* Node table in creation order (root = 0, children created in pairs).
  * geom: 4 doubles per node {x, y, h, w}; ints: 6 int32 per node
  * {v0, v1, ch1, parent, depth, nseg}; cm: 6 doubles {CMp.x, CMp.y, CMp.g,

@@ -103,6 +103,18 @@ class Oracle:
             _sig(L, "vvo_far_moments", I32, [P, C.c_void_p])
             _sig(L, "vvo_far_pairs", C.c_int, [P, ip, ip, I64, dp, dp, dp, C.c_void_p, C.c_void_p])
             _sig(L, "vvo_far_point_features", C.c_int, [P, dp, dp, ip, I64, dp, dp, dp])
+            _sig(L, "vvo_set_list", C.c_int, [P, C.c_int, C.c_void_p, I64])
+            _sig(L, "vvo_list_size", I64, [P, C.c_int])
+            _sig(L, "vvo_get_list", C.c_int, [P, C.c_int, C.c_void_p])
+            _sig(L, "vvo_set_pr", None, [P, F64])
+            _sig(L, "vvo_set_heat", C.c_int, [P, C.c_void_p, C.c_void_p])
+            _sig(L, "vvo_get_heat_layer", None, [P, C.c_void_p])
+            _sig(L, "vvo_set_heat_layer", None, [P, C.c_void_p])
+            _sig(L, "vvo_heat_shed", C.c_int, [P])
+            _sig(L, "vvo_heat_crop", C.c_int, [P, F64])
+            _sig(L, "vvo_streak_shed", C.c_int, [P])
+            _sig(L, "vvo_diffusive_heat", C.c_int, [P, C.c_int])
+            _sig(L, "vvo_tree_leaf_hs", None, [P, ip])
             cls._lib = L
         return cls._lib
 
@@ -250,6 +262,55 @@ class Oracle:
     def diffusive(self, threads=0, leaf_begin=0, leaf_end=-1):
         self._check(self.L.vvo_diffusive(self.h, threads, leaf_begin, leaf_end))
 
+    # ---- heat and streak lists: 1 HeatList, 2 StreakList, 3 StreakSourceList
+    HEAT, STREAK, STREAK_SOURCE = 1, 2, 3
+
+    def set_list(self, list_id, objs):
+        objs = np.ascontiguousarray(objs, dtype=np.float64).reshape(-1, 6)
+        self._check(self.L.vvo_set_list(self.h, list_id, objs.ctypes.data, objs.shape[0]))
+
+    def get_list(self, list_id):
+        n = self.L.vvo_list_size(self.h, list_id)
+        out = np.zeros((n, 6))
+        if n:
+            self._check(self.L.vvo_get_list(self.h, list_id, out.ctypes.data))
+        return out
+
+    def set_pr(self, pr):
+        self.L.vvo_set_pr(self.h, pr)
+
+    def set_heat(self, conditions, heat_const):
+        cond = np.ascontiguousarray(conditions, dtype=np.int32)
+        hc = np.ascontiguousarray(heat_const, dtype=np.float32)
+        self._check(self.L.vvo_set_heat(self.h, cond.ctypes.data, hc.ctypes.data))
+
+    def heat_layer(self):
+        out = np.zeros(self.L.vvo_num_segments(self.h), dtype=np.int64)
+        self.L.vvo_get_heat_layer(self.h, out.ctypes.data)
+        return out
+
+    def set_heat_layer(self, v):
+        v = np.ascontiguousarray(v, dtype=np.int64)
+        self.L.vvo_set_heat_layer(self.h, v.ctypes.data)
+
+    def heat_shed(self):
+        self._check(self.L.vvo_heat_shed(self.h))
+
+    def heat_crop(self, scale=16.0):
+        self._check(self.L.vvo_heat_crop(self.h, scale))
+
+    def streak_shed(self):
+        self._check(self.L.vvo_streak_shed(self.h))
+
+    def diffusive_heat(self, threads=0):
+        self._check(self.L.vvo_diffusive_heat(self.h, threads))
+
+    def leaf_hs(self):
+        nl = self.L.vvo_tree_num_leaves(self.h)
+        out = np.zeros((nl, 4), dtype=np.int32)
+        self.L.vvo_tree_leaf_hs(self.h, out)
+        return out
+
     def move_and_clean(self, remove=True):
         cl = I64(0)
         self._check(self.L.vvo_move_and_clean(self.h, int(remove), C.byref(cl)))
@@ -352,6 +413,12 @@ class RefHarness:
             _sig(L, "vvr_far_cm", None, [P, dp])
             _sig(L, "vvr_epsilon", I32, [P, C.c_int])
             _sig(L, "vvr_diffusive", None, [P])
+            _sig(L, "vvr_set_pr", None, [P, F64])
+            _sig(L, "vvr_set_list", C.c_int, [P, C.c_int, dp, I64])
+            _sig(L, "vvr_list_size", I64, [P, C.c_int])
+            _sig(L, "vvr_get_list", C.c_int, [P, C.c_int, dp])
+            _sig(L, "vvr_leaves_hs", None, [P, ip])
+            _sig(L, "vvr_diffusive_heat", None, [P])
             cls._lib = L
         return cls._lib
 
@@ -413,3 +480,26 @@ class RefHarness:
 
     def diffusive(self):
         self.L.vvr_diffusive(self.h)
+
+    def set_pr(self, pr):
+        self.L.vvr_set_pr(self.h, pr)
+
+    def set_list(self, list_id, objs):
+        """list_id 1: HeatList, 2: StreakList."""
+        objs = np.ascontiguousarray(objs, dtype=np.float64).reshape(-1, 6)
+        self.L.vvr_set_list(self.h, list_id, objs, objs.shape[0])
+
+    def get_list(self, list_id):
+        n = self.L.vvr_list_size(self.h, list_id)
+        out = np.zeros((max(n, 1), 6))
+        self.L.vvr_get_list(self.h, list_id, out)
+        return out[:n]
+
+    def leaves_hs(self):
+        nl = self.L.vvr_num_leaves(self.h)
+        out = np.zeros((max(nl, 1), 4), dtype=np.int32)
+        self.L.vvr_leaves_hs(self.h, out)
+        return out[:nl]
+
+    def diffusive_heat(self):
+        self.L.vvr_diffusive_heat(self.h)
