@@ -86,6 +86,40 @@ int64_t vvhd_num_vortices(vvhd_ctx* ctx);
 int     vvhd_snapshot(vvhd_ctx* ctx);
 int     vvhd_restore(vvhd_ctx* ctx);
 
+/* ---- heat and streak lists (Space::HeatList, StreakList,
+ * StreakSourceList, TSpace.hpp:40-43) ---------------------------------------
+ * The tree build partitions the heat and streak lists with the vortices
+ * (hRange / sRange, TSortedTree.cpp:68-69); vvhd_epsilon also runs epsh on
+ * the heat list (MEpsilonFast.cpp:53-60, 175-212; its merges count in
+ * *merged), vvhd_convective and vvhd_convective_learned also move the heat
+ * (g != 0) and streak targets (MConvectiveFast.cpp:87-110; the learned path
+ * gives them the reference's Taylor far field), vvhd_move_and_clean also
+ * moves and cleans both lists (MFlowmove.cpp:107-198).  One simulation per
+ * context: with vvhd_set_batch in force the tree build rejects them. */
+enum { VVHD_LIST_VORTEX = 0, VVHD_LIST_HEAT = 1, VVHD_LIST_STREAK = 2, VVHD_LIST_STREAK_SOURCE = 3 };
+int     vvhd_upload_list(vvhd_ctx* ctx, int list, const double* objs6, int64_t n);
+int     vvhd_download_list(vvhd_ctx* ctx, int list, double* objs6, int64_t n);
+int64_t vvhd_list_size(vvhd_ctx* ctx, int list);
+/* Space::pr (TSpace.hpp:25), used by process_heat_list and calc_forces. */
+int     vvhd_set_pr(vvhd_ctx* ctx, double pr);
+/* TBody::heat_condition per body (hc_t, TBody.hpp:15) and TAtt::heat_const
+ * per segment (TBody.hpp:27) of the uploaded bodies; both start at zero
+ * (neglect) when bodies are uploaded. */
+enum { VVHD_HC_NEGLECT = 0, VVHD_HC_ISOLATE = 1, VVHD_HC_CONST_T = 2, VVHD_HC_CONST_W = 3 };
+int     vvhd_set_heat(vvhd_ctx* ctx, const int32_t* heat_condition, const float* heat_const);
+/* TAtt::heat_layer_obj_no per segment (TBody.hpp:26): -1 after an upload
+ * and after zero_forces (TSpace.cpp:131), set by move_and_clean. */
+int     vvhd_heat_layer(vvhd_ctx* ctx, int64_t* obj_no, int32_t nseg);
+int     vvhd_set_heat_layer(vvhd_ctx* ctx, const int64_t* obj_no, int32_t nseg);
+/* MDiffusiveFast::process_heat_list (MDiffusiveFast.cpp:50-89). */
+int     vvhd_diffusive_heat(vvhd_ctx* ctx);
+/* MFlowmove::heat_shed, heat_crop(scale), streak_shed (MFlowmove.cpp:
+ * 237-291).  streak_shed appends the sources unconditionally: the caller
+ * tests Space::time.divisibleBy(dt_streak). */
+int     vvhd_heat_shed(vvhd_ctx* ctx);
+int     vvhd_heat_crop(vvhd_ctx* ctx, double scale);
+int     vvhd_streak_shed(vvhd_ctx* ctx);
+
 /* ---- bodies (Space::BodyList, TSpace.hpp:38) ------------------------- */
 int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg,
                        const vvhd_body* bodies, int32_t nbody);
@@ -130,6 +164,9 @@ int vvhd_tree_leaves(vvhd_ctx* ctx, int32_t* nleaves, double* geom,
  * Call with entries == NULL to get the total count. */
 int vvhd_tree_lists(vvhd_ctx* ctx, int far, int64_t* count, int64_t* offsets,
                     int32_t* depth, uint64_t* path, double* cm);
+/* hRange and sRange of every leaf (DFS order) as heat / streak list
+ * indices: 4 int32 per leaf {h0, h1, s0, s1}. */
+int vvhd_tree_leaf_lists(vvhd_ctx* ctx, int32_t* ranges4);
 /* stree::findNode for n points: leaf index (DFS order) or -1. */
 int vvhd_tree_find(vvhd_ctx* ctx, const double* xy, int64_t n, int32_t* leaf_out);
 /* Every node of the built tree in device numbering (inspection):

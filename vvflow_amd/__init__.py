@@ -33,6 +33,8 @@ VVHD_OK, VVHD_EINVAL, VVHD_ESTATE, VVHD_EHIP, VVHD_ENOMEM, VVHD_ESINGULAR, VVHD_
 FAR_TRAINABLE, FAR_PARAMS = 5122, 5148   # VVHD_FAR_TRAINABLE, VVHD_FAR_PARAMS
 NOALLOC_SCRATCH_BYTES = 64 << 20          # VVHD_NOALLOC_SCRATCH_BYTES
 FIELD_VX, FIELD_VY, FIELD_VORTICITY, FIELD_STREAMFUNCTION, FIELD_PRESSURE = range(5)   # VVHD_FIELD_*
+LIST_VORTEX, LIST_HEAT, LIST_STREAK, LIST_STREAK_SOURCE = range(4)                        # VVHD_LIST_*
+HC_NEGLECT, HC_ISOLATE, HC_CONST_T, HC_CONST_W = range(4)                                 # VVHD_HC_*
 
 SEG_DTYPE = np.dtype([(n, np.float64) for n in
                       ("x", "y", "g", "inv_eps", "cx", "cy", "dlx", "dly",
@@ -102,6 +104,18 @@ ABI = {
     "vvhd_work_counts": (_I, [_P, _VP]),
     "vvhd_gen_wake": (_I, [_I64, _U64, _VP]),
     "vvhd_gen_cylinder": (_I, [_D, _I32, _VP]),
+    "vvhd_upload_list": (_I, [_P, _I, _VP, _I64]),
+    "vvhd_download_list": (_I, [_P, _I, _VP, _I64]),
+    "vvhd_list_size": (_I64, [_P, _I]),
+    "vvhd_set_pr": (_I, [_P, _D]),
+    "vvhd_set_heat": (_I, [_P, _VP, _VP]),
+    "vvhd_heat_layer": (_I, [_P, _VP, _I32]),
+    "vvhd_set_heat_layer": (_I, [_P, _VP, _I32]),
+    "vvhd_diffusive_heat": (_I, [_P]),
+    "vvhd_heat_shed": (_I, [_P]),
+    "vvhd_heat_crop": (_I, [_P, _D]),
+    "vvhd_streak_shed": (_I, [_P]),
+    "vvhd_tree_leaf_lists": (_I, [_P, _VP]),
 }
 
 _lib = None
@@ -333,6 +347,53 @@ class Context:
 
     def vortex_shed(self):
         self._check(self.L.vvhd_vortex_shed(self.h))
+
+    # ---- heat and streak lists (LIST_HEAT, LIST_STREAK, LIST_STREAK_SOURCE)
+    def upload_list(self, list_id, objs):
+        objs = np.ascontiguousarray(objs, dtype=np.float64).reshape(-1, 6)
+        self._check(self.L.vvhd_upload_list(self.h, list_id, _ptr(objs), objs.shape[0]))
+
+    def download_list(self, list_id):
+        n = self.L.vvhd_list_size(self.h, list_id)
+        out = np.zeros((n, 6))
+        self._check(self.L.vvhd_download_list(self.h, list_id, _ptr(out), n))
+        return out
+
+    def set_pr(self, pr):
+        self._check(self.L.vvhd_set_pr(self.h, pr))
+
+    def set_heat(self, conditions, heat_const):
+        cond = np.ascontiguousarray(conditions, dtype=np.int32)
+        hc = np.ascontiguousarray(heat_const, dtype=np.float32)
+        self._check(self.L.vvhd_set_heat(self.h, _ptr(cond), _ptr(hc)))
+
+    def heat_layer(self):
+        out = np.zeros(self.nseg, dtype=np.int64)
+        self._check(self.L.vvhd_heat_layer(self.h, _ptr(out), self.nseg))
+        return out
+
+    def set_heat_layer(self, v):
+        v = np.ascontiguousarray(v, dtype=np.int64)
+        self._check(self.L.vvhd_set_heat_layer(self.h, _ptr(v), len(v)))
+
+    def diffusive_heat(self):
+        self._check(self.L.vvhd_diffusive_heat(self.h))
+
+    def heat_shed(self):
+        self._check(self.L.vvhd_heat_shed(self.h))
+
+    def heat_crop(self, scale=16.0):
+        self._check(self.L.vvhd_heat_crop(self.h, scale))
+
+    def streak_shed(self):
+        self._check(self.L.vvhd_streak_shed(self.h))
+
+    def tree_leaf_lists(self):
+        """hRange and sRange of every leaf as list indices {h0, h1, s0, s1}."""
+        nl = self.tree_leaves()["geom"].shape[0]
+        out = np.zeros((max(nl, 1), 4), dtype=np.int32)
+        self._check(self.L.vvhd_tree_leaf_lists(self.h, _ptr(out)))
+        return out[:nl]
 
     def calc_circulation(self, inf=(0.0, 0.0)):
         self._check(self.L.vvhd_calc_circulation(self.h, inf[0], inf[1]))

@@ -378,10 +378,10 @@ __global__ void k_shed_write(const vvhd_seg* segs, int nseg, const int32_t* flag
 }
 
 // Space::calc_forces (TSpace.cpp:94-120); fric holds the diffusive friction.
-__global__ void k_calc_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double* fric, double re, double dt) {
+__global__ void k_calc_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double* fric, double re, double dt,
+                              double pr) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbody) return;
-    const double pr = 0;
     const double C_NyuDt_Pi = dt / (M_PI * re);
     const double C_Nyu_Pi = 1. / (M_PI * re);
     vvhd_body& B = bodies[b];
@@ -720,6 +720,8 @@ __global__ void k_writeback(SlaeMap M, vvhd_body* bodies, vvhd_seg* segs, const 
     }
 }
 
+}  // namespace
+
 void sync_bodies_to_host(Ctx& c) {
     const int ns = (int)c.segs_h.size(), nb = (int)c.bodies_h.size();
     if (ns)
@@ -731,14 +733,12 @@ void sync_bodies_to_host(Ctx& c) {
     VVHD_HIP(hipStreamSynchronize(c.stream));
 }
 
-void exclusive_scan_i32(Ctx& c, const int32_t* in, int32_t* out, size_t n) {
+static void exclusive_scan_i32(Ctx& c, const int32_t* in, int32_t* out, size_t n) {
     size_t bytes = 0;
     VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int32_t)0, n, rocprim::plus<int32_t>(), c.stream));
     c.tmp.grow(bytes + 16);
     VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, in, out, (int32_t)0, n, rocprim::plus<int32_t>(), c.stream));
 }
-
-}  // namespace
 
 // Geometry of every uploaded body from its corners (the reference computes
 // it when a body is loaded: TBody::doUpdateSegments + doFillProperties).
@@ -804,6 +804,7 @@ void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
     c.n = c.fm_counts[0];
     if (cleaned) *cleaned = c.fm_counts[1];
     c.t.built = false;
+    if (c.heat.n || c.streak.n) passive_move_and_clean(c);
     if (nb) sync_bodies_to_host(c);
 }
 
@@ -831,7 +832,7 @@ void calc_forces(Ctx& c) {
     const int nb = (int)c.bodies_h.size();
     if (!nb) return;
     hipLaunchKernelGGL(k_calc_forces, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, c.bodies_d.p, nb, c.segs.p,
-                       c.fric.p, c.re, c.dt);
+                       c.fric.p, c.re, c.dt, c.pr);
     sync_bodies_to_host(c);
 }
 
@@ -840,6 +841,7 @@ void zero_forces(Ctx& c) {
     if (!nb) return;
     hipLaunchKernelGGL(k_zero_forces, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, c.bodies_d.p, nb, c.segs.p,
                        c.fric.p);
+    heat_layer_reset(c);
     sync_bodies_to_host(c);
 }
 

@@ -127,6 +127,11 @@ struct TreeArrays {
     DBuf<double> x, y, h, w;              // Stretch() geometry
     DBuf<uint64_t> blx, bly, trx, try_;   // ordered-uint64 bbox (atomics)
     DBuf<int32_t> v0, v1, ch1, parent, depth, nseg;
+    // hRange / sRange (TSortedTree.hpp:39-40) in element numbering: the tree
+    // is built over the vortices, then the heat particles (from hbase), then
+    // the streak particles (from sbase)
+    DBuf<int32_t> h0, h1, s0, s1;
+    int32_t hbase = 0, sbase = 0, nelem = 0;
     DBuf<uint64_t> path;                  // path code (bit per level)
     DBuf<double> cmp_x, cmp_y, cmp_g, cmm_x, cmm_y, cmm_g;
     DBuf<float> cmp_ie, cmm_ie;           // _1_eps of monopoles: +inf or 0
@@ -142,7 +147,7 @@ struct TreeArrays {
     DBuf<int64_t> seg_off;                // per leaf
     DBuf<int32_t> seg_list;               // segment ids
     DBuf<int32_t> seg_leaf;               // per segment: leaf (DFS)
-    DBuf<int32_t> vleaf;                  // per vortex: leaf (DFS)
+    DBuf<int32_t> vleaf;                  // per element (vortex, heat, streak): leaf (DFS)
     DBuf<int32_t> node_L, node_m, leaf_tmp, seg_node;   // build scratch
     DBuf<int32_t> cnt_near, cnt_far, err;               // list scratch
     DBuf<int64_t> slot_near, slot_far;                  // per (leaf, top entry) list offsets
@@ -160,6 +165,49 @@ struct TreeArrays {
     bool built = false;
     int far_criteria = 8;
     double min_node = 0, max_node = DBL_MAX;
+};
+
+// A list of passive particles (Space::HeatList or StreakList, TSpace.hpp:
+// 40-43): TObj records as structure of arrays, FP64, with double buffers
+// for reordering and compaction.
+struct PList {
+    int64_t n = 0;
+    DBuf<double> x, y, g, vx, vy, ie;
+    DBuf<double> x2, y2, g2, vx2, vy2, ie2;
+    void reserve(int64_t m, bool keep, hipStream_t s) {
+        if ((size_t)m <= x.cap) return;
+        const size_t cap = std::max<size_t>((size_t)m, x.cap + x.cap / 2);
+        for (DBuf<double>* b : {&x, &y, &g, &vx, &vy, &ie}) {
+            DBuf<double> nb;
+            nb.grow(cap);
+            if (keep && n > 0) VVHD_HIP(hipMemcpyAsync(nb.p, b->p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+            VVHD_HIP(hipStreamSynchronize(s));
+            b->release();
+            *b = nb;
+        }
+        for (DBuf<double>* b : {&x2, &y2, &g2, &vx2, &vy2, &ie2}) {
+            b->release();
+            b->grow(cap);
+        }
+    }
+    void swap_buffers() {
+        std::swap(x.p, x2.p);
+        std::swap(y.p, y2.p);
+        std::swap(g.p, g2.p);
+        std::swap(vx.p, vx2.p);
+        std::swap(vy.p, vy2.p);
+        std::swap(ie.p, ie2.p);
+        std::swap(x.cap, x2.cap);
+        std::swap(y.cap, y2.cap);
+        std::swap(g.cap, g2.cap);
+        std::swap(vx.cap, vx2.cap);
+        std::swap(vy.cap, vy2.cap);
+        std::swap(ie.cap, ie2.cap);
+    }
+    void release() {
+        for (DBuf<double>* b : {&x, &y, &g, &vx, &vy, &ie, &x2, &y2, &g2, &vx2, &vy2, &ie2}) b->release();
+        n = 0;
+    }
 };
 
 struct Ctx {
@@ -202,6 +250,20 @@ struct Ctx {
     DBuf<int32_t> sim_body;
     DBuf<double> sim_re;
     DBuf<int32_t> seg_root;                  // per segment: its simulation
+
+    // heat and streak lists, the streak sources (host), Space::pr, and the
+    // bodies' heat conditions: TBody::heat_condition per body (hc_t,
+    // TBody.hpp:15), TAtt::heat_const and heat_layer_obj_no per segment
+    PList heat, streak;
+    std::vector<double> streak_src_h;        // StreakSourceList, 6 doubles per record
+    double pr = 0;
+    std::vector<int32_t> heat_cond_h;        // per body
+    DBuf<int32_t> heat_cond;
+    DBuf<float> heat_const;                  // per segment
+    DBuf<int64_t> heat_layer;                // per segment
+    DBuf<double> hres;                       // per heat particle: epsh result
+    DBuf<uint8_t> hflag;
+    DBuf<uint8_t> hdirty;                    // per leaf: a heat merge touched it
 
     // segments and bodies (host copy is authoritative between calls)
     std::vector<vvhd_seg> segs_h;
@@ -283,6 +345,8 @@ struct TreeDev {
     const float *cmp_ie, *cmm_ie;
     int nleaf;
     const double4* pbox;   // per node {blx, bly, trx, try} as doubles (pack_boxes)
+    const int32_t *h0, *h1, *s0, *s1;   // heat / streak ranges (element numbering)
+    int hbase, sbase;                   // first heat and first streak element
 };
 
 inline TreeDev tree_dev(const TreeArrays& t) {
@@ -290,7 +354,8 @@ inline TreeDev tree_dev(const TreeArrays& t) {
                    t.try_.p,    t.v0.p,     t.v1.p,      t.ch1.p,     t.leaf_rank.p, t.leaf_node.p, t.near_off.p,
                    t.far_off.p, t.seg_off.p, t.near_list.p, t.far_list.p, t.seg_list.p, t.vleaf.p,  t.cmp_x.p,
                    t.cmp_y.p,   t.cmp_g.p,  t.cmm_x.p,   t.cmm_y.p,   t.cmm_g.p,   t.cmp_ie.p,  t.cmm_ie.p,
-                   t.nleaves_h, t.pbox.p};
+                   t.nleaves_h, t.pbox.p, t.h0.p,   t.h1.p,      t.s0.p,      t.s1.p,      t.hbase,
+                   t.sbase};
 }
 
 // box_dist2 from the packed copy of the boxes: one 32-byte load per node.
@@ -345,6 +410,20 @@ void bodies_release(Ctx& c);
 void farfield_release(Ctx& c);
 void direct_sum(Ctx& c, const int64_t* targets, int64_t nt, double* v_out);
 void pack_sources(Ctx& c);
+// heat and streak lists (passive.hip)
+int heat_epsilon(Ctx& c, int merge);
+void passive_convective(Ctx& c, const double4* taylor, double inf_x, double inf_y);
+void heat_diffusive(Ctx& c);
+void passive_move_and_clean(Ctx& c);
+void heat_shed(Ctx& c);
+void heat_crop(Ctx& c, double scale);
+void streak_shed(Ctx& c);
+void heat_layer_reset(Ctx& c);
+void list_append(Ctx& c, PList& L, const double* o6, int64_t m);
+void list_download(Ctx& c, PList& L, double* o6);
+void far_taylor(Ctx& c, double4* taylor);
+void sync_bodies_to_host(Ctx& c);
+__global__ void k_leaf_lists(TreeDev T, int32_t* out);
 void eps2h_h2(Ctx& c, bool h2, const double* xy, const int32_t* leaf, int64_t n, double* out);
 void grid_field(Ctx& c, int kind, const vvhd_grid& gr, double m, double dl, double inf_x, double inf_y, int with_ref,
                 double ref_x, double ref_y, float* map);

@@ -473,6 +473,17 @@ static void convective_into(Ctx& c, double inf_x, double inf_y, int kahan, doubl
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
                        taylor, c.x.p, c.y.p, c.g.p, near_out, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     VVHD_HIP(hipGetLastError());
+    if (c.heat.n || c.streak.n) passive_convective(c, taylor, inf_x, inf_y);
+}
+
+// The per-leaf Taylor terms alone (the heat and streak targets of the learned
+// path use the reference's far field).
+void far_taylor(Ctx& c, double4* taylor) {
+    const TreeDev T = tree_dev(c.t);
+    const int nl = T.nleaf;
+    if (nl == 0) return;
+    const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, taylor);
 }
 
 void convective(Ctx& c, double inf_x, double inf_y, int kahan) {

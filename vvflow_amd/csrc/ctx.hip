@@ -240,6 +240,15 @@ void vvhd_destroy(vvhd_ctx* ctx) {
         }
     c.dvx.release();
     c.dvy.release();
+    c.heat.release();
+    c.streak.release();
+    c.heat_cond.release();
+    c.heat_const.release();
+    c.heat_layer.release();
+    c.hres.release();
+    c.hflag.release();
+    c.hdirty.release();
+    for (auto* b : {&t.h0, &t.h1, &t.s0, &t.s1}) b->release();
     for (auto* b : {&c.fld_a, &c.fld_b, &c.fld_gtmp}) b->release();
     c.fld_map.release();
     for (auto& e : c.lev_ev)
@@ -373,6 +382,14 @@ int vvhd_upload_bodies(vvhd_ctx* ctx, const vvhd_seg* segs, int32_t nseg, const 
         VVHD_HIP(hipStreamSynchronize(c.stream));
     }
     bodies_prepare(c);
+    // heat conditions start at neglect, heat_const at 0, no layer particles
+    c.heat_cond_h.assign(nbody, 0);
+    c.heat_cond.grow(std::max(nbody, 1));
+    c.heat_const.grow(std::max(nseg, 1));
+    if (nbody) VVHD_HIP(hipMemsetAsync(c.heat_cond.p, 0, (size_t)nbody * 4, c.stream));
+    if (nseg) VVHD_HIP(hipMemsetAsync(c.heat_const.p, 0, (size_t)nseg * 4, c.stream));
+    heat_layer_reset(c);
+    VVHD_HIP(hipStreamSynchronize(c.stream));
     c.t.built = false;
     API_END(ctx)
 }

@@ -860,7 +860,8 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
     VVHD_HIP(hipStreamSynchronize(s));
     VVHD_CHECK(hc[2] == 0, VVHD_EUNSUPPORTED, "epsilon: too many targets to rescan after one merge");
     if (hc[1] > 0) pack_boxes(c);   // merges grew leaf boxes
-    if (merged_out) *merged_out = hc[1];
+    const int heat_merged = heat_epsilon(c, merge);   // epsh on the heat list (MEpsilonFast.cpp:53-60)
+    if (merged_out) *merged_out = hc[1] + heat_merged;
 }
 
 }  // namespace vvhd

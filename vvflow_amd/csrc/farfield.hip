@@ -1414,6 +1414,11 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
         if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream, c.ev_side, 0));
         hipLaunchKernelGGL(k_finish_learned, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T, c.x.p, c.y.p, c.g.p,
                            c.near_v.p, c.far_v.p, c.far_leaf.p, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
+        if (c.heat.n || c.streak.n) {   // heat and streak targets: the reference's Taylor far field
+            c.taylor.grow(T.nleaf + 1);
+            far_taylor(c, c.taylor.p);
+            passive_convective(c, c.taylor.p, inf_x, inf_y);
+        }
     }
     VVHD_HIP(hipEventRecord(c.ev[5], c.stream));
     c.phase_recorded[2] = true;

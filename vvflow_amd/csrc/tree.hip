@@ -13,6 +13,10 @@
 // centres, leaf decisions and the resulting permutation are bit-identical to
 // the reference.  Segments (body panels) follow a stable partition
 // (:139-148), i.e. each segment descends like findNode and keeps its order.
+// The heat and streak lists (hRange, sRange) are partitioned the same way
+// as the vortex list, each in place within its own range: the elements of
+// the build are the vortices, then the heat particles, then the streak
+// particles, and every node holds one range per list.
 //
 // Node ids are allocated level by level, so every level is a contiguous id
 // range.  After the levels: subtree leaf counts (bottom-up) and leaf DFS
@@ -46,19 +50,50 @@ struct NodeView {
     uint64_t *blx, *bly, *trx, *try_;
     int32_t *v0, *v1, *ch1, *parent, *depth, *nseg, *split;
     uint64_t* path;
+    int32_t *h0, *h1, *s0, *s1;
 };
 
 NodeView view(TreeArrays& t) {
     return NodeView{t.x.p,  t.y.p,   t.h.p,      t.w.p,     t.blx.p,   t.bly.p,  t.trx.p,   t.try_.p,
-                    t.v0.p, t.v1.p,  t.ch1.p,    t.parent.p, t.depth.p, t.nseg.p, t.split.p, t.path.p};
+                    t.v0.p, t.v1.p,  t.ch1.p,    t.parent.p, t.depth.p, t.nseg.p, t.split.p, t.path.p,
+                    t.h0.p, t.h1.p,  t.s0.p,     t.s1.p};
 }
 
-__device__ inline void init_node(NodeView nv, int id, int parent, int depth, uint64_t path, int v0, int v1) {
+// List of element k: 0 vortex, 1 heat, 2 streak.
+__device__ inline int elem_list(int k, int hbase, int sbase) { return k < hbase ? 0 : (k < sbase ? 1 : 2); }
+
+// A node's range of list cls.
+__device__ inline void list_range(const NodeView& nv, int node, int cls, int& r0, int& r1) {
+    if (cls == 0) {
+        r0 = nv.v0[node];
+        r1 = nv.v1[node];
+    } else if (cls == 1) {
+        r0 = nv.h0[node];
+        r1 = nv.h1[node];
+    } else {
+        r0 = nv.s0[node];
+        r1 = nv.s1[node];
+    }
+}
+
+__device__ inline void set_list_range(const NodeView& nv, int node, int cls, int r0, int r1) {
+    int32_t* a = cls == 0 ? nv.v0 : (cls == 1 ? nv.h0 : nv.s0);
+    int32_t* b = cls == 0 ? nv.v1 : (cls == 1 ? nv.h1 : nv.s1);
+    a[node] = r0;
+    b[node] = r1;
+}
+
+__device__ inline void init_node(NodeView nv, int id, int parent, int depth, uint64_t path, int v0, int v1, int h0,
+                                 int h1, int s0, int s1) {
     nv.parent[id] = parent;
     nv.depth[id] = depth;
     nv.path[id] = path;
     nv.v0[id] = v0;
     nv.v1[id] = v1;
+    nv.h0[id] = h0;
+    nv.h1[id] = h1;
+    nv.s0[id] = s0;
+    nv.s1[id] = s1;
     nv.ch1[id] = -1;
     nv.nseg[id] = 0;
     nv.split[id] = 0;
@@ -69,9 +104,17 @@ __device__ inline void init_node(NodeView nv, int id, int parent, int depth, uin
 }
 
 // One root per simulation: root s holds the vortices [sim_off[s], sim_off[s+1]).
-__global__ void k_init_roots(NodeView nv, int nroots, const int64_t* sim_off, int32_t* counters) {
+// The heat elements [hbase, sbase) and streak elements [sbase, nelem) belong
+// to root 0 (one simulation only).
+__global__ void k_init_roots(NodeView nv, int nroots, const int64_t* sim_off, int32_t* counters, int hbase,
+                             int sbase, int nelem) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nroots) init_node(nv, s, -1, 0, 0, (int)sim_off[s], (int)sim_off[s + 1]);
+    if (s < nroots) {
+        if (s == 0)
+            init_node(nv, s, -1, 0, 0, (int)sim_off[s], (int)sim_off[s + 1], hbase, sbase, sbase, nelem);
+        else
+            init_node(nv, s, -1, 0, 0, (int)sim_off[s], (int)sim_off[s + 1], nelem, nelem, nelem, nelem);
+    }
     if (s == 0) {
         counters[0] = nroots;   // nodes
         counters[1] = 0;        // leaves
@@ -239,19 +282,22 @@ __global__ void k_child_boxes(NodeView nv, const int32_t* counters, const double
     for (int base = lb; base < le; base += slots) {   // uniform trip count: shuffles stay converged
         const int id = first + (base - lb);
         const bool ok = id < le;
-        const int v0 = ok ? nv.v0[id] : 0, v1 = ok ? nv.v1[id] : 0;
         uint64_t lx = lo0, hx = hi0, ly = lo0, hy = hi0;
-        for (int k = v0 + sub; k < v1; k += LPN) {
-            const double px = x[k], py = y[k];
-            if (!isnan(px)) {
-                const uint64_t o = ord_from_double(px);
-                lx = o < lx ? o : lx;
-                hx = o > hx ? o : hx;
-            }
-            if (!isnan(py)) {
-                const uint64_t o = ord_from_double(py);
-                ly = o < ly ? o : ly;
-                hy = o > hy ? o : hy;
+        for (int cls = 0; cls < 3; cls++) {
+            int r0 = 0, r1 = 0;
+            if (ok) list_range(nv, id, cls, r0, r1);
+            for (int k = r0 + sub; k < r1; k += LPN) {
+                const double px = x[k], py = y[k];
+                if (!isnan(px)) {
+                    const uint64_t o = ord_from_double(px);
+                    lx = o < lx ? o : lx;
+                    hx = o > hx ? o : hx;
+                }
+                if (!isnan(py)) {
+                    const uint64_t o = ord_from_double(py);
+                    ly = o < ly ? o : ly;
+                    hy = o > hy ? o : hy;
+                }
             }
         }
         for (int o = LPN / 2; o > 0; o >>= 1) {
@@ -334,7 +380,7 @@ __global__ void k_decide(NodeView nv, int32_t* counters, int32_t* leaf_tmp, int 
         const double mx = (h < w) ? w : h;   // std::max(h, w)
         const double mn = (w < h) ? w : h;   // std::min(h, w)
         const int nvort = nv.v1[id] - nv.v0[id];
-        const int nlist = max(nvort, nv.nseg[id]);
+        const int nlist = max(max(nvort, nv.nseg[id]), max(nv.h1[id] - nv.h0[id], nv.s1[id] - nv.s0[id]));
         const bool leaf = (mx < max_node && mn <= min_node) || (mx < max_node && nlist < kLeafMaxList);
         if (leaf) {
             nv.split[id] = 1;
@@ -352,8 +398,9 @@ __global__ void k_decide(NodeView nv, int32_t* counters, int32_t* leaf_tmp, int 
         nv.ch1[id] = c;
         const int d = nv.depth[id] + 1;
         atomicMax(&counters[5], d);
-        init_node(nv, c, id, d, nv.path[id] << 1, nv.v0[id], nv.v0[id]);
-        init_node(nv, c + 1, id, d, (nv.path[id] << 1) | 1ull, nv.v1[id], nv.v1[id]);
+        init_node(nv, c, id, d, nv.path[id] << 1, nv.v0[id], nv.v0[id], nv.h0[id], nv.h0[id], nv.s0[id], nv.s0[id]);
+        init_node(nv, c + 1, id, d, (nv.path[id] << 1) | 1ull, nv.v1[id], nv.v1[id], nv.h1[id], nv.h1[id], nv.s1[id],
+                  nv.s1[id]);
     }
 }
 
@@ -376,16 +423,19 @@ struct GoesLeftFlag {
     }
 };
 
-// Ranks of misplaced elements: left_pos[v0 + r] / right_pos[v0 + q].  P is
-// the exclusive prefix of the flags over all elements (n + 1 entries).  The
-// first element of a split node also records its split position and number
-// of misplaced pairs and sets the children's ranges.  Thread 0 advances the
-// level range (the nodes of the next level were allocated by k_decide).
+// Ranks of misplaced elements: left_pos[r0 + r] / right_pos[r0 + q], with
+// [r0, r1) the range of the element's list in its node.  P is the exclusive
+// prefix of the flags over all elements (n + 1 entries).  The first element
+// of a split node's range also records that list's split position and number
+// of misplaced pairs (node_L / node_m at list * cap + node) and sets the
+// children's ranges.  Thread 0 advances the level range (the nodes of the
+// next level were allocated by k_decide).
 // The flags come from k_flag_count, before any swap of this level: k_swap
 // reads them too and must not derive them from coordinates that the swaps
 // may be rewriting.
-__global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* P, int n, int32_t* node_L, int32_t* node_m,
-                        int32_t* left_pos, int32_t* right_pos, int32_t* counters, const int32_t* flag) {
+__global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* P, int n, int hbase, int sbase, int cap,
+                        int32_t* node_L, int32_t* node_m, int32_t* left_pos, int32_t* right_pos, int32_t* counters,
+                        const int32_t* flag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {
         counters[2] = counters[3];
@@ -394,18 +444,18 @@ __global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* P, int
     if (k >= n) return;
     const int node = enode[k];
     if (nv.split[node] != 2) return;
-    const int v0 = nv.v0[node], v1 = nv.v1[node];   // a split node's own range is unchanged
+    const int cls = elem_list(k, hbase, sbase);
+    int v0, v1;   // a split node's own range is unchanged
+    list_range(nv, node, cls, v0, v1);
     const int P0 = P[v0];
     const int Lc = P[v1] - P0;
     const int Labs = v0 + Lc;
     if (k == v0) {
-        node_L[node] = Labs;
-        node_m[node] = Lc - (P[Labs] - P0);   // left-goers sitting right of Labs
+        node_L[cls * cap + node] = Labs;
+        node_m[cls * cap + node] = Lc - (P[Labs] - P0);   // left-goers sitting right of Labs
         const int c = nv.ch1[node];
-        nv.v0[c] = v0;
-        nv.v1[c] = Labs;
-        nv.v0[c + 1] = Labs;
-        nv.v1[c + 1] = v1;
+        set_list_range(nv, c, cls, v0, Labs);
+        set_list_range(nv, c + 1, cls, Labs, v1);
     }
     const bool f = flag[k] != 0;
     if (k < Labs && !f) {
@@ -419,19 +469,22 @@ __global__ void k_ranks(NodeView nv, const int32_t* enode, const int32_t* P, int
 
 // Hoare swaps, child bounding boxes and element -> child node.
 // With bbox false the child boxes are left to k_bbox_top.
-__global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const int32_t* P, int n,
-                       const int32_t* node_L, const int32_t* node_m, const int32_t* left_pos,
+__global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const int32_t* P, int n, int hbase,
+                       int sbase, int cap, const int32_t* node_L, const int32_t* node_m, const int32_t* left_pos,
                        const int32_t* right_pos, double* x, double* y, int32_t* id, bool bbox) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
-    int node = 0, Labs = 0, v0 = 0, f = 0, c = 0;
+    int node = 0, Labs = 0, v0 = 0, f = 0, c = 0, slot = 0;
     if (k < n) {
         node = enode[k];
         active = nv.split[node] == 2;
     }
     if (active) {
-        v0 = nv.v0[node];
-        Labs = node_L[node];
+        const int cls = elem_list(k, hbase, sbase);
+        int v1;
+        list_range(nv, node, cls, v0, v1);
+        slot = cls * cap + node;
+        Labs = node_L[slot];
         f = flag[k];
         c = nv.ch1[node];
     }
@@ -445,7 +498,7 @@ __global__ void k_swap(NodeView nv, int32_t* enode, const int32_t* flag, const i
     int partner = -1;
     if (left_mis) {
         const int r = (k - v0) - (P[k] - P[v0]);
-        partner = right_pos[v0 + node_m[node] - 1 - r];
+        partner = right_pos[v0 + node_m[slot] - 1 - r];
         qx = x[partner];
         qy = y[partner];
     }
@@ -479,6 +532,23 @@ __global__ void k_seg_step(NodeView nv, const vvhd_seg* segs, int nseg, int32_t*
     bbox_add(nv, c, segs[s].x, segs[s].y);
 }
 
+
+// A heat or streak list in tree order: positions from the element arrays,
+// the other fields gathered from the list's own arrays (id holds element
+// numbers, base is the list's first element).
+__global__ void k_gather_list(const int32_t* id, int n, int base, const double* ex, const double* ey, const double* g,
+                              const double* vx, const double* vy, const double* ie, double* x2, double* y2, double* g2,
+                              double* vx2, double* vy2, double* ie2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = id[k] - base;
+    x2[k] = ex[k];
+    y2[k] = ey[k];
+    g2[k] = g[s];
+    vx2[k] = vx[s];
+    vy2[k] = vy[s];
+    ie2[k] = ie[s];
+}
 
 // Gathers the non-key vortex fields into tree order.
 __global__ void k_gather(const int32_t* id, int n, const double* g, const double* vx, const double* vy,
@@ -596,6 +666,8 @@ __global__ void k_down_level(NodeView nv, int lb, int le, const int32_t* nleaves
         leaf_rank[id] = f;
         leaf_node[f] = id;
         for (int k = nv.v0[id]; k < nv.v1[id]; k++) vleaf[k] = f;
+        for (int k = nv.h0[id]; k < nv.h1[id]; k++) vleaf[k] = f;
+        for (int k = nv.s0[id]; k < nv.s1[id]; k++) vleaf[k] = f;
     }
 }
 
@@ -928,20 +1000,39 @@ void scan_flags(Ctx& c, const GoesLeftFlag& f, int32_t* flag, int32_t* P, int32_
 
 void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int32_t* perm_out) {
     TreeArrays& t = c.t;
-    const int n = (int)c.n;
+    const int nvort = (int)c.n;
+    const int nh = (int)c.heat.n, nst = (int)c.streak.n;
     const int nseg = (int)c.segs_h.size();
-    VVHD_CHECK(c.n < INT32_MAX / 2, VVHD_EINVAL, "tree_build: too many vortices");
+    VVHD_CHECK(c.n + c.heat.n + c.streak.n < INT32_MAX / 2, VVHD_EINVAL, "tree_build: too many particles");
     VVHD_CHECK(far_criteria >= 0, VVHD_EINVAL, "tree_build: far_criteria must be non-negative");
     t.far_criteria = far_criteria;
     t.min_node = min_node;
     t.max_node = max_node;
     sim_sync(c);
     const int nroots = c.nsim;
+    VVHD_CHECK(nroots == 1 || nh + nst == 0, VVHD_EUNSUPPORTED,
+               "tree_build: heat and streak lists with several simulations in one context");
+    // elements: the vortices, then the heat and the streak particles
+    const int n = nvort + nh + nst;
+    t.hbase = nvort;
+    t.sbase = nvort + nh;
+    t.nelem = n;
+    if (nh + nst) {
+        reserve_vortices(c, n + 1, true);
+        if (nh) {
+            VVHD_HIP(hipMemcpyAsync(c.x.p + nvort, c.heat.x.p, (size_t)nh * 8, hipMemcpyDeviceToDevice, c.stream));
+            VVHD_HIP(hipMemcpyAsync(c.y.p + nvort, c.heat.y.p, (size_t)nh * 8, hipMemcpyDeviceToDevice, c.stream));
+        }
+        if (nst) {
+            VVHD_HIP(hipMemcpyAsync(c.x.p + t.sbase, c.streak.x.p, (size_t)nst * 8, hipMemcpyDeviceToDevice, c.stream));
+            VVHD_HIP(hipMemcpyAsync(c.y.p + t.sbase, c.streak.y.p, (size_t)nst * 8, hipMemcpyDeviceToDevice, c.stream));
+        }
+    }
     const int cap = 2 * (n + nseg) + 64 + nroots;
     for (auto* b : {&t.x, &t.y, &t.h, &t.w}) b->grow(cap);
     for (auto* b : {&t.blx, &t.bly, &t.trx, &t.try_, &t.path}) b->grow(cap);
     for (auto* b : {&t.v0, &t.v1, &t.ch1, &t.parent, &t.depth, &t.nseg, &t.split, &t.nleaves, &t.arrive, &t.leaf_rank,
-                    &t.pre})
+                    &t.pre, &t.h0, &t.h1, &t.s0, &t.s1})
         b->grow(cap);
     for (auto* b : {&t.cmp_x, &t.cmp_y, &t.cmp_g, &t.cmm_x, &t.cmm_y, &t.cmm_g}) b->grow(cap);
     t.cmp_ie.grow(cap);
@@ -964,15 +1055,15 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     DBuf<int32_t>& node_m = t.node_m;
     DBuf<int32_t>& leaf_tmp = t.leaf_tmp;
     DBuf<int32_t>& seg_node = t.seg_node;
-    node_L.grow(cap);
-    node_m.grow(cap);
+    node_L.grow(3 * (size_t)cap);
+    node_m.grow(3 * (size_t)cap);
     leaf_tmp.grow(cap);
     seg_node.grow(nseg + 1);
     NodeView nv = view(t);
     hipStream_t s = c.stream;
 
     hipLaunchKernelGGL(k_init_roots, dim3(nblocks(nroots)), dim3(kBlock), 0, s, nv, nroots, (const int64_t*)c.sim_off.p,
-                       t.counters.p);
+                       t.counters.p, t.hbase, t.sbase, n);
     hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(kBlock), 0, s, c.id.p, n);
     hipLaunchKernelGGL(k_fill_roots, dim3(nblocks(n)), dim3(kBlock), 0, s, enode.p, n, (const int64_t*)c.sim_off.p,
                        nroots);
@@ -1003,12 +1094,12 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
         hipLaunchKernelGGL(k_decide, dim3(node_blocks), dim3(kBlock), 0, s, nv, t.counters.p, leaf_tmp.p, cap,
                            min_node, max_node);
         scan_flags(c, GoesLeftFlag{nv, enode.p, c.x.p, c.y.p, n}, flag, P, reinterpret_cast<int32_t*>(c.tmp.p), n + 1);
-        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, P, n, node_L.p, node_m.p,
-                           left_pos, right_pos, t.counters.p, (const int32_t*)flag);
+        hipLaunchKernelGGL(k_ranks, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, nv, enode.p, P, n, t.hbase, t.sbase, cap,
+                           node_L.p, node_m.p, left_pos, right_pos, t.counters.p, (const int32_t*)flag);
         // the children of level d number at most nroots 2^(d+1)
         const bool top = ((int64_t)nroots << std::min<int>(d + 1, 30)) <= kTopNodes;
-        hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, node_L.p, node_m.p,
-                           left_pos, right_pos, c.x.p, c.y.p, c.id.p, false);
+        hipLaunchKernelGGL(k_swap, dim3(nblocks(n)), dim3(kBlock), 0, s, nv, enode.p, flag, P, n, t.hbase, t.sbase,
+                           cap, node_L.p, node_m.p, left_pos, right_pos, c.x.p, c.y.p, c.id.p, false);
         if (top && n)
             hipLaunchKernelGGL(k_bbox_top, dim3(nblocks(n, kTopChunk)), dim3(kBlock), 0, s, nv,
                                (const int32_t*)(t.counters.p + 2), (const int32_t*)enode.p, c.x.p, c.y.p, n);
@@ -1049,13 +1140,23 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     const int nl = t.nleaves_h;
 
     // tree order of the remaining fields
-    hipLaunchKernelGGL(k_gather, dim3(nblocks(n)), dim3(kBlock), 0, s, c.id.p, n, c.g.p, c.vx.p, c.vy.p, c.ie.p,
-                       c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p);
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(nvort)), dim3(kBlock), 0, s, c.id.p, nvort, c.g.p, c.vx.p, c.vy.p,
+                       c.ie.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p);
     swap_buf(c.g, c.g2);
     swap_buf(c.vx, c.vx2);
     swap_buf(c.vy, c.vy2);
     swap_buf(c.ie, c.ie2);
-    if (perm_out && n) VVHD_HIP(hipMemcpyAsync(perm_out, c.id.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    if (perm_out && nvort) VVHD_HIP(hipMemcpyAsync(perm_out, c.id.p, (size_t)nvort * 4, hipMemcpyDeviceToHost, s));
+    // the heat and streak lists in their new order
+    for (int q = 0; q < 2; q++) {
+        PList& L = q == 0 ? c.heat : c.streak;
+        const int base = q == 0 ? t.hbase : t.sbase, m = q == 0 ? nh : nst;
+        if (!m) continue;
+        hipLaunchKernelGGL(k_gather_list, dim3(nblocks(m)), dim3(kBlock), 0, s, c.id.p + base, m, base, c.x.p + base,
+                           c.y.p + base, L.g.p, L.vx.p, L.vy.p, L.ie.p, L.x2.p, L.y2.p, L.g2.p, L.vx2.p, L.vy2.p,
+                           L.ie2.p);
+        L.swap_buffers();
+    }
 
     // leaf counts and monopoles bottom-up, DFS ranks top-down
     CMView cm{t.cmp_x.p, t.cmp_y.p, t.cmp_g.p, t.cmm_x.p, t.cmm_y.p, t.cmm_g.p, t.cmp_ie.p, t.cmm_ie.p};
