@@ -26,7 +26,8 @@
  * Errors of the device library are thrown as std::invalid_argument for
  * calls out of order (the reference's own convention for an unbuilt tree)
  * and std::runtime_error otherwise.  Bodies are rigid and static (the
- * configurations of this hot path); heat and streak lists are not carried.
+ * configurations of this hot path).  The heat and streak lists (HeatList,
+ * StreakList, StreakSourceList) are carried like the vortex list.
  */
 #ifndef VVHD_MODULES_HPP
 #define VVHD_MODULES_HPP
@@ -77,9 +78,14 @@ class TObj {
 };
 static_assert(sizeof(TObj) == 6 * sizeof(double), "TObj is the 48-byte record of the device ABI");
 
+// Heat boundary condition of a body (TBody.hpp:15).
+enum class hc_t { neglect, isolate, const_t, const_w };
+
 class TAtt : public TObj {
    public:
     TVec corner;
+    int64_t heat_layer_obj_no = -1;   // index of its layer particle in Space::HeatList, or -1
+    float heat_const = 0;
     uint32_t slip = 0;
     TVec dl;
     double gsum = 0, hsum = 0, fric = 0, Cp = 0, Fr = 0, Nu = 0;
@@ -96,6 +102,7 @@ class TBody {
     double g_dead = 0, nusselt = 0;
     int special_segment_no = 0;
     bool kutta = false;
+    hc_t heat_condition = hc_t::neglect;
     // geometry, computed on the device (TBody::doFillProperties)
     double get_slen() const { return slen_; }
     double get_area() const { return area_; }
@@ -123,8 +130,12 @@ class Space {
     double time = 0;
     double dt = 1;
     double finish = 0;
+    double dt_streak = 0;   // period of streak shedding (the reference keeps it as a TTime)
     std::vector<std::shared_ptr<TBody>> BodyList;
     std::vector<TObj> VortexList;
+    std::vector<TObj> HeatList;
+    std::vector<TObj> StreakSourceList;
+    std::vector<TObj> StreakList;
 
     TVec inf_speed() const { return inf_vel; }
     void calc_forces();
@@ -161,15 +172,17 @@ class Space {
     void ensure_pushed();
     void set_params();
     void pull_bodies();
+    void pull_lists();
     void check(int rc, bool state_error = false) const;
 };
 
 // Host view of one node of the device tree (TSortedTree.hpp:20-57): centre
 // and size, depth i, path code j, per-sign monopoles CMp/CMm, the vortex
 // range as indices [v0, v1) into Space::VortexList (and as TObj pointers in
-// vRange while the host mirror holds the device's list), and for a leaf its
-// near (leaves) and far (any level) interaction lists in the reference's
-// order.  Views of every node and the lists are loaded from the device when
+// vRange while the host mirror holds the device's list), a leaf's heat and
+// streak ranges [h0, h1), [s0, s1) (hRange, sRange likewise), and for a leaf
+// its near (leaves) and far (any level) interaction lists in the
+// reference's order.  Views of every node and the lists are loaded from the device when
 // the tree is first asked for its leaves.
 class stree;
 class TSortedNode {
@@ -183,6 +196,8 @@ class TSortedNode {
     TObj CMp, CMm;
     int64_t v0 = 0, v1 = 0;
     range vRange;
+    int64_t h0 = 0, h1 = 0, s0 = 0, s1 = 0;
+    range hRange, sRange;
     std::vector<TSortedNode*>* NearNodes = nullptr;
     std::vector<TSortedNode*>* FarNodes = nullptr;
     TSortedNode *ch1 = nullptr, *ch2 = nullptr;
@@ -271,7 +286,7 @@ class MDiffusiveFast {
    public:
     MDiffusiveFast(Space* S_, const TSortedTree* tree_) : S(S_), tree(tree_) {}
     void process_vort_list();
-    void process_heat_list() {}   // no heat list in this implementation
+    void process_heat_list();
 
    private:
     Space* S;
@@ -283,9 +298,9 @@ class MFlowmove {
     MFlowmove(Space* S_, double remove_eps_ = 1E-10);
     void move_and_clean(bool remove, const void** collision, size_t* cleaned_v = nullptr);
     void vortex_shed();
-    void streak_shed() {}   // no streak lists in this implementation
-    void heat_shed() {}     // no heat list in this implementation
-    void heat_crop(double scale = 16) { (void)scale; }
+    void streak_shed();
+    void heat_shed();
+    void heat_crop(double scale = 16);
 
    private:
     Space* S;

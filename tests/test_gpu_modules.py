@@ -41,6 +41,36 @@ def test_cpp_driver_matches_python_driver():
     print(f"  n after {steps} steps: {counts[-1]}; force_hydro x {f[-1, 0]:.6f}", flush=True)
 
 
+def test_cpp_driver_heat_and_streak_match_python_driver():
+    """A heated cylinder (const_w, heat_const 0.5, Pr 0.7) with 11 streak
+    sources shed every 4 steps: the module classes (heat_shed, streak_shed,
+    process_heat_list, heat_crop and the lists carried by Space) against the
+    Python driver's direct ABI calls in the reference's order."""
+    steps = 20
+    print(f"case C++ cylinder driver with heat and streak, {steps} steps", flush=True)
+    r = subprocess.run([EXE, "--steps", str(steps), "--heat", "3:0.5:0.7", "--streak", "11:0.02"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    heat_line = [ln for ln in r.stdout.splitlines() if ln.startswith("# heat")][0].split()
+    ctx = V.Context(0)
+    run = CylinderRun(ctx, heat=(V.HC_CONST_W, 0.5, 0.7), streak=(11, 0.02))
+    for _ in range(steps):
+        run.step()
+    h = ctx.download_list(V.LIST_HEAT)
+    st = ctx.download_list(V.LIST_STREAK)
+    hsum = ctx.download_bodies()[0]["hsum"].sum()
+    ctx.close()
+    print(f"  heat {len(h)} (sum {h[:, 2].sum():.6e}), streak {len(st)}", flush=True)
+    assert int(heat_line[2]) == len(h)
+    assert float(heat_line[3]) == pytest.approx(h[:, 2].sum(), rel=1e-12)
+    assert float(heat_line[5]) == pytest.approx(hsum, rel=1e-12, abs=1e-15)
+    assert int(heat_line[7]) == len(st)
+    assert len(st) == 11 * 5   # shed at t = 0, 0.02, 0.04, 0.06, 0.08
+    # const_w sheds one particle per segment and step; epsh merges the
+    # layers shed at the same places and the wall absorbs some of them
+    assert 500 * 2 < len(h) < 500 * steps
+
+
 def read_xfield(path):
     """XField's binary matrix (XField.cpp:124-152) -> (xs, ys, map)."""
     a = np.fromfile(path, dtype=np.float32)

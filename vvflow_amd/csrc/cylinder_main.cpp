@@ -6,7 +6,8 @@
 //   vvflow_cylinder [--steps 400] [--re 600] [--dt 0.005] [--nseg 500]
 //                   [--learned model.npy] [--far-criteria 8] [--every 1]
 //                   [--sensors points.txt --sensors-out out.txt]
-//                   [--fields PREFIX]
+//                   [--fields PREFIX] [--heat COND:HEAT_CONST:PR]
+//                   [--streak N:DT_STREAK]
 //
 // Prints one line per `every` steps: step, time, vortex count, force_hydro
 // (x, y, o), friction (x, y, o), step wall time in ms; then a summary line.
@@ -18,7 +19,10 @@
 // PREFIX_{velocity,vorticity,streamfunction,pressure}.bin (and the state
 // they start from to PREFIX_state.bin), and prints
 // MEpsilonFast::eps2h / h2 at three points with the near and far list sizes
-// of their leaves.
+// of their leaves.  --heat gives the body a heat condition (0 neglect,
+// 1 isolate, 2 const_t, 3 const_w), every segment heat_const and Space::pr;
+// --streak puts N streak sources on the segment x = -1, y in [-1, 1], shed
+// every DT_STREAK.  The summary then also reports the heat and streak lists.
 
 #include <algorithm>
 #include <chrono>
@@ -73,6 +77,8 @@ int main(int argc, char** argv) {
     int steps = 400, nseg = 500, fc = 8, every = 1;
     double re = 600, dt = 0.005, radius = 0.5;
     std::string learned, sensors_in, sensors_out, fields;
+    int heat_cond = 0, nstreak = 0;
+    double heat_const = 0, pr = 0, dt_streak = 0;
     for (int k = 1; k < argc; k++) {
         const std::string a = argv[k];
         auto next = [&]() -> const char* {
@@ -89,6 +95,13 @@ int main(int argc, char** argv) {
         else if (a == "--sensors") sensors_in = next();
         else if (a == "--sensors-out") sensors_out = next();
         else if (a == "--fields") fields = next();
+        else if (a == "--heat") {
+            if (std::sscanf(next(), "%d:%lf:%lf", &heat_cond, &heat_const, &pr) != 3)
+                throw std::invalid_argument("--heat expects COND:HEAT_CONST:PR");
+        } else if (a == "--streak") {
+            if (std::sscanf(next(), "%d:%lf", &nstreak, &dt_streak) != 2)
+                throw std::invalid_argument("--streak expects N:DT_STREAK");
+        }
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -103,7 +116,13 @@ int main(int argc, char** argv) {
         std::vector<double> xy(2 * (size_t)nseg);
         if (vvhd_gen_cylinder(radius, nseg, xy.data()) != VVHD_OK) throw std::runtime_error("gen_cylinder failed");
         for (int k = 0; k < nseg; k++) body->alist.emplace_back(xy[2 * k], xy[2 * k + 1]);
+        body->heat_condition = static_cast<hc_t>(heat_cond);
+        for (TAtt& a : body->alist) a.heat_const = (float)heat_const;
         S.BodyList.push_back(body);
+        S.pr = pr;
+        S.dt_streak = dt_streak;
+        for (int k = 0; k < nstreak; k++)
+            S.StreakSourceList.emplace_back(-1.0, nstreak > 1 ? -1.0 + 2.0 * k / (nstreak - 1) : 0.0, 0.0);
         S.push();
 
         const double dl = S.average_segment_length();
@@ -158,6 +177,15 @@ int main(int argc, char** argv) {
         }
         printf("# steps %d vortices %lld total %.1f ms mean %.3f ms/step\n", steps, (long long)S.vortex_count(),
                total_ms, total_ms / std::max(1, steps));
+        if (heat_cond || nstreak) {
+            S.pull();
+            double hg = 0;
+            for (const TObj& o : S.HeatList) hg += o.g;
+            double hsum = 0;
+            for (const TAtt& a : S.BodyList[0]->alist) hsum += a.hsum;
+            printf("# heat %zu %.17g hsum %.17g streak %zu nusselt %.17g\n", S.HeatList.size(), hg, hsum,
+                   S.StreakList.size(), S.BodyList[0]->nusselt);
+        }
         if (!fields.empty()) {
             // the state the evaluators copy: vortices (6 doubles each), then
             // per segment its vvhd_seg fields as 16 doubles, then the body's

@@ -5,6 +5,7 @@
 #include "vvhd_modules.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 
@@ -33,7 +34,10 @@ void Space::check(int rc, bool state_error) const {
     throw std::runtime_error(msg);
 }
 
-void Space::set_params() { check(vvhd_set_params(ctx_, re, dt, inf_g, time)); }
+void Space::set_params() {
+    check(vvhd_set_params(ctx_, re, dt, inf_g, time));
+    check(vvhd_set_pr(ctx_, pr));
+}
 
 void Space::push() {
     std::vector<vvhd_seg> segs;
@@ -85,6 +89,25 @@ void Space::push() {
     check(vvhd_upload_bodies(ctx_, segs.data(), nseg_, bodies.data(), (int32_t)bodies.size()));
     check(vvhd_upload_vortices(ctx_, reinterpret_cast<const double*>(VortexList.data()),
                                (int64_t)VortexList.size()));
+    // heat conditions, heat constants and layer particles of the bodies
+    std::vector<int32_t> cond;
+    std::vector<float> hconst;
+    std::vector<int64_t> layer;
+    for (const auto& bp : BodyList) {
+        cond.push_back((int32_t)bp->heat_condition);
+        for (const TAtt& a : bp->alist) {
+            hconst.push_back(a.heat_const);
+            layer.push_back(a.heat_layer_obj_no);
+        }
+    }
+    check(vvhd_set_heat(ctx_, cond.data(), hconst.data()));
+    check(vvhd_set_heat_layer(ctx_, layer.data(), nseg_));
+    check(vvhd_set_pr(ctx_, pr));
+    const std::vector<TObj>* lists[3] = {&HeatList, &StreakList, &StreakSourceList};
+    const int ids[3] = {VVHD_LIST_HEAT, VVHD_LIST_STREAK, VVHD_LIST_STREAK_SOURCE};
+    for (int q = 0; q < 3; q++)
+        check(vvhd_upload_list(ctx_, ids[q], reinterpret_cast<const double*>(lists[q]->data()),
+                               (int64_t)lists[q]->size()));
     pushed_ = true;
     pull_bodies();
 }
@@ -98,7 +121,9 @@ void Space::pull_bodies() {
     if (!nb) return;
     std::vector<vvhd_seg> segs(nseg_);
     std::vector<vvhd_body> bodies(nb);
+    std::vector<int64_t> layer(nseg_);
     check(vvhd_download_bodies(ctx_, segs.data(), nseg_, bodies.data(), nb));
+    check(vvhd_heat_layer(ctx_, layer.data(), nseg_));
     for (int32_t bi = 0; bi < nb; bi++) {
         TBody& b = *BodyList[bi];
         const vvhd_body& r = bodies[bi];
@@ -132,7 +157,18 @@ void Space::pull_bodies() {
             a.Nu = s.Nu;
             a.slip = s.slip;
             a.eq_no = s.eq_no;
+            a.heat_layer_obj_no = layer[r.first_seg + k];
         }
+    }
+}
+
+void Space::pull_lists() {
+    std::vector<TObj>* lists[2] = {&HeatList, &StreakList};
+    const int ids[2] = {VVHD_LIST_HEAT, VVHD_LIST_STREAK};
+    for (int q = 0; q < 2; q++) {
+        const int64_t n = vvhd_list_size(ctx_, ids[q]);
+        lists[q]->resize((size_t)n);
+        check(vvhd_download_list(ctx_, ids[q], reinterpret_cast<double*>(lists[q]->data()), n));
     }
 }
 
@@ -141,6 +177,7 @@ void Space::pull() {
     const int64_t n = vvhd_num_vortices(ctx_);
     VortexList.resize((size_t)n);
     check(vvhd_download_vortices(ctx_, reinterpret_cast<double*>(VortexList.data()), n));
+    pull_lists();
     pull_bodies();
 }
 
@@ -208,8 +245,7 @@ stree::stree(Space* sS, int sFarCriteria, double sMinNodeSize, double sMaxNodeSi
     : S(sS), farCriteria_(sFarCriteria), minNodeSize(sMinNodeSize), maxNodeSize(sMaxNodeSize) {}
 
 void stree::build(bool IncludeVortexes, bool IncludeBody, bool IncludeHeat) {
-    (void)IncludeHeat;   // no heat list
-    if (!IncludeVortexes || !IncludeBody)
+    if (!IncludeVortexes || !IncludeBody || (!IncludeHeat && vvhd_list_size(S->ctx_, VVHD_LIST_HEAT) > 0))
         throw std::invalid_argument("stree::build: partial trees are not supported by the device path");
     if (built_) {
         fprintf(stderr, "Tree is already built\n");   // TSortedTree.cpp:234
@@ -271,12 +307,28 @@ void stree::load_leaves() const {
     std::vector<int32_t> lints(4 * (size_t)nl);
     std::vector<uint64_t> lpath(nl);
     S->check(vvhd_tree_leaves(ctx, &nl, nullptr, lints.data(), lpath.data()));
+    std::vector<int32_t> hs(4 * (size_t)std::max(nl, 1));
+    if (nl) S->check(vvhd_tree_leaf_lists(ctx, hs.data()));
+    const bool hmirror = (int64_t)S->HeatList.size() == vvhd_list_size(ctx, VVHD_LIST_HEAT);
+    const bool smirror = (int64_t)S->StreakList.size() == vvhd_list_size(ctx, VVHD_LIST_STREAK);
     leaf_ptrs_.resize(nl);
     near_.assign(nl, {});
     far_.assign(nl, {});
     for (int32_t l = 0; l < nl; l++) {
         TSortedNode* t = &nodes_[by_key.at(key(lints[4 * l + 2], lpath[l]))];
         t->leaf = l;
+        t->h0 = hs[4 * l];
+        t->h1 = hs[4 * l + 1];
+        t->s0 = hs[4 * l + 2];
+        t->s1 = hs[4 * l + 3];
+        if (hmirror && t->h1 > t->h0) {
+            t->hRange.first = S->HeatList.data() + t->h0;
+            t->hRange.last = S->HeatList.data() + t->h1;
+        }
+        if (smirror && t->s1 > t->s0) {
+            t->sRange.first = S->StreakList.data() + t->s0;
+            t->sRange.last = S->StreakList.data() + t->s1;
+        }
         t->NearNodes = &near_[l];
         t->FarNodes = &far_[l];
         leaf_ptrs_[l] = t;
@@ -390,6 +442,11 @@ void MDiffusiveFast::process_vort_list() {
     S->check(vvhd_diffusive(S->ctx_));
 }
 
+void MDiffusiveFast::process_heat_list() {
+    S->set_params();
+    S->check(vvhd_diffusive_heat(S->ctx_));
+}
+
 // ------------------------------------------------------------- MFlowmove --
 
 MFlowmove::MFlowmove(Space* S_, double remove_eps_) : S(S_) {
@@ -408,6 +465,29 @@ void MFlowmove::move_and_clean(bool remove, const void** collision, size_t* clea
 void MFlowmove::vortex_shed() {
     S->check(vvhd_vortex_shed(S->ctx_));
     S->pull_bodies();
+}
+
+void MFlowmove::heat_shed() {
+    S->ensure_pushed();
+    S->check(vvhd_heat_shed(S->ctx_));
+    S->pull_bodies();
+}
+
+void MFlowmove::heat_crop(double scale) {
+    S->ensure_pushed();
+    S->check(vvhd_heat_crop(S->ctx_, scale));
+}
+
+// MFlowmove::streak_shed (MFlowmove.cpp:284-291): when the time is a whole
+// multiple of dt_streak.  The reference's TTime is an exact fraction; the
+// time here is a double on the dt grid, so a multiple is recognised within
+// 1e-9 of a step count.
+void MFlowmove::streak_shed() {
+    S->ensure_pushed();
+    if (!(S->dt_streak != 0)) return;   // TTime::divisibleBy: a zero divisor divides nothing
+    const double k = S->time / S->dt_streak;
+    if (!(std::fabs(k - std::nearbyint(k)) <= 1e-9 * std::max(1.0, std::fabs(k)))) return;
+    S->check(vvhd_streak_shed(S->ctx_));
 }
 
 // ---------------------------------------------------------------- XField --
