@@ -90,8 +90,8 @@ def parse():
                          "2000-segment body SLAE filled and solved in FP64 (one JSON line)")
     ap.add_argument("--sweep-fcs", default="8,64,256")
     ap.add_argument("--batched", action="store_true",
-                    help="64 independent cylinder simulations of uneven size in one context vs one at a time "
-                         "(one JSON line)")
+                    help="64 independent cylinder simulations of uneven size, whole steps, in one context vs one at "
+                         "a time (one JSON line)")
     ap.add_argument("--nsim", type=int, default=64)
     return ap.parse_args()
 
@@ -363,77 +363,138 @@ def precision_sweep(a):
     print(json.dumps(out), flush=True)
 
 
+def cylinder_wake(n, seed, nseg=500):
+    """A cylinder simulation's state of n vortices (TObj records): a
+    boundary layer round the cylinder (R = 0.5; 30 % of the vortices,
+    exponential in the distance from the wall) and a Karman street behind it
+    (staggered rows of Gaussian clusters, sigma 0.12, at y = +-0.35, spaced
+    0.5 in x from x = 1, one cluster per 4000 vortices, opposite signs per
+    row), circulations of the order of the body's shed vortices."""
+    rng = np.random.default_rng(seed)
+    nb = int(0.3 * n)
+    th = rng.uniform(0, 2 * np.pi, nb)
+    rr = 0.5 + 2e-3 + rng.exponential(0.15, nb)
+    out = np.zeros((n, 6))
+    out[:nb, 0], out[:nb, 1] = rr * np.cos(th), rr * np.sin(th)
+    out[:nb, 2] = rng.normal(size=nb) * 1e-3
+    nw = n - nb
+    ncl = max(4, nw // 4000)
+    lab = rng.integers(0, ncl, nw)
+    top = lab % 2 == 0
+    out[nb:, 0] = 1.0 + 0.5 * lab + 0.12 * rng.normal(size=nw)
+    out[nb:, 1] = np.where(top, 0.35, -0.35) + 0.12 * rng.normal(size=nw)
+    out[nb:, 2] = np.where(top, -1.0, 1.0) * rng.uniform(0.5, 1.5, nw) * 1e-3
+    return out
+
+
 def batched(a):
-    """Many independent simulations advanced together: nsim Karman-street
-    wakes of n_s vortices (n_s log-uniform in [10k, 400k], SplitMix64 seeds
-    0x5EED1000 + s) with Reynolds numbers log-uniform in [100, 3000].  One
-    step = tree build (far criterion 8) + epsilon (merging) + convective
-    (plain, FP32 Kahan) + diffusive, from a device-resident copy restored
-    each step.  Sequential: each simulation alone in the context, its mean
-    step time.  Batched: all simulations in one context (vvhd_set_batch), one
-    step for all."""
+    """Config 4: nsim independent cylinder simulations (R = 0.5, 500
+    segments, U = (1, 0), dt = 0.005) with Reynolds numbers log-uniform in
+    [100, 3000] and n_s vortices each, log-uniform in [10k, 400k] and scaled
+    to about 8M in all (cylinder_wake states, seeds 0x5EED2000 + s).  One
+    step is the reference's whole loop (utils/vvflow/vvflow.cpp:214-262):
+    tree, boundary system (FP64), shedding, forces, tree, epsilon (merging),
+    convective (plain, FP32 Kahan), diffusive, flow-move.
+    Sequential: each simulation alone in the context, W + K steps from its
+    state.  Batched: all simulations in one context (vvhd_set_batch), W + K
+    steps; every simulation evolves bit-identically to its sequential run,
+    so both do the same work.  Per-simulation figures: the time of its own
+    step (sequential: its mean step; batched: the step of all) and the time
+    at which its step is done when every simulation takes one step
+    (sequential: one after another, in order; batched: the step of all)."""
     import vvflow_amd as V
     rng = np.random.default_rng(0x5EED0040)
-    ns = np.exp(rng.uniform(np.log(10_000), np.log(400_000), a.nsim)).astype(np.int64)
+    raw = np.exp(rng.uniform(np.log(10_000), np.log(400_000), a.nsim))
+    ns = np.clip(raw * (8_000_000 / raw.sum()), 10_000, 400_000).astype(np.int64)
     res = np.exp(rng.uniform(np.log(100.0), np.log(3000.0), a.nsim))
-    wakes = [V.gen_wake(int(ns[s]), 0x5EED1000 + s) for s in range(a.nsim)]
+    nseg, dt = 500, 0.005
+    corners = V.gen_cylinder(0.5, nseg)
+    segs1 = np.ascontiguousarray(V.segments_from_corners(corners), dtype=V.SEG_DTYPE)
+    body1 = np.zeros(1, dtype=V.BODY_DTYPE)
+    body1["nseg"] = nseg
+    states = [cylinder_wake(int(ns[s]), 0x5EED2000 + s, nseg) for s in range(a.nsim)]
     ctx = V.Context(0)
-    ctx.set_params(re=600.0, dt=0.005)
-    ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
-    fc = 8
+    ctx.set_params(re=600.0, dt=dt)
+    ctx.upload_bodies(segs1, body1)
+    dl = ctx.download_bodies()[1][0]["slen"] / (nseg - 1)
+    tree_args = (8, 5 * dl, 100 * dl)
 
-    def step():
-        ctx.restore()
-        ctx.tree_build(fc)
+    def step(k, re):
+        ctx.set_params(re=re, dt=dt, time=k * dt)
+        ctx.tree_build(*tree_args)
+        ctx.calc_circulation((1.0, 0.0))
+        ctx.tree_destroy()
+        ctx.vortex_shed()
+        ctx.calc_forces()
+        ctx.zero_forces()
+        ctx.tree_build(*tree_args)
         ctx.epsilon(True)
         ctx.convective((1.0, 0.0), kahan=True)
         ctx.diffusive()
+        ctx.tree_destroy()
+        ctx.move_and_clean(True)
 
-    def timed(k, w):
-        for _ in range(w):
-            step()
+    def run(re, w, k):
+        """w untimed and k timed steps; particles stepped in the timed ones."""
+        for j in range(w):
+            step(j, re)
         ctx.phase_times()   # synchronises
+        parts = 0
         t0 = time.perf_counter()
-        for _ in range(k):
-            step()
+        for j in range(w, w + k):
+            parts += ctx.n
+            step(j, re)
         ctx.phase_times()
-        return (time.perf_counter() - t0) / k
+        return (time.perf_counter() - t0) / k, parts
 
-    seq = []
+    seq, seq_parts = [], 0
     for s in range(a.nsim):
-        ctx.set_params(re=float(res[s]), dt=0.005)
-        ctx.upload(wakes[s])
-        ctx.snapshot()
-        seq.append(timed(a.steps, 1))
+        ctx.upload_bodies(segs1, body1)
+        ctx.upload(states[s])
+        t, p = run(float(res[s]), a.warmup, a.steps)
+        seq.append(t)
+        seq_parts += p
         if (s + 1) % 8 == 0:
             print(f"sequential: {s + 1}/{a.nsim} simulations", file=sys.stderr, flush=True)
     seq = np.array(seq)
-    ctx.set_params(re=600.0, dt=0.005)
-    ctx.upload(np.concatenate(wakes))
+    segs = np.concatenate([segs1] * a.nsim)
+    bodies = np.concatenate([body1] * a.nsim)
+    bodies["first_seg"] = np.arange(a.nsim) * nseg
+    ctx.set_params(re=600.0, dt=dt)
+    ctx.upload_bodies(segs, bodies)
+    ctx.upload(np.concatenate(states))
     off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
-    ctx.set_batch(off, np.zeros(a.nsim + 1, dtype=np.int32), res)
-    ctx.snapshot()
-    t_batch = timed(a.steps, a.warmup)
+    ctx.set_batch(off, np.arange(a.nsim + 1, dtype=np.int32), res)
+    t_batch, batch_parts = run(600.0, a.warmup, a.steps)
     print("batched: done", file=sys.stderr, flush=True)
     ph = ctx.phase_times()
+    final = np.diff(ctx.batch_offsets())
     ctx.close()
-    total = int(ns.sum())
-    out = {"metric": "aggregate particle-steps/s, independent simulations advanced together",
-           "value": total / t_batch, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+    done_seq = np.cumsum(seq)   # each simulation's step done, one after another
+    value = batch_parts / (t_batch * a.steps)
+    out = {"metric": "aggregate particle-steps/s, independent cylinder simulations advanced together",
+           "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": t_batch * 1e3, "higher_is_better": True, "dtype": "fp32",
            "data": "synthetic",
-           "config": {"workload": f"{a.nsim} Karman-street wakes of {int(ns.min())}-{int(ns.max())} vortices "
-                                  f"({total} in all), Re {res.min():.0f}-{res.max():.0f}; one step = tree + "
-                                  "epsilon (merge) + convective (plain, Kahan FP32) + diffusive",
-                      "nsim": a.nsim, "far_criteria": fc},
-           "batched": {"ms_per_step": t_batch * 1e3, "per_simulation_step_ms_p50": t_batch * 1e3,
-                       "per_simulation_step_ms_p99": t_batch * 1e3},
-           "sequential": {"ms_all_simulations": float(seq.sum() * 1e3), "value": total / float(seq.sum()),
+           "config": {"workload": f"{a.nsim} cylinder simulations (500-segment cylinder, boundary layer and "
+                                  f"Karman street) of {int(ns.min())}-{int(ns.max())} vortices ({int(ns.sum())} "
+                                  f"in all), Re {res.min():.0f}-{res.max():.0f}; one step = the whole loop: "
+                                  "tree, boundary system (FP64), shedding, forces, tree, epsilon (merge), "
+                                  "convective (plain, Kahan FP32), diffusive, flow-move",
+                      "nsim": a.nsim, "far_criteria": 8, "vortices_after": int(final.sum())},
+           "batched": {"ms_per_step": t_batch * 1e3,
+                       "per_simulation_step_ms_p50": t_batch * 1e3, "per_simulation_step_ms_p99": t_batch * 1e3,
+                       "step_done_ms_p50": t_batch * 1e3, "step_done_ms_p99": t_batch * 1e3},
+           "sequential": {"ms_all_simulations": float(seq.sum() * 1e3),
+                          "value": seq_parts / (float(seq.sum()) * a.steps),
                           "per_simulation_step_ms_p50": float(np.percentile(seq, 50) * 1e3),
-                          "per_simulation_step_ms_p99": float(np.percentile(seq, 99) * 1e3)},
+                          "per_simulation_step_ms_p99": float(np.percentile(seq, 99) * 1e3),
+                          "step_done_ms_p50": float(np.percentile(done_seq, 50) * 1e3),
+                          "step_done_ms_p99": float(np.percentile(done_seq, 99) * 1e3)},
            "speedup_vs_sequential": float(seq.sum() / t_batch),
            "phases_ms_batched": {"tree": float(ph[0]), "epsilon": float(ph[1]), "convective": float(ph[2]),
-                                 "diffusive": float(ph[3])}}
+                                 "diffusive": float(ph[3]), "flow_move": float(ph[4]),
+                                 "boundary_system": float(ph[5])}}
     print(json.dumps(out), flush=True)
 
 

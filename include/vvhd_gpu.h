@@ -136,12 +136,36 @@ int vvhd_download_bodies(vvhd_ctx* ctx, vvhd_seg* segs, int32_t nseg,
  * epsilon, convective (plain and learned) and diffusive advance every
  * simulation in the same launches; no list, merge or sum crosses
  * simulations, and the direct sum of a target runs over its own simulation.
- * Holds until the next vortex or body upload (back to one simulation).
- * calc_circulation, move_and_clean, vortex_shed, calc_forces, zero_forces,
- * integrals, velocity_points and tree_find, whose reference counterparts
- * act on the whole Space, return VVHD_EUNSUPPORTED meanwhile. */
+ * calc_circulation solves one boundary system per simulation (all of one
+ * order: strided-batched LU), vortex_shed appends each simulation's new
+ * vortices after its own, move_and_clean removes each vortex only against
+ * its own simulation's bodies, and calc_forces uses each simulation's
+ * Reynolds number; the ranges then move (vvhd_batch_offsets).  Holds until
+ * the next vortex or body upload (back to one simulation).  integrals,
+ * velocity_points, tree_find, the point queries, the grid fields and the
+ * heat and streak lists, whose reference counterparts act on the whole
+ * Space, return VVHD_EUNSUPPORTED meanwhile. */
 int vvhd_set_batch(vvhd_ctx* ctx, int32_t nsim, const int64_t* vortex_off, const int32_t* body_off,
                    const double* re);
+/* The simulations' current vortex ranges (nsim + 1 offsets; vortex_off may
+ * be NULL to get the count). */
+int vvhd_batch_offsets(vvhd_ctx* ctx, int32_t* nsim, int64_t* vortex_off);
+
+/* ---- launch forms (no counterpart in the reference) ------------------
+ * The near-field kernel runs one wave per leaf (form 0) or as a persistent
+ * grid of 8, 16 or 32 waves per compute unit taking {leaf, 16 targets} work
+ * items from a device queue; every form gives the same bits.  With autotune
+ * set (the default) the form is the fastest of the four measured on the
+ * first call of each workload class (one or several simulations, leaves and
+ * targets per leaf in powers of two); otherwise near_form.  With the graph
+ * set, vvhd_convective (plain path, no heat or streak particles) runs as one
+ * captured HIP graph when the form is persistent: its kernels read every
+ * size from device memory, and the graph is captured once per set of
+ * buffers and replayed.  vvhd_tuning: {last form, workload classes tuned,
+ * graphs captured, graph launches}. */
+int vvhd_set_tuning(vvhd_ctx* ctx, int autotune, int near_form);
+int vvhd_set_graph(vvhd_ctx* ctx, int enable);
+int vvhd_tuning(vvhd_ctx* ctx, int64_t* out4);
 
 /* ---- tree: stree (TSortedTree.hpp:60-92, TSortedTree.cpp:232-303) ----
  * Builds the adaptive binary tree over the vortex list and the body

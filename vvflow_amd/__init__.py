@@ -66,6 +66,10 @@ ABI = {
     "vvhd_restore": (_I, [_P]),
     "vvhd_upload_bodies": (_I, [_P, _VP, _I32, _VP, _I32]),
     "vvhd_set_batch": (_I, [_P, _I32, _VP, _VP, _VP]),
+    "vvhd_batch_offsets": (_I, [_P, _VP, _VP]),
+    "vvhd_set_tuning": (_I, [_P, _I, _I]),
+    "vvhd_set_graph": (_I, [_P, _I]),
+    "vvhd_tuning": (_I, [_P, _VP]),
     "vvhd_download_bodies": (_I, [_P, _VP, _I32, _VP, _I32]),
     "vvhd_tree_build": (_I, [_P, _I, _D, _D, _VP]),
     "vvhd_tree_destroy": (_I, [_P]),
@@ -233,6 +237,29 @@ class Context:
         bo = np.ascontiguousarray(body_off, dtype=np.int32)
         r = np.ascontiguousarray(re, dtype=np.float64)
         self._check(self.L.vvhd_set_batch(self.h, len(r), _ptr(vo), _ptr(bo), _ptr(r)))
+
+    def set_tuning(self, autotune=True, near_form=0):
+        """Autotuned near-field form (default), or a fixed one: 0 one wave per
+        leaf, 8/16/32 persistent waves per compute unit."""
+        self._check(self.L.vvhd_set_tuning(self.h, int(autotune), int(near_form)))
+
+    def set_graph(self, enable):
+        """Run the plain convective phase as a captured HIP graph (persistent form)."""
+        self._check(self.L.vvhd_set_graph(self.h, int(enable)))
+
+    def tuning(self):
+        """{near_form, classes_tuned, graph_captures, graph_launches}."""
+        out = np.zeros(4, dtype=np.int64)
+        self._check(self.L.vvhd_tuning(self.h, _ptr(out)))
+        return dict(zip(("near_form", "classes_tuned", "graph_captures", "graph_launches"), out.tolist()))
+
+    def batch_offsets(self):
+        """Current vortex ranges of the simulations sharing the context."""
+        ns = C.c_int32(0)
+        self._check(self.L.vvhd_batch_offsets(self.h, C.addressof(ns), None))
+        out = np.zeros(ns.value + 1, dtype=np.int64)
+        self._check(self.L.vvhd_batch_offsets(self.h, C.addressof(ns), _ptr(out)))
+        return out
 
     @property
     def n(self):

@@ -180,7 +180,25 @@ struct FmArgs {
     int nbody;
     const vvhd_seg* segs;
     const double* geo;
+    // simulations sharing the context: vortex i of simulation s meets only
+    // the bodies [sim_body[s], sim_body[s + 1]) (nsim = 1: all bodies)
+    const int64_t* sim_off;
+    const int32_t* sim_body;
+    int nsim;
 };
+
+// Simulation of vortex i: the last s with sim_off[s] <= i.
+__device__ inline int sim_of(const int64_t* sim_off, int nsim, int64_t i) {
+    int lo = 0, hi = nsim - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sim_off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 // Moved position and removal class of vortex i: 1 kept, 2 removed in a body
 // (segment in *bs), 0 removed for |g| < remove_eps or past the list.
@@ -190,8 +208,14 @@ __device__ inline int fm_classify(const FmArgs& A, int64_t i, double& px, double
     px = A.x[i] + A.vx[i] * A.dt;
     py = A.y[i] + A.vy[i] * A.dt;
     if (fabs(A.g[i]) < kRemoveEps) return 0;
+    int b0 = 0, b1 = A.nbody;
+    if (A.nsim > 1) {
+        const int sm = sim_of(A.sim_off, A.nsim, i);
+        b0 = A.sim_body[sm];
+        b1 = A.sim_body[sm + 1];
+    }
     if (A.remove)
-        for (int b = 0; b < A.nbody; b++) {
+        for (int b = b0; b < b1; b++) {
             const int sgi = point_invalid(A.bodies[b], A.segs, A.geo + 5 * b, px, py);
             if (sgi >= 0) {
                 bs = sgi;
@@ -234,7 +258,8 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_scatter(FmArgs A, const int32_t
                                                          double* __restrict__ x2, double* __restrict__ y2,
                                                          double* __restrict__ g2, double* __restrict__ vx2,
                                                          double* __restrict__ vy2, double* __restrict__ ie2,
-                                                         double4* __restrict__ dead, int32_t* counts) {
+                                                         double4* __restrict__ dead, int32_t* counts,
+                                                         int32_t* __restrict__ kept_before) {
     __shared__ uint32_t s_cnt[2][kFmItems][kFmWaves];
     __shared__ uint32_t s_red[2][kFmWaves];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
@@ -292,6 +317,8 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_scatter(FmArgs A, const int32_t
             rd += s_cnt[1][k][v];
         }
         const int64_t i = base + k * kFmBlock + t;
+        // survivors before vortex i (the new offset of a simulation starting at i)
+        if (kept_before && i <= A.n) kept_before[i] = (int32_t)(rk + (uint32_t)__popcll(mk[k] & lt));
         if (mk[k] >> lane & 1) {
             const uint32_t q = rk + (uint32_t)__popcll(mk[k] & lt);
             x2[q] = px[k];
@@ -363,6 +390,41 @@ __global__ void k_shed_flags(vvhd_seg* segs, int nseg, int32_t* flag) {
     flag[s] = (fabs(r.g) >= kRemoveEps && !r.slip) ? 1 : 0;
 }
 
+// vortex_shed with several simulations: every vortex of simulation s moves
+// up by the shed count of the simulations before it (pos at the first
+// segment of s), and segment k of simulation s writes its vortex after the
+// (moved) vortices of s: at sim_off[s + 1] + pos[k].
+__global__ void k_shed_batch_move(const int64_t* sim_off, int nsim, const int32_t* sim_seg0, const int32_t* pos,
+                                  int64_t n, const double* x, const double* y, const double* g, const double* vx,
+                                  const double* vy, const double* ie, double* x2, double* y2, double* g2,
+                                  double* vx2, double* vy2, double* ie2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sm = sim_of(sim_off, nsim, i);
+    const int64_t w = i + pos[sim_seg0[sm]];
+    x2[w] = x[i];
+    y2[w] = y[i];
+    g2[w] = g[i];
+    vx2[w] = vx[i];
+    vy2[w] = vy[i];
+    ie2[w] = ie[i];
+}
+
+__global__ void k_shed_batch_write(const vvhd_seg* segs, int nseg, const int32_t* seg_sim, const int64_t* sim_off,
+                                   const int32_t* flag, const int32_t* pos, double* x, double* y, double* g,
+                                   double* vx, double* vy, double* ie) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg || !flag[k]) return;
+    const vvhd_seg& r = segs[k];
+    const int64_t w = sim_off[seg_sim[k] + 1] + pos[k];
+    x[w] = r.cx + (-r.dly) * 1e-4;
+    y[w] = r.cy + r.dlx * 1e-4;
+    g[w] = r.g;
+    vx[w] = 0;
+    vy[w] = 0;
+    ie[w] = 0;
+}
+
 __global__ void k_shed_write(const vvhd_seg* segs, int nseg, const int32_t* flag, const int32_t* pos, int64_t n,
                              double* x, double* y, double* g, double* vx, double* vy, double* ie) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,10 +440,11 @@ __global__ void k_shed_write(const vvhd_seg* segs, int nseg, const int32_t* flag
 }
 
 // Space::calc_forces (TSpace.cpp:94-120); fric holds the diffusive friction.
-__global__ void k_calc_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double* fric, double re, double dt,
-                              double pr) {
+__global__ void k_calc_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double* fric, double re_one,
+                              double dt, double pr, const double* sim_re, const int32_t* body_sim) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbody) return;
+    const double re = sim_re ? sim_re[body_sim[b]] : re_one;   // the body's simulation's Reynolds number
     const double C_NyuDt_Pi = dt / (M_PI * re);
     const double C_Nyu_Pi = 1. / (M_PI * re);
     vvhd_body& B = bodies[b];
@@ -458,28 +521,47 @@ __device__ double xi_g(double px, double py, const vvhd_seg& s, double rd) {
            xi_g_near(px, py, 0.5 * (c2x + p3x), 0.5 * (c2y + p3y), c2x - p3x, c2y - p3y, rd);
 }
 
+// One system per simulation (one in all without a batch), each of order n,
+// stored one after another (column-major, stride n * n; right-hand sides
+// and solutions stride n).  base holds each body's first equation within
+// its own system.
 struct SlaeMap {
     const vvhd_body* bodies;
-    const int32_t* base;   // first equation (and column) of each body
+    const int32_t* base;      // first equation (and column) of each body in its system
     int nbody;
-    int n;                 // order of the system
+    int n;                    // order of every system
+    const int32_t* sys_body;  // per system: first body (nsys + 1 entries)
+    const int32_t* seg_sys;   // per segment: its system
+    int nsys;
+    int max_sys_segs;         // most segments in one system
 };
+
+__device__ inline int body_of_seg_in(const vvhd_body* bodies, int b0, int b1, int s) {
+    int b = b0;
+    while (b + 1 < b1 && bodies[b + 1].first_seg <= s) b++;
+    return b;
+}
 
 // Influence cells of the slip rows: A(row of segment i, column of segment m)
 // = Xi_g(corner_m; segment i, |dl_m| / 4) / 2 pi  (:459-482).
 __global__ void k_slip_cells(SlaeMap M, const vvhd_seg* segs, int nseg, double* A) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nseg * nseg) return;
-    const int i = (int)(t / nseg), m = (int)(t % nseg);
-    const int bi = body_of_seg(M.bodies, M.nbody, i);
+    if (t >= (int64_t)nseg * M.max_sys_segs) return;
+    const int i = (int)(t / M.max_sys_segs), mm = (int)(t % M.max_sys_segs);
+    const int sys = M.seg_sys[i];
+    const int b0 = M.sys_body[sys], b1 = M.sys_body[sys + 1];
+    const int f0 = M.bodies[b0].first_seg, f1 = M.bodies[b1 - 1].first_seg + M.bodies[b1 - 1].nseg;
+    const int m = f0 + mm;
+    if (m >= f1) return;
+    const int bi = body_of_seg_in(M.bodies, b0, b1, i);
     const vvhd_body& Bi = M.bodies[bi];
     if (i - Bi.first_seg == Bi.special_segment_no) return;
     const int row = M.base[bi] + (i - Bi.first_seg);
-    const int bm = body_of_seg(M.bodies, M.nbody, m);
+    const int bm = body_of_seg_in(M.bodies, b0, b1, m);
     const int col = M.base[bm] + (m - M.bodies[bm].first_seg);
     const vvhd_seg& sm = segs[m];
     const double rd = sqrt(sm.dlx * sm.dlx + sm.dly * sm.dly) * 0.25;
-    A[(size_t)col * M.n + row] = xi_g(sm.cx, sm.cy, segs[i], rd) * k1_2Pi;
+    A[(size_t)sys * M.n * M.n + (size_t)col * M.n + row] = xi_g(sm.cx, sm.cy, segs[i], rd) * k1_2Pi;
 }
 
 // Right-hand side of the slip rows, one wave (= one block) per segment:
@@ -496,12 +578,16 @@ __global__ __launch_bounds__(64) void k_slip_rhs(SlaeMap M, TreeDev T, const vvh
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     if (i >= nseg) return;
-    const int bi = body_of_seg(M.bodies, M.nbody, i);
+    const int sys = M.seg_sys[i];
+    const int b0 = M.sys_body[sys], b1 = M.sys_body[sys + 1];
+    const int bi = body_of_seg_in(M.bodies, b0, b1, i);
     const vvhd_body& Bi = M.bodies[bi];
     if (i - Bi.first_seg == Bi.special_segment_no) return;
     const int row = M.base[bi] + (i - Bi.first_seg);
+    A += (size_t)sys * M.n * M.n;
+    rhs += (size_t)sys * M.n;
     const vvhd_seg s = segs[i];
-    int node = 0;
+    int node = sys;   // the root of the segment's simulation
     while (T.ch1[node] >= 0) {
         const int c = T.ch1[node];
         if (T.nh[node] < T.nw[node])
@@ -556,7 +642,7 @@ __global__ __launch_bounds__(64) void k_slip_rhs(SlaeMap M, TreeDev T, const vvh
         r += dot2(-0.0, 0.0, s.dlx, s.dly);   // rotl(body speed 0 / 2 pi) . dl
         r -= res * k1_2Pi;
         rhs[row] = r;
-        for (int jb = 0; jb < M.nbody; jb++) {
+        for (int jb = b0; jb < b1; jb++) {
             const double A1 = (jb == bi) ? k2Pi * s.dly : 0;
             const double A2 = (jb == bi) ? -k2Pi * s.dlx : 0;
             const double A3 = 0;
@@ -569,14 +655,20 @@ __global__ __launch_bounds__(64) void k_slip_rhs(SlaeMap M, TreeDev T, const vvh
 }
 
 // Special-segment rows and the nine body rows (:484-862, :896-975), one
-// thread: the steady-circulation rows depend on a counter over bodies.
+// thread per system: the steady-circulation rows depend on a counter over
+// the system's bodies.  gsum_all holds each system's vortex circulation.
 __global__ void k_body_rows(SlaeMap M, const vvhd_seg* segs, const double* gsum_all, double inf_g, double dt,
                             double* A, double* rhs) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys >= M.nsys) return;
     const int n = M.n;
+    A += (size_t)sys * n * n;
+    rhs += (size_t)sys * n;
+    gsum_all += sys;
+    const int b0 = M.sys_body[sys], b1 = M.sys_body[sys + 1];
     auto cell = [&](int row, int col) -> double& { return A[(size_t)col * n + row]; };
     int once = 0;
-    for (int b = 0; b < M.nbody; b++) {
+    for (int b = b0; b < b1; b++) {
         const vvhd_body& B = M.bodies[b];
         const int base = M.base[b], ns = B.nseg;
         const int cs = base + ns, ch = base + ns + 3, ck = base + ns + 6;   // speed, hydro, holder columns
@@ -587,7 +679,7 @@ __global__ void k_body_rows(SlaeMap M, const vvhd_seg* segs, const double* gsum_
                 cell(row, base + B.special_segment_no) = 1;
             } else if (!once++) {   // fillInfSteadyEquationForSegment (:508-522)
                 rhs[row] = -(*gsum_all) - inf_g;
-                for (int jb = 0; jb < M.nbody; jb++) {
+                for (int jb = b0; jb < b1; jb++) {
                     for (int m = 0; m < M.bodies[jb].nseg; m++) cell(row, M.base[jb] + m) = 1;
                     cell(row, M.base[jb] + M.bodies[jb].nseg + 2) = 2 * M.bodies[jb].area;
                 }
@@ -671,12 +763,16 @@ __global__ void k_body_rows(SlaeMap M, const vvhd_seg* segs, const double* gsum_
     }
 }
 
-// Total vortex circulation (Space::gsum, TSpace.hpp:93-99): per-thread
-// Neumaier sums over contiguous chunks combined in a fixed tree, so the
-// result is deterministic and at least as accurate as the serial sum.
-__global__ __launch_bounds__(1024) void k_gsum(const double* g, int64_t n, double* out) {
+// Total vortex circulation (Space::gsum, TSpace.hpp:93-99) of every
+// simulation, one block each over its range [sim_off[s], sim_off[s + 1]):
+// per-thread Neumaier sums over contiguous chunks combined in a fixed tree,
+// so the result is deterministic and at least as accurate as the serial sum.
+__global__ __launch_bounds__(1024) void k_gsum(const double* g, const int64_t* sim_off, double* out) {
     __shared__ double ss[1024], sc[1024];
     const int t = threadIdx.x;
+    g += sim_off[blockIdx.x];
+    const int64_t n = sim_off[blockIdx.x + 1] - sim_off[blockIdx.x];
+    out += blockIdx.x;
     const int64_t chunk = (n + 1023) / 1024;
     const int64_t a = min(n, (int64_t)t * chunk), b = min(n, a + chunk);
     double s = 0, c = 0;
@@ -707,11 +803,13 @@ __global__ void k_differ(const double* a, const double* b, size_t n, int32_t* di
     if (i < n && __double_as_longlong(a[i]) != __double_as_longlong(b[i])) atomicOr(differ, 1);
 }
 
-__global__ void k_writeback(SlaeMap M, vvhd_body* bodies, vvhd_seg* segs, const double* xsol) {
+__global__ void k_writeback(SlaeMap M, vvhd_body* bodies, vvhd_seg* segs, const double* xsol,
+                            const int32_t* body_sys) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= M.nbody) return;
     vvhd_body& B = bodies[b];
     const int base = M.base[b];
+    xsol += (size_t)body_sys[b] * M.n;
     for (int k = 0; k < B.nseg; k++) segs[B.first_seg + k].g = xsol[base + k];
     for (int i = 0; i < 3; i++) {
         B.speed_slae[i] = xsol[base + B.nseg + i];
@@ -767,6 +865,7 @@ void bodies_release(Ctx& c) {
 void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
     const int nb = (int)c.bodies_h.size();
     hipStream_t s = c.stream;
+    if (c.nsim > 1) sim_sync(c);   // the device copy of the simulations' ranges
     if (nb)
         hipLaunchKernelGGL(k_body_move, dim3(nblocks(nb, 64)), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p,
                            c.body_geo.p, c.dt);
@@ -782,11 +881,14 @@ void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
     if (n == 0) {
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, (uint64_t*)counts, (int64_t)1);
     } else {
+        const bool batch = c.nsim > 1;
+        if (batch) c.iscratch2.grow((size_t)ntiles * kFmTile + 1);
         const FmArgs A{c.x.p, c.y.p, c.vx.p, c.vy.p, c.g.p, c.ie.p, n, c.dt, remove, c.bodies_d.p, nb, c.segs.p,
-                       c.body_geo.p};
+                       c.body_geo.p, (const int64_t*)c.sim_off.p, (const int32_t*)c.sim_body.p, c.nsim};
         hipLaunchKernelGGL(k_fm_count, dim3(ntiles), dim3(kFmBlock), 0, s, A, tile_cnt);
         hipLaunchKernelGGL(k_fm_scatter, dim3(ntiles), dim3(kFmBlock), 0, s, A, (const int32_t*)tile_cnt, ntiles,
-                           c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p, dead, counts);
+                           c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p, dead, counts,
+                           batch ? c.iscratch2.p : (int32_t*)nullptr);
     }
     if (nb) hipLaunchKernelGGL(k_dead, dim3(1), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p, dead, counts);
     VVHD_HIP(hipGetLastError());
@@ -804,6 +906,15 @@ void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
     c.n = c.fm_counts[0];
     if (cleaned) *cleaned = c.fm_counts[1];
     c.t.built = false;
+    if (c.nsim > 1) {   // each simulation's new range: the survivors before its first vortex
+        std::vector<int32_t> kb(c.nsim);
+        for (int q = 1; q < c.nsim; q++)
+            if (c.sim_off_h[q] < n)
+                VVHD_HIP(hipMemcpyAsync(&kb[q], c.iscratch2.p + c.sim_off_h[q], 4, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipStreamSynchronize(s));
+        for (int q = 1; q < c.nsim; q++) c.sim_off_h[q] = c.sim_off_h[q] < n ? kb[q] : c.n;
+        c.sim_off_h[c.nsim] = c.n;
+    }
     if (c.heat.n || c.streak.n) passive_move_and_clean(c);
     if (nb) sync_bodies_to_host(c);
 }
@@ -818,6 +929,38 @@ void vortex_shed(Ctx& c) {
     hipLaunchKernelGGL(k_shed_flags, dim3(nblocks(nseg + 1)), dim3(kBlock), 0, c.stream, c.segs.p, nseg,
                        c.iscratch0.p);
     exclusive_scan_i32(c, c.iscratch0.p, c.iscratch1.p, (size_t)nseg + 1);
+    if (c.nsim > 1) {
+        // segments come in simulation order: simulation s sheds pos[seg0[s + 1]] - pos[seg0[s]]
+        sim_sync(c);
+        const int ns = c.nsim;
+        std::vector<int32_t> seg0(ns + 1, nseg);
+        for (int q = ns - 1; q >= 0; q--)
+            seg0[q] = c.sim_body_h[q] < c.sim_body_h[q + 1] ? c.bodies_h[c.sim_body_h[q]].first_seg : seg0[q + 1];
+        c.iscratch2.grow(ns + 1);
+        VVHD_HIP(hipMemcpyAsync(c.iscratch2.p, seg0.data(), (size_t)(ns + 1) * 4, hipMemcpyHostToDevice, c.stream));
+        if (c.n)
+            hipLaunchKernelGGL(k_shed_batch_move, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream,
+                               (const int64_t*)c.sim_off.p, ns, (const int32_t*)c.iscratch2.p,
+                               (const int32_t*)c.iscratch1.p, c.n, c.x.p, c.y.p, c.g.p, c.vx.p, c.vy.p, c.ie.p, c.x2.p,
+                               c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p);
+        hipLaunchKernelGGL(k_shed_batch_write, dim3(nblocks(nseg)), dim3(kBlock), 0, c.stream, c.segs.p, nseg,
+                           (const int32_t*)c.seg_root.p, (const int64_t*)c.sim_off.p, (const int32_t*)c.iscratch0.p,
+                           (const int32_t*)c.iscratch1.p, c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p);
+        std::vector<int32_t> pos(nseg + 1);
+        VVHD_HIP(hipMemcpyAsync(pos.data(), c.iscratch1.p, (size_t)(nseg + 1) * 4, hipMemcpyDeviceToHost, c.stream));
+        VVHD_HIP(hipStreamSynchronize(c.stream));
+        std::swap(c.x.p, c.x2.p);
+        std::swap(c.y.p, c.y2.p);
+        std::swap(c.g.p, c.g2.p);
+        std::swap(c.vx.p, c.vx2.p);
+        std::swap(c.vy.p, c.vy2.p);
+        std::swap(c.ie.p, c.ie2.p);
+        for (int q = 1; q <= ns; q++) c.sim_off_h[q] += pos[seg0[q]];
+        c.n += pos[nseg];
+        c.t.built = false;
+        sync_bodies_to_host(c);
+        return;
+    }
     hipLaunchKernelGGL(k_shed_write, dim3(nblocks(nseg)), dim3(kBlock), 0, c.stream, c.segs.p, nseg, c.iscratch0.p,
                        c.iscratch1.p, c.n, c.x.p, c.y.p, c.g.p, c.vx.p, c.vy.p, c.ie.p);
     int32_t added = 0;
@@ -831,8 +974,20 @@ void vortex_shed(Ctx& c) {
 void calc_forces(Ctx& c) {
     const int nb = (int)c.bodies_h.size();
     if (!nb) return;
+    const bool batch = c.nsim > 1;
+    int32_t* body_sim = nullptr;
+    if (batch) {
+        sim_sync(c);
+        std::vector<int32_t> bs(nb);
+        for (int q = 0; q < c.nsim; q++)
+            for (int b = c.sim_body_h[q]; b < c.sim_body_h[q + 1]; b++) bs[b] = q;
+        c.iscratch3.grow(nb);
+        body_sim = c.iscratch3.p;
+        VVHD_HIP(hipMemcpyAsync(body_sim, bs.data(), (size_t)nb * 4, hipMemcpyHostToDevice, c.stream));
+    }
     hipLaunchKernelGGL(k_calc_forces, dim3(nblocks(nb, 64)), dim3(64), 0, c.stream, c.bodies_d.p, nb, c.segs.p,
-                       c.fric.p, c.re, c.dt, c.pr);
+                       c.fric.p, c.re, c.dt, c.pr, batch ? (const double*)c.sim_re.p : (const double*)nullptr,
+                       (const int32_t*)body_sim);
     sync_bodies_to_host(c);
 }
 
@@ -849,39 +1004,59 @@ void calc_circulation(Ctx& c, double infx, double infy) {
     const int nb = (int)c.bodies_h.size();
     const int nseg = (int)c.segs_h.size();
     VVHD_CHECK(nb > 0, VVHD_ESTATE, "calc_circulation: no bodies");
-    const int n = nseg + 9 * nb;
     hipStream_t s = c.stream;
-    std::vector<int32_t> base(nb);
-    int acc = 0;
-    for (int b = 0; b < nb; b++) {
-        base[b] = acc;
-        acc += c.bodies_h[b].nseg + 9;
+    // one system per simulation, all of the same order
+    sim_sync(c);
+    const int nsys = c.nsim;
+    std::vector<int32_t> base(nb), body_sys(nb);
+    int n = -1, max_segs = 0;
+    for (int q = 0; q < nsys; q++) {
+        const int b0 = c.sim_body_h[q], b1 = c.sim_body_h[q + 1];
+        VVHD_CHECK(b1 > b0, VVHD_EUNSUPPORTED, "calc_circulation: a simulation without bodies in the batch");
+        int acc = 0, segs_q = 0;
+        for (int b = b0; b < b1; b++) {
+            base[b] = acc;
+            body_sys[b] = q;
+            acc += c.bodies_h[b].nseg + 9;
+            segs_q += c.bodies_h[b].nseg;
+        }
+        VVHD_CHECK(n < 0 || acc == n, VVHD_EUNSUPPORTED,
+                   "calc_circulation: the simulations of a batch must have systems of one order");
+        n = acc;
+        max_segs = std::max(max_segs, segs_q);
     }
-    c.slae_base.grow(nb);
-    VVHD_HIP(hipMemcpyAsync(c.slae_base.p, base.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s));
-    const size_t nn = (size_t)n * n;
-    c.slae_A.grow(nn);
-    c.slae_b.grow(n + 1);
-    c.slae_x.grow(n + 1);
-    VVHD_HIP(hipMemsetAsync(c.slae_A.p, 0, nn * 8, s));
-    VVHD_HIP(hipMemsetAsync(c.slae_b.p, 0, (size_t)(n + 1) * 8, s));
-    SlaeMap M{c.bodies_d.p, c.slae_base.p, nb, n};
-    hipLaunchKernelGGL(k_slip_cells, dim3(nblocks((int64_t)nseg * nseg)), dim3(kBlock), 0, s, M, c.segs.p, nseg,
+    c.slae_base.grow(2 * (size_t)nb + nsys + 1);
+    int32_t* d_base = c.slae_base.p;
+    int32_t* d_body_sys = c.slae_base.p + nb;
+    int32_t* d_sys_body = c.slae_base.p + 2 * nb;
+    VVHD_HIP(hipMemcpyAsync(d_base, base.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s));
+    VVHD_HIP(hipMemcpyAsync(d_body_sys, body_sys.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s));
+    VVHD_HIP(hipMemcpyAsync(d_sys_body, c.sim_body_h.data(), (size_t)(nsys + 1) * 4, hipMemcpyHostToDevice, s));
+    const size_t nn = (size_t)n * n, nall = nn * nsys;
+    c.slae_A.grow(nall);
+    c.slae_b.grow((size_t)n * nsys + 1);
+    c.slae_x.grow((size_t)n * nsys + nsys + 1);
+    VVHD_HIP(hipMemsetAsync(c.slae_A.p, 0, nall * 8, s));
+    VVHD_HIP(hipMemsetAsync(c.slae_b.p, 0, ((size_t)n * nsys + 1) * 8, s));
+    SlaeMap M{c.bodies_d.p, d_base, nb, n, d_sys_body, c.seg_root.p, nsys, max_segs};
+    double* gsum = c.slae_x.p + (size_t)n * nsys;   // per system, after the solutions
+    hipLaunchKernelGGL(k_slip_cells, dim3(nblocks((int64_t)nseg * max_segs)), dim3(kBlock), 0, s, M, c.segs.p, nseg,
                        c.slae_A.p);
     hipLaunchKernelGGL(k_slip_rhs, dim3(nseg), dim3(kWave), 0, s, M, tree_dev(c.t), c.segs.p, nseg, c.x.p, c.y.p,
                        c.g.p, c.ie.p, infx, infy, c.slae_A.p, c.slae_b.p);
-    hipLaunchKernelGGL(k_gsum, dim3(1), dim3(1024), 0, s, c.g.p, c.n, c.slae_x.p + n);
-    hipLaunchKernelGGL(k_body_rows, dim3(1), dim3(64), 0, s, M, c.segs.p, c.slae_x.p + n, c.inf_g, c.dt, c.slae_A.p,
-                       c.slae_b.p);
+    hipLaunchKernelGGL(k_gsum, dim3(nsys), dim3(1024), 0, s, c.g.p, (const int64_t*)c.sim_off.p, gsum);
+    hipLaunchKernelGGL(k_body_rows, dim3(nblocks(nsys, 64)), dim3(64), 0, s, M, c.segs.p, (const double*)gsum,
+                       c.inf_g, c.dt, c.slae_A.p, c.slae_b.p);
     c.slae_filled = n;
-    bool refactor = !(c.time > 0) || c.slae_n != n;
-    c.slae_info.grow(4);
+    c.slae_nsys = nsys;
+    bool refactor = !(c.time > 0) || c.slae_n != n || c.slae_nsys_factored != nsys;
+    c.slae_info.grow((size_t)nsys + 2);
     if (!refactor) {
-        VVHD_HIP(hipMemsetAsync(c.slae_info.p + 1, 0, 4, s));
-        hipLaunchKernelGGL(k_differ, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, s, c.slae_A.p, c.slae_Acached.p, nn,
-                           c.slae_info.p + 1);
+        VVHD_HIP(hipMemsetAsync(c.slae_info.p + nsys, 0, 4, s));
+        hipLaunchKernelGGL(k_differ, dim3(nblocks((int64_t)nall)), dim3(kBlock), 0, s, c.slae_A.p, c.slae_Acached.p,
+                           nall, c.slae_info.p + nsys);
         int32_t differ = 0;
-        VVHD_HIP(hipMemcpyAsync(&differ, c.slae_info.p + 1, 4, hipMemcpyDeviceToHost, s));
+        VVHD_HIP(hipMemcpyAsync(&differ, c.slae_info.p + nsys, 4, hipMemcpyDeviceToHost, s));
         VVHD_HIP(hipStreamSynchronize(s));
         refactor = differ != 0;
     }
@@ -894,24 +1069,31 @@ void calc_circulation(Ctx& c, double infx, double infy) {
     VVHD_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, VVHD_EHIP, "rocblas_set_stream failed");
     if (refactor) {
         c.slae_n = -1;
-        c.slae_Acached.grow(nn);
-        c.slae_LU.grow(nn);
-        c.slae_ipiv.grow(n + 1);
-        VVHD_HIP(hipMemcpyAsync(c.slae_Acached.p, c.slae_A.p, nn * 8, hipMemcpyDeviceToDevice, s));
-        VVHD_HIP(hipMemcpyAsync(c.slae_LU.p, c.slae_A.p, nn * 8, hipMemcpyDeviceToDevice, s));
-        VVHD_CHECK(rocsolver_dgetrf(h, n, n, c.slae_LU.p, n, c.slae_ipiv.p, c.slae_info.p) == rocblas_status_success,
-                   VVHD_EHIP, "rocsolver_dgetrf failed");
-        int32_t info = 0;
-        VVHD_HIP(hipMemcpyAsync(&info, c.slae_info.p, 4, hipMemcpyDeviceToHost, s));
+        c.slae_Acached.grow(nall);
+        c.slae_LU.grow(nall);
+        c.slae_ipiv.grow((size_t)n * nsys + 1);
+        VVHD_HIP(hipMemcpyAsync(c.slae_Acached.p, c.slae_A.p, nall * 8, hipMemcpyDeviceToDevice, s));
+        VVHD_HIP(hipMemcpyAsync(c.slae_LU.p, c.slae_A.p, nall * 8, hipMemcpyDeviceToDevice, s));
+        VVHD_CHECK(rocsolver_dgetrf_strided_batched(h, n, n, c.slae_LU.p, n, (rocblas_stride)nn, c.slae_ipiv.p,
+                                                    (rocblas_stride)n, c.slae_info.p, nsys) == rocblas_status_success,
+                   VVHD_EHIP, "rocsolver_dgetrf_strided_batched failed");
+        std::vector<int32_t> info(nsys);
+        VVHD_HIP(hipMemcpyAsync(info.data(), c.slae_info.p, (size_t)nsys * 4, hipMemcpyDeviceToHost, s));
         VVHD_HIP(hipStreamSynchronize(s));
-        if (info != 0) throw Error(VVHD_ESINGULAR, "calc_circulation: singular matrix, column " + std::to_string(info));
+        for (int q = 0; q < nsys; q++)
+            if (info[q] != 0)
+                throw Error(VVHD_ESINGULAR, "calc_circulation: singular matrix of simulation " + std::to_string(q) +
+                                                ", column " + std::to_string(info[q]));
         c.slae_n = n;
+        c.slae_nsys_factored = nsys;
     }
-    VVHD_HIP(hipMemcpyAsync(c.slae_x.p, c.slae_b.p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
-    VVHD_CHECK(rocsolver_dgetrs(h, rocblas_operation_none, n, 1, c.slae_LU.p, n, c.slae_ipiv.p, c.slae_x.p, n) ==
-                   rocblas_status_success,
-               VVHD_EHIP, "rocsolver_dgetrs failed");
-    hipLaunchKernelGGL(k_writeback, dim3(nblocks(nb, 64)), dim3(64), 0, s, M, c.bodies_d.p, c.segs.p, c.slae_x.p);
+    VVHD_HIP(hipMemcpyAsync(c.slae_x.p, c.slae_b.p, (size_t)n * nsys * 8, hipMemcpyDeviceToDevice, s));
+    VVHD_CHECK(rocsolver_dgetrs_strided_batched(h, rocblas_operation_none, n, 1, c.slae_LU.p, n, (rocblas_stride)nn,
+                                                c.slae_ipiv.p, (rocblas_stride)n, c.slae_x.p, n, (rocblas_stride)n,
+                                                nsys) == rocblas_status_success,
+               VVHD_EHIP, "rocsolver_dgetrs_strided_batched failed");
+    hipLaunchKernelGGL(k_writeback, dim3(nblocks(nb, 64)), dim3(64), 0, s, M, c.bodies_d.p, c.segs.p, c.slae_x.p,
+                       (const int32_t*)d_body_sys);
     sync_bodies_to_host(c);
 }
 
@@ -924,7 +1106,6 @@ extern "C" {
 int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned) {
     REQUIRE_CTX(ctx);
     API_BEGIN
-    REQUIRE_ONE_SIM(ctx->c, "move_and_clean");
     Ctx& c = ctx->c;
     VVHD_HIP(hipSetDevice(c.device));
     VVHD_HIP(hipEventRecord(c.ev[8], c.stream));
@@ -936,7 +1117,6 @@ int vvhd_move_and_clean(vvhd_ctx* ctx, int remove, int64_t* cleaned) {
 int vvhd_vortex_shed(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
     API_BEGIN
-    REQUIRE_ONE_SIM(ctx->c, "vortex_shed");
     VVHD_HIP(hipSetDevice(ctx->c.device));
     vortex_shed(ctx->c);
     API_END(ctx)
@@ -945,7 +1125,6 @@ int vvhd_vortex_shed(vvhd_ctx* ctx) {
 int vvhd_calc_circulation(vvhd_ctx* ctx, double inf_x, double inf_y) {
     REQUIRE_CTX(ctx);
     API_BEGIN
-    REQUIRE_ONE_SIM(ctx->c, "calc_circulation");
     Ctx& c = ctx->c;
     VVHD_HIP(hipSetDevice(c.device));
     VVHD_CHECK(c.t.built, VVHD_ESTATE, "calc_circulation: tree is not built");
@@ -982,7 +1161,6 @@ int vvhd_slae_system(vvhd_ctx* ctx, int32_t* n, double* A, double* rhs) {
 int vvhd_calc_forces(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
     API_BEGIN
-    REQUIRE_ONE_SIM(ctx->c, "calc_forces");
     VVHD_HIP(hipSetDevice(ctx->c.device));
     calc_forces(ctx->c);
     API_END(ctx)
@@ -991,7 +1169,6 @@ int vvhd_calc_forces(vvhd_ctx* ctx) {
 int vvhd_zero_forces(vvhd_ctx* ctx) {
     REQUIRE_CTX(ctx);
     API_BEGIN
-    REQUIRE_ONE_SIM(ctx->c, "zero_forces");
     VVHD_HIP(hipSetDevice(ctx->c.device));
     zero_forces(ctx->c);
     API_END(ctx)

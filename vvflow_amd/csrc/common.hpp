@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -278,6 +279,7 @@ struct Ctx {
     DBuf<double> slae_A, slae_Acached, slae_LU, slae_b, slae_x;
     DBuf<int32_t> slae_base, slae_ipiv, slae_info;
     int slae_n = -1;                         // order of the cached factorisation
+    int slae_nsys = 1, slae_nsys_factored = 0;   // systems filled / factorised (one per simulation)
     int slae_filled = 0;                     // order of the last filled system
     void* rocblas = nullptr;                 // rocblas_handle
 
@@ -320,6 +322,20 @@ struct Ctx {
     // field evaluators: per-vortex weights, the grid map, running gsum
     DBuf<double> fld_a, fld_b, fld_gtmp;
     DBuf<float> fld_map;
+
+    // near-field work items and queue (persistent form), the autotuner's
+    // choices per workload class, and the captured convective graphs
+    DBuf<int2> near_items;
+    DBuf<int32_t> near_q;
+    int n_cu = 256;                          // compute units of the device
+    bool autotune = true;                    // vvhd_set_tuning
+    int near_form = 0;                       // the form used with the autotuner off
+    int near_last = 0;                       // the form of the last near-field launch
+    std::map<uint64_t, int> tune_near;
+    hipEvent_t tune_ev[2] = {};
+    bool use_graph = false;                  // vvhd_set_graph
+    std::vector<std::pair<std::vector<uint8_t>, void*>> conv_graphs;
+    int64_t graph_captures = 0, graph_launches = 0;
 
     // timing
     hipEvent_t ev[16] = {};

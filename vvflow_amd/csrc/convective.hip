@@ -20,11 +20,25 @@
 //     are combined in FP64.
 //  3. k_conv_finish, one thread per vortex: v += U_inf + near + far terms in
 //     the reference's order, FP64.
+//
+// The near kernel has a persistent form: a fixed grid whose waves take work
+// items -- a leaf and the first of up to 16 of its targets -- from a device
+// queue, so that a leaf of many targets spreads over many waves and no block
+// waits for one long leaf.  Each item runs exactly the arithmetic of the
+// one-wave-per-leaf form for its targets, so both give the same bits.  A
+// small autotuner times the one-wave-per-leaf form and the persistent form
+// at several grid sizes on the first call of every workload class (number of
+// simulations, leaves and targets per leaf, in powers of two) and keeps the
+// fastest.  With the persistent form every size the phase needs is read from
+// device memory, so the whole phase (queue reset, work items, Taylor terms,
+// near field, finish) is captured once as a HIP graph and replayed while its
+// buffers stay where they are.
 
 #include "common.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -76,6 +90,34 @@ __global__ __launch_bounds__(kBlock) void k_far_taylor(TreeDev T, double4* taylo
     a3 = wave_sum(a3);
     a4 = wave_sum(a4);
     if (lane_id() == 0) taylor[l] = make_double4(-a1 * k1_2Pi, a2 * k1_2Pi, a3 * k1_Pi, a4 * k1_2Pi);
+}
+
+// k_far_taylor over a fixed grid, the leaf count read from the device.
+__global__ __launch_bounds__(kBlock) void k_far_taylor_gs(TreeDev T, const int32_t* nleaf_dev, double4* taylor) {
+    const int nl = *nleaf_dev;
+    const int waves = gridDim.x * kWavesPerBlock;
+    for (int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; l < nl; l += waves) {
+        const int L = T.leaf_node[l];
+        const double cx = T.nx[L], cy = T.ny[L];
+        double a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+        for (int64_t e = T.far_off[l] + lane_id(); e < T.far_off[l + 1]; e += kWave) {
+            const int F = T.far_list[e];
+            const double dPx = cx - T.cmp_x[F], dPy = cy - T.cmp_y[F];
+            const double dMx = cx - T.cmm_x[F], dMy = cy - T.cmm_y[F];
+            const double aP = dPx * dPx + dPy * dPy, aM = dMx * dMx + dMy * dMy;
+            const double fP1 = T.cmp_g[F] / aP, fM1 = T.cmm_g[F] / aM;
+            const double fP2 = T.cmp_g[F] / (aP * aP), fM2 = T.cmm_g[F] / (aM * aM);
+            a1 += (fP1 * dPy + fM1 * dMy);
+            a2 += (fP1 * dPx + fM1 * dMx);
+            a3 += (fP2 * dPy * dPx + fM2 * dMy * dMx);
+            a4 += (fP2 * (dPy * dPy - dPx * dPx) + fM2 * (dMy * dMy - dMx * dMx));
+        }
+        a1 = wave_sum(a1);
+        a2 = wave_sum(a2);
+        a3 = wave_sum(a3);
+        a4 = wave_sum(a4);
+        if (lane_id() == 0) taylor[l] = make_double4(-a1 * k1_2Pi, a2 * k1_2Pi, a3 * k1_Pi, a4 * k1_2Pi);
+    }
 }
 
 // Only what the near-field kernel reads (keeps its scalar registers free for
@@ -275,6 +317,57 @@ __device__ inline void near_chunk(const NearArgs& A, const double* __restrict__ 
     }
 }
 
+// Near-field work items: one per leaf and chunk of kTargets targets,
+// {leaf, first target}; q[0] counts them (the order of the items does not
+// matter: each writes only its own targets' sums).
+__global__ void k_near_items(NearArgs A, const int32_t* nleaf_dev, int2* items, int32_t* q) {
+    const int nl = *nleaf_dev;
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nl; l += gridDim.x * blockDim.x) {
+        const int L = A.leaf_node[l];
+        const int v0 = A.v0[L], v1 = A.v1[L];
+        const int k = (v1 - v0 + kTargets - 1) / kTargets;
+        if (k == 0) continue;
+        const int b = atomicAdd(&q[0], k);
+        for (int c = 0; c < k; c++) items[b + c] = make_int2(l, v0 + c * kTargets);
+    }
+}
+
+__global__ void k_fill_i32q(int32_t* a, int n) {
+    if (threadIdx.x < n) a[threadIdx.x] = 0;
+}
+
+template <bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_near_persist(NearArgs A, const int2* __restrict__ items, int32_t* q,
+                                                         const double* __restrict__ x, const double* __restrict__ y,
+                                                         const float2* __restrict__ src_ge,
+                                                         double2* __restrict__ near_out) {
+    __shared__ int s_tab[kWavesPerBlock][kNearTab];
+    const int w = threadIdx.x / kWave;
+    const int total = q[0];
+    for (;;) {
+        int it = 0;
+        if (lane_id() == 0) it = atomicAdd(&q[1], 1);
+        it = __shfl(it, 0, kWave);
+        if (it >= total) break;   // every wave reaches this: the queue only grows past total
+        const int2 item = items[it];
+        const int l = item.x, base = item.y;
+        const int L = A.leaf_node[l];
+        const int nt = min(kTargets, A.v1[L] - base);
+        const double cx = A.nx[L], cy = A.ny[L];
+        const int64_t e_beg = A.near_off[l], e_end = A.near_off[l + 1];
+        switch ((nt + 1) / 2) {
+            case 1: near_chunk<2, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 2: near_chunk<4, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 3: near_chunk<6, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 4: near_chunk<8, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 5: near_chunk<10, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 6: near_chunk<12, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            case 7: near_chunk<14, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+            default: near_chunk<16, KAHAN>(A, x, y, src_ge, near_out, base, nt, cx, cy, e_beg, e_end, s_tab[w]); break;
+        }
+    }
+}
+
 template <bool KAHAN>
 __global__ __launch_bounds__(kBlock) void k_near(NearArgs A, const double* __restrict__ x,
                                                  const double* __restrict__ y, const float2* __restrict__ src_ge,
@@ -328,6 +421,35 @@ __global__ void k_conv_finish(const int32_t* vleaf, const double* nx, const doub
     ay += T.w * dxl + (-T.z) * dyl;
     vx[i] = ax;
     vy[i] = ay;
+}
+
+// k_conv_finish over a fixed grid, the vortex count read from the device.
+__global__ void k_conv_finish_gs(const int32_t* vleaf, const double* nx, const double* ny, const int32_t* leaf_node,
+                                 const double4* taylor, const double* x, const double* y, const double* g,
+                                 const double2* near_in, double* vx, double* vy, const int64_t* n_dev, double inf_x,
+                                 double inf_y) {
+    const int64_t n = *n_dev;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (g[i] == 0) continue;
+        const int l = vleaf[i];
+        const int L = leaf_node[l];
+        const double4 T = taylor[l];
+        const double2 nr = near_in[i];
+        double ax = vx[i] + inf_x, ay = vy[i] + inf_y;
+        ax += nr.x * k1_2Pi;
+        ay += nr.y * k1_2Pi;
+        ax += 0.0;
+        ay += 0.0;
+        ax += 0.0;
+        ay += 0.0;
+        ax += T.x;
+        ay += T.y;
+        const double dxl = x[i] - nx[L], dyl = y[i] - ny[L];
+        ax += T.z * dxl + T.w * dyl;
+        ay += T.w * dxl + (-T.z) * dyl;
+        vx[i] = ax;
+        vy[i] = ay;
+    }
 }
 
 // MConvectiveFast::velocity for arbitrary points, FP64 (one wave per point).
@@ -444,32 +566,180 @@ __global__ __launch_bounds__(kDsBlock) void k_direct_sum(const double* x, const 
 
 }  // namespace
 
+namespace {
+
+constexpr int kPersistWaves[] = {8, 16, 32};   // persistent grid: waves per compute unit
+
+// Persistent-grid blocks for waves_per_cu waves on every compute unit.
+int persist_blocks(const Ctx& c, int waves_per_cu) {
+    return std::max(1, c.n_cu * waves_per_cu / kWavesPerBlock);
+}
+
+// The near field in the form `choice` names: 0 one wave per leaf, k > 0 the
+// persistent grid with k waves per compute unit (items built first).
+void launch_near(Ctx& c, int choice, int kahan, hipStream_t s, double2* near_out) {
+    const TreeDev T = tree_dev(c.t);
+    const int nl = T.nleaf;
+    NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
+    if (choice == 0) {
+        const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (kahan)
+            hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, near_out);
+        else
+            hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, near_out);
+        return;
+    }
+    const int32_t* nleaf_dev = c.t.counters.p + 1;
+    hipLaunchKernelGGL(k_fill_i32q, dim3(1), dim3(64), 0, s, c.near_q.p, 2);
+    hipLaunchKernelGGL(k_near_items, dim3(nblocks(nl)), dim3(kBlock), 0, s, A, nleaf_dev, c.near_items.p, c.near_q.p);
+    const int g = persist_blocks(c, choice);
+    if (kahan)
+        hipLaunchKernelGGL(k_near_persist<true>, dim3(g), dim3(kBlock), 0, s, A, (const int2*)c.near_items.p,
+                           c.near_q.p, c.x.p, c.y.p, c.src_ge.p, near_out);
+    else
+        hipLaunchKernelGGL(k_near_persist<false>, dim3(g), dim3(kBlock), 0, s, A, (const int2*)c.near_items.p,
+                           c.near_q.p, c.x.p, c.y.p, c.src_ge.p, near_out);
+}
+
+// Workload class of the near field: simulations (one or several), leaves and
+// targets per leaf in powers of two.
+uint64_t near_key(const Ctx& c, int kahan) {
+    const int64_t nl = std::max<int64_t>(1, c.t.nleaves_h);
+    const int lg_nl = 63 - __builtin_clzll((unsigned long long)nl);
+    const int lg_per = 63 - __builtin_clzll((unsigned long long)std::max<int64_t>(1, c.n / nl));
+    return ((uint64_t)(c.nsim > 1) << 32) | ((uint64_t)kahan << 24) | ((uint64_t)lg_nl << 8) | (uint64_t)lg_per;
+}
+
+// The form of the near field for this workload: the one measured fastest the
+// first time the class was seen (each candidate once, HIP events on s), or
+// one wave per leaf with the autotuner off.
+int near_choice(Ctx& c, int kahan, hipStream_t s, double2* near_out) {
+    if (!c.autotune) return c.near_form;
+    const uint64_t key = near_key(c, kahan);
+    auto it = c.tune_near.find(key);
+    if (it != c.tune_near.end()) return it->second;
+    int best = 0;
+    float best_ms = 0;
+    const int cands[] = {0, kPersistWaves[0], kPersistWaves[1], kPersistWaves[2]};
+    for (int k = 0; k < 4; k++) {
+        VVHD_HIP(hipEventRecord(c.tune_ev[0], s));
+        launch_near(c, cands[k], kahan, s, near_out);
+        VVHD_HIP(hipEventRecord(c.tune_ev[1], s));
+        VVHD_HIP(hipEventSynchronize(c.tune_ev[1]));
+        float ms = 0;
+        VVHD_HIP(hipEventElapsedTime(&ms, c.tune_ev[0], c.tune_ev[1]));
+        if (k == 0 || ms < best_ms) {
+            best = cands[k];
+            best_ms = ms;
+        }
+    }
+    c.tune_near[key] = best;
+    return best;
+}
+
+void grow_near_items(Ctx& c) {
+    c.near_items.grow((size_t)(c.n / kTargets + c.t.nleaves_h + 2));
+    c.near_q.grow(4);
+}
+
+}  // namespace
+
 // Near-field sums of every vortex into near_out (n + 1 entries; the plain and
 // the learned path share it).
 void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out) {
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf == 0) return;
-    const int nl = T.nleaf;
-    const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
-    NearArgs A{T.leaf_node, T.v0, T.v1, T.nx, T.ny, T.near_off, T.near_list, nl};
+    grow_near_items(c);
+    const int choice = near_choice(c, kahan, s, near_out);
+    c.near_last = choice;
     VVHD_HIP(hipEventRecord(c.ev[12], s));
-    if (kahan)
-        hipLaunchKernelGGL(k_near<true>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, near_out);
-    else
-        hipLaunchKernelGGL(k_near<false>, dim3(grid), dim3(kBlock), 0, s, A, c.x.p, c.y.p, c.src_ge.p, near_out);
+    launch_near(c, choice, kahan, s, near_out);
     VVHD_HIP(hipEventRecord(c.ev[13], s));
     c.phase_recorded[6] = true;
 }
 
+namespace {
+
+// The convective phase as one graph: queue reset, work items, Taylor terms,
+// persistent near field, finish -- every size read from the device.  A graph
+// is kept per set of buffers and arguments (every pointer its kernels read
+// or write, the free stream and the form), since the flow-move swaps the
+// vortex arrays with their double buffers every step and buffers move when
+// they grow; at most four are kept.
+void convective_graph(Ctx& c, double inf_x, double inf_y, int kahan, int choice, double4* taylor, double2* near_out) {
+    const TreeDev T = tree_dev(c.t);
+    const void* ps[] = {c.x.p,       c.y.p,      c.g.p,       c.vx.p,       c.vy.p,      c.src_ge.p,  taylor,
+                        near_out,    c.near_items.p, c.near_q.p, c.t.counters.p, c.sim_off.p + c.nsim, T.leaf_node,
+                        T.v0,        T.v1,       T.nx,        T.ny,         T.near_off,  T.near_list, T.far_off,
+                        T.far_list,  T.cmp_x,    T.cmp_y,     T.cmp_g,      T.cmm_x,     T.cmm_y,     T.cmm_g,
+                        T.vleaf};
+    const double fs[2] = {inf_x, inf_y};
+    const int is[3] = {kahan, choice, c.n_cu};
+    std::vector<uint8_t> key(sizeof(ps) + sizeof(fs) + sizeof(is));
+    std::memcpy(key.data(), ps, sizeof(ps));
+    std::memcpy(key.data() + sizeof(ps), fs, sizeof(fs));
+    std::memcpy(key.data() + sizeof(ps) + sizeof(fs), is, sizeof(is));
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : c.conv_graphs)
+        if (e.first == key) exec = (hipGraphExec_t)e.second;
+    if (!exec) {
+        VVHD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        const int g = persist_blocks(c, choice);
+        const int32_t* nleaf_dev = c.t.counters.p + 1;
+        hipLaunchKernelGGL(k_far_taylor_gs, dim3(g), dim3(kBlock), 0, c.stream, T, nleaf_dev, taylor);
+        launch_near(c, choice, kahan, c.stream, near_out);
+        hipLaunchKernelGGL(k_conv_finish_gs, dim3(g), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
+                           (const double4*)taylor, c.x.p, c.y.p, c.g.p, (const double2*)near_out, c.vx.p, c.vy.p,
+                           (const int64_t*)(c.sim_off.p + c.nsim), inf_x, inf_y);
+        hipGraph_t graph = nullptr;
+        VVHD_HIP(hipStreamEndCapture(c.stream, &graph));
+        VVHD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        VVHD_HIP(hipGraphDestroy(graph));
+        if (c.conv_graphs.size() >= 4) {
+            (void)hipGraphExecDestroy((hipGraphExec_t)c.conv_graphs.front().second);
+            c.conv_graphs.erase(c.conv_graphs.begin());
+        }
+        c.conv_graphs.emplace_back(key, (void*)exec);
+        c.graph_captures++;
+    }
+    VVHD_HIP(hipEventRecord(c.ev[12], c.stream));
+    VVHD_HIP(hipGraphLaunch(exec, c.stream));
+    VVHD_HIP(hipEventRecord(c.ev[13], c.stream));
+    c.phase_recorded[6] = true;
+    c.graph_launches++;
+}
+
+}  // namespace
+
 // process_all_lists with the per-leaf Taylor terms in taylor (nleaf + 1) and
 // the per-vortex near sums in near_out (n + 1): every entry is written before
 // it is read, so neither needs clearing.
-static void convective_into(Ctx& c, double inf_x, double inf_y, int kahan, double4* taylor, double2* near_out) {
+// noalloc: the allocation-free caller -- one wave per leaf, no work-item
+// buffer, no tuning, no graph.
+static void convective_into(Ctx& c, double inf_x, double inf_y, int kahan, double4* taylor, double2* near_out,
+                            bool noalloc = false) {
     const TreeDev T = tree_dev(c.t);
     const int nl = T.nleaf;
+    if (!noalloc) grow_near_items(c);
+    if (!noalloc && c.use_graph && !(c.heat.n || c.streak.n)) {
+        const int choice = near_choice(c, kahan, c.stream, near_out);
+        c.near_last = choice;
+        if (choice > 0) {
+            convective_graph(c, inf_x, inf_y, kahan, choice, taylor, near_out);
+            VVHD_HIP(hipGetLastError());
+            return;
+        }
+    }
     const int grid = (nl + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_far_taylor, dim3(grid), dim3(kBlock), 0, c.stream, T, taylor);
-    convective_near(c, kahan, c.stream, near_out);
+    if (noalloc) {
+        VVHD_HIP(hipEventRecord(c.ev[12], c.stream));
+        launch_near(c, 0, kahan, c.stream, near_out);
+        VVHD_HIP(hipEventRecord(c.ev[13], c.stream));
+        c.phase_recorded[6] = true;
+    } else {
+        convective_near(c, kahan, c.stream, near_out);
+    }
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
                        taylor, c.x.p, c.y.p, c.g.p, near_out, c.vx.p, c.vy.p, c.n, inf_x, inf_y);
     VVHD_HIP(hipGetLastError());
@@ -545,7 +815,7 @@ void convective_noalloc(Ctx& c, double inf_x, double inf_y, int kahan) {
     VVHD_CHECK(base, VVHD_ESTATE, "convective_noalloc: scratch of this device not set up");
     double4* taylor = (double4*)base;
     double2* near_out = (double2*)(base + align256((size_t)(c.t.nleaves_h + 1) * sizeof(double4)));
-    convective_into(c, inf_x, inf_y, kahan, taylor, near_out);
+    convective_into(c, inf_x, inf_y, kahan, taylor, near_out, true);
 }
 
 void velocity_points(Ctx& c, const double* xy, int64_t n, double inf_x, double inf_y, double* v_out) {

@@ -162,6 +162,16 @@ int vvhd_create(int device, vvhd_ctx** out) {
             delete ctx;
             return VVHD_EHIP;
         }
+    for (auto& e : ctx->c.tune_ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return VVHD_EHIP;
+        }
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+            ctx->c.n_cu = ncu;
+    }
     if (hipStreamCreateWithFlags(&ctx->c.stream2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->c.stream3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->c.ev_pre_conv, hipEventDisableTiming) != hipSuccess ||
@@ -231,6 +241,12 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     t.cmm_ie.release();
     for (auto& e : c.ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c.tune_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& g : c.conv_graphs) (void)hipGraphExecDestroy((hipGraphExec_t)g.second);
+    c.conv_graphs.clear();
+    c.near_items.release();
+    c.near_q.release();
     if (c.ev_pre_conv) (void)hipEventDestroy(c.ev_pre_conv);
     if (c.ev_side) (void)hipEventDestroy(c.ev_side);
     for (hipStream_t* st : {&c.stream2, &c.stream3})
@@ -414,6 +430,51 @@ int vvhd_set_batch(vvhd_ctx* ctx, int32_t nsim, const int64_t* vortex_off, const
     c.sim_body_h.assign(body_off, body_off + nsim + 1);
     c.sim_re_h.assign(re, re + nsim);
     c.t.built = false;
+    API_END(ctx)
+}
+
+int vvhd_set_tuning(vvhd_ctx* ctx, int autotune, int near_form) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    VVHD_CHECK(near_form == 0 || near_form == 8 || near_form == 16 || near_form == 32, VVHD_EINVAL,
+               "set_tuning: near_form must be 0 (one wave per leaf) or 8, 16, 32 (persistent waves per CU)");
+    ctx->c.autotune = autotune != 0;
+    ctx->c.near_form = near_form;
+    ctx->c.tune_near.clear();
+    API_END(ctx)
+}
+
+int vvhd_set_graph(vvhd_ctx* ctx, int enable) {
+    REQUIRE_CTX(ctx);
+    ctx->c.use_graph = enable != 0;
+    return VVHD_OK;
+}
+
+int vvhd_tuning(vvhd_ctx* ctx, int64_t* out4) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    VVHD_CHECK(out4, VVHD_EINVAL, "tuning: null output");
+    out4[0] = ctx->c.near_last;
+    out4[1] = (int64_t)ctx->c.tune_near.size();
+    out4[2] = ctx->c.graph_captures;
+    out4[3] = ctx->c.graph_launches;
+    API_END(ctx)
+}
+
+int vvhd_batch_offsets(vvhd_ctx* ctx, int32_t* nsim, int64_t* vortex_off) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_CHECK(nsim, VVHD_EINVAL, "batch_offsets: null count");
+    *nsim = c.nsim;
+    if (vortex_off) {
+        if (c.nsim == 1) {
+            vortex_off[0] = 0;
+            vortex_off[1] = c.n;
+        } else {
+            std::memcpy(vortex_off, c.sim_off_h.data(), (size_t)(c.nsim + 1) * 8);
+        }
+    }
     API_END(ctx)
 }
 
