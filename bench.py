@@ -93,6 +93,8 @@ def parse():
                     help="64 independent cylinder simulations of uneven size, whole steps, in one context vs one at "
                          "a time (one JSON line)")
     ap.add_argument("--nsim", type=int, default=64)
+    ap.add_argument("--graph", action="store_true",
+                    help="batched: run the plain convective phase as a captured HIP graph")
     return ap.parse_args()
 
 
@@ -363,27 +365,42 @@ def precision_sweep(a):
     print(json.dumps(out), flush=True)
 
 
-def cylinder_wake(n, seed, nseg=500):
-    """A cylinder simulation's state of n vortices (TObj records): a
-    boundary layer round the cylinder (R = 0.5; 30 % of the vortices,
-    exponential in the distance from the wall) and a Karman street behind it
-    (staggered rows of Gaussian clusters, sigma 0.12, at y = +-0.35, spaced
-    0.5 in x from x = 1, one cluster per 4000 vortices, opposite signs per
-    row), circulations of the order of the body's shed vortices."""
+def cylinder_wake(n, seed):
+    """A cylinder simulation's state of n vortices (TObj records) near the
+    balance the method keeps by itself: the merge distance of epsilon grows
+    with the distance d from the body as 0.4 |dl| sqrt(1 + d)
+    (MEpsilonFast.cpp:29-43), so the vortices sit on jittered lattices of
+    spacing 0.006 sqrt(1 + d) -- rings round the cylinder (R = 0.5; 10 % of
+    them) and columns behind it across |y| <= w, w = max(1, n / 1e5) -- with
+    circulation -sign(y) U(0.5, 1.5) 1e-3 (negative above the axis, as the
+    upper side sheds).  Few of them merge on the first step (about 0.1 %
+    against the FP64 oracle at 100k and 400k)."""
     rng = np.random.default_rng(seed)
-    nb = int(0.3 * n)
-    th = rng.uniform(0, 2 * np.pi, nb)
-    rr = 0.5 + 2e-3 + rng.exponential(0.15, nb)
+    c, jit = 0.006, 0.2
+    pts = []
+    nb = int(0.1 * n)
+    r, cnt = 0.502, 0
+    while cnt < nb:
+        h = c * np.sqrt(1.0 + (r - 0.5))
+        m = int(2 * np.pi * r / h)
+        th = (np.arange(m) + rng.uniform(-jit, jit, m)) * (2 * np.pi / m)
+        rr = r + rng.uniform(-jit, jit, m) * h
+        pts.append(np.stack([rr * np.cos(th), rr * np.sin(th)], 1))
+        cnt += m
+        r += h
+    layer = np.concatenate(pts)[:nb]
+    w = max(1.0, n / 1e5)
+    pts, x, cnt = [], max(1.0, r + 0.1), 0
+    while cnt < n - nb:
+        h = c * np.sqrt(1.0 + x)
+        m = int(2 * w / h)
+        y = -w + (np.arange(m) + 0.5 + rng.uniform(-jit, jit, m)) * (2 * w / m)
+        pts.append(np.stack([x + rng.uniform(-jit, jit, m) * h, y], 1))
+        cnt += m
+        x += h
     out = np.zeros((n, 6))
-    out[:nb, 0], out[:nb, 1] = rr * np.cos(th), rr * np.sin(th)
-    out[:nb, 2] = rng.normal(size=nb) * 1e-3
-    nw = n - nb
-    ncl = max(4, nw // 4000)
-    lab = rng.integers(0, ncl, nw)
-    top = lab % 2 == 0
-    out[nb:, 0] = 1.0 + 0.5 * lab + 0.12 * rng.normal(size=nw)
-    out[nb:, 1] = np.where(top, 0.35, -0.35) + 0.12 * rng.normal(size=nw)
-    out[nb:, 2] = np.where(top, -1.0, 1.0) * rng.uniform(0.5, 1.5, nw) * 1e-3
+    out[:, :2] = np.concatenate([layer, np.concatenate(pts)[:n - nb]])
+    out[:, 2] = np.where(out[:, 1] > 0, -1.0, 1.0) * rng.uniform(0.5, 1.5, n) * 1e-3
     return out
 
 
@@ -391,7 +408,7 @@ def batched(a):
     """Config 4: nsim independent cylinder simulations (R = 0.5, 500
     segments, U = (1, 0), dt = 0.005) with Reynolds numbers log-uniform in
     [100, 3000] and n_s vortices each, log-uniform in [10k, 400k] and scaled
-    to about 8M in all (cylinder_wake states, seeds 0x5EED2000 + s).  One
+    to about 8M in all (cylinder_wake lattices, seeds 0x5EED2000 + s).  One
     step is the reference's whole loop (utils/vvflow/vvflow.cpp:214-262):
     tree, boundary system (FP64), shedding, forces, tree, epsilon (merging),
     convective (plain, FP32 Kahan), diffusive, flow-move.
@@ -412,8 +429,9 @@ def batched(a):
     segs1 = np.ascontiguousarray(V.segments_from_corners(corners), dtype=V.SEG_DTYPE)
     body1 = np.zeros(1, dtype=V.BODY_DTYPE)
     body1["nseg"] = nseg
-    states = [cylinder_wake(int(ns[s]), 0x5EED2000 + s, nseg) for s in range(a.nsim)]
+    states = [cylinder_wake(int(ns[s]), 0x5EED2000 + s) for s in range(a.nsim)]
     ctx = V.Context(0)
+    ctx.set_graph(a.graph)
     ctx.set_params(re=600.0, dt=dt)
     ctx.upload_bodies(segs1, body1)
     dl = ctx.download_bodies()[1][0]["slen"] / (nseg - 1)
@@ -469,6 +487,7 @@ def batched(a):
     print("batched: done", file=sys.stderr, flush=True)
     ph = ctx.phase_times()
     final = np.diff(ctx.batch_offsets())
+    tuning = ctx.tuning()
     ctx.close()
     done_seq = np.cumsum(seq)   # each simulation's step done, one after another
     value = batch_parts / (t_batch * a.steps)
@@ -476,12 +495,14 @@ def batched(a):
            "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": t_batch * 1e3, "higher_is_better": True, "dtype": "fp32",
            "data": "synthetic",
-           "config": {"workload": f"{a.nsim} cylinder simulations (500-segment cylinder, boundary layer and "
-                                  f"Karman street) of {int(ns.min())}-{int(ns.max())} vortices ({int(ns.sum())} "
+           "config": {"workload": f"{a.nsim} cylinder simulations (500-segment cylinder, vortex lattices round "
+                                  f"and behind it) of {int(ns.min())}-{int(ns.max())} vortices ({int(ns.sum())} "
                                   f"in all), Re {res.min():.0f}-{res.max():.0f}; one step = the whole loop: "
                                   "tree, boundary system (FP64), shedding, forces, tree, epsilon (merge), "
                                   "convective (plain, Kahan FP32), diffusive, flow-move",
-                      "nsim": a.nsim, "far_criteria": 8, "vortices_after": int(final.sum())},
+                      "nsim": a.nsim, "far_criteria": 8, "vortices_after": int(final.sum()),
+                      "near_form": tuning["near_form"], "graph": bool(a.graph),
+                      "graph_launches": tuning["graph_launches"]},
            "batched": {"ms_per_step": t_batch * 1e3,
                        "per_simulation_step_ms_p50": t_batch * 1e3, "per_simulation_step_ms_p99": t_batch * 1e3,
                        "step_done_ms_p50": t_batch * 1e3, "step_done_ms_p99": t_batch * 1e3},

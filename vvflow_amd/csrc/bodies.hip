@@ -259,7 +259,8 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_scatter(FmArgs A, const int32_t
                                                          double* __restrict__ g2, double* __restrict__ vx2,
                                                          double* __restrict__ vy2, double* __restrict__ ie2,
                                                          double4* __restrict__ dead, int32_t* counts,
-                                                         int32_t* __restrict__ kept_before) {
+                                                         int32_t* __restrict__ kept_before,
+                                                         int32_t* __restrict__ dead_before) {
     __shared__ uint32_t s_cnt[2][kFmItems][kFmWaves];
     __shared__ uint32_t s_red[2][kFmWaves];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
@@ -318,7 +319,10 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_scatter(FmArgs A, const int32_t
         }
         const int64_t i = base + k * kFmBlock + t;
         // survivors before vortex i (the new offset of a simulation starting at i)
-        if (kept_before && i <= A.n) kept_before[i] = (int32_t)(rk + (uint32_t)__popcll(mk[k] & lt));
+        if (kept_before && i <= A.n) {
+            kept_before[i] = (int32_t)(rk + (uint32_t)__popcll(mk[k] & lt));
+            dead_before[i] = (int32_t)(rd + (uint32_t)__popcll(md[k] & lt));
+        }
         if (mk[k] >> lane & 1) {
             const uint32_t q = rk + (uint32_t)__popcll(mk[k] & lt);
             x2[q] = px[k];
@@ -345,13 +349,26 @@ __global__ void k_fill_u64(uint64_t* p, int64_t n) {
 
 // Dead force, moment and circulation of the removed vortices in vortex order
 // (MFlowmove.cpp:124-144), then the zero/slip segment loop (:205-214).
-__global__ void k_dead(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double4* dead, const int32_t* counts) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int nd = counts[1];
-    for (int q = 0; q < nd; q++) {
+// One thread per body: the dead records of its simulation's vortices (a
+// contiguous run of the records, in vortex order) whose segment is its own,
+// then its zero and slip segments.  With one simulation the run is every
+// record.  dead_off: per simulation, the records before its first vortex
+// (nullptr: one simulation).
+__global__ void k_dead(vvhd_body* bodies, int nbody, vvhd_seg* segs, const double4* dead, const int32_t* counts,
+                       const int32_t* body_sim, const int32_t* dead_off) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbody) return;
+    vvhd_body& B = bodies[b];
+    const int f0 = B.first_seg, f1 = B.first_seg + B.nseg;
+    int q0 = 0, q1 = counts[1];
+    if (dead_off) {
+        q0 = dead_off[body_sim[b]];
+        q1 = dead_off[body_sim[b] + 1];
+    }
+    for (int q = q0; q < q1; q++) {
         const double4 r = dead[q];
         const int s = (int)r.w;
-        vvhd_body& B = bodies[body_of_seg(bodies, nbody, s)];
+        if (s < f0 || s >= f1) continue;
         const double gi = r.z, xi = r.x, yi = r.y;
         B.fdt_dead[0] += (-yi) * gi;
         B.fdt_dead[1] += xi * gi;
@@ -360,20 +377,27 @@ __global__ void k_dead(vvhd_body* bodies, int nbody, vvhd_seg* segs, const doubl
         segs[s].gsum -= gi;
         B.g_dead += gi;
     }
-    for (int b = 0; b < nbody; b++) {
-        vvhd_body& B = bodies[b];
-        const double ax = B.holder[0] + B.dpos[0], ay = B.holder[1] + B.dpos[1];
-        for (int k = 0; k < B.nseg; k++) {
-            vvhd_seg& s = segs[B.first_seg + k];
-            if (fabs(s.g) < kRemoveEps || s.slip) {
-                B.fdt_dead[0] += (-s.cy) * s.g;
-                B.fdt_dead[1] += s.cx * s.g;
-                B.fdt_dead[2] += ((s.cx - ax) * (s.cx - ax) + (s.cy - ay) * (s.cy - ay)) * s.g;
-                s.gsum -= s.g;
-                B.g_dead += s.g;
-            }
+    const double ax = B.holder[0] + B.dpos[0], ay = B.holder[1] + B.dpos[1];
+    for (int k = 0; k < B.nseg; k++) {
+        vvhd_seg& s = segs[B.first_seg + k];
+        if (fabs(s.g) < kRemoveEps || s.slip) {
+            B.fdt_dead[0] += (-s.cy) * s.g;
+            B.fdt_dead[1] += s.cx * s.g;
+            B.fdt_dead[2] += ((s.cx - ax) * (s.cx - ax) + (s.cy - ay) * (s.cy - ay)) * s.g;
+            s.gsum -= s.g;
+            B.g_dead += s.g;
         }
     }
+}
+
+// Per simulation s: the dead records before its first vortex, from the
+// per-vortex counts the scatter wrote (dead_before), and the total last.
+__global__ void k_dead_off(const int64_t* sim_off, int nsim, const int32_t* dead_before, const int32_t* counts,
+                           int64_t n, int32_t* dead_off) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nsim) return;
+    const int64_t i = sim_off[s];
+    dead_off[s] = (s == nsim || i >= n) ? counts[1] : dead_before[i];
 }
 
 // Where each segment's shed vortex lands: a flag per segment, positions by an
@@ -882,15 +906,34 @@ void move_and_clean(Ctx& c, int remove, int64_t* cleaned) {
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, (uint64_t*)counts, (int64_t)1);
     } else {
         const bool batch = c.nsim > 1;
-        if (batch) c.iscratch2.grow((size_t)ntiles * kFmTile + 1);
+        if (batch) {
+            c.iscratch2.grow((size_t)ntiles * kFmTile + 1);
+            c.iscratch3.grow((size_t)ntiles * kFmTile + 1);
+        }
         const FmArgs A{c.x.p, c.y.p, c.vx.p, c.vy.p, c.g.p, c.ie.p, n, c.dt, remove, c.bodies_d.p, nb, c.segs.p,
                        c.body_geo.p, (const int64_t*)c.sim_off.p, (const int32_t*)c.sim_body.p, c.nsim};
         hipLaunchKernelGGL(k_fm_count, dim3(ntiles), dim3(kFmBlock), 0, s, A, tile_cnt);
         hipLaunchKernelGGL(k_fm_scatter, dim3(ntiles), dim3(kFmBlock), 0, s, A, (const int32_t*)tile_cnt, ntiles,
                            c.x2.p, c.y2.p, c.g2.p, c.vx2.p, c.vy2.p, c.ie2.p, dead, counts,
-                           batch ? c.iscratch2.p : (int32_t*)nullptr);
+                           batch ? c.iscratch2.p : (int32_t*)nullptr, batch ? c.iscratch3.p : (int32_t*)nullptr);
     }
-    if (nb) hipLaunchKernelGGL(k_dead, dim3(1), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p, dead, counts);
+    if (nb) {
+        const int32_t* body_sim = nullptr;
+        int32_t* dead_off = nullptr;
+        if (c.nsim > 1 && n) {
+            std::vector<int32_t> bs(nb);
+            for (int q = 0; q < c.nsim; q++)
+                for (int b = c.sim_body_h[q]; b < c.sim_body_h[q + 1]; b++) bs[b] = q;
+            c.iscratch4.grow((size_t)nb + c.nsim + 2);
+            VVHD_HIP(hipMemcpyAsync(c.iscratch4.p, bs.data(), (size_t)nb * 4, hipMemcpyHostToDevice, s));
+            body_sim = c.iscratch4.p;
+            dead_off = c.iscratch4.p + nb;
+            hipLaunchKernelGGL(k_dead_off, dim3(nblocks(c.nsim + 1)), dim3(kBlock), 0, s, (const int64_t*)c.sim_off.p,
+                               c.nsim, (const int32_t*)c.iscratch3.p, (const int32_t*)counts, n, dead_off);
+        }
+        hipLaunchKernelGGL(k_dead, dim3(nblocks(nb, 64)), dim3(64), 0, s, c.bodies_d.p, nb, c.segs.p, dead, counts,
+                           body_sim, (const int32_t*)dead_off);
+    }
     VVHD_HIP(hipGetLastError());
     VVHD_HIP(hipEventRecord(c.ev[9], s));   // end of the device work of the phase
     VVHD_HIP(hipMemcpyAsync(c.fm_counts, counts, 8, hipMemcpyDeviceToHost, s));

@@ -148,6 +148,7 @@ struct TreeArrays {
     DBuf<int64_t> seg_off;                // per leaf
     DBuf<int32_t> seg_list;               // segment ids
     DBuf<int32_t> seg_leaf;               // per segment: leaf (DFS)
+    DBuf<int32_t> seg_keys, seg_ids;      // sort scratch of the per-leaf segment lists
     DBuf<int32_t> vleaf;                  // per element (vortex, heat, streak): leaf (DFS)
     DBuf<int32_t> node_L, node_m, leaf_tmp, seg_node;   // build scratch
     DBuf<int32_t> cnt_near, cnt_far, err;               // list scratch

@@ -264,7 +264,7 @@ void vvhd_destroy(vvhd_ctx* ctx) {
     c.hres.release();
     c.hflag.release();
     c.hdirty.release();
-    for (auto* b : {&t.h0, &t.h1, &t.s0, &t.s1}) b->release();
+    for (auto* b : {&t.h0, &t.h1, &t.s0, &t.s1, &t.seg_keys, &t.seg_ids}) b->release();
     for (auto* b : {&c.fld_a, &c.fld_b, &c.fld_gtmp}) b->release();
     c.fld_map.release();
     for (auto& e : c.lev_ev)

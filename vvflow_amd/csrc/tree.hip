@@ -28,6 +28,7 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -875,19 +876,13 @@ __global__ void k_leaf_offsets(const int64_t* slot_off, int nleaf, int64_t* leaf
 }
 
 // Segments: leaf of each segment, then stable order by (leaf, index).
-__global__ void k_seg_sort(const int32_t* seg_node, const int32_t* leaf_rank, int nseg, int32_t* seg_leaf,
-                           int32_t* seg_list) {
-    for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_leaf[s] = leaf_rank[seg_node[s]];
-    __syncthreads();
-    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-        const int ls = seg_leaf[s];
-        int pos = 0;
-        for (int t = 0; t < nseg; t++) {
-            const int lt = seg_leaf[t];
-            pos += (lt < ls) || (lt == ls && t < s);
-        }
-        seg_list[pos] = s;
-    }
+// Leaf (DFS rank) of every segment, and the segment numbers to sort by it.
+__global__ void k_seg_leaf(const int32_t* seg_node, const int32_t* leaf_rank, int nseg, int32_t* seg_leaf,
+                           int32_t* ids) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    seg_leaf[s] = leaf_rank[seg_node[s]];
+    ids[s] = s;
 }
 
 __global__ void k_seg_off(const int32_t* seg_leaf, const int32_t* seg_list, int nseg, int nleaf, int64_t* seg_off) {
@@ -1198,8 +1193,21 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
     t.seg_leaf.grow(nseg + 1);
     t.seg_list.grow(nseg + 1);
     if (nseg) {
-        hipLaunchKernelGGL(k_seg_sort, dim3(1), dim3(1024), 0, s, seg_node.p, t.leaf_rank.p, nseg, t.seg_leaf.p,
-                           t.seg_list.p);
+        // segments of every leaf in their own order (the stable partition of
+        // the segment list, TSortedTree.cpp:139-148): a stable radix sort of
+        // the segment numbers by leaf
+        t.seg_keys.grow(nseg + 1);
+        t.seg_ids.grow(nseg + 1);
+        hipLaunchKernelGGL(k_seg_leaf, dim3(nblocks(nseg)), dim3(kBlock), 0, s, seg_node.p, t.leaf_rank.p, nseg,
+                           t.seg_leaf.p, t.seg_ids.p);
+        int bits = 1;
+        while (bits < 31 && (1 << bits) <= nl) bits++;
+        size_t bytes = 0;
+        VVHD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, t.seg_leaf.p, t.seg_keys.p, t.seg_ids.p, t.seg_list.p,
+                                           (unsigned)nseg, 0, bits, s));
+        c.tmp.grow(bytes + 16);
+        VVHD_HIP(rocprim::radix_sort_pairs(c.tmp.p, bytes, t.seg_leaf.p, t.seg_keys.p, t.seg_ids.p, t.seg_list.p,
+                                           (unsigned)nseg, 0, bits, s));
         hipLaunchKernelGGL(k_seg_off, dim3(nblocks(nl + 1)), dim3(kBlock), 0, s, t.seg_leaf.p, t.seg_list.p, nseg, nl,
                            t.seg_off.p);
     } else {
