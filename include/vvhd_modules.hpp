@@ -131,6 +131,11 @@ class Space {
     double dt = 1;
     double finish = 0;
     double dt_streak = 0;   // period of streak shedding (the reference keeps it as a TTime)
+    double dt_save = 0, dt_profile = 0;
+    std::string caption;
+    std::string inf_vx = "1", inf_vy = "0";   // the free stream's expressions, kept for snapshots
+    TVec inf_marker{0, 0};
+    TVec gravity{0, 0};
     std::vector<std::shared_ptr<TBody>> BodyList;
     std::vector<TObj> VortexList;
     std::vector<TObj> HeatList;
@@ -147,6 +152,14 @@ class Space {
     double gmax();
     double integral();
     TVec hydrodynamic_momentum();
+
+    // HDF5 snapshots (TSpace.hpp: save / load, the format of
+    // TSpace_save_hdf.cpp and TSpace_load_hdf.cpp) through libvvhd_hdf.so:
+    // save pulls the device state first; load replaces the lists and bodies
+    // (every vortex with v = 0 and _1_eps = 0, as the reference's reader
+    // leaves them) and pushes them to the device.
+    void save(const char* path);
+    void load(const char* path);
 
     // device synchronisation (no counterpart in the reference)
     void push();

@@ -71,6 +71,40 @@ def test_cpp_driver_heat_and_streak_match_python_driver():
     assert 500 * 2 < len(h) < 500 * steps
 
 
+def test_cpp_driver_snapshot_save_and_load(tmp_path):
+    """Space::save after ten steps of the C++ driver holds the state the
+    Python driver reaches with the same calls (vortices and the body's
+    corners, bound circulation and gsum, bit for bit); Space::load restarts
+    the driver from it (every vortex with _1_eps = 0, as the reference's
+    reader leaves them)."""
+    from vvflow_amd import hdf
+    print("case C++ driver snapshot, 10 steps", flush=True)
+    path = str(tmp_path / "cyl.h5")
+    r = subprocess.run([EXE, "--steps", "10", "--save", path], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sp, lists, bodies = hdf.load(path)
+    ctx = V.Context(0)
+    run = CylinderRun(ctx)
+    for _ in range(10):
+        run.step()
+    v = ctx.download()
+    segs = ctx.download_bodies()[0]
+    ctx.close()
+    assert sp["caption"] == "cylinder" and sp["re"] == 600.0 and sp["dt"] == 0.005
+    assert sp["time"] == pytest.approx(0.05, abs=1e-12)
+    assert np.array_equal(lists["vort"][:, :3], v[:, :3])
+    assert len(bodies) == 1 and bodies[0]["nseg"] == 500
+    seg4 = bodies[0]["seg4"]
+    assert np.array_equal(seg4[:, 0], segs["cx"]) and np.array_equal(seg4[:, 1], segs["cy"])
+    assert np.array_equal(seg4[:, 2], segs["g"]) and np.array_equal(seg4[:, 3], segs["gsum"])
+    r2 = subprocess.run([EXE, "--load", path, "--steps", "3"], capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    rows = [ln.split() for ln in r2.stdout.splitlines() if not ln.startswith("#")]
+    assert len(rows) == 3 and float(rows[0][1]) == pytest.approx(0.055, abs=1e-12)
+    assert int(rows[-1][2]) > len(v)
+    print(f"  snapshot of {len(v)} vortices; restarted for 3 steps: {rows[-1][2]} vortices", flush=True)
+
+
 def read_xfield(path):
     """XField's binary matrix (XField.cpp:124-152) -> (xs, ys, map)."""
     a = np.fromfile(path, dtype=np.float32)

@@ -7,7 +7,7 @@
 //                   [--learned model.npy] [--far-criteria 8] [--every 1]
 //                   [--sensors points.txt --sensors-out out.txt]
 //                   [--fields PREFIX] [--heat COND:HEAT_CONST:PR]
-//                   [--streak N:DT_STREAK]
+//                   [--streak N:DT_STREAK] [--save PATH] [--load PATH]
 //
 // Prints one line per `every` steps: step, time, vortex count, force_hydro
 // (x, y, o), friction (x, y, o), step wall time in ms; then a summary line.
@@ -23,6 +23,8 @@
 // 1 isolate, 2 const_t, 3 const_w), every segment heat_const and Space::pr;
 // --streak puts N streak sources on the segment x = -1, y in [-1, 1], shed
 // every DT_STREAK.  The summary then also reports the heat and streak lists.
+// --save writes an HDF5 snapshot of the final state (Space::save); --load
+// starts from a snapshot instead of an empty cylinder (Space::load).
 
 #include <algorithm>
 #include <chrono>
@@ -76,7 +78,7 @@ std::vector<float> load_npy_f32(const std::string& path) {
 int main(int argc, char** argv) {
     int steps = 400, nseg = 500, fc = 8, every = 1;
     double re = 600, dt = 0.005, radius = 0.5;
-    std::string learned, sensors_in, sensors_out, fields;
+    std::string learned, sensors_in, sensors_out, fields, save_path, load_path;
     int heat_cond = 0, nstreak = 0;
     double heat_const = 0, pr = 0, dt_streak = 0;
     for (int k = 1; k < argc; k++) {
@@ -95,6 +97,8 @@ int main(int argc, char** argv) {
         else if (a == "--sensors") sensors_in = next();
         else if (a == "--sensors-out") sensors_out = next();
         else if (a == "--fields") fields = next();
+        else if (a == "--save") save_path = next();
+        else if (a == "--load") load_path = next();
         else if (a == "--heat") {
             if (std::sscanf(next(), "%d:%lf:%lf", &heat_cond, &heat_const, &pr) != 3)
                 throw std::invalid_argument("--heat expects COND:HEAT_CONST:PR");
@@ -109,21 +113,26 @@ int main(int argc, char** argv) {
     }
     try {
         Space S(0);
-        S.re = re;
-        S.dt = dt;
-        S.inf_vel = TVec(1, 0);
-        auto body = std::make_shared<TBody>();
-        std::vector<double> xy(2 * (size_t)nseg);
-        if (vvhd_gen_cylinder(radius, nseg, xy.data()) != VVHD_OK) throw std::runtime_error("gen_cylinder failed");
-        for (int k = 0; k < nseg; k++) body->alist.emplace_back(xy[2 * k], xy[2 * k + 1]);
-        body->heat_condition = static_cast<hc_t>(heat_cond);
-        for (TAtt& a : body->alist) a.heat_const = (float)heat_const;
-        S.BodyList.push_back(body);
-        S.pr = pr;
-        S.dt_streak = dt_streak;
-        for (int k = 0; k < nstreak; k++)
-            S.StreakSourceList.emplace_back(-1.0, nstreak > 1 ? -1.0 + 2.0 * k / (nstreak - 1) : 0.0, 0.0);
-        S.push();
+        if (!load_path.empty()) {
+            S.load(load_path.c_str());
+        } else {
+            S.re = re;
+            S.dt = dt;
+            S.inf_vel = TVec(1, 0);
+            S.caption = "cylinder";
+            auto body = std::make_shared<TBody>();
+            std::vector<double> xy(2 * (size_t)nseg);
+            if (vvhd_gen_cylinder(radius, nseg, xy.data()) != VVHD_OK) throw std::runtime_error("gen_cylinder failed");
+            for (int k = 0; k < nseg; k++) body->alist.emplace_back(xy[2 * k], xy[2 * k + 1]);
+            body->heat_condition = static_cast<hc_t>(heat_cond);
+            for (TAtt& a : body->alist) a.heat_const = (float)heat_const;
+            S.BodyList.push_back(body);
+            S.pr = pr;
+            S.dt_streak = dt_streak;
+            for (int k = 0; k < nstreak; k++)
+                S.StreakSourceList.emplace_back(-1.0, nstreak > 1 ? -1.0 + 2.0 * k / (nstreak - 1) : 0.0, 0.0);
+            S.push();
+        }
 
         const double dl = S.average_segment_length();
         const double min_node_size = dl > 0 ? dl * 5 : 0;
@@ -177,6 +186,7 @@ int main(int argc, char** argv) {
         }
         printf("# steps %d vortices %lld total %.1f ms mean %.3f ms/step\n", steps, (long long)S.vortex_count(),
                total_ms, total_ms / std::max(1, steps));
+        if (!save_path.empty()) S.save(save_path.c_str());
         if (heat_cond || nstreak) {
             S.pull();
             double hg = 0;

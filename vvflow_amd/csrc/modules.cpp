@@ -4,10 +4,14 @@
 
 #include "vvhd_modules.hpp"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+
+#include "vvhd_hdf.h"
 
 namespace {
 
@@ -237,6 +241,188 @@ TVec Space::hydrodynamic_momentum() {
     double out[5];
     check(vvhd_integrals(ctx_, out));
     return TVec(out[2], out[3]);
+}
+
+// ------------------------------------------------------------ snapshots --
+
+namespace {
+
+// libvvhd_hdf.so beside this library, opened on first use.
+struct HdfLib {
+    decltype(&vvhd_hdf_save) save = nullptr;
+    decltype(&vvhd_hdf_sizes) sizes = nullptr;
+    decltype(&vvhd_hdf_load) load = nullptr;
+    decltype(&vvhd_hdf_error) error = nullptr;
+};
+
+const HdfLib& hdf_lib() {
+    static HdfLib L;
+    static bool done = false;
+    if (done) return L;
+    Dl_info info;
+    std::string dir;
+    if (dladdr(reinterpret_cast<void*>(&hdf_lib), &info) && info.dli_fname) {
+        dir = info.dli_fname;
+        dir = dir.substr(0, dir.find_last_of('/') + 1);
+    }
+    void* h = dlopen((dir + "libvvhd_hdf.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (!h) throw std::runtime_error(std::string("Space::save/load: ") + dlerror());
+    L.save = reinterpret_cast<decltype(&vvhd_hdf_save)>(dlsym(h, "vvhd_hdf_save"));
+    L.sizes = reinterpret_cast<decltype(&vvhd_hdf_sizes)>(dlsym(h, "vvhd_hdf_sizes"));
+    L.load = reinterpret_cast<decltype(&vvhd_hdf_load)>(dlsym(h, "vvhd_hdf_load"));
+    L.error = reinterpret_cast<decltype(&vvhd_hdf_error)>(dlsym(h, "vvhd_hdf_error"));
+    if (!L.save || !L.sizes || !L.load || !L.error) throw std::runtime_error("Space::save/load: incomplete libvvhd_hdf");
+    done = true;
+    return L;
+}
+
+void copy_str(char* dst, const std::string& src) { std::snprintf(dst, VVHD_HDF_STR, "%s", src.c_str()); }
+
+}  // namespace
+
+void Space::save(const char* path) {
+    pull();
+    const HdfLib& H = hdf_lib();
+    vvhd_hdf_space sp;
+    std::memset(&sp, 0, sizeof(sp));
+    copy_str(sp.caption, caption);
+    sp.time = time;
+    sp.dt = dt;
+    sp.dt_save = dt_save;
+    sp.dt_streak = dt_streak;
+    sp.dt_profile = dt_profile;
+    sp.re = re;
+    sp.pr = pr;
+    sp.inf_g = inf_g;
+    sp.finish = finish;
+    sp.inf_marker[0] = inf_marker.x;
+    sp.inf_marker[1] = inf_marker.y;
+    sp.gravity[0] = gravity.x;
+    sp.gravity[1] = gravity.y;
+    copy_str(sp.inf_speed_x, inf_vx);
+    copy_str(sp.inf_speed_y, inf_vy);
+    const std::vector<TObj> empty;
+    const std::vector<TObj>* lists[VVHD_HDF_LISTS] = {&VortexList, &HeatList, &StreakList, &StreakSourceList, &empty};
+    const double* ptrs[VVHD_HDF_LISTS];
+    int64_t counts[VVHD_HDF_LISTS];
+    for (int k = 0; k < VVHD_HDF_LISTS; k++) {
+        ptrs[k] = reinterpret_cast<const double*>(lists[k]->data());
+        counts[k] = (int64_t)lists[k]->size();
+    }
+    std::vector<vvhd_hdf_body> bodies;
+    std::vector<double> seg4;
+    std::vector<uint32_t> slip;
+    for (const auto& bp : BodyList) {
+        const TBody& b = *bp;
+        vvhd_hdf_body r;
+        std::memset(&r, 0, sizeof(r));
+        const TVec3D* v3[5] = {&b.holder, &b.dpos, &b.speed_slae, &b.speed_slae_prev, &b.force_hydro};
+        double* d3[5] = {r.holder, r.dpos, r.speed_slae, r.speed_slae_prev, r.force_hydro};
+        for (int k = 0; k < 5; k++) {
+            d3[k][0] = v3[k]->r.x;
+            d3[k][1] = v3[k]->r.y;
+            d3[k][2] = v3[k]->o;
+        }
+        const TVec3D* w3[3] = {&b.force_holder, &b.friction_prev, &b.fdt_dead};
+        double* e3[3] = {r.force_holder, r.friction_prev, r.fdt_dead};
+        for (int k = 0; k < 3; k++) {
+            e3[k][0] = w3[k]->r.x;
+            e3[k][1] = w3[k]->r.y;
+            e3[k][2] = w3[k]->o;
+        }
+        for (int k = 0; k < 3; k++) {   // rigid: infinite springs (TBody.cpp:24-30)
+            r.kspring[k] = -1;
+            r.damping[k] = 0;
+        }
+        copy_str(r.speed_x, "0");
+        copy_str(r.speed_y, "0");
+        copy_str(r.speed_o, "0");
+        r.density = 1;
+        r.g_dead = b.g_dead;
+        r.area = b.get_area();
+        r.cofm[0] = b.get_cofm().x;
+        r.cofm[1] = b.get_cofm().y;
+        r.moi_cofm = b.get_moi_cofm();
+        const TVec ax = b.get_axis();
+        r.moi_axis = b.get_moi_cofm() + (ax - b.get_cofm()).abs2() * b.get_area();
+        r.special_segment_no = b.special_segment_no;
+        r.boundary_condition = b.kutta ? 1 : 0;
+        r.heat_condition = (uint32_t)b.heat_condition;
+        r.heat_const = b.alist.empty() ? 0 : (double)b.alist.front().heat_const;
+        r.nseg = (int32_t)b.alist.size();
+        for (const TAtt& a : b.alist) {
+            seg4.insert(seg4.end(), {a.corner.x, a.corner.y, a.g, a.gsum});
+            slip.push_back(a.slip);
+        }
+        bodies.push_back(r);
+    }
+    if (H.save(path, &sp, ptrs, counts, (int32_t)bodies.size(), bodies.data(), seg4.data(), slip.data()) != 0)
+        throw std::runtime_error(std::string("Space::save: ") + H.error());
+}
+
+void Space::load(const char* path) {
+    const HdfLib& H = hdf_lib();
+    int64_t counts[VVHD_HDF_LISTS];
+    int32_t nb = 0, ns = 0;
+    if (H.sizes(path, counts, &nb, &ns) != 0) throw std::runtime_error(std::string("Space::load: ") + H.error());
+    std::vector<TObj> src(counts[4]);
+    std::vector<TObj>* lists[VVHD_HDF_LISTS] = {&VortexList, &HeatList, &StreakList, &StreakSourceList, &src};
+    double* ptrs[VVHD_HDF_LISTS];
+    for (int k = 0; k < VVHD_HDF_LISTS; k++) {
+        lists[k]->assign((size_t)counts[k], TObj());
+        ptrs[k] = reinterpret_cast<double*>(lists[k]->data());
+    }
+    std::vector<vvhd_hdf_body> bodies(std::max(nb, 1));
+    std::vector<double> seg4(4 * (size_t)std::max(ns, 1));
+    std::vector<uint32_t> slip(std::max(ns, 1));
+    vvhd_hdf_space sp;
+    if (H.load(path, &sp, ptrs, bodies.data(), seg4.data(), slip.data()) != 0)
+        throw std::runtime_error(std::string("Space::load: ") + H.error());
+    caption = sp.caption;
+    time = sp.time;
+    dt = sp.dt;
+    dt_save = sp.dt_save;
+    dt_streak = sp.dt_streak;
+    dt_profile = sp.dt_profile;
+    re = sp.re;
+    pr = sp.pr;
+    inf_g = sp.inf_g;
+    finish = sp.finish;
+    inf_marker = TVec(sp.inf_marker[0], sp.inf_marker[1]);
+    gravity = TVec(sp.gravity[0], sp.gravity[1]);
+    inf_vx = sp.inf_speed_x;
+    inf_vy = sp.inf_speed_y;
+    char* end = nullptr;
+    const double ux = std::strtod(sp.inf_speed_x, &end);
+    if (!(end && *end == 0 && end != sp.inf_speed_x)) throw std::invalid_argument("Space::load: inf_speed_x is not a constant");
+    const double uy = std::strtod(sp.inf_speed_y, &end);
+    if (!(end && *end == 0 && end != sp.inf_speed_y)) throw std::invalid_argument("Space::load: inf_speed_y is not a constant");
+    inf_vel = TVec(ux, uy);
+    BodyList.clear();
+    size_t s = 0;
+    for (int32_t b = 0; b < nb; b++) {
+        const vvhd_hdf_body& r = bodies[b];
+        auto body = std::make_shared<TBody>();
+        for (int32_t k = 0; k < r.nseg; k++, s++) {
+            body->alist.emplace_back(seg4[4 * s], seg4[4 * s + 1], slip[s] != 0);
+            TAtt& a = body->alist.back();
+            a.g = seg4[4 * s + 2];
+            a.gsum = seg4[4 * s + 3];
+            a.heat_const = (float)r.heat_const;
+        }
+        const double* d3[5] = {r.holder, r.dpos, r.speed_slae, r.speed_slae_prev, r.force_hydro};
+        TVec3D* v3[5] = {&body->holder, &body->dpos, &body->speed_slae, &body->speed_slae_prev, &body->force_hydro};
+        for (int k = 0; k < 5; k++) *v3[k] = TVec3D{TVec(d3[k][0], d3[k][1]), d3[k][2]};
+        body->force_holder = TVec3D{TVec(r.force_holder[0], r.force_holder[1]), r.force_holder[2]};
+        body->friction_prev = TVec3D{TVec(r.friction_prev[0], r.friction_prev[1]), r.friction_prev[2]};
+        body->fdt_dead = TVec3D{TVec(r.fdt_dead[0], r.fdt_dead[1]), r.fdt_dead[2]};
+        body->g_dead = r.g_dead;
+        body->special_segment_no = r.special_segment_no;
+        body->kutta = r.boundary_condition == 1;
+        body->heat_condition = static_cast<hc_t>(r.heat_condition);
+        BodyList.push_back(body);
+    }
+    push();   // segments and body geometry are rebuilt from the corners on the device
 }
 
 // ---------------------------------------------------------------- stree --
