@@ -143,3 +143,65 @@ def test_graph_whole_cylinder_steps(ctx):
         assert np.array_equal(a, b), f"step {k}"
     ctx.set_graph(False)
     ctx.set_tuning(True)
+
+
+def test_graph_batched_cylinder_steps(ctx):
+    """Four cylinder simulations in one context (lattice states as in the
+    batched benchmark, Re 150-2800), whole steps: two eager runs (the tuned
+    form) and a run with the captured convective phase (persistent form 16)
+    give the same bits after every step."""
+    from bench import cylinder_wake
+    print("case graph in batched cylinder steps", flush=True)
+    nseg = 500
+    corners = V.gen_cylinder(0.5, nseg)
+    segs1 = np.ascontiguousarray(V.segments_from_corners(corners), dtype=V.SEG_DTYPE)
+    body1 = np.zeros(1, dtype=V.BODY_DTYPE)
+    body1["nseg"] = nseg
+    ns = [20_000, 50_000, 12_000, 35_000]
+    res = np.array([150.0, 600.0, 1500.0, 2800.0])
+    states = np.concatenate([cylinder_wake(n, 0x5EED2100 + s) for s, n in enumerate(ns)])
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    segs = np.concatenate([segs1] * len(ns))
+    bodies = np.concatenate([body1] * len(ns))
+    bodies["first_seg"] = np.arange(len(ns)) * nseg
+    dl = nseg * np.sin(np.pi / nseg) / (nseg - 1)
+
+    def run(graph, steps=4):
+        ctx.set_graph(graph)
+        if graph:
+            ctx.set_tuning(False, 16)
+        else:
+            ctx.set_tuning(True)
+        ctx.set_params(re=600.0, dt=0.005, time=0.0)
+        ctx.upload_bodies(segs, bodies)
+        ctx.upload(states)
+        ctx.set_batch(off, np.arange(len(ns) + 1, dtype=np.int32), res)
+        out = []
+        for k in range(steps):
+            ctx.set_params(re=600.0, dt=0.005, time=k * 0.005)
+            ctx.tree_build(8, 5 * dl, 100 * dl)
+            ctx.calc_circulation((1.0, 0.0))
+            ctx.tree_destroy()
+            ctx.vortex_shed()
+            ctx.calc_forces()
+            ctx.zero_forces()
+            ctx.tree_build(8, 5 * dl, 100 * dl)
+            ctx.epsilon(True)
+            ctx.convective((1.0, 0.0))
+            ctx.diffusive()
+            ctx.tree_destroy()
+            ctx.move_and_clean(True)
+            out.append((ctx.download(), ctx.batch_offsets(), ctx.download_bodies()[0]["g"].copy()))
+        return out
+
+    eager = run(False)
+    again = run(False)
+    graphed = run(True)
+    for k in range(len(eager)):
+        for name, other in (("eager again", again), ("graph", graphed)):
+            assert np.array_equal(eager[k][1], other[k][1]), f"{name}: step {k} offsets"
+            assert np.array_equal(eager[k][0], other[k][0]), f"{name}: step {k} vortices"
+            assert np.array_equal(eager[k][2], other[k][2]), f"{name}: step {k} segment circulations"
+    print(f"  vortices after {len(eager)} steps: {eager[-1][1][-1]}", flush=True)
+    ctx.set_graph(False)
+    ctx.set_tuning(True)

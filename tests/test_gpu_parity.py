@@ -220,6 +220,35 @@ def test_velocity_points_vs_oracle(ctx):
     assert np.allclose(vg, vo, rtol=1e-10, atol=1e-12)
 
 
+def test_tree_find_and_integrals_vs_oracle(ctx):
+    """stree::findNode (TSortedTree.cpp:284-303) for vortex positions, random
+    points and points far outside the root box (the descent still ends in a
+    leaf, as the reference's does), and Space::integral / the impulse
+    integrals after merges: the same leaf and the same sums as the oracle."""
+    print("case tree find and integrals", flush=True)
+    objs = random_cluster_case(20000, 21, "mixed")
+    o = oracle_for(objs)
+    load_case(ctx, objs)
+    o.tree_build(8)
+    ctx.tree_build(8)
+    leaves = o.tree()["leaves"]
+    rng = np.random.default_rng(4)
+    v = o.vortices()
+    pts = np.concatenate([v[rng.choice(len(v), 500, replace=False), :2],
+                          rng.uniform(-0.2, 1.2, (500, 2)),
+                          np.array([[50.0, 50.0], [-40.0, 3.0], [0.5, -60.0]])])
+    got = ctx.tree_find(pts)
+    want = np.array([o.find(x, y) for x, y in pts])
+    assert np.all(got >= 0)
+    assert np.array_equal(leaves[got], want)
+    o.epsilon(True)
+    ctx.epsilon(True)
+    gi, oi = ctx.integrals(), o.integrals()
+    # mixed signs: the sums may cancel, so the floor scales with sum |g|
+    assert np.allclose(gi, oi, rtol=1e-12, atol=1e-13 * np.abs(objs[:, 2]).sum()), (gi, oi)
+    assert gi[4] == oi[4]
+
+
 def test_direct_sum_vs_oracle(ctx):
     print("case direct sum", flush=True)
     objs = random_cluster_case(3000, 15, "mixed")
