@@ -2,8 +2,10 @@
  * vvhd_modules.hpp -- the libvvhd module classes over the device context.
  *
  * The classes keep the names and the public signatures of the reference's
- * modules so that its time loop (utils/vvflow/vvflow.cpp:214-262) compiles
- * against them unchanged:
+ * modules, so that its time loop (utils/vvflow/vvflow.cpp:214-262) is
+ * written against them call for call.  Where they differ: Space::time is a
+ * double (no TTime), the free stream is set as numbers (no TEval), and a
+ * Space names its device.  The classes:
  *   stree / TSortedTree   libvvhd/headers/TSortedTree.hpp:60-92
  *   MConvectiveFast       libvvhd/headers/MConvectiveFast.hpp:9-27
  *   MEpsilonFast          libvvhd/headers/MEpsilonFast.hpp:7-31
@@ -25,8 +27,8 @@
  *                  sensors, scripting).
  * Errors of the device library are thrown as std::invalid_argument for
  * calls out of order (the reference's own convention for an unbuilt tree)
- * and std::runtime_error otherwise.  Bodies are rigid and static (the
- * configurations of this hot path).  The heat and streak lists (HeatList,
+ * and std::runtime_error otherwise.  Bodies are rigid (no springs or
+ * collisions).  The heat and streak lists (HeatList,
  * StreakList, StreakSourceList) are carried like the vortex list.
  */
 #ifndef VVHD_MODULES_HPP
