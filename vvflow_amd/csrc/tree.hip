@@ -199,12 +199,15 @@ __device__ inline void bbox_add_wave(NodeView nv, int node, double px, double py
 // Bounding boxes of the nodes of a level with at most kTopNodes nodes (the
 // root, and the children of the first levels), from the elements in their
 // final positions: every element k with enode[k] - base in [0, kTopNodes)
-// adds to that node.  Few nodes share many elements there, so the lanes
-// reduce per node in the wave, then in LDS per block, and one lane per block
-// and node issues the global atomics.  Each block covers kTopChunk
-// consecutive elements.
+// adds to that node.  A node's elements form one contiguous range, so each
+// thread keeps a running box for the node of its current element (order-
+// preserving integer images, NaN coordinates skipped) and adds it to the
+// block's LDS slot of that node only when the node changes; one lane per
+// block and node then issues the global atomics.  Each block covers
+// kTopChunk consecutive elements, kTopPer per thread, loaded before use.
 constexpr int kTopNodes = 512;
-constexpr int kTopChunk = 4096;
+constexpr int kTopPer = 4;
+constexpr int kTopChunk = kBlock * kTopPer;
 
 __global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t* base_ptr, const int32_t* enode,
                                                      const double* x, const double* y, int n) {
@@ -218,45 +221,50 @@ __global__ __launch_bounds__(kBlock) void k_bbox_top(NodeView nv, const int32_t*
     }
     __syncthreads();
     const int base = base_ptr ? *base_ptr : 0;
-    const int k0 = blockIdx.x * kTopChunk;
-    const int k1 = min(n, k0 + kTopChunk);
-    for (int kb = k0; kb < k1; kb += blockDim.x) {
-        const int k = kb + threadIdx.x;
-        int node = -1;
-        double px = 0, py = 0;
-        if (k < k1) {
-            node = (enode ? enode[k] : 0) - base;
-            px = x[k];
-            py = y[k];
-        }
-        bool active = node >= 0 && node < kTopNodes;
-        uint64_t left = __ballot(active);
-        while (left) {
-            const int leader = (int)__builtin_ctzll(left);
-            const int lead_node = __shfl(node, leader, kWave);
-            const bool mine = active && node == lead_node;
-            left &= ~__ballot(mine);
-            const bool vx = mine && !isnan(px), vy = mine && !isnan(py);
-            double lx = vx ? px : DBL_MAX, hx = vx ? px : -DBL_MAX;
-            double ly = vy ? py : DBL_MAX, hy = vy ? py : -DBL_MAX;
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                lx = fmin(lx, __shfl_xor(lx, o, kWave));
-                hx = fmax(hx, __shfl_xor(hx, o, kWave));
-                ly = fmin(ly, __shfl_xor(ly, o, kWave));
-                hy = fmax(hy, __shfl_xor(hy, o, kWave));
-            }
-            if (lane_id() == leader) {
-                if (lx <= hx) {
-                    atomicMin(&s_lx[lead_node], (unsigned long long)ord_from_double(lx));
-                    atomicMax(&s_hx[lead_node], (unsigned long long)ord_from_double(hx));
-                }
-                if (ly <= hy) {
-                    atomicMin(&s_ly[lead_node], (unsigned long long)ord_from_double(ly));
-                    atomicMax(&s_hy[lead_node], (unsigned long long)ord_from_double(hy));
-                }
-            }
+    const int k0 = blockIdx.x * kTopChunk + threadIdx.x;
+    int nodes[kTopPer];
+    double px[kTopPer], py[kTopPer];
+#pragma unroll
+    for (int j = 0; j < kTopPer; j++) {
+        const int k = k0 + j * kBlock;
+        nodes[j] = -1;
+        if (k < n) {
+            nodes[j] = (enode ? enode[k] : 0) - base;
+            px[j] = x[k];
+            py[j] = y[k];
         }
     }
+    int cur = -1;
+    unsigned long long lx = lo0, hx = hi0, ly = lo0, hy = hi0;
+    auto flush = [&]() {
+        if (cur < 0) return;
+        if (lx != lo0) atomicMin(&s_lx[cur], lx);
+        if (hx != hi0) atomicMax(&s_hx[cur], hx);
+        if (ly != lo0) atomicMin(&s_ly[cur], ly);
+        if (hy != hi0) atomicMax(&s_hy[cur], hy);
+    };
+#pragma unroll
+    for (int j = 0; j < kTopPer; j++) {
+        const int node = nodes[j];
+        if (node < 0 || node >= kTopNodes) continue;
+        if (node != cur) {
+            flush();
+            cur = node;
+            lx = ly = lo0;
+            hx = hy = hi0;
+        }
+        if (!isnan(px[j])) {
+            const unsigned long long o = ord_from_double(px[j]);
+            lx = o < lx ? o : lx;
+            hx = o > hx ? o : hx;
+        }
+        if (!isnan(py[j])) {
+            const unsigned long long o = ord_from_double(py[j]);
+            ly = o < ly ? o : ly;
+            hy = o > hy ? o : hy;
+        }
+    }
+    flush();
     __syncthreads();
     for (int t = threadIdx.x; t < kTopNodes; t += blockDim.x) {
         const int id = base + t;
