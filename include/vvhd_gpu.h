@@ -297,6 +297,9 @@ int vvhd_slae_system(vvhd_ctx* ctx, int32_t* n, double* A, double* rhs);
 int vvhd_calc_forces(vvhd_ctx* ctx);
 int vvhd_zero_forces(vvhd_ctx* ctx);
 int vvhd_integrals(vvhd_ctx* ctx, double* out5);
+/* Stepdata::write's reset of every segment's Cp, Fr and Nu accumulators
+ * once the profiles are written (MStepdata.cpp:166-176). */
+int vvhd_zero_profiles(vvhd_ctx* ctx);
 
 /* ---- point queries and grid fields -----------------------------------
  * MEpsilonFast::eps2h / h2 (MEpsilonFast.cpp:66-107, MEpsilonFast.hpp:15-18)

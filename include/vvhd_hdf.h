@@ -74,6 +74,34 @@ int vvhd_hdf_load(const char* path, vvhd_hdf_space* space, double* const* lists,
                   double* seg4, uint32_t* slip);
 const char* vvhd_hdf_error(void);
 
+/* ---- step data: Stepdata (MStepdata.hpp:7-39, MStepdata.cpp:10-290) ----
+ * One row per step in stepdata_<caption>.h5: a dataset "time" (rows x 1)
+ * and per body a group bodyNN with force_hydro, force_holder,
+ * force_friction, holder_position, delta_position, speed_slae (rows x 3
+ * {x, y, o}) and, with profiles, pressure and friction (rows x nseg).
+ * Datasets are stored as float, chunked by 128 rows, deflate 9, NaN fill,
+ * unlimited rows.  A new file gets the attributes caption, git_rev,
+ * git_info, git_diff and the committed type string_t.  Opening an existing
+ * file keeps the leading rows whose time is below `time` (a restart) and
+ * cuts every dataset there; without profiles, existing pressure and
+ * friction datasets are cut too.  The file is flushed every 20 s of wall
+ * time, and at the next write after vvhd_stepdata_flush. */
+typedef struct vvhd_stepdata vvhd_stepdata;
+typedef struct vvhd_stepdata_body {
+    double force_hydro[3], force_holder[3], friction[3];
+    double holder[3], dpos[3], speed_slae[3];
+} vvhd_stepdata_body;
+/* NULL on failure (message in vvhd_hdf_error()); nseg[b] is body b's
+ * segment count. */
+vvhd_stepdata* vvhd_stepdata_open(const char* path, const char* caption, double time, int32_t nbody,
+                                  const int32_t* nseg, int32_t save_profile);
+/* Appends one row: time, every body's record, and with profiles the
+ * per-segment pressure and friction of all bodies in order (else NULL). */
+int vvhd_stepdata_write(vvhd_stepdata* sd, double time, const vvhd_stepdata_body* bodies, const double* pressure,
+                        const double* friction);
+int vvhd_stepdata_flush(vvhd_stepdata* sd);
+int vvhd_stepdata_close(vvhd_stepdata* sd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@
  *                         libvvhd/headers/XField.hpp:18-63, XVelocity.hpp,
  *                         XVorticity.hpp, XStreamfunction.hpp, XPressure.hpp
  *   sensors               utils/vvflow/sensors.cpp:8-70
+ *   Stepdata              libvvhd/headers/MStepdata.hpp:7-39
  *
  * The vortex list and the bodies live on the device between calls; the host
  * containers of Space are a mirror that is synchronised explicitly:
@@ -181,6 +182,7 @@ class Space {
     friend class XStreamfunction;
     friend class XPressure;
     friend class sensors;
+    friend class Stepdata;
     vvhd_ctx* ctx_ = nullptr;
     bool pushed_ = false;
     int32_t nseg_ = 0;
@@ -319,6 +321,30 @@ class MFlowmove {
 
    private:
     Space* S;
+};
+
+// --------------------------------------------------------- step data --
+// The per-step time series (MStepdata.hpp:7-39, MStepdata.cpp:10-290) in the
+// reference's HDF5 layout, through libvvhd_hdf.so: every write() appends the
+// time and each body's force_hydro, force_holder, friction, holder, dpos and
+// speed_slae; with b_save_profile also every segment's Cp / dt and Fr / dt,
+// after which the segments' Cp, Fr and Nu accumulators are reset (on the
+// device too).  An existing file is continued from S->time (its later rows
+// are dropped).
+struct vvhd_stepdata;
+class Stepdata {
+   public:
+    Stepdata(Space* S, const char* fname, bool b_save_profile);
+    Stepdata(const Stepdata&) = delete;
+    Stepdata& operator=(const Stepdata&) = delete;
+    ~Stepdata();
+    void write();
+    void flush();
+
+   private:
+    Space* S;
+    bool b_save_profile;
+    vvhd_stepdata* h_ = nullptr;
 };
 
 // ------------------------------------------------------------ fields --

@@ -105,6 +105,55 @@ def test_cpp_driver_snapshot_save_and_load(tmp_path):
     print(f"  snapshot of {len(v)} vortices; restarted for 3 steps: {rows[-1][2]} vortices", flush=True)
 
 
+def h5_values(path, dset, tmp_path):
+    """A dataset's values through h5dump's binary output (float32)."""
+    import shutil
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    out = tmp_path / (dset.strip("/").replace("/", "_") + ".bin")
+    subprocess.run([h5dump, "-d", dset, "-b", "LE", "-o", str(out), path], capture_output=True, check=True)
+    return np.fromfile(out, dtype="<f4")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="vvflow_cylinder not built")
+def test_cpp_driver_stepdata(tmp_path):
+    """Stepdata (MStepdata.cpp) in the C++ driver's loop: one row per step
+    after the forces, holding the force_hydro and friction the driver prints
+    (as float); the friction profile Fr / dt of every segment reproduces the
+    friction force, F = -sum dl (Fr / dt) / |dl| (TSpace.cpp:94-120), which
+    holds only if the accumulators are reset after every row; a restart from
+    a snapshot continues the series from the snapshot's time."""
+    print("case C++ driver step data, 8 + 4 steps", flush=True)
+    sd, snap = str(tmp_path / "stepdata_cylinder.h5"), str(tmp_path / "cyl.h5")
+    r = subprocess.run([EXE, "--steps", "8", "--stepdata", sd, "--profile", "--save", snap], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = np.array([[float(x) for x in ln.split()] for ln in r.stdout.splitlines() if not ln.startswith("#")])
+    t = h5_values(sd, "/time", tmp_path)
+    assert np.array_equal(t, (0.005 * np.arange(8)).astype(np.float32))
+    fh = h5_values(sd, "/body00/force_hydro", tmp_path).reshape(8, 3)
+    ff = h5_values(sd, "/body00/force_friction", tmp_path).reshape(8, 3)
+    assert np.array_equal(fh, rows[:, 3:6].astype(np.float32))
+    assert np.array_equal(ff, rows[:, 6:9].astype(np.float32))
+    fr = h5_values(sd, "/body00/friction", tmp_path).reshape(8, 500).astype(np.float64)
+    cp = h5_values(sd, "/body00/pressure", tmp_path).reshape(8, 500)
+    assert np.all(np.isfinite(cp)) and np.all(np.isfinite(fr))
+    c = V.gen_cylinder(0.5, 500)
+    dl = np.roll(c, -1, axis=0) - c
+    fx = -(dl[:, 0] * fr / np.hypot(dl[:, 0], dl[:, 1])).sum(1)
+    fy = -(dl[:, 1] * fr / np.hypot(dl[:, 0], dl[:, 1])).sum(1)
+    scale = np.abs(rows[:, 6:8]).max()
+    assert np.abs(fx - rows[:, 6]).max() < 1e-4 * scale and np.abs(fy - rows[:, 7]).max() < 1e-4 * scale
+    # restart at the snapshot's time: eight rows kept, four appended
+    r2 = subprocess.run([EXE, "--load", snap, "--steps", "4", "--stepdata", sd], capture_output=True, text=True,
+                        timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    t = h5_values(sd, "/time", tmp_path)
+    assert np.array_equal(t, (0.005 * np.arange(12)).astype(np.float32))
+    assert h5_values(sd, "/body00/pressure", tmp_path).size == 8 * 500   # cut at the restart, no profiles after
+    print(f"  {len(t)} rows; friction force from the profile within "
+          f"{np.abs(fx - rows[:, 6]).max() / scale:.1e} of the force", flush=True)
+
+
 def read_xfield(path):
     """XField's binary matrix (XField.cpp:124-152) -> (xs, ys, map)."""
     a = np.fromfile(path, dtype=np.float32)

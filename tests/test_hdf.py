@@ -110,3 +110,65 @@ def test_file_structure_as_h5dump_sees_it(tmp_path):
     vort = subprocess.run([H5DUMP, "-p", "-H", "-d", "/vort", path], capture_output=True, text=True,
                           check=True).stdout
     assert "CHUNKED ( 512, 3 )" in vort and "DEFLATE { LEVEL 9 }" in vort
+
+
+def dump(path, dset, tmp_path):
+    """A dataset's values through h5dump's binary output (float32, little endian)."""
+    out = tmp_path / (dset.replace("/", "_") + ".bin")
+    subprocess.run([H5DUMP, "-d", dset, "-b", "LE", "-o", str(out), path], capture_output=True, check=True)
+    return np.fromfile(out, dtype="<f4")
+
+
+def step_bodies(k, nb=2):
+    return [dict(force_hydro=[k + 0.5, -k, 0.25 * b], force_holder=[k, b, 1.0], friction=[1e-3 * k, 2e-3, b],
+                 holder=[b, 0.0, 0.1 * k], dpos=[0.0, 0.0, 0.0], speed_slae=[0.0, 0.0, 0.01 * k]) for b in range(nb)]
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="h5dump not in the image")
+def test_stepdata_rows_restart_and_structure(tmp_path):
+    """Stepdata (MStepdata.cpp): one row per write; reopening at a time keeps
+    the rows written before it (a restart) and cuts every dataset there;
+    float datasets chunked by 128 rows, deflate 9, NaN fill, unlimited rows."""
+    path = str(tmp_path / "stepdata_cyl.h5")
+    nseg = [64, 20]
+    rng = np.random.default_rng(5)
+    cps = [rng.normal(size=84) for _ in range(5)]
+    with hdf.Stepdata(path, "cyl", 0.0, nseg, profile=True) as sd:
+        for k in range(5):
+            sd.write(0.005 * k, step_bodies(k), pressure=cps[k], friction=-cps[k])
+    t = dump(path, "/time", tmp_path)
+    assert np.array_equal(t, (0.005 * np.arange(5)).astype(np.float32))
+    fh = dump(path, "/body01/force_hydro", tmp_path).reshape(5, 3)
+    assert np.array_equal(fh, np.array([b["force_hydro"] for b in (step_bodies(k)[1] for k in range(5))],
+                                       dtype=np.float32))
+    pr = dump(path, "/body00/pressure", tmp_path).reshape(5, 64)
+    assert np.array_equal(pr, np.array([c[:64] for c in cps], dtype=np.float32))
+    fr = dump(path, "/body01/friction", tmp_path).reshape(5, 20)
+    assert np.array_equal(fr, np.array([-c[64:] for c in cps], dtype=np.float32))
+    # restart at t = 0.0125: rows 0.000, 0.005, 0.010 stay; then two new rows
+    with hdf.Stepdata(path, "cyl", 0.0125, nseg, profile=False) as sd:
+        for k in (7, 8):
+            sd.write(0.005 * k, step_bodies(k))
+    t = dump(path, "/time", tmp_path)
+    assert np.array_equal(t, np.float32([0.0, 0.005, 0.01, 0.035, 0.04]))
+    sp = dump(path, "/body00/speed_slae", tmp_path).reshape(5, 3)
+    assert np.array_equal(sp[:, 2], np.float32([0.0, 0.01, 0.02, 0.07, 0.08]))
+    # without profiles the profile datasets were cut at the restart row
+    assert dump(path, "/body00/pressure", tmp_path).size == 3 * 64
+    head = subprocess.run([H5DUMP, "-H", "-p", path], capture_output=True, text=True, check=True).stdout
+    assert 'DATATYPE "string_t"' in head and 'ATTRIBUTE "caption"' in head and 'ATTRIBUTE "git_rev"' in head
+    blk = head[head.index('DATASET "force_holder"'):][:900]
+    assert "H5T_IEEE_F32LE" in blk and "H5S_UNLIMITED" in blk and "CHUNKED ( 128, 3 )" in blk
+    assert "DEFLATE { LEVEL 9 }" in blk and "nan" in blk.lower()
+    for g in ("body00", "body01"):
+        assert f'GROUP "{g}"' in head
+    for dset in ("force_hydro", "force_holder", "force_friction", "holder_position", "delta_position", "speed_slae"):
+        assert f'DATASET "{dset}"' in head
+
+
+def test_stepdata_bad_arguments(tmp_path):
+    with pytest.raises(RuntimeError):
+        hdf.Stepdata(str(tmp_path / "no_such_dir" / "s.h5"), "x", 0.0, [4])
+    with hdf.Stepdata(str(tmp_path / "s.h5"), "x", 0.0, [4], profile=True) as sd:
+        with pytest.raises(RuntimeError):
+            sd.write(0.0, step_bodies(0, 1))   # profiles without pressure / friction

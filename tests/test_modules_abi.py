@@ -29,7 +29,10 @@ def test_exports_reference_methods():
                 "MConvectiveFast::calc_circulation(void const**)", "MEpsilonFast::CalcEpsilonFast(bool)",
                 "MDiffusiveFast::process_vort_list()",
                 "MFlowmove::move_and_clean(bool, void const**, unsigned long*)", "MFlowmove::vortex_shed()",
-                "Space::calc_forces()", "Space::zero_forces()"]:
+                "Space::calc_forces()", "Space::zero_forces()", "Space::save(char const*)",
+                "Space::load(char const*)", "Stepdata::Stepdata(Space*, char const*, bool)", "Stepdata::write()",
+                "Stepdata::flush()", "MDiffusiveFast::process_heat_list()", "MFlowmove::heat_shed()",
+                "MFlowmove::streak_shed()", "MEpsilonFast::eps2h(TSortedNode const&, TVec)"]:
         assert sym in out, sym
 
 

@@ -88,6 +88,7 @@ ABI = {
     "vvhd_slae_system": (_I, [_P, _VP, _VP, _VP]),
     "vvhd_calc_forces": (_I, [_P]),
     "vvhd_zero_forces": (_I, [_P]),
+    "vvhd_zero_profiles": (_I, [_P]),
     "vvhd_far_model_set": (_I, [_P, _VP, _I32]),
     "vvhd_far_model_get": (_I, [_P, _VP, _I32]),
     "vvhd_convective_learned": (_I, [_P, _D, _D, _I]),
@@ -439,6 +440,11 @@ class Context:
 
     def zero_forces(self):
         self._check(self.L.vvhd_zero_forces(self.h))
+
+    def zero_profiles(self):
+        """Resets every segment's Cp, Fr and Nu accumulators (Stepdata::write
+        after writing the profiles)."""
+        self._check(self.L.vvhd_zero_profiles(self.h))
 
     # ---- learned far field
     def far_model_set(self, params):

@@ -512,6 +512,16 @@ __global__ void k_zero_forces(vvhd_body* bodies, int nbody, vvhd_seg* segs, doub
     B.nusselt = 0.;
 }
 
+// Stepdata::write's reset of the per-segment pressure, friction and Nusselt
+// accumulators after writing the profiles (MStepdata.cpp:166-176).
+__global__ void k_zero_profiles(vvhd_seg* segs, int nseg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    segs[i].Cp = 0;
+    segs[i].Fr = 0;
+    segs[i].Nu = 0;
+}
+
 // ------------------------------------------------------------------- SLAE
 
 // MConvectiveFast::_2PI_Xi_g and its near / dist parts (MConvectiveFast.cpp:
@@ -1214,6 +1224,19 @@ int vvhd_zero_forces(vvhd_ctx* ctx) {
     API_BEGIN
     VVHD_HIP(hipSetDevice(ctx->c.device));
     zero_forces(ctx->c);
+    API_END(ctx)
+}
+
+int vvhd_zero_profiles(vvhd_ctx* ctx) {
+    REQUIRE_CTX(ctx);
+    API_BEGIN
+    Ctx& c = ctx->c;
+    VVHD_HIP(hipSetDevice(c.device));
+    const int nseg = (int)c.segs_h.size();
+    if (nseg) {
+        hipLaunchKernelGGL(k_zero_profiles, dim3(nblocks(nseg)), dim3(kBlock), 0, c.stream, c.segs.p, nseg);
+        sync_bodies_to_host(c);
+    }
     API_END(ctx)
 }
 

@@ -8,6 +8,7 @@
 //                   [--sensors points.txt --sensors-out out.txt]
 //                   [--fields PREFIX] [--heat COND:HEAT_CONST:PR]
 //                   [--streak N:DT_STREAK] [--save PATH] [--load PATH]
+//                   [--stepdata PATH [--profile]]
 //
 // Prints one line per `every` steps: step, time, vortex count, force_hydro
 // (x, y, o), friction (x, y, o), step wall time in ms; then a summary line.
@@ -25,6 +26,9 @@
 // every DT_STREAK.  The summary then also reports the heat and streak lists.
 // --save writes an HDF5 snapshot of the final state (Space::save); --load
 // starts from a snapshot instead of an empty cylinder (Space::load).
+// --stepdata writes the per-step time series (Stepdata, MStepdata.cpp) after
+// the forces of every step, as the reference's loop does (vvflow.cpp:235-237),
+// with the segments' pressure and friction profiles under --profile.
 
 #include <algorithm>
 #include <chrono>
@@ -78,8 +82,9 @@ std::vector<float> load_npy_f32(const std::string& path) {
 int main(int argc, char** argv) {
     int steps = 400, nseg = 500, fc = 8, every = 1;
     double re = 600, dt = 0.005, radius = 0.5;
-    std::string learned, sensors_in, sensors_out, fields, save_path, load_path;
+    std::string learned, sensors_in, sensors_out, fields, save_path, load_path, stepdata_path;
     int heat_cond = 0, nstreak = 0;
+    bool profile = false;
     double heat_const = 0, pr = 0, dt_streak = 0;
     for (int k = 1; k < argc; k++) {
         const std::string a = argv[k];
@@ -99,6 +104,8 @@ int main(int argc, char** argv) {
         else if (a == "--fields") fields = next();
         else if (a == "--save") save_path = next();
         else if (a == "--load") load_path = next();
+        else if (a == "--stepdata") stepdata_path = next();
+        else if (a == "--profile") profile = true;
         else if (a == "--heat") {
             if (std::sscanf(next(), "%d:%lf:%lf", &heat_cond, &heat_const, &pr) != 3)
                 throw std::invalid_argument("--heat expects COND:HEAT_CONST:PR");
@@ -148,6 +155,8 @@ int main(int argc, char** argv) {
         if (!sensors_in.empty())
             probes.reset(new sensors(&S, &convective, sensors_in.c_str(),
                                      sensors_out.empty() ? "sensors.txt" : sensors_out.c_str()));
+        std::unique_ptr<Stepdata> stepdata;
+        if (!stepdata_path.empty()) stepdata.reset(new Stepdata(&S, stepdata_path.c_str(), profile));
 
         const void* collision = nullptr;
         double total_ms = 0;
@@ -164,6 +173,7 @@ int main(int argc, char** argv) {
             S.calc_forces();
             const TBody& b = *S.BodyList[0];
             const TVec3D fh = b.force_hydro, fr = b.friction;
+            if (stepdata) stepdata->write();
             S.zero_forces();
             tr.build();
             epsilon.CalcEpsilonFast(is_viscous);

@@ -285,7 +285,75 @@ int guarded(F f) {
     }
 }
 
+// ----------------------------------------------------------- step data --
+
+// A float dataset of rows x cols with unlimited rows (chunks of 128 rows,
+// deflate 9, NaN fill); an existing one is cut or grown to rows
+// (MStepdata.cpp:255-289).
+hid_t step_dataset(hid_t loc, const char* name, hsize_t rows, hsize_t cols) {
+    if (H5Lexists(loc, name, H5P_DEFAULT) > 0) {
+        const hid_t d = checkid(H5Dopen2(loc, name, H5P_DEFAULT), "H5Dopen", name);
+        hsize_t dims[2] = {rows, cols};
+        check(H5Dset_extent(d, dims), "H5Dset_extent", name);
+        return d;
+    }
+    static const float nanf_ = std::nanf("");
+    const hid_t prop = H5Pcreate(H5P_DATASET_CREATE);
+    hsize_t chunk[2] = {128, cols};
+    H5Pset_chunk(prop, 2, chunk);
+    H5Pset_deflate(prop, 9);
+    H5Pset_fill_value(prop, H5T_NATIVE_FLOAT, &nanf_);
+    H5Pset_fill_time(prop, H5D_FILL_TIME_ALLOC);
+    hsize_t cur[2] = {rows, cols}, mx[2] = {H5S_UNLIMITED, cols};
+    const hid_t sp = checkid(H5Screate_simple(2, cur, mx), "H5Screate_simple", name);
+    const hid_t d = checkid(H5Dcreate2(loc, name, H5T_NATIVE_FLOAT, sp, H5P_DEFAULT, prop, H5P_DEFAULT), "H5Dcreate",
+                            name);
+    H5Sclose(sp);
+    H5Pclose(prop);
+    return d;
+}
+
+// One more row of doubles at the end of a dataset (MStepdata.cpp:196-240).
+void step_append(hid_t d, const double* row) {
+    if (d < 0) return;
+    hid_t sf = checkid(H5Dget_space(d), "H5Dget_space", "append");
+    hsize_t dims[2] = {0, 0};
+    check(H5Sget_simple_extent_dims(sf, dims, nullptr), "H5Sget_simple_extent_dims", "append");
+    H5Sclose(sf);
+    hsize_t off[2] = {dims[0], 0}, ext[2] = {1, dims[1]};
+    dims[0]++;
+    check(H5Dset_extent(d, dims), "H5Dset_extent", "append");
+    sf = checkid(H5Dget_space(d), "H5Dget_space", "append");
+    check(H5Sselect_hyperslab(sf, H5S_SELECT_SET, off, nullptr, ext, nullptr), "H5Sselect_hyperslab", "append");
+    const hid_t sm = checkid(H5Screate_simple(2, ext, nullptr), "H5Screate_simple", "append");
+    check(H5Dwrite(d, H5T_NATIVE_DOUBLE, sm, sf, H5P_DEFAULT, row), "H5Dwrite", "append");
+    H5Sclose(sm);
+    H5Sclose(sf);
+}
+
+struct BodySets {
+    hid_t force_hydro = -1, force_holder = -1, force_friction = -1, position = -1, spring = -1, speed = -1;
+    hid_t pressure = -1, friction = -1;
+    int32_t nseg = 0;
+};
+
 }  // namespace
+
+struct vvhd_stepdata {
+    hid_t file = -1;
+    hid_t time = -1;
+    bool profile = false;
+    std::vector<BodySets> bodies;
+    long long last_flush = 0;
+    ~vvhd_stepdata() {
+        if (time >= 0) H5Dclose(time);
+        for (BodySets& b : bodies)
+            for (hid_t d : {b.force_hydro, b.force_holder, b.force_friction, b.position, b.spring, b.speed, b.pressure,
+                            b.friction})
+                if (d >= 0) H5Dclose(d);
+        if (file >= 0) H5Fclose(file);
+    }
+};
 
 extern "C" {
 
@@ -424,6 +492,114 @@ int vvhd_hdf_load(const char* path, vvhd_hdf_space* sp, double* const* lists, vv
             s += B.nseg;
         }
     });
+}
+
+vvhd_stepdata* vvhd_stepdata_open(const char* path, const char* caption, double time, int32_t nbody,
+                                  const int32_t* nseg, int32_t save_profile) {
+    vvhd_stepdata* sd = new vvhd_stepdata;
+    const int rc = guarded([&] {
+        if (!path || nbody < 0 || (nbody && !nseg)) throw H5Fail("vvhd_stepdata_open: bad arguments");
+        sd->profile = save_profile != 0;
+        hsize_t rows = 0;
+        if (H5Fis_hdf5(path) <= 0) {
+            sd->file = checkid(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), "H5Fcreate", path);
+            Types T;
+            attr_str(sd->file, "caption", caption, T);
+            attr_str(sd->file, "git_rev", "vvflow_amd", T);
+            attr_str(sd->file, "git_info", "MI355X device path", T);
+            attr_str(sd->file, "git_diff", " ", T);
+            check(H5Tcommit2(sd->file, "string_t", T.str, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), "H5Tcommit",
+                  "string_t");
+        } else {
+            sd->file = checkid(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), "H5Fopen", path);
+            if (H5Lexists(sd->file, "time", H5P_DEFAULT) > 0) {
+                // keep the rows written before this time (MStepdata.cpp:54-75)
+                const hid_t d = checkid(H5Dopen2(sd->file, "time", H5P_DEFAULT), "H5Dopen", "time");
+                const hid_t sp = H5Dget_space(d);
+                hsize_t dims[2] = {0, 0};
+                H5Sget_simple_extent_dims(sp, dims, nullptr);
+                std::vector<float> t((size_t)(dims[0] * dims[1]) + 1);
+                check(H5Dread(d, H5T_NATIVE_FLOAT, sp, sp, H5P_DEFAULT, t.data()), "H5Dread", "time");
+                while (rows < dims[0] && t[(size_t)(rows * dims[1])] < (float)time) rows++;
+                H5Sclose(sp);
+                H5Dclose(d);
+            }
+        }
+        sd->time = step_dataset(sd->file, "time", rows, 1);
+        sd->bodies.resize((size_t)nbody);
+        for (int32_t b = 0; b < nbody; b++) {
+            char name[32];
+            std::snprintf(name, sizeof(name), "body%02d", b);
+            const hid_t g = H5Lexists(sd->file, name, H5P_DEFAULT) > 0
+                                ? checkid(H5Gopen2(sd->file, name, H5P_DEFAULT), "H5Gopen", name)
+                                : checkid(H5Gcreate2(sd->file, name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                                          "H5Gcreate", name);
+            BodySets& B = sd->bodies[(size_t)b];
+            B.nseg = nseg[b];
+            B.force_hydro = step_dataset(g, "force_hydro", rows, 3);
+            B.force_holder = step_dataset(g, "force_holder", rows, 3);
+            B.force_friction = step_dataset(g, "force_friction", rows, 3);
+            B.position = step_dataset(g, "holder_position", rows, 3);
+            B.spring = step_dataset(g, "delta_position", rows, 3);
+            B.speed = step_dataset(g, "speed_slae", rows, 3);
+            const hsize_t ns = (hsize_t)std::max<int32_t>(nseg[b], 1);
+            if (sd->profile) {
+                B.pressure = step_dataset(g, "pressure", rows, ns);
+                B.friction = step_dataset(g, "friction", rows, ns);
+            } else {
+                for (const char* n : {"pressure", "friction"})
+                    if (H5Lexists(g, n, H5P_DEFAULT) > 0) H5Dclose(step_dataset(g, n, rows, ns));
+            }
+            H5Gclose(g);
+        }
+    });
+    if (rc != 0) {
+        delete sd;
+        return nullptr;
+    }
+    return sd;
+}
+
+int vvhd_stepdata_write(vvhd_stepdata* sd, double time, const vvhd_stepdata_body* bodies, const double* pressure,
+                        const double* friction) {
+    return guarded([&] {
+        if (!sd || (!sd->bodies.empty() && !bodies) || (sd->profile && (!pressure || !friction)))
+            throw H5Fail("vvhd_stepdata_write: bad arguments");
+        step_append(sd->time, &time);
+        size_t s = 0;
+        for (size_t b = 0; b < sd->bodies.size(); b++) {
+            const BodySets& B = sd->bodies[b];
+            const vvhd_stepdata_body& r = bodies[b];
+            step_append(B.force_hydro, r.force_hydro);
+            step_append(B.force_holder, r.force_holder);
+            step_append(B.force_friction, r.friction);
+            step_append(B.position, r.holder);
+            step_append(B.spring, r.dpos);
+            step_append(B.speed, r.speed_slae);
+            if (sd->profile && B.nseg > 0) {
+                step_append(B.pressure, pressure + s);
+                step_append(B.friction, friction + s);
+            }
+            s += (size_t)B.nseg;
+        }
+        timespec clk;
+        clock_gettime(CLOCK_MONOTONIC, &clk);
+        if (clk.tv_sec - sd->last_flush > 20) {
+            check(H5Fflush(sd->file, H5F_SCOPE_GLOBAL), "H5Fflush", "stepdata");
+            sd->last_flush = clk.tv_sec;
+        }
+    });
+}
+
+int vvhd_stepdata_flush(vvhd_stepdata* sd) {
+    if (!sd) return -1;
+    sd->last_flush = 0;
+    return 0;
+}
+
+int vvhd_stepdata_close(vvhd_stepdata* sd) {
+    delete sd;
+    return 0;
 }
 
 }  // extern "C"

@@ -253,6 +253,10 @@ struct HdfLib {
     decltype(&vvhd_hdf_sizes) sizes = nullptr;
     decltype(&vvhd_hdf_load) load = nullptr;
     decltype(&vvhd_hdf_error) error = nullptr;
+    decltype(&vvhd_stepdata_open) step_open = nullptr;
+    decltype(&vvhd_stepdata_write) step_write = nullptr;
+    decltype(&vvhd_stepdata_flush) step_flush = nullptr;
+    decltype(&vvhd_stepdata_close) step_close = nullptr;
 };
 
 const HdfLib& hdf_lib() {
@@ -266,12 +270,17 @@ const HdfLib& hdf_lib() {
         dir = dir.substr(0, dir.find_last_of('/') + 1);
     }
     void* h = dlopen((dir + "libvvhd_hdf.so").c_str(), RTLD_NOW | RTLD_LOCAL);
-    if (!h) throw std::runtime_error(std::string("Space::save/load: ") + dlerror());
+    if (!h) throw std::runtime_error(std::string("libvvhd_hdf.so: ") + dlerror());
     L.save = reinterpret_cast<decltype(&vvhd_hdf_save)>(dlsym(h, "vvhd_hdf_save"));
     L.sizes = reinterpret_cast<decltype(&vvhd_hdf_sizes)>(dlsym(h, "vvhd_hdf_sizes"));
     L.load = reinterpret_cast<decltype(&vvhd_hdf_load)>(dlsym(h, "vvhd_hdf_load"));
     L.error = reinterpret_cast<decltype(&vvhd_hdf_error)>(dlsym(h, "vvhd_hdf_error"));
-    if (!L.save || !L.sizes || !L.load || !L.error) throw std::runtime_error("Space::save/load: incomplete libvvhd_hdf");
+    L.step_open = reinterpret_cast<decltype(&vvhd_stepdata_open)>(dlsym(h, "vvhd_stepdata_open"));
+    L.step_write = reinterpret_cast<decltype(&vvhd_stepdata_write)>(dlsym(h, "vvhd_stepdata_write"));
+    L.step_flush = reinterpret_cast<decltype(&vvhd_stepdata_flush)>(dlsym(h, "vvhd_stepdata_flush"));
+    L.step_close = reinterpret_cast<decltype(&vvhd_stepdata_close)>(dlsym(h, "vvhd_stepdata_close"));
+    if (!L.save || !L.sizes || !L.load || !L.error || !L.step_open || !L.step_write || !L.step_flush || !L.step_close)
+        throw std::runtime_error("libvvhd_hdf.so: missing entry points");
     done = true;
     return L;
 }
@@ -911,4 +920,47 @@ void sensors::output() {
     for (size_t k = 0; k < slist.size(); k++) fprintf(fout, " \t%lg \t%lg", v[2 * k], v[2 * k + 1]);
     fprintf(fout, "\n");
     fflush(fout);
+}
+
+// ------------------------------------------------------------ step data --
+
+Stepdata::Stepdata(Space* S_, const char* fname, bool profile) : S(S_), b_save_profile(profile) {
+    const HdfLib& H = hdf_lib();
+    std::vector<int32_t> nseg;
+    for (const auto& b : S->BodyList) nseg.push_back((int32_t)b->alist.size());
+    h_ = H.step_open(fname, S->caption.c_str(), S->time, (int32_t)nseg.size(), nseg.data(), profile ? 1 : 0);
+    if (!h_) throw std::runtime_error(std::string("Stepdata: ") + H.error());
+}
+
+Stepdata::~Stepdata() {
+    if (h_) hdf_lib().step_close(h_);
+}
+
+void Stepdata::flush() { hdf_lib().step_flush(h_); }
+
+void Stepdata::write() {
+    const HdfLib& H = hdf_lib();
+    std::vector<vvhd_stepdata_body> rows(S->BodyList.size());
+    std::vector<double> pressure, friction;
+    for (size_t bi = 0; bi < S->BodyList.size(); bi++) {
+        TBody& b = *S->BodyList[bi];
+        const TVec3D* src[6] = {&b.force_hydro, &b.force_holder, &b.friction, &b.holder, &b.dpos, &b.speed_slae};
+        double* dst[6] = {rows[bi].force_hydro, rows[bi].force_holder, rows[bi].friction,
+                          rows[bi].holder,      rows[bi].dpos,         rows[bi].speed_slae};
+        for (int k = 0; k < 6; k++) {
+            dst[k][0] = src[k]->r.x;
+            dst[k][1] = src[k]->r.y;
+            dst[k][2] = src[k]->o;
+        }
+        if (b_save_profile)
+            for (TAtt& a : b.alist) {
+                pressure.push_back(a.Cp / S->dt);
+                friction.push_back(a.Fr / S->dt);
+                a.Cp = a.Fr = a.Nu = 0;
+            }
+    }
+    if (H.step_write(h_, S->time, rows.data(), b_save_profile ? pressure.data() : nullptr,
+                     b_save_profile ? friction.data() : nullptr) != 0)
+        throw std::runtime_error(std::string("Stepdata::write: ") + H.error());
+    if (b_save_profile && !S->BodyList.empty()) S->check(vvhd_zero_profiles(S->ctx_));
 }

@@ -1,9 +1,13 @@
-"""hdf.py -- the reference's HDF5 snapshot format (Space::save_hdf /
-load_hdf, libvvhd/src/TSpace_save_hdf.cpp, TSpace_load_hdf.cpp) through
-libvvhd_hdf.so (include/vvhd_hdf.h).  Host code: no device is needed.
+"""hdf.py -- the reference's HDF5 formats through libvvhd_hdf.so
+(include/vvhd_hdf.h): snapshots (Space::save_hdf / load_hdf,
+libvvhd/src/TSpace_save_hdf.cpp, TSpace_load_hdf.cpp) and the per-step time
+series (Stepdata, libvvhd/src/MStepdata.cpp).  Host code: no device is
+needed.
 
     save(path, space, lists, bodies)
     space, lists, bodies = load(path)
+    with Stepdata(path, caption, time, nseg=[500]) as sd:
+        sd.write(time, bodies)      # one row per step
 
 space: dict of the header fields of vvhd_hdf_space (caption, time, dt,
 dt_save, dt_streak, dt_profile, re, pr, inf_g, finish, inf_marker, gravity,
@@ -44,6 +48,11 @@ class Body(C.Structure):
                 ("heat_condition", C.c_uint32), ("nseg", C.c_int32)]
 
 
+class StepBody(C.Structure):
+    _fields_ = [("force_hydro", _V3), ("force_holder", _V3), ("friction", _V3), ("holder", _V3), ("dpos", _V3),
+                ("speed_slae", _V3)]
+
+
 _lib = None
 
 
@@ -63,6 +72,15 @@ def lib():
                                     C.c_void_p, C.c_void_p]
         L.vvhd_hdf_error.restype = C.c_char_p
         L.vvhd_hdf_error.argtypes = []
+        L.vvhd_stepdata_open.restype = C.c_void_p
+        L.vvhd_stepdata_open.argtypes = [C.c_char_p, C.c_char_p, C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                                         C.c_int32]
+        L.vvhd_stepdata_write.restype = C.c_int
+        L.vvhd_stepdata_write.argtypes = [C.c_void_p, C.c_double, C.POINTER(StepBody), C.c_void_p, C.c_void_p]
+        L.vvhd_stepdata_flush.restype = C.c_int
+        L.vvhd_stepdata_flush.argtypes = [C.c_void_p]
+        L.vvhd_stepdata_close.restype = C.c_int
+        L.vvhd_stepdata_close.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -142,3 +160,43 @@ def load(path):
         s += n
         bodies.append(b)
     return _get(sp), lists, bodies
+
+
+class Stepdata:
+    """The reference's per-step time series (Stepdata, MStepdata.cpp): one
+    row per write of the time and of every body's forces, holder position,
+    spring displacement and SLAE speed, and with profile=True the
+    per-segment pressure and friction.  Opening an existing file keeps the
+    rows before `time` (a restart).  bodies: dicts with force_hydro,
+    force_holder, friction, holder, dpos, speed_slae (3 numbers each);
+    pressure / friction: one array of all bodies' segments."""
+
+    def __init__(self, path, caption="", time=0.0, nseg=(), profile=False):
+        self.nb = len(nseg)
+        ns = (C.c_int32 * max(self.nb, 1))(*nseg)
+        self.h = lib().vvhd_stepdata_open(os.fsencode(path), caption.encode(), float(time), self.nb, ns, int(profile))
+        if not self.h:
+            raise RuntimeError(lib().vvhd_hdf_error().decode())
+
+    def write(self, time, bodies, pressure=None, friction=None):
+        bs = (StepBody * max(self.nb, 1))()
+        for k, b in enumerate(bodies):
+            _set(bs[k], b)
+        p = None if pressure is None else np.ascontiguousarray(pressure, dtype=np.float64)
+        f = None if friction is None else np.ascontiguousarray(friction, dtype=np.float64)
+        _check(lib().vvhd_stepdata_write(self.h, float(time), bs, None if p is None else p.ctypes.data,
+                                         None if f is None else f.ctypes.data))
+
+    def flush(self):
+        lib().vvhd_stepdata_flush(self.h)
+
+    def close(self):
+        if self.h:
+            lib().vvhd_stepdata_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
