@@ -1,8 +1,8 @@
 /*
- * vvhd_hdf.h -- the reference's HDF5 snapshot format (Space::save_hdf /
- * load_hdf, vvflow/vvflow libvvhd/src/TSpace_save_hdf.cpp:11-221,
- * TSpace_load_hdf.cpp:11-228, vvhdf.cpp, vvhdf_h5t.cpp), host code in
- * libvvhd_hdf.so over libhdf5.
+ * vvhd_hdf.h -- the reference's HDF5 formats, host code in libvvhd_hdf.so
+ * over libhdf5: snapshots (Space::save_hdf / load_hdf, vvflow/vvflow
+ * libvvhd/src/TSpace_save_hdf.cpp:11-221, TSpace_load_hdf.cpp:11-228,
+ * vvhdf.cpp, vvhdf_h5t.cpp) and the per-step time series (Stepdata, below).
  *
  * A file holds:
  *  - root attributes: caption, time, dt, dt_save, dt_streak, dt_profile
@@ -21,8 +21,9 @@
  *    force_holder, friction_prev, fdt_dead, g_dead, collision_min,
  *    collision_max, bounce, area, cofm, moi_cofm, moi_axis,
  *    special_segment_no, boundary_condition, heat_condition).
- * Times are written as fractions in lowest terms (the reference's TTime
- * may carry a larger common timescale for the same value).
+ * Times are written as the fractions the caller gives (frac_value /
+ * frac_scale, as the reference writes its TTime), or from the doubles in
+ * lowest terms.
  * Loading gives every TObj v = 0 and _1_eps = 0, as the reference's
  * strided read does (TSpace_load_hdf.cpp:11-45); the caller rebuilds the
  * segments from the corners (TBody::doUpdateSegments).
@@ -45,6 +46,12 @@ typedef struct vvhd_hdf_space {
     double re, pr, inf_g, finish;
     double inf_marker[2], gravity[2];
     char inf_speed_x[VVHD_HDF_STR], inf_speed_y[VVHD_HDF_STR];
+    /* the same five times as exact fractions value / timescale, in the order
+     * time, dt, dt_save, dt_streak, dt_profile: written as given when the
+     * timescale is non-zero (else the double, in lowest terms), and filled
+     * by the loader */
+    int32_t frac_value[5];
+    uint32_t frac_scale[5];
 } vvhd_hdf_space;
 
 typedef struct vvhd_hdf_body {

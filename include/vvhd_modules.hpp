@@ -3,9 +3,8 @@
  *
  * The classes keep the names and the public signatures of the reference's
  * modules, so that its time loop (utils/vvflow/vvflow.cpp:214-262) is
- * written against them call for call.  Where they differ: Space::time is a
- * double (no TTime), the free stream is set as numbers (no TEval), and a
- * Space names its device.  The classes:
+ * written against them call for call.  Where they differ: the free stream is
+ * set as numbers (no TEval), and a Space names its device.  The classes:
  *   stree / TSortedTree   libvvhd/headers/TSortedTree.hpp:60-92
  *   MConvectiveFast       libvvhd/headers/MConvectiveFast.hpp:9-27
  *   MEpsilonFast          libvvhd/headers/MEpsilonFast.hpp:7-31
@@ -40,12 +39,60 @@
 #include <cstdio>
 #include <limits>
 #include <memory>
+#include <numeric>
 #include <ostream>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "vvhd_gpu.h"
+
+// Simulation time as an exact fraction value / timescale (TTime.hpp): time
+// steps add without rounding, and "every dt_save" is exact divisibility.
+class TTime {
+   public:
+    int32_t value = 0;
+    uint32_t timescale = 1;
+
+    TTime() = default;
+    TTime(int32_t v, uint32_t ts) : value(v), timescale(ts) {}
+
+    static TTime makeWithSeconds(double seconds, uint32_t preferredTimescale) {
+        return TTime((int32_t)(seconds * preferredTimescale), preferredTimescale);
+    }
+    // The smallest power-of-ten timescale (below 0x19999999) at which the
+    // truncated value gives `seconds` back, then reduced by tens while the
+    // value stays a multiple of ten.
+    static TTime makeWithSecondsDecimal(double seconds) {
+        uint32_t ts = 1;
+        while ((double)(int32_t)(seconds * ts) / (double)ts != seconds && ts < 0x19999999u) ts *= 10;
+        while (ts > 1 && (int32_t)(seconds * ts) % 10 == 0) ts /= 10;
+        return TTime((int32_t)(seconds * ts), ts);
+    }
+    static TTime add(TTime a, TTime b) {
+        const uint32_t ts = lcm(a.timescale, b.timescale);
+        return TTime(a.value * (int32_t)(ts / a.timescale) + b.value * (int32_t)(ts / b.timescale), ts);
+    }
+    // value / timescale is a whole multiple of the divisor (never for a zero
+    // divisor)
+    bool divisibleBy(TTime d) const {
+        if (!d.value || !d.timescale) return false;
+        const uint32_t ts = lcm(timescale, d.timescale);
+        return ((int64_t)value * ts / timescale) % ((int64_t)d.value * ts / d.timescale) == 0;
+    }
+    operator double() const { return double(value) / double(timescale); }
+    operator std::string() const { return std::to_string(value) + "/" + std::to_string(timescale); }
+
+   private:
+    // least common multiple; 1 when either is zero, and the first factor
+    // again when the multiple does not fit 32 bits
+    static uint32_t lcm(uint32_t x, uint32_t y) {
+        if (x == y) return x;
+        if (!x || !y) return 1;
+        const uint64_t l = (uint64_t)(x / std::gcd(x, y)) * y;
+        return l > 0xffffffffu ? x : (uint32_t)l;
+    }
+};
 
 struct TVec {
     double x = 0, y = 0;
@@ -130,11 +177,10 @@ class Space {
     double pr = 0;
     double inf_g = 0;
     TVec inf_vel{0, 0};   // U_inf (the reference evaluates inf_vx/inf_vy expressions)
-    double time = 0;
-    double dt = 1;
-    double finish = 0;
-    double dt_streak = 0;   // period of streak shedding (the reference keeps it as a TTime)
-    double dt_save = 0, dt_profile = 0;
+    TTime time;                        // current simulation time
+    TTime dt{1, 1};                    // time step
+    TTime dt_save, dt_streak, dt_profile;   // periods of saving, streak shedding, profiles
+    double finish = 0;                 // time to finish the simulation
     std::string caption;
     std::string inf_vx = "1", inf_vy = "0";   // the free stream's expressions, kept for snapshots
     TVec inf_marker{0, 0};
@@ -148,7 +194,7 @@ class Space {
     TVec inf_speed() const { return inf_vel; }
     void calc_forces();
     void zero_forces();
-    void finish_step() { time = time + dt; }
+    void finish_step() { time = TTime::add(time, dt); }
     double average_segment_length() const;
     // sums over the vortex list on the device (TSpace.hpp:85-117)
     double gsum();

@@ -50,3 +50,69 @@ def test_space_without_gpu_throws(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0
     assert r.stdout.strip() in ("gpu", "no gpu")
+
+
+TTIME_PROBE = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <string>
+#include HEADER
+int main() {
+    // fixed cases, then a seeded sweep of decimal times and sums
+    const double ds[] = {0.005, 0.01, 0.02, 0.125, 0.1, 1.0, 2.5, 0.0, 0.3, 7e-4, 1e-3};
+    for (double d : ds) {
+        TTime t = TTime::makeWithSecondsDecimal(d);
+        std::printf("dec %d %u\n", t.value, t.timescale);
+    }
+    std::srand(7);
+    for (int k = 0; k < 400; k++) {
+        const double a = (std::rand() % 2000) * 0.005, b = ((std::rand() % 50) + 1) * 0.005;
+        TTime ta = TTime::makeWithSecondsDecimal(a), tb = TTime::makeWithSecondsDecimal(b);
+        TTime s = TTime::add(ta, tb);
+        std::printf("add %d %u div %d %d %.17g\n", s.value, s.timescale, (int)ta.divisibleBy(tb),
+                    (int)s.divisibleBy(TTime(1, 100)), double(s));
+    }
+    TTime z;
+    std::printf("zero %d %d\n", (int)z.divisibleBy(TTime(0, 1)), (int)TTime(3, 8).divisibleBy(TTime(1, 8)));
+    TTime m = TTime::add(TTime(3, 8), TTime(5, 1000));
+    std::printf("mixed %d %u %s\n", m.value, m.timescale, std::string(m).c_str());
+    return 0;
+}
+"""
+
+
+def run_ttime_probe(tmp_path, header, extra):
+    src = tmp_path / "t.cpp"
+    src.write_text(TTIME_PROBE.replace("HEADER", f'"{header}"'))
+    exe = tmp_path / ("t_" + os.path.basename(header).split(".")[0])
+    subprocess.run(["g++", "-std=c++17", "-O1", *extra, str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+
+def test_ttime_known_answers(tmp_path):
+    """TTime (TTime.hpp): decimal construction, exact sums over the least
+    common timescale, divisibility, conversions."""
+    out = run_ttime_probe(tmp_path, os.path.join(INC, "vvhd_modules.hpp"), [f"-I{INC}"]).splitlines()
+    dec = [tuple(map(int, ln.split()[1:])) for ln in out if ln.startswith("dec")]
+    assert dec[:9] == [(5, 1000), (1, 100), (2, 100), (125, 1000), (1, 10), (1, 1), (25, 10), (0, 1), (3, 10)]
+    zero = [ln for ln in out if ln.startswith("zero")][0].split()
+    assert zero[1:] == ["0", "1"]
+    assert [ln for ln in out if ln.startswith("mixed")][0].split()[1:] == ["380", "1000", "380/1000"]
+    for ln in (l for l in out if l.startswith("add")):
+        p = ln.split()
+        assert int(p[1]) / int(p[2]) == float(p[6])
+
+
+REF_TTIME = "/root/reference/libvvhd/headers/TTime.hpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TTIME), reason="the reference tree is not here")
+def test_ttime_matches_reference_header(tmp_path):
+    """The same probe built on the reference's own header-only TTime gives the
+    same lines (value / timescale of every construction and sum, every
+    divisibility answer)."""
+    ours = run_ttime_probe(tmp_path, os.path.join(INC, "vvhd_modules.hpp"), [f"-I{INC}"])
+    ref = run_ttime_probe(tmp_path, REF_TTIME, ["-include", "cstdio"])
+    assert ours == ref

@@ -124,7 +124,7 @@ int main(int argc, char** argv) {
             S.load(load_path.c_str());
         } else {
             S.re = re;
-            S.dt = dt;
+            S.dt = TTime::makeWithSecondsDecimal(dt);
             S.inf_vel = TVec(1, 0);
             S.caption = "cylinder";
             auto body = std::make_shared<TBody>();
@@ -135,7 +135,7 @@ int main(int argc, char** argv) {
             for (TAtt& a : body->alist) a.heat_const = (float)heat_const;
             S.BodyList.push_back(body);
             S.pr = pr;
-            S.dt_streak = dt_streak;
+            S.dt_streak = TTime::makeWithSecondsDecimal(dt_streak);
             for (int k = 0; k < nstreak; k++)
                 S.StreakSourceList.emplace_back(-1.0, nstreak > 1 ? -1.0 + 2.0 * k / (nstreak - 1) : 0.0, 0.0);
             S.push();
@@ -191,7 +191,7 @@ int main(int argc, char** argv) {
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             total_ms += ms;
             if (every > 0 && (step + 1) % every == 0)
-                printf("%d %.6f %lld %.10e %.10e %.10e %.10e %.10e %.10e %.3f\n", step + 1, S.time,
+                printf("%d %.6f %lld %.10e %.10e %.10e %.10e %.10e %.10e %.3f\n", step + 1, double(S.time),
                        (long long)S.vortex_count(), fh.r.x, fh.r.y, fh.o, fr.r.x, fr.r.y, fr.o, ms);
         }
         printf("# steps %d vortices %lld total %.1f ms mean %.3f ms/step\n", steps, (long long)S.vortex_count(),

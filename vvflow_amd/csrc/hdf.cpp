@@ -130,11 +130,6 @@ double read_double(hid_t h, const char* n) {
     attr_read(h, n, H5T_NATIVE_DOUBLE, &v);
     return v;
 }
-double read_frac(hid_t h, const char* n, const Types& T) {
-    Frac f{0, 1};
-    if (!attr_read(h, n, T.frac, &f) || f.timescale == 0) return 0;
-    return (double)f.value / (double)f.timescale;
-}
 void read_str(hid_t h, const char* n, char* out, const Types& T) {
     out[0] = 0;
     char* p = nullptr;
@@ -368,8 +363,10 @@ int vvhd_hdf_save(const char* path, const vvhd_hdf_space* sp, const double* cons
         f.id = checkid(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), "H5Fcreate", path);
         Types T;
         attr_str(f.id, "caption", sp->caption, T);
-        const Frac fr[5] = {to_frac(sp->time), to_frac(sp->dt), to_frac(sp->dt_save), to_frac(sp->dt_streak),
-                            to_frac(sp->dt_profile)};
+        const double tv[5] = {sp->time, sp->dt, sp->dt_save, sp->dt_streak, sp->dt_profile};
+        Frac fr[5];
+        for (int k = 0; k < 5; k++)
+            fr[k] = sp->frac_scale[k] ? Frac{sp->frac_value[k], sp->frac_scale[k]} : to_frac(tv[k]);
         const char* fn[5] = {"time", "dt", "dt_save", "dt_streak", "dt_profile"};
         for (int k = 0; k < 5; k++) attr_write(f.id, fn[k], T.frac, &fr[k]);
         attr_double(f.id, "re", sp->re);
@@ -422,11 +419,15 @@ int vvhd_hdf_load(const char* path, vvhd_hdf_space* sp, double* const* lists, vv
         Types T;
         std::memset(sp, 0, sizeof(*sp));
         read_str(f.id, "caption", sp->caption, T);
-        sp->time = read_frac(f.id, "time", T);
-        sp->dt = read_frac(f.id, "dt", T);
-        sp->dt_save = read_frac(f.id, "dt_save", T);
-        sp->dt_streak = read_frac(f.id, "dt_streak", T);
-        sp->dt_profile = read_frac(f.id, "dt_profile", T);
+        const char* fn[5] = {"time", "dt", "dt_save", "dt_streak", "dt_profile"};
+        double* tv[5] = {&sp->time, &sp->dt, &sp->dt_save, &sp->dt_streak, &sp->dt_profile};
+        for (int k = 0; k < 5; k++) {
+            Frac q{0, 1};
+            if (!attr_read(f.id, fn[k], T.frac, &q) || q.timescale == 0) q = Frac{0, 1};
+            sp->frac_value[k] = q.value;
+            sp->frac_scale[k] = q.timescale;
+            *tv[k] = (double)q.value / (double)q.timescale;
+        }
         sp->re = read_double(f.id, "re");
         sp->pr = read_double(f.id, "pr");
         attr_read(f.id, "inf_marker", T.vec, sp->inf_marker);

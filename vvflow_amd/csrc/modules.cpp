@@ -295,11 +295,13 @@ void Space::save(const char* path) {
     vvhd_hdf_space sp;
     std::memset(&sp, 0, sizeof(sp));
     copy_str(sp.caption, caption);
-    sp.time = time;
-    sp.dt = dt;
-    sp.dt_save = dt_save;
-    sp.dt_streak = dt_streak;
-    sp.dt_profile = dt_profile;
+    const TTime* tt[5] = {&time, &dt, &dt_save, &dt_streak, &dt_profile};
+    double* tv[5] = {&sp.time, &sp.dt, &sp.dt_save, &sp.dt_streak, &sp.dt_profile};
+    for (int k = 0; k < 5; k++) {
+        *tv[k] = double(*tt[k]);
+        sp.frac_value[k] = tt[k]->value;
+        sp.frac_scale[k] = tt[k]->timescale;
+    }
     sp.re = re;
     sp.pr = pr;
     sp.inf_g = inf_g;
@@ -388,11 +390,8 @@ void Space::load(const char* path) {
     if (H.load(path, &sp, ptrs, bodies.data(), seg4.data(), slip.data()) != 0)
         throw std::runtime_error(std::string("Space::load: ") + H.error());
     caption = sp.caption;
-    time = sp.time;
-    dt = sp.dt;
-    dt_save = sp.dt_save;
-    dt_streak = sp.dt_streak;
-    dt_profile = sp.dt_profile;
+    TTime* tt[5] = {&time, &dt, &dt_save, &dt_streak, &dt_profile};
+    for (int k = 0; k < 5; k++) *tt[k] = TTime(sp.frac_value[k], sp.frac_scale[k]);
     re = sp.re;
     pr = sp.pr;
     inf_g = sp.inf_g;
@@ -674,14 +673,10 @@ void MFlowmove::heat_crop(double scale) {
 }
 
 // MFlowmove::streak_shed (MFlowmove.cpp:284-291): when the time is a whole
-// multiple of dt_streak.  The reference's TTime is an exact fraction; the
-// time here is a double on the dt grid, so a multiple is recognised within
-// 1e-9 of a step count.
+// multiple of dt_streak (exact, on the fractions).
 void MFlowmove::streak_shed() {
     S->ensure_pushed();
-    if (!(S->dt_streak != 0)) return;   // TTime::divisibleBy: a zero divisor divides nothing
-    const double k = S->time / S->dt_streak;
-    if (!(std::fabs(k - std::nearbyint(k)) <= 1e-9 * std::max(1.0, std::fabs(k)))) return;
+    if (!S->time.divisibleBy(S->dt_streak)) return;
     S->check(vvhd_streak_shed(S->ctx_));
 }
 

@@ -33,7 +33,8 @@ class Space(C.Structure):
     _fields_ = [("caption", C.c_char * STR), ("time", C.c_double), ("dt", C.c_double), ("dt_save", C.c_double),
                 ("dt_streak", C.c_double), ("dt_profile", C.c_double), ("re", C.c_double), ("pr", C.c_double),
                 ("inf_g", C.c_double), ("finish", C.c_double), ("inf_marker", C.c_double * 2),
-                ("gravity", C.c_double * 2), ("inf_speed_x", C.c_char * STR), ("inf_speed_y", C.c_char * STR)]
+                ("gravity", C.c_double * 2), ("inf_speed_x", C.c_char * STR), ("inf_speed_y", C.c_char * STR),
+                ("frac_value", C.c_int32 * 5), ("frac_scale", C.c_uint32 * 5)]
 
 
 class Body(C.Structure):
