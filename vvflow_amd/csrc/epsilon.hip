@@ -406,6 +406,32 @@ __global__ void k_rnear_count(TreeDev T, int32_t* cnt, const uint8_t* mark) {
     }
 }
 
+// The same lists for a few marked leaves, flat over all near-list entries
+// (one thread per entry, coalesced; the owner leaf of a kept entry is found
+// by binary search over the list offsets).
+__global__ void k_rnear_count_flat(TreeDev T, int64_t nnear, int32_t* cnt, const uint8_t* mark) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnear; e += (int64_t)gridDim.x * blockDim.x) {
+        const int m = T.leaf_rank[T.near_list[e]];
+        if (mark[m]) atomicAdd(&cnt[m], 1);
+    }
+}
+__global__ void k_rnear_fill_flat(TreeDev T, int64_t nnear, const int64_t* roff, int32_t* fillc, int32_t* rlist,
+                                  const uint8_t* mark) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnear; e += (int64_t)gridDim.x * blockDim.x) {
+        const int m = T.leaf_rank[T.near_list[e]];
+        if (!mark[m]) continue;
+        int lo = 0, hi = T.nleaf;   // the leaf l with near_off[l] <= e < near_off[l + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (T.near_off[mid] <= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        rlist[roff[m] + atomicAdd(&fillc[m], 1)] = lo;
+    }
+}
+
 // Leaves of the flagged targets and of their first neighbours: the commits
 // of pass 1's flags need their reverse lists.
 __global__ void k_mark_flag_leaves(TreeDev T, const int32_t* flag_list, const int32_t* ecount, const int32_t* nn1,
@@ -798,8 +824,13 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
     // the full build)
     auto build_rnear = [&](const uint8_t* mark) {
         VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
-        hipLaunchKernelGGL(k_rnear_count, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_cnt.p,
-                           mark);
+        const int flat_grid = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (c.t.nnear + kBlock - 1) / kBlock));
+        if (mark)
+            hipLaunchKernelGGL(k_rnear_count_flat, dim3(flat_grid), dim3(kBlock), 0, s, T, (int64_t)c.t.nnear,
+                               c.rnear_cnt.p, mark);
+        else
+            hipLaunchKernelGGL(k_rnear_count, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T,
+                               c.rnear_cnt.p, mark);
         size_t bytes = 0;
         VVHD_HIP(rocprim::exclusive_scan(nullptr, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
                                          rocprim::plus<int64_t>(), s));
@@ -807,8 +838,12 @@ void epsilon(Ctx& c, int merge, int32_t* nn1_out, int32_t* nn2_out, int32_t* mer
         VVHD_HIP(rocprim::exclusive_scan(c.tmp.p, bytes, c.rnear_cnt.p, c.rnear_off.p, (int64_t)0, (size_t)nl + 1,
                                          rocprim::plus<int64_t>(), s));
         VVHD_HIP(hipMemsetAsync(c.rnear_cnt.p, 0, (size_t)(nl + 1) * 4, s));
-        hipLaunchKernelGGL(k_rnear_fill, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T, c.rnear_off.p,
-                           c.rnear_cnt.p, c.rnear_list.p, mark);
+        if (mark)
+            hipLaunchKernelGGL(k_rnear_fill_flat, dim3(flat_grid), dim3(kBlock), 0, s, T, (int64_t)c.t.nnear,
+                               c.rnear_off.p, c.rnear_cnt.p, c.rnear_list.p, mark);
+        else
+            hipLaunchKernelGGL(k_rnear_fill, dim3(nblocks(nl, kWavesPerBlock)), dim3(kBlock), 0, s, T,
+                               c.rnear_off.p, c.rnear_cnt.p, c.rnear_list.p, mark);
     };
     // (buffers grown first: the arguments hold their pointers)
     auto repair_args = [&] {

@@ -375,29 +375,53 @@ __global__ void k_leaf_sim(const int32_t* root_first, int nroots, int nleaf, int
 __global__ void k_decide(NodeView nv, int32_t* counters, int32_t* leaf_tmp, int cap, double min_node,
                          double max_node) {
     const int lb = counters[2], le = counters[3];
-    for (int id = lb + blockIdx.x * blockDim.x + threadIdx.x; id < le; id += gridDim.x * blockDim.x) {
-        const double blx = double_from_ord(nv.blx[id]), bly = double_from_ord(nv.bly[id]);
-        const double trx = double_from_ord(nv.trx[id]), try_ = double_from_ord(nv.try_[id]);
-        const double x = (blx + trx) * 0.5;
-        const double y = (bly + try_) * 0.5;
-        const double h = try_ - bly;
-        const double w = trx - blx;
-        nv.x[id] = x;
-        nv.y[id] = y;
-        nv.h[id] = h;
-        nv.w[id] = w;
-        const double mx = (h < w) ? w : h;   // std::max(h, w)
-        const double mn = (w < h) ? w : h;   // std::min(h, w)
-        const int nvort = nv.v1[id] - nv.v0[id];
-        const int nlist = max(max(nvort, nv.nseg[id]), max(nv.h1[id] - nv.h0[id], nv.s1[id] - nv.s0[id]));
-        const bool leaf = (mx < max_node && mn <= min_node) || (mx < max_node && nlist < kLeafMaxList);
+    const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1;
+    // whole waves step through the level, so the counters are claimed once
+    // per wave (leaf slots, pairs of child ids) and lanes take their ranks
+    const int stride = gridDim.x * blockDim.x;
+    for (int b0 = lb + blockIdx.x * blockDim.x + threadIdx.x - lane; b0 < le; b0 += stride) {
+        const int id = b0 + lane;
+        const bool valid = id < le;
+        bool leaf = false;
+        int depth = 0;
+        if (valid) {
+            const double blx = double_from_ord(nv.blx[id]), bly = double_from_ord(nv.bly[id]);
+            const double trx = double_from_ord(nv.trx[id]), try_ = double_from_ord(nv.try_[id]);
+            const double x = (blx + trx) * 0.5;
+            const double y = (bly + try_) * 0.5;
+            const double h = try_ - bly;
+            const double w = trx - blx;
+            nv.x[id] = x;
+            nv.y[id] = y;
+            nv.h[id] = h;
+            nv.w[id] = w;
+            const double mx = (h < w) ? w : h;   // std::max(h, w)
+            const double mn = (w < h) ? w : h;   // std::min(h, w)
+            const int nvort = nv.v1[id] - nv.v0[id];
+            const int nlist = max(max(nvort, nv.nseg[id]), max(nv.h1[id] - nv.h0[id], nv.s1[id] - nv.s0[id]));
+            leaf = (mx < max_node && mn <= min_node) || (mx < max_node && nlist < kLeafMaxList);
+            depth = nv.depth[id] + 1;
+        }
+        const uint64_t lm = __ballot(valid && leaf), sm = __ballot(valid && !leaf);
+        int lbase = 0, sbase = 0, dmax = depth;
+        for (int o = kWave / 2; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, kWave));
+        if (lane == 0) {
+            if (lm) lbase = atomicAdd(&counters[1], __popcll(lm));
+            if (sm) {
+                sbase = atomicAdd(&counters[0], 2 * __popcll(sm));
+                atomicMax(&counters[5], dmax);
+            }
+        }
+        lbase = __shfl(lbase, 0, kWave);
+        sbase = __shfl(sbase, 0, kWave);
+        if (!valid) continue;
         if (leaf) {
             nv.split[id] = 1;
-            const int li = atomicAdd(&counters[1], 1);
-            leaf_tmp[li] = id;
+            leaf_tmp[lbase + __popcll(lm & below)] = id;
             continue;
         }
-        const int c = atomicAdd(&counters[0], 2);
+        const int c = sbase + 2 * __popcll(sm & below);
         if (c + 2 > cap) {
             counters[4] = 1;
             nv.split[id] = 1;   // stop here; reported as an error by the host
@@ -405,11 +429,9 @@ __global__ void k_decide(NodeView nv, int32_t* counters, int32_t* leaf_tmp, int 
         }
         nv.split[id] = 2;
         nv.ch1[id] = c;
-        const int d = nv.depth[id] + 1;
-        atomicMax(&counters[5], d);
-        init_node(nv, c, id, d, nv.path[id] << 1, nv.v0[id], nv.v0[id], nv.h0[id], nv.h0[id], nv.s0[id], nv.s0[id]);
-        init_node(nv, c + 1, id, d, (nv.path[id] << 1) | 1ull, nv.v1[id], nv.v1[id], nv.h1[id], nv.h1[id], nv.s1[id],
-                  nv.s1[id]);
+        init_node(nv, c, id, depth, nv.path[id] << 1, nv.v0[id], nv.v0[id], nv.h0[id], nv.h0[id], nv.s0[id], nv.s0[id]);
+        init_node(nv, c + 1, id, depth, (nv.path[id] << 1) | 1ull, nv.v1[id], nv.v1[id], nv.h1[id], nv.h1[id],
+                  nv.s1[id], nv.s1[id]);
     }
 }
 
