@@ -141,29 +141,59 @@ def feature_tangents(X, ux, uy, rinv):
     return TX, TY
 
 
+LOG2E_F32 = np.float32(1.44269504088896341)
+LN2_F32 = np.float32(0.693147180559945309)
+
+
+def folded_inference_params(p):
+    """The device's inference weights (farfield.hip, k_pack_model and
+    k_far_learned): with L = log2 e, W1' = bf16((-L W1) / sd) applied to the
+    centred features x - mu, b1' = -L b1, W2' = bf16(W2), b2' = -L b2,
+    W3' = bf16(-W3 / L), b3; so layer 1 gives -L z, 2^(-L z) = e^-z and the
+    hidden values carry the factor -L.  FP32 operations where the device
+    uses FP32."""
+    W1, b1, W2, b2, W3, b3, mu, sd, _ = unpack(np.asarray(p, dtype=np.float32))
+    isd = (np.float32(1.0) / sd).astype(np.float32)
+    W1f = bf16(((-LOG2E_F32) * W1).astype(np.float32) * isd[None, :]).astype(np.float64)
+    b1f = ((-LOG2E_F32) * b1).astype(np.float32).astype(np.float64)
+    b2f = ((-LOG2E_F32) * b2).astype(np.float32).astype(np.float64)
+    W3f = bf16((W3 * (-LN2_F32)).astype(np.float32)).astype(np.float64)
+    return W1f, b1f, bf16(W2).astype(np.float64), b2f, W3f, b3.astype(np.float64)
+
+
 def forward_jvp(p, X, TX, TY, mirror=True):
     """Raw outputs and their forward-mode tangents along TX and TY (inputs
-    before standardisation).  mirror=True rounds the tangents to bf16 where
-    the device rounds them: standardised inputs and hidden activations."""
-    W1, b1, W2, b2, W3, b3, mu, sd, _ = unpack(p)
+    before standardisation).  mirror=True follows the device's inference
+    path: the folded weights of folded_inference_params, bf16 centred
+    features, bf16 raw tangents, and bf16 hidden values and tangents (which
+    carry the factor -L); mirror=False is the plain network in float64."""
     if mirror:
         r = lambda a: bf16(np.asarray(a, dtype=np.float32)).astype(np.float64)
-        xs = r(((np.asarray(X, np.float32) - mu) / sd).astype(np.float32))
-        txs = r(np.asarray(TX, np.float64) / sd)
-        tys = r(np.asarray(TY, np.float64) / sd)
-    else:
-        r = lambda a: np.asarray(a, dtype=np.float64)
-        xs, txs, tys = (X - mu) / sd, TX / sd, TY / sd
-    W1r, W2r, W3r = (r(W) for W in (W1, W2, W3))
+        W1f, b1f, W2f, b2f, W3f, b3 = folded_inference_params(p)
+        mu = unpack(np.asarray(p, dtype=np.float32))[6]
+        xs, txs, tys = r((np.asarray(X, np.float32) - mu).astype(np.float32)), r(TX), r(TY)
+        ln2 = np.float64(LN2_F32)
+
+        def layer(zp, tx, ty):
+            s = 1.0 / (1.0 + np.exp2(zp))
+            h = zp * s
+            d = s + h * (s * ln2 - ln2)
+            return r(h), r(d * tx), r(d * ty)
+
+        h1, t1x, t1y = layer(xs @ W1f.T + b1f, txs @ W1f.T, tys @ W1f.T)
+        h2, t2x, t2y = layer(h1 @ W2f.T + b2f, t1x @ W2f.T, t1y @ W2f.T)
+        return h2 @ W3f.T + b3, t2x @ W3f.T, t2y @ W3f.T
+    W1, b1, W2, b2, W3, b3, mu, sd, _ = unpack(p)
+    xs, txs, tys = (X - mu) / sd, TX / sd, TY / sd
 
     def layer(z, tx, ty):
         s = 1.0 / (1.0 + np.exp(-z))
         d = s * (1.0 + z * (1.0 - s))
-        return r(z * s), r(d * tx), r(d * ty)
+        return z * s, d * tx, d * ty
 
-    h1, t1x, t1y = layer(xs @ W1r.T + b1, txs @ W1r.T, tys @ W1r.T)
-    h2, t2x, t2y = layer(h1 @ W2r.T + b2, t1x @ W2r.T, t1y @ W2r.T)
-    return h2 @ W3r.T + b3, t2x @ W3r.T, t2y @ W3r.T
+    h1, t1x, t1y = layer(xs @ W1.T + b1, txs @ W1.T, tys @ W1.T)
+    h2, t2x, t2y = layer(h1 @ W2.T + b2, t1x @ W2.T, t1y @ W2.T)
+    return h2 @ W3.T + b3, t2x @ W3.T, t2y @ W3.T
 
 
 def correction_jvp(p, X, TX, TY, scale, ux, uy, rinv, mirror=True):

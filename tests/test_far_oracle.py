@@ -265,3 +265,27 @@ def test_correction_gradient_finite_differences():
 def _point(o, P, nd):
     X, sc, u = o.far_point_features(P[:, 0], P[:, 1], nd)
     return X, sc, u[:, 0], u[:, 1]
+
+
+def test_folded_inference_is_the_same_network():
+    """The device's inference weights fold log2 e and the features' 1/sd
+    into the layers (folded_inference_params); through them,
+    with the device's bf16 rounding points, the outputs and both tangents
+    stay within bf16 precision of the plain float64 network (a wrong sign or
+    factor in the fold is off by order one)."""
+    rng = np.random.default_rng(21)
+    mu = rng.normal(size=M.N_IN).astype(np.float32)
+    sd = rng.uniform(0.05, 2.0, M.N_IN).astype(np.float32)
+    p = M.init_params(rng, mu=mu, sd=sd, ysd=np.float32([0.03, 0.07]))
+    X = (mu + sd * rng.normal(size=(3000, M.N_IN))).astype(np.float32)
+    TX, TY = rng.normal(size=(2, 3000, M.N_IN)) * sd
+    got = M.forward_jvp(p, X, TX, TY, mirror=True)
+    want = M.forward_jvp(p, X.astype(np.float64), TX, TY, mirror=False)
+    for g, w in zip(got, want):
+        assert np.linalg.norm(g - w) < 2e-2 * np.linalg.norm(w)
+    W1f, b1f, W2f, b2f, W3f, b3 = M.folded_inference_params(p)
+    W1, b1 = M.unpack(p)[:2]
+    # layer 1 of the folded network is -log2(e) times the standardised one
+    z = ((X.astype(np.float64) - mu) / sd) @ W1.T.astype(np.float64) + b1
+    zf = (X.astype(np.float64) - mu) @ W1f.T + b1f
+    assert np.linalg.norm(zf + np.log2(np.e) * z) < 1e-2 * np.linalg.norm(z)

@@ -64,6 +64,16 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double k1_2Pi = 1. / (2. * kPi);
 
+// The inference weights are folded so that the SiLU exponent needs no scaling
+// and the features go in centred but not scaled (L = log2 e): layer 1 takes
+// W1' = -L W1 diag(1/sd) and b1' = -L b1 on x - mu, so its outputs are
+// z' = -L z and 2^z' = e^-z; the hidden values -L h feed layer 2 unchanged
+// (W2' = W2, b2' = -L b2) and layer 3 takes W3' = -W3 / L.  The tangent
+// chains (raw feature tangents) carry the same -L factor.  Training uses the
+// plain parameters.
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kLn2 = 0.693147180559945309f;   // 1 / L
+
 // parameter layout (also documented in vvhd_gpu.h)
 constexpr int kFeat = 12, kHid = 64, kOut = 2;
 constexpr int oW1 = 0;                    // [64][12]
@@ -385,14 +395,14 @@ __global__ void k_pack_model(const float* P, uint16_t* frag) {
     const int lane = rem / 8, e = rem % 8, m = lane % 32, g = lane / 32;
     if (t >= fB1) {   // 16x16x32 inference sets
         const int m16 = lane % 16, g16 = lane / 16;
-        if (t < fB2) {   // W1 block set: rows 16 set + m16, K = feature 8 g16 + e
+        if (t < fB2) {   // W1' block set: rows 16 set + m16, K = feature 8 g16 + e
             const int k = 8 * g16 + e;
-            if (k < kFeat) v = P[oW1 + (16 * set + m16) * kFeat + k];
+            if (k < kFeat) v = (-kLog2e * P[oW1 + (16 * set + m16) * kFeat + k]) * (1.f / P[oSd + k]);
         } else if (t < fB3) {   // W2: block b = set / 2, K step st = set % 2
             const int b = set / 2, st = set % 2;
             v = P[oW2 + (16 * b + m16) * kHid + chain16(st, g16, e)];
-        } else {   // W3 rows 0, 1, K step set
-            if (m16 < kOut) v = P[oW3 + m16 * kHid + chain16(set, g16, e)];
+        } else {   // W3' rows 0, 1, K step set
+            if (m16 < kOut) v = P[oW3 + m16 * kHid + chain16(set, g16, e)] * -kLn2;
         }
         frag[t] = bf16_bits(v);
         return;
@@ -473,9 +483,10 @@ __global__ void k_far_nodes(TreeDev T, MomView M, int nnodes, FarNode* out) {
 }
 
 // Inference forward of one tile of 16 columns with two forward-mode
-// tangents, on v_mfma_f32_16x16x32_bf16: the tangent inputs tx, ty
-// (standardised by the deviations only) pass through the same weights, and
-// each SiLU scales them by its derivative s (1 + z (1 - s)), s = sigmoid(z);
+// tangents, on v_mfma_f32_16x16x32_bf16, with the folded weights: the raw
+// features and their tangents tx, ty pass through the same weights, and
+// each SiLU scales the tangents by its derivative s (1 + z (1 - s)),
+// s = sigmoid(z);
 // tangents are rounded to bf16 where the activations are.  A layer's four
 // output blocks (16 units each) feed the next layer's two K steps without a
 // transpose (chain16).  Outputs (value, x tangent, y tangent) in lanes 0..15.
@@ -495,7 +506,10 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
         cy = mfma16(a, ty, cy);
         for (int i = 0; i < 4; i++) {
             // SiLU and its derivative s (1 + z (1 - s)) = s + h (1 - s), h = z s
-            const float z = c[i], sg = sigmoid_f(z), hv = z * sg, d = __builtin_fmaf(hv, 1.f - sg, sg);
+            // z' = -L z: sigmoid(z) = 1 / (1 + 2^z'); h' = -L h; the derivative
+            // s + h (1 - s) = s + h' (1 - s) (-1/L)
+            const float zp = c[i], sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(zp)), hv = zp * sg;
+            const float d = __builtin_fmaf(hv, __builtin_fmaf(sg, kLn2, -kLn2), sg);
             const int st = b >> 1, e = 4 * (b & 1) + i;
             h[st][e] = (__bf16)hv;
             hx[st][e] = (__bf16)(d * cx[i]);
@@ -513,7 +527,10 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
             cy = mfma16(a, hy[st], cy);
         }
         for (int i = 0; i < 4; i++) {
-            const float z = c[i], sg = sigmoid_f(z), hv = z * sg, d = __builtin_fmaf(hv, 1.f - sg, sg);
+            // z' = -L z: sigmoid(z) = 1 / (1 + 2^z'); h' = -L h; the derivative
+            // s + h (1 - s) = s + h' (1 - s) (-1/L)
+            const float zp = c[i], sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(zp)), hv = zp * sg;
+            const float d = __builtin_fmaf(hv, __builtin_fmaf(sg, kLn2, -kLn2), sg);
             const int st = b >> 1, e = 4 * (b & 1) + i;
             h2[st][e] = (__bf16)hv;
             h2x[st][e] = (__bf16)(d * cx[i]);
@@ -547,14 +564,16 @@ __device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const
         f32x4 c;
         for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i];
         c = mfma16(W((fB1 - fB1) / kSet + b), x, c);
-        for (int i = 0; i < 4; i++) h[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * sigmoid_f(c[i]));
+        for (int i = 0; i < 4; i++)
+            h[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(c[i])));
     }
     bf16x8 h2[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c;
         for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i];
         for (int st = 0; st < 2; st++) c = mfma16(W((fB2 - fB1) / kSet + 2 * b + st), h[st], c);
-        for (int i = 0; i < 4; i++) h2[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * sigmoid_f(c[i]));
+        for (int i = 0; i < 4; i++)
+            h2[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(c[i])));
     }
     f32x4 c = {};
     c[0] = g == 0 ? b30 : 0.f;
@@ -589,15 +608,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     TreeDev T, const FarNode* __restrict__ nodes, const double* __restrict__ g, const uint16_t* frag, const float* P,
     double* __restrict__ leaf_corr) {
     constexpr int kInfSets = (kFragElems - fB1) / kSet;   // W1, W2, W3 inference sets
-    __shared__ float sb1[kHid], sb2[kHid], smu[16], sisd[16];
+    __shared__ float sb1[kHid], sb2[kHid], smu[16];
     __shared__ uint4 sw[kInfSets * kWave];
     for (int k = threadIdx.x; k < kHid; k += blockDim.x) {
-        sb1[k] = P[oB1 + k];
-        sb2[k] = P[oB2 + k];
-        if (k < 16) {
-            smu[k] = k < kFeat ? P[oMu + k] : 0.f;
-            sisd[k] = k < kFeat ? 1.f / P[oSd + k] : 0.f;
-        }
+        sb1[k] = -kLog2e * P[oB1 + k];   // b1' = -L b1, b2' = -L b2
+        sb2[k] = -kLog2e * P[oB2 + k];
+        if (k < 16) smu[k] = k < kFeat ? P[oMu + k] : 0.f;
     }
     for (int k = threadIdx.x; k < kInfSets * kWave; k += blockDim.x)
         sw[k] = reinterpret_cast<const uint4*>(frag + fB1)[k];
@@ -621,6 +637,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
     const double px = T.nx[L], py = T.ny[L];
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, cmp[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // Kahan sums
+    float mu8[8];   // the means of this lane's eight inputs
+    for (int q = 0; q < 8; q++) mu8[q] = smu[(8 * grp + q) & 15];
     // one tile: 16 far nodes (column n)
     auto tile = [&](const FarNode& nd, bool valid) {
         const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
@@ -658,21 +676,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 fy[2] = -f[2] * (uy * rinv);
             }
         }
-        float xs[8], txs[8], tys[8];
-        for (int q = 0; q < 8; q++) {
-            const int k = (8 * grp + q) & 15;
-            const float is = sisd[k];
-            xs[q] = (f[q] - smu[k]) * is;
-            if constexpr (kJvp) {
-                txs[q] = fx[q] * is;
-                tys[q] = fy[q] * is;
-            }
-        }
-        const bf16x8 xin = pack_bf16x8(xs, ok);
+        // centred features; the 1/sd scale is in the folded layer-1 weights
+        for (int q = 0; q < 8; q++) f[q] -= mu8[q];
+        const bf16x8 xin = pack_bf16x8(f, ok);
         bf16x8 txin, tyin;
         if constexpr (kJvp) {
-            txin = pack_bf16x8(txs, ok);
-            tyin = pack_bf16x8(tys, ok);
+            txin = pack_bf16x8(fx, ok);
+            tyin = pack_bf16x8(fy, ok);
         }
         float y[2], yx[2] = {0.f, 0.f}, yy[2] = {0.f, 0.f};
         if constexpr (kJvp)
