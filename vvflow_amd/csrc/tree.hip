@@ -845,25 +845,6 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const TravNode* _
     const size_t slot = (size_t)l * kMaxEntries + q;
     int64_t pn = (valid && near_off) ? near_off[slot] : 0, pf = (valid && far_off) ? far_off[slot] : 0;
     int cn = 0, cf = 0;
-    const int P0 = E->node[q];
-    const int end = min(tn[P0].skip, nnodes);
-    // a lane takes part at positions >= resume
-    int resume = valid ? P0 : end;
-    // ancestors of the entry: the first far one (if any) cuts the whole entry
-    const int dE = E->depth[q];
-    for (int k = 0; k < dE && valid; k++) {
-        const TravNode a = tn[E->anc[q][k]];
-        const double dx = a.x - lx, dy = a.y - ly;
-        const double hp = a.hw + lh + lw;
-        if (dx * dx + dy * dy > fc * hp * hp) {
-            if ((E->lead[q] >> k) & 1) {
-                if (far_list) far_list[pf++] = a.id;
-                cf++;
-            }
-            resume = end;
-            break;
-        }
-    }
     // the position is uniform: records come through the scalar cache, all
     // 32 bytes in one load
     auto rec = [&](int p) {
@@ -874,6 +855,33 @@ __global__ __launch_bounds__(kBlock) void k_lists(NodeView nv, const TravNode* _
         __builtin_memcpy(reinterpret_cast<char*>(&r) + 16, &u1, 16);
         return r;
     };
+    const int P0 = __builtin_amdgcn_readfirstlane(E->node[q]);
+    const int dE = __builtin_amdgcn_readfirstlane(E->depth[q]);
+    const int lead = __builtin_amdgcn_readfirstlane(E->lead[q]);
+    // the entry's own record and its ancestors' are all loaded before any is
+    // used, so their latencies overlap
+    TravNode anc[kSplitDepth];
+#pragma unroll
+    for (int k = 0; k < kSplitDepth; k++)
+        if (k < dE) anc[k] = rec(__builtin_amdgcn_readfirstlane(E->anc[q][k]));
+    const int end = min(rec(P0).skip, nnodes);
+    // a lane takes part at positions >= resume
+    int resume = valid ? P0 : end;
+    // ancestors of the entry: the first far one (if any) cuts the whole entry
+#pragma unroll
+    for (int k = 0; k < kSplitDepth; k++) {
+        if (k >= dE || !valid || resume == end) continue;
+        const TravNode& a = anc[k];
+        const double dx = a.x - lx, dy = a.y - ly;
+        const double hp = a.hw + lh + lw;
+        if (dx * dx + dy * dy > fc * hp * hp) {
+            if ((lead >> k) & 1) {
+                if (far_list) far_list[pf++] = a.id;
+                cf++;
+            }
+            resume = end;
+        }
+    }
     int P = __builtin_amdgcn_readfirstlane(__ballot(resume < end) != 0 ? P0 : end);
     while (P < end) {
         const TravNode r = rec(P);
