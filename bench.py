@@ -13,7 +13,7 @@ device-to-device copy inside the timed region).
                     [--no-cpu] [--error-sample S] [--model PATH]
 
 value: particle-steps/s of the learned path (far criterion --learned-fc,
-default 8 as in the reference, learned far-field correction, FP32 Kahan pair
+default 8 as in the reference, learned far-field correction, FP32 compensated pair
 sums) = N * K / (time of K steps).  The plain tree path (the reference's method) is timed at the
 smallest far criterion of --plain-fcs whose convective error reaches 1e-5,
 and at the reference's far criterion 8.  Errors: relative L2 of the induced
@@ -95,6 +95,10 @@ def parse():
     ap.add_argument("--nsim", type=int, default=64)
     ap.add_argument("--graph", action="store_true",
                     help="batched: run the plain convective phase as a captured HIP graph")
+    ap.add_argument("--near-form", type=int, default=-1, choices=(-1, 0, 8, 16, 32),
+                    help="near-field launch form: -1 autotuned (default), 0 one wave per leaf, 8/16/32 persistent "
+                         "waves per compute unit (fixes the form for counter runs, whose timings would steer the "
+                         "autotuner elsewhere)")
     return ap.parse_args()
 
 
@@ -544,6 +548,8 @@ def main():
     ctx.upload_bodies(np.zeros(0, V.SEG_DTYPE), np.zeros(0, V.BODY_DTYPE))
     ctx.upload(objs)
     ctx.snapshot()
+    if a.near_form >= 0:
+        ctx.set_tuning(False, a.near_form)
     B = Bench(V, ctx, n)
     learned = not a.plain_only
     if learned:
@@ -563,7 +569,10 @@ def main():
     acc = B.error(fc, learned, a.error_sample)
     near_ms, far_ms = float(ph_alone[6]), float(ph_alone[7]) if learned else 0.0
     near_tf = acc["near_pairs"] * FLOPS_PER_NEAR_PAIR / (near_ms * 1e-3) / 1e12
-    rl_near = roofline("k_near (convective near field, FP32 VALU)", "fp32_valu", near_tf, FP32_PEAK_TFLOPS, "TFLOP/s",
+    near_form = int(ctx.tuning()["near_form"])
+    near_name = "k_near" if near_form == 0 else f"k_near_persist ({near_form} waves per compute unit)"
+    rl_near = roofline(f"{near_name} (convective near field, the autotuned form, FP32 VALU)", "fp32_valu", near_tf,
+                       FP32_PEAK_TFLOPS, "TFLOP/s",
                        traffic_bytes("k_near"),
                        {"compute_unit": "FP32 VALU (no matrix instructions); peak is the published FP32 figure",
                         "algorithmic_flops": acc["near_pairs"] * FLOPS_PER_NEAR_PAIR, "near_pairs": acc["near_pairs"],
