@@ -563,14 +563,18 @@ def main():
     # The timed step runs kernels concurrently (near field beside the far
     # field, diffusive beside both): a kernel's own rate is measured in a
     # second, untimed pass with the kernels one at a time (same results).
+    # (the near field in the form the timed steps used)
+    timed_form = int(ctx.tuning()["near_form"])
     ctx.set_overlap(False)
+    ctx.set_tuning(False, timed_form)
     _, ph_alone = B.time(fc, learned, ROOFLINE_STEPS, 1)
     ctx.set_overlap(True)
+    if a.near_form < 0:
+        ctx.set_tuning(True)
     acc = B.error(fc, learned, a.error_sample)
     near_ms, far_ms = float(ph_alone[6]), float(ph_alone[7]) if learned else 0.0
     near_tf = acc["near_pairs"] * FLOPS_PER_NEAR_PAIR / (near_ms * 1e-3) / 1e12
-    near_form = int(ctx.tuning()["near_form"])
-    near_name = "k_near" if near_form == 0 else f"k_near_persist ({near_form} waves per compute unit)"
+    near_name = "k_near" if timed_form == 0 else f"k_near_persist ({timed_form} waves per compute unit)"
     rl_near = roofline(f"{near_name} (convective near field, the autotuned form, FP32 VALU)", "fp32_valu", near_tf,
                        FP32_PEAK_TFLOPS, "TFLOP/s",
                        traffic_bytes("k_near"),

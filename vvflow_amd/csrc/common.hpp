@@ -412,7 +412,9 @@ void tree_build(Ctx& c, int far_criteria, double min_node, double max_node, int3
 void tree_lists(Ctx& c);
 void epsilon(Ctx& c, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 void convective(Ctx& c, double inf_x, double inf_y, int kahan);
-void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out);
+// beside: the near field runs beside the far-field kernels (tuned among the
+// persistent forms only)
+void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out, bool beside);
 // Allocation-free convective (vvhd_convective_noalloc): the scratch lives in a
 // fixed device array of kNoallocScratchBytes; noalloc_init looks its address
 // up once per device (the device must be current).

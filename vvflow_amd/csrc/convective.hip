@@ -603,25 +603,31 @@ void launch_near(Ctx& c, int choice, int kahan, hipStream_t s, double2* near_out
 
 // Workload class of the near field: simulations (one or several), leaves and
 // targets per leaf in powers of two.
-uint64_t near_key(const Ctx& c, int kahan) {
+uint64_t near_key(const Ctx& c, int kahan, bool beside) {
     const int64_t nl = std::max<int64_t>(1, c.t.nleaves_h);
     const int lg_nl = 63 - __builtin_clzll((unsigned long long)nl);
     const int lg_per = 63 - __builtin_clzll((unsigned long long)std::max<int64_t>(1, c.n / nl));
-    return ((uint64_t)(c.nsim > 1) << 32) | ((uint64_t)kahan << 24) | ((uint64_t)lg_nl << 8) | (uint64_t)lg_per;
+    return ((uint64_t)beside << 40) | ((uint64_t)(c.nsim > 1) << 32) | ((uint64_t)kahan << 24) |
+           ((uint64_t)lg_nl << 8) | (uint64_t)lg_per;
 }
 
 // The form of the near field for this workload: the one measured fastest the
 // first time the class was seen (each candidate once, HIP events on s), or
-// one wave per leaf with the autotuner off.
-int near_choice(Ctx& c, int kahan, hipStream_t s, double2* near_out) {
+// one wave per leaf with the autotuner off.  With `beside` (the near field
+// shares the GPU with the far-field kernels or the diffusive) only the
+// persistent forms are candidates: the candidates are timed alone, where one
+// wave per leaf may tie with them, but beside other kernels the persistent
+// grid keeps its pace and one wave per leaf falls behind (1.6 against 3.4 ms
+// at 1M).
+int near_choice(Ctx& c, int kahan, hipStream_t s, double2* near_out, bool beside) {
     if (!c.autotune) return c.near_form;
-    const uint64_t key = near_key(c, kahan);
+    const uint64_t key = near_key(c, kahan, beside);
     auto it = c.tune_near.find(key);
     if (it != c.tune_near.end()) return it->second;
     int best = 0;
     float best_ms = 0;
-    const int cands[] = {0, kPersistWaves[0], kPersistWaves[1], kPersistWaves[2]};
-    for (int k = 0; k < 4; k++) {
+    const int cands[] = {kPersistWaves[0], kPersistWaves[1], kPersistWaves[2], 0};
+    for (int k = 0; k < (beside ? 3 : 4); k++) {
         VVHD_HIP(hipEventRecord(c.tune_ev[0], s));
         launch_near(c, cands[k], kahan, s, near_out);
         VVHD_HIP(hipEventRecord(c.tune_ev[1], s));
@@ -646,11 +652,11 @@ void grow_near_items(Ctx& c) {
 
 // Near-field sums of every vortex into near_out (n + 1 entries; the plain and
 // the learned path share it).
-void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out) {
+void convective_near(Ctx& c, int kahan, hipStream_t s, double2* near_out, bool beside) {
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf == 0) return;
     grow_near_items(c);
-    const int choice = near_choice(c, kahan, s, near_out);
+    const int choice = near_choice(c, kahan, s, near_out, beside);
     c.near_last = choice;
     VVHD_HIP(hipEventRecord(c.ev[12], s));
     launch_near(c, choice, kahan, s, near_out);
@@ -722,7 +728,7 @@ static void convective_into(Ctx& c, double inf_x, double inf_y, int kahan, doubl
     const int nl = T.nleaf;
     if (!noalloc) grow_near_items(c);
     if (!noalloc && c.use_graph && !(c.heat.n || c.streak.n)) {
-        const int choice = near_choice(c, kahan, c.stream, near_out);
+        const int choice = near_choice(c, kahan, c.stream, near_out, c.overlap);
         c.near_last = choice;
         if (choice > 0) {
             convective_graph(c, inf_x, inf_y, kahan, choice, taylor, near_out);
@@ -738,7 +744,7 @@ static void convective_into(Ctx& c, double inf_x, double inf_y, int kahan, doubl
         VVHD_HIP(hipEventRecord(c.ev[13], c.stream));
         c.phase_recorded[6] = true;
     } else {
-        convective_near(c, kahan, c.stream, near_out);
+        convective_near(c, kahan, c.stream, near_out, c.overlap);
     }
     hipLaunchKernelGGL(k_conv_finish, dim3(nblocks(c.n)), dim3(kBlock), 0, c.stream, T.vleaf, T.nx, T.ny, T.leaf_node,
                        taylor, c.x.p, c.y.p, c.g.p, near_out, c.vx.p, c.vy.p, c.n, inf_x, inf_y);

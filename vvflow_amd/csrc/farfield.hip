@@ -1390,7 +1390,7 @@ int vvhd_convective_learned(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan
     const hipStream_t ns = c.overlap ? c.stream3 : c.stream;
     if (c.overlap) VVHD_HIP(hipStreamWaitEvent(c.stream3, c.ev_pre_conv, 0));
     c.near_v.grow(c.n + 1);
-    convective_near(c, kahan, ns, c.near_v.p);
+    convective_near(c, kahan, ns, c.near_v.p, c.overlap);
     const TreeDev T = tree_dev(c.t);
     if (T.nleaf > 0) {
         MomView M = moments(c);
