@@ -13,7 +13,8 @@
 //   * plus a correction predicted by a small MLP from the node's moments and
 //     the target offset: 12 inputs, two hidden layers of 64 SiLU units, 2
 //     outputs, bf16 weights and activations with FP32 accumulation on the
-//     matrix cores (v_mfma_f32_32x32x16_bf16).  The network runs per
+//     matrix cores (inference on v_mfma_f32_16x16x32_bf16, training on
+//     v_mfma_f32_32x32x16_bf16).  The network runs per
 //     (leaf, far node) at the leaf centre with forward-mode tangents, and
 //     each target takes the first-order expansion of the correction about
 //     the centre (k_far_learned, k_finish_learned).
@@ -37,7 +38,8 @@
 // "rotated" = multiplied by conj(u), u the unit vector from the node centre
 // to the target.  The correction is (y0 + i y1) * ysd * S d^2/(2 pi r^3) * u.
 //
-// MFMA data flow of one wave (32 pairs / samples per tile): every layer is
+// MFMA data flow of one wave (training: 32 samples per tile on 32x32x16;
+// inference: 16 columns per tile on 16x16x32, the same scheme): every layer is
 // computed transposed, H^T = W X^T, so the hidden units are the M dimension
 // and the samples the N dimension.  The accumulator of one layer (lane holds
 // sample l%32, units 8j + 4(l/32) + i) feeds the next layer's B operand
@@ -482,6 +484,20 @@ __global__ void k_far_nodes(TreeDev T, MomView M, int nnodes, FarNode* out) {
     out[F] = r;
 }
 
+// A zero the compiler cannot see through, renewed at every call.  Adding it
+// to the LDS indices of the weight fragments and biases makes every tile
+// read them from LDS again instead of keeping all 14 fragments and 32 biases
+// in registers across the far-list loop: that frees about 90 VGPRs per lane,
+// and four waves per SIMD instead of two hide the latency of the MFMA ->
+// SiLU -> MFMA chains (2.58 against 2.84 ms at the 1M wake).  The LDS
+// traffic this adds (22 x 1 KiB per tile and wave) is a quarter of the
+// LDS bandwidth.
+__device__ inline int lds_offset_zero() {
+    int o = 0;
+    asm volatile("" : "+v"(o));
+    return o;
+}
+
 // Inference forward of one tile of 16 columns with two forward-mode
 // tangents, on v_mfma_f32_16x16x32_bf16, with the folded weights: the raw
 // features and their tangents tx, ty pass through the same weights, and
@@ -494,12 +510,13 @@ __global__ void k_far_nodes(TreeDev T, MomView M, int nnodes, FarNode* out) {
 __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
                                        bf16x8 x, bf16x8 tx, bf16x8 ty, float (&y)[2], float (&yx)[2],
                                        float (&yy)[2]) {
-    const int ln = lane_id(), g = ln >> 4;
+    const int o = lds_offset_zero();
+    const int ln = lane_id() + o, g = lane_id() >> 4;
     auto W = [&](int set) { return sw[set * kWave + ln]; };
     bf16x8 h[2], hx[2], hy[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c, cx = {}, cy = {};
-        for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i];
+        for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i + o];
         const bf16x8 a = W((fB1 - fB1) / kSet + b);
         c = mfma16(a, x, c);
         cx = mfma16(a, tx, cx);
@@ -519,7 +536,7 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
     bf16x8 h2[2], h2x[2], h2y[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c, cx = {}, cy = {};
-        for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i];
+        for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i + o];
         for (int st = 0; st < 2; st++) {
             const bf16x8 a = W((fB2 - fB1) / kSet + 2 * b + st);
             c = mfma16(a, h[st], c);
@@ -557,12 +574,13 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
 // The value chain alone (no tangents), same tiles and fragments.
 __device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
                                        bf16x8 x, float (&y)[2]) {
-    const int ln = lane_id(), g = ln >> 4;
+    const int o = lds_offset_zero();
+    const int ln = lane_id() + o, g = lane_id() >> 4;
     auto W = [&](int set) { return sw[set * kWave + ln]; };
     bf16x8 h[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c;
-        for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i];
+        for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i + o];
         c = mfma16(W((fB1 - fB1) / kSet + b), x, c);
         for (int i = 0; i < 4; i++)
             h[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(c[i])));
@@ -570,7 +588,7 @@ __device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const
     bf16x8 h2[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c;
-        for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i];
+        for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i + o];
         for (int st = 0; st < 2; st++) c = mfma16(W((fB2 - fB1) / kSet + 2 * b + st), h[st], c);
         for (int i = 0; i < 4; i++)
             h2[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(c[i])));
@@ -604,7 +622,7 @@ __device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const
 // tangents, three chains); otherwise the network output is held fixed over
 // the leaf and the gradient keeps the rotation and 1/r^3 terms (one chain).
 template <bool kJvp>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_far_learned(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_far_learned(
     TreeDev T, const FarNode* __restrict__ nodes, const double* __restrict__ g, const uint16_t* frag, const float* P,
     double* __restrict__ leaf_corr) {
     constexpr int kInfSets = (kFragElems - fB1) / kSet;   // W1, W2, W3 inference sets
@@ -715,17 +733,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             }
         }
     };
-    // tiles of 16 consecutive far-list entries; the next tile's records load
-    // meanwhile, into two record sets used in turn
-    FarNode nd0 = nodes[(fb + n < fe) ? T.far_list[fb + n] : 0];
-    for (int64_t f0 = fb; f0 < fe; f0 += 32) {
-        const int64_t e1 = f0 + 16 + n;
-        const FarNode nd1 = nodes[(e1 < fe) ? T.far_list[e1] : 0];
-        tile(nd0, f0 + n < fe);
-        if (f0 + 16 >= fe) break;
-        const int64_t e2 = f0 + 32 + n;
-        nd0 = nodes[(e2 < fe) ? T.far_list[e2] : 0];
-        tile(nd1, e1 < fe);
+    // tiles of 16 consecutive far-list entries; the next tile's far-list
+    // index loads meanwhile (the records themselves load at the tile start:
+    // a second record set would cost 24 VGPRs, and the other waves of the
+    // SIMD cover the latency).  Past the list end a lane reads entry fb.
+    int idx = T.far_list[(fb + n < fe) ? fb + n : fb];
+    for (int64_t f0 = fb; f0 < fe; f0 += 16) {
+        const int64_t e0 = f0 + n, e1 = f0 + 16 + n;
+        const FarNode nd = nodes[idx];
+        idx = T.far_list[(e1 < fe) ? e1 : fb];
+        tile(nd, e0 < fe);
     }
     // sums over the 16 columns (lanes 0..15), FP64, fixed order
     for (int q = 0; q < 6; q++) {
