@@ -657,6 +657,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, cmp[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // Kahan sums
     float mu8[8];   // the means of this lane's eight inputs
     for (int q = 0; q < 8; q++) mu8[q] = smu[(8 * grp + q) & 15];
+    // outputs of one tile held by one lane group until they are expanded
+    struct Pending {
+        float y0, y1, yx0, yx1, yy0, yy1, ux, uy, rinv, scl;
+        bool ok;
+    } pend{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, false};
+    int tc = 0;   // tiles done (uniform)
+    // the correction of the held tile in the target frame and its gradient,
+    // added to this lane's Kahan sums
+    auto expand = [&]() {
+        if (pend.ok) {
+            const float ux = pend.ux, uy = pend.uy, rinv = pend.rinv;
+            const float thx = -uy * rinv, thy = ux * rinv;
+            const float sc = pend.scl * rinv * rinv * rinv;
+            const float a = pend.y0 * ysd0, b = pend.y1 * ysd1;
+            const float U = (a * ux - b * uy) * sc, V = (a * uy + b * ux) * sc;
+            const float kx = -3.f * ux * rinv, ky = -3.f * uy * rinv;
+            float t[6];
+            t[0] = U;
+            t[1] = V;
+            t[2] = kx * U - thx * V;
+            t[3] = kx * V + thx * U;
+            t[4] = ky * U - thy * V;
+            t[5] = ky * V + thy * U;
+            if constexpr (kJvp) {
+                const float ax = pend.yx0 * ysd0, bx = pend.yx1 * ysd1, ay = pend.yy0 * ysd0, by = pend.yy1 * ysd1;
+                t[2] += (ax * ux - bx * uy) * sc;
+                t[3] += (ax * uy + bx * ux) * sc;
+                t[4] += (ay * ux - by * uy) * sc;
+                t[5] += (ay * uy + by * ux) * sc;
+            }
+            for (int q = 0; q < 6; q++) {
+                const float yk = t[q] - cmp[q], tk = acc[q] + yk;
+                cmp[q] = (tk - acc[q]) - yk;
+                acc[q] = tk;
+            }
+        }
+        pend.ok = false;
+    };
     // one tile: 16 far nodes (column n)
     auto tile = [&](const FarNode& nd, bool valid) {
         const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
@@ -707,31 +745,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             mlp_infer_jvp16(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
         else
             mlp_infer_val16(swf, sb1, sb2, b30, b31, xin, y);
-        if (grp == 0 && ok) {
-            const float sc = nd.scl * rinv * rinv * rinv;
-            const float a = y[0] * ysd0, b = y[1] * ysd1;
-            const float U = (a * ux - b * uy) * sc, V = (a * uy + b * ux) * sc;
-            const float kx = -3.f * ux * rinv, ky = -3.f * uy * rinv;
-            float t[6];
-            t[0] = U;
-            t[1] = V;
-            t[2] = kx * U - thx * V;
-            t[3] = kx * V + thx * U;
-            t[4] = ky * U - thy * V;
-            t[5] = ky * V + thy * U;
-            if constexpr (kJvp) {
-                const float ax = yx[0] * ysd0, bx = yx[1] * ysd1, ay = yy[0] * ysd0, by = yy[1] * ysd1;
-                t[2] += (ax * ux - bx * uy) * sc;
-                t[3] += (ax * uy + bx * ux) * sc;
-                t[4] += (ay * ux - by * uy) * sc;
-                t[5] += (ay * uy + by * ux) * sc;
-            }
-            for (int q = 0; q < 6; q++) {
-                const float yk = t[q] - cmp[q], tk = acc[q] + yk;
-                cmp[q] = (tk - acc[q]) - yk;
-                acc[q] = tk;
-            }
+        // lane group tc % 4 keeps this tile's outputs (from lanes 0..15) and
+        // geometry; every fourth tile all 64 lanes expand four tiles at once
+        const int k = tc & 3;
+        const float o0 = __shfl(y[0], n, kWave), o1 = __shfl(y[1], n, kWave);
+        float ox0 = 0.f, ox1 = 0.f, oy0 = 0.f, oy1 = 0.f;
+        if constexpr (kJvp) {
+            ox0 = __shfl(yx[0], n, kWave);
+            ox1 = __shfl(yx[1], n, kWave);
+            oy0 = __shfl(yy[0], n, kWave);
+            oy1 = __shfl(yy[1], n, kWave);
         }
+        if (grp == k) {
+            pend.y0 = o0;
+            pend.y1 = o1;
+            pend.yx0 = ox0;
+            pend.yx1 = ox1;
+            pend.yy0 = oy0;
+            pend.yy1 = oy1;
+            pend.ux = ux;
+            pend.uy = uy;
+            pend.rinv = rinv;
+            pend.scl = nd.scl;
+            pend.ok = valid && nd.ok != 0.f;
+        }
+        tc++;
+        if (k == 3) expand();
     };
     // tiles of 16 consecutive far-list entries; the next tile's far-list
     // index loads meanwhile (the records themselves load at the tile start:
@@ -744,10 +783,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         idx = T.far_list[(e1 < fe) ? e1 : fb];
         tile(nd, e0 < fe);
     }
-    // sums over the 16 columns (lanes 0..15), FP64, fixed order
+    expand();   // tiles still held (fewer than four since the last expansion)
+    // sums over the 64 lanes (four tiles of 16 columns each), FP64, fixed order
     for (int q = 0; q < 6; q++) {
-        double v = grp == 0 ? (double)acc[q] - (double)cmp[q] : 0.;
-        for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        double v = (double)acc[q] - (double)cmp[q];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
         if (lane == 0) out[q] = v;
     }
 }
