@@ -141,6 +141,23 @@ def feature_tangents(X, ux, uy, rinv):
     return TX, TY
 
 
+def feature_directions(X):
+    """The two input directions the device propagates tangents along:
+    d1, the rotated moments' derivative in theta (rot of feature_tangents),
+    and d2, -d/r at input 10.  The feature tangents along x and y are
+    thx d1 + (ux/r) d2 and thy d1 + (uy/r) d2 (thx, thy = -uy/r, ux/r), so
+    by linearity the network's tangents along x and y are the same
+    combinations of its tangents along d1 and d2."""
+    X = np.asarray(X, dtype=np.float64)
+    D1 = np.zeros_like(X)
+    D1[:, 2], D1[:, 3] = X[:, 3], -X[:, 2]
+    D1[:, 4], D1[:, 5] = X[:, 5], -X[:, 4]
+    D1[:, 6], D1[:, 7] = 2 * X[:, 7], -2 * X[:, 6]
+    D2 = np.zeros_like(X)
+    D2[:, 10] = -X[:, 10]
+    return D1, D2
+
+
 LOG2E_F32 = np.float32(1.44269504088896341)
 LN2_F32 = np.float32(0.693147180559945309)
 
@@ -196,12 +213,28 @@ def forward_jvp(p, X, TX, TY, mirror=True):
     return h2 @ W3.T + b3, t2x @ W3.T, t2y @ W3.T
 
 
+def correction_jvp_dirs(p, X, D1, D2, scale, ux, uy, rinv, mirror=True):
+    """correction_jvp with the network's tangents taken along the directions
+    D1, D2 of feature_directions (as the device runs them) and combined into
+    the tangents along x and y after the network."""
+    y, j1, j2 = forward_jvp(p, X, D1, D2, mirror)
+    thx, thy = -uy * rinv, ux * rinv
+    yx = thx[:, None] * j1 + (ux * rinv)[:, None] * j2
+    yy = thy[:, None] * j1 + (uy * rinv)[:, None] * j2
+    return _expand(p, y, yx, yy, scale, ux, uy, rinv)
+
+
 def correction_jvp(p, X, TX, TY, scale, ux, uy, rinv, mirror=True):
     """Correction (u, v) of pairs at their target points and its gradient
     with respect to the target position: columns u, v, du/dx, dv/dx, du/dy,
     dv/dy (network tangents, rotation back to the target frame and the 1/r^3
     of the scale)."""
     y, yx, yy = forward_jvp(p, X, TX, TY, mirror)
+    return _expand(p, y, yx, yy, scale, ux, uy, rinv)
+
+
+def _expand(p, y, yx, yy, scale, ux, uy, rinv):
+    """Outputs and their tangents along x and y -> u, v and their gradient."""
     ysd = unpack(p)[8].astype(np.float64)
     a, b = y[:, 0] * ysd[0], y[:, 1] * ysd[1]
     U, V = (a * ux - b * uy) * scale, (a * uy + b * ux) * scale

@@ -260,6 +260,10 @@ def test_correction_gradient_finite_differences():
         fd = (cp - cm) / (2 * h[:, None])
         ref = np.abs(C[:, 2 + 2 * k:4 + 2 * k]).max()
         assert np.max(np.abs(fd - C[:, 2 + 2 * k:4 + 2 * k])) < 1e-5 * ref
+    # the device's form: tangents along the two fixed directions, combined
+    # into x and y after the network, give the same gradient
+    Cd = M.correction_jvp_dirs(p, X, *M.feature_directions(X), sc, u[:, 0], u[:, 1], rinv, mirror=False)
+    assert np.max(np.abs(Cd - C)) < 1e-10 * np.abs(C).max()
 
 
 def _point(o, P, nd):

@@ -173,10 +173,10 @@ def _oracle_learned(o, targets, p=None, network_gradient=True):
             ok = sc > 0
             D = np.array([cx, cy]) - geom[far, :2]
             rinv = 1.0 / np.hypot(D[:, 0], D[:, 1])
-            TX, TY = M.feature_tangents(X, u[:, 0], u[:, 1], rinv)
+            D1, D2 = M.feature_directions(X)
             if not network_gradient:
-                TX, TY = np.zeros_like(TX), np.zeros_like(TY)
-            C = M.correction_jvp(p, X[ok], TX[ok], TY[ok], sc[ok], u[ok, 0], u[ok, 1], rinv[ok]).sum(0)
+                D1, D2 = np.zeros_like(D1), np.zeros_like(D2)
+            C = M.correction_jvp_dirs(p, X[ok], D1[ok], D2[ok], sc[ok], u[ok, 0], u[ok, 1], rinv[ok]).sum(0)
             out[q] += C[:2] + (v[t, 0] - cx) * C[2:4] + (v[t, 1] - cy) * C[4:6]
     return out
 

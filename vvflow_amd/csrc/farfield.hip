@@ -612,7 +612,12 @@ __device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const
 // a leaf, which lies far from the node.
 // Feature derivatives: with theta the angle of u, d theta / dp = (-uy, ux)/r;
 // the rotated moments turn with theta (d(f2 + i f3)/d theta = f3 - i f2, the
-// quadrupole twice as fast), d/r scales as 1/r, the rest is constant.
+// quadrupole twice as fast), d/r scales as 1/r, the rest is constant.  So
+// the feature tangents along x and y are thx d1 + (ux/r) d2 and
+// thy d1 + (uy/r) d2 with two directions fixed per node: d1, the rotated
+// moments' derivative in theta, and d2, -d/r at input 10.  The network's
+// tangents run along d1 and d2 (no products per tile), and the expansion
+// combines them.
 // A tile holds 16 far nodes (column l%16); lane groups 0 and 1 (lanes
 // 16g..16g+15) build inputs 8g..8g+7 and their tangents, groups 2 and 3 the
 // zero padding of the K step.  Output per leaf: 6 doubles
@@ -681,7 +686,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             t[4] = ky * U - thy * V;
             t[5] = ky * V + thy * U;
             if constexpr (kJvp) {
-                const float ax = pend.yx0 * ysd0, bx = pend.yx1 * ysd1, ay = pend.yy0 * ysd0, by = pend.yy1 * ysd1;
+                // the feature tangents along x and y are thx d1 + (ux / r) d2 and
+                // thy d1 + (uy / r) d2, so the output's are the same
+                // combinations of its tangents along d1 and d2
+                const float urx = ux * rinv, ury = uy * rinv;
+                const float gx0 = __builtin_fmaf(thx, pend.yx0, urx * pend.yy0);
+                const float gx1 = __builtin_fmaf(thx, pend.yx1, urx * pend.yy1);
+                const float gy0 = __builtin_fmaf(thy, pend.yx0, ury * pend.yy0);
+                const float gy1 = __builtin_fmaf(thy, pend.yx1, ury * pend.yy1);
+                const float ax = gx0 * ysd0, bx = gx1 * ysd1, ay = gy0 * ysd0, by = gy1 * ysd1;
                 t[2] += (ax * ux - bx * uy) * sc;
                 t[3] += (ax * uy + bx * ux) * sc;
                 t[4] += (ay * ux - by * uy) * sc;
@@ -700,7 +713,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const float dx = (float)(px - nd.cx), dy = (float)(py - nd.cy);
         const float rinv = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
         const float ux = dx * rinv, uy = dy * rinv;
-        const float thx = -uy * rinv, thy = ux * rinv;   // d theta / dp
         const bool ok = valid && nd.ok != 0.f && grp < 2;
         float f[8], fx[8], fy[8];
         if (grp == 0) {
@@ -714,10 +726,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             f[6] = nd.qqx * c2r - nd.qqy * c2i;
             f[7] = nd.qqx * c2i + nd.qqy * c2r;
             if constexpr (kJvp) {
-                const float rot[8] = {0.f, 0.f, f[3], -f[2], f[5], -f[4], 2.f * f[7], -2.f * f[6]};
+                // direction 1: the rotated moments' derivative in theta
+                const float rot[8] = {0.f, 0.f, f[3], -f[2], f[5], -f[4], f[7] + f[7], -(f[6] + f[6])};
                 for (int q = 0; q < 8; q++) {
-                    fx[q] = rot[q] * thx;
-                    fy[q] = rot[q] * thy;
+                    fx[q] = rot[q];
+                    fy[q] = 0.f;
                 }
             }
         } else {
@@ -727,9 +740,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             f[3] = nd.asp;
             f[4] = f[5] = f[6] = f[7] = 0.f;
             if constexpr (kJvp) {
+                // direction 2: d/r's derivative in r, times r
                 for (int q = 0; q < 8; q++) fx[q] = fy[q] = 0.f;
-                fx[2] = -f[2] * (ux * rinv);
-                fy[2] = -f[2] * (uy * rinv);
+                fy[2] = -f[2];
             }
         }
         // centred features; the 1/sd scale is in the folded layer-1 weights
