@@ -7,9 +7,11 @@ CPU-baseline leg of bench.py may import it.
 The network and its flat parameter vector (include/vvhd_gpu.h):
     W1[64][12] b1[64] W2[64][64] b2[64] W3[2][64] b3[2] | mu[12] sd[12] ysd[2]
     y = W3 silu(W2 silu(W1 xs + b1) + b2) + b3,   xs = (x - mu) / sd.
-With mirror=True the bf16 rounding points of the device are reproduced
-(round to nearest even): standardised inputs, weights, hidden activations
-fed to the next product, and the deltas fed to the backward products.  With
+With mirror=True the rounding points of the device are reproduced (round
+to nearest even): in training (bf16) the standardised inputs, weights,
+hidden activations fed to the next product and the deltas fed to the
+backward products; in inference (forward_jvp) bf16 inputs and layer-1
+weights, fp16 hidden values, tangents and layer-2/3 weights.  With
 mirror=False everything is float64 (for finite-difference checks).
 This network has no counterpart in the reference, so no reference line is
 cited; the checks that pin it are the finite-difference gradient test and the
@@ -162,11 +164,16 @@ LOG2E_F32 = np.float32(1.44269504088896341)
 LN2_F32 = np.float32(0.693147180559945309)
 
 
+def f16(a):
+    """Round to fp16 (nearest even), returned as float64."""
+    return np.asarray(a, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
 def folded_inference_params(p):
     """The device's inference weights (farfield.hip, k_pack_model and
     k_far_learned): with L = log2 e, W1' = bf16((-L W1) / sd) applied to the
-    centred features x - mu, b1' = -L b1, W2' = bf16(W2), b2' = -L b2,
-    W3' = bf16(-W3 / L), b3; so layer 1 gives -L z, 2^(-L z) = e^-z and the
+    centred features x - mu, b1' = -L b1, W2' = fp16(W2), b2' = -L b2,
+    W3' = fp16(-W3 / L), b3; so layer 1 gives -L z, 2^(-L z) = e^-z and the
     hidden values carry the factor -L.  FP32 operations where the device
     uses FP32."""
     W1, b1, W2, b2, W3, b3, mu, sd, _ = unpack(np.asarray(p, dtype=np.float32))
@@ -174,28 +181,33 @@ def folded_inference_params(p):
     W1f = bf16(((-LOG2E_F32) * W1).astype(np.float32) * isd[None, :]).astype(np.float64)
     b1f = ((-LOG2E_F32) * b1).astype(np.float32).astype(np.float64)
     b2f = ((-LOG2E_F32) * b2).astype(np.float32).astype(np.float64)
-    W3f = bf16((W3 * (-LN2_F32)).astype(np.float32)).astype(np.float64)
-    return W1f, b1f, bf16(W2).astype(np.float64), b2f, W3f, b3.astype(np.float64)
+    W3f = f16((W3 * (-LN2_F32)).astype(np.float32))
+    return W1f, b1f, f16(W2), b2f, W3f, b3.astype(np.float64)
 
 
 def forward_jvp(p, X, TX, TY, mirror=True):
     """Raw outputs and their forward-mode tangents along TX and TY (inputs
     before standardisation).  mirror=True follows the device's inference
     path: the folded weights of folded_inference_params, bf16 centred
-    features, bf16 raw tangents, and bf16 hidden values and tangents (which
-    carry the factor -L); mirror=False is the plain network in float64."""
+    features and bf16 raw tangents into layer 1, and fp16 hidden values and
+    tangents (which carry the factor -L): per unit the matrix output z' and
+    the sigmoid s = 1 / (1 + 2^z') (FP32) rounded to fp16, then
+    h = fp16(z' s), d = fp16(h fp16(s ln2 - ln2) + s) (fused) and the
+    tangents fp16(d fp16(t)), as the device's packed fp16 instructions do;
+    mirror=False is the plain network in float64."""
     if mirror:
         r = lambda a: bf16(np.asarray(a, dtype=np.float32)).astype(np.float64)
         W1f, b1f, W2f, b2f, W3f, b3 = folded_inference_params(p)
         mu = unpack(np.asarray(p, dtype=np.float32))[6]
         xs, txs, tys = r((np.asarray(X, np.float32) - mu).astype(np.float32)), r(TX), r(TY)
-        ln2 = np.float64(LN2_F32)
+        ln2 = f16(np.float32(LN2_F32))
 
         def layer(zp, tx, ty):
-            s = 1.0 / (1.0 + np.exp2(zp))
-            h = zp * s
-            d = s + h * (s * ln2 - ln2)
-            return r(h), r(d * tx), r(d * ty)
+            z32 = np.asarray(zp, dtype=np.float32)
+            s = f16((np.float32(1.0) / (np.float32(1.0) + np.exp2(z32))).astype(np.float32))
+            h = f16(f16(z32) * s)
+            d = f16(h * f16(s * ln2 - ln2) + s)
+            return h, f16(d * f16(np.asarray(tx, np.float32))), f16(d * f16(np.asarray(ty, np.float32)))
 
         h1, t1x, t1y = layer(xs @ W1f.T + b1f, txs @ W1f.T, tys @ W1f.T)
         h2, t2x, t2y = layer(h1 @ W2f.T + b2f, t1x @ W2f.T, t1y @ W2f.T)

@@ -113,6 +113,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 // Eight floats to bf16 (round to nearest even), converted in pairs, and the
 // whole vector zeroed when keep is false (one select per packed word).
@@ -130,6 +132,9 @@ inline int nblocks(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1,
 
 __device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline f32x4 mfma16h(f16x8 a, f16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // 16x16x32 layout: A row l%16, K 8(l/16)+e; B column l%16, K 8(l/16)+e;
@@ -406,7 +411,8 @@ __global__ void k_pack_model(const float* P, uint16_t* frag) {
         } else {   // W3' rows 0, 1, K step set
             if (m16 < kOut) v = P[oW3 + m16 * kHid + chain16(set, g16, e)] * -kLn2;
         }
-        frag[t] = bf16_bits(v);
+        // layer 1 takes bf16 features; layers 2 and 3 take the fp16 hidden values
+        frag[t] = t < fB2 ? bf16_bits(v) : __builtin_bit_cast(uint16_t, (_Float16)v);
         return;
     }
     if (t < fA2) {   // W1: A[m = unit][k = feature]
@@ -499,69 +505,84 @@ __device__ inline int lds_offset_zero() {
 }
 
 // Inference forward of one tile of 16 columns with two forward-mode
-// tangents, on v_mfma_f32_16x16x32_bf16, with the folded weights: the raw
-// features and their tangents tx, ty pass through the same weights, and
+// tangents, on the matrix cores, with the folded weights: the bf16 features
+// and their tangents tx, ty pass through W1 (v_mfma_f32_16x16x32_bf16);
 // each SiLU scales the tangents by its derivative s (1 + z (1 - s)),
-// s = sigmoid(z);
-// tangents are rounded to bf16 where the activations are.  A layer's four
-// output blocks (16 units each) feed the next layer's two K steps without a
-// transpose (chain16).  Outputs (value, x tangent, y tangent) in lanes 0..15.
+// s = sigmoid(z).  The hidden values and tangents are fp16 and feed layers
+// 2 and 3 (v_mfma_f32_16x16x32_f16, fp16 weights): an activation pair takes
+// its exponentials and reciprocals in FP32, then the product, the
+// derivative and the tangent products as packed fp16 pairs, whose results
+// are already the next layer's operand.  A layer's four output blocks (16
+// units each) feed the next layer's two K steps without a transpose
+// (chain16).  Outputs (value, x tangent, y tangent) in lanes 0..15.
 // The weight fragments come from LDS (sw: W1, W2, W3 sets, 64 lanes each).
-__device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
+//
+// The SiLU of z' = -L z: sigmoid(z) = 1 / (1 + 2^z'), h' = z' s = -L h, and
+// the derivative s + h (1 - s) = s + h' (s ln2 - ln2).
+__device__ inline void silu_pair(float z0, float z1, f16x2& H, f16x2& D) {
+    const float s0 = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z0));
+    const float s1 = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z1));
+    const f16x2 Z = __builtin_convertvector((f32x2){z0, z1}, f16x2);
+    const f16x2 S = __builtin_convertvector((f32x2){s0, s1}, f16x2);
+    const f16x2 ln2 = {(_Float16)kLn2, (_Float16)kLn2};
+    H = Z * S;
+    D = __builtin_elementwise_fma(H, __builtin_elementwise_fma(S, ln2, -ln2), S);
+}
+__device__ inline f16x2 tangent_pair(f16x2 D, float c0, float c1) {
+    return D * __builtin_convertvector((f32x2){c0, c1}, f16x2);
+}
+
+__device__ inline void mlp_infer_jvp16(const uint4* sw, const float* sb1, const float* sb2, float b30, float b31,
                                        bf16x8 x, bf16x8 tx, bf16x8 ty, float (&y)[2], float (&yx)[2],
                                        float (&yy)[2]) {
     const int o = lds_offset_zero();
     const int ln = lane_id() + o, g = lane_id() >> 4;
-    auto W = [&](int set) { return sw[set * kWave + ln]; };
-    bf16x8 h[2], hx[2], hy[2];
+    auto Wb = [&](int set) { return __builtin_bit_cast(bf16x8, sw[set * kWave + ln]); };
+    auto Wh = [&](int set) { return __builtin_bit_cast(f16x8, sw[set * kWave + ln]); };
+    u32x4 h[2], hx[2], hy[2];   // fp16 pairs: word q of K step st holds elements 2q, 2q + 1
     for (int b = 0; b < 4; b++) {
         f32x4 c, cx = {}, cy = {};
         for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i + o];
-        const bf16x8 a = W((fB1 - fB1) / kSet + b);
+        const bf16x8 a = Wb((fB1 - fB1) / kSet + b);
         c = mfma16(a, x, c);
         cx = mfma16(a, tx, cx);
         cy = mfma16(a, ty, cy);
-        for (int i = 0; i < 4; i++) {
-            // SiLU and its derivative s (1 + z (1 - s)) = s + h (1 - s), h = z s
-            // z' = -L z: sigmoid(z) = 1 / (1 + 2^z'); h' = -L h; the derivative
-            // s + h (1 - s) = s + h' (1 - s) (-1/L)
-            const float zp = c[i], sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(zp)), hv = zp * sg;
-            const float d = __builtin_fmaf(hv, __builtin_fmaf(sg, kLn2, -kLn2), sg);
-            const int st = b >> 1, e = 4 * (b & 1) + i;
-            h[st][e] = (__bf16)hv;
-            hx[st][e] = (__bf16)(d * cx[i]);
-            hy[st][e] = (__bf16)(d * cy[i]);
+        for (int i = 0; i < 4; i += 2) {
+            f16x2 H, D;
+            silu_pair(c[i], c[i + 1], H, D);
+            const int st = b >> 1, q = 2 * (b & 1) + i / 2;
+            h[st][q] = __builtin_bit_cast(uint32_t, H);
+            hx[st][q] = __builtin_bit_cast(uint32_t, tangent_pair(D, cx[i], cx[i + 1]));
+            hy[st][q] = __builtin_bit_cast(uint32_t, tangent_pair(D, cy[i], cy[i + 1]));
         }
     }
-    bf16x8 h2[2], h2x[2], h2y[2];
+    u32x4 h2[2], h2x[2], h2y[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c, cx = {}, cy = {};
         for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i + o];
         for (int st = 0; st < 2; st++) {
-            const bf16x8 a = W((fB2 - fB1) / kSet + 2 * b + st);
-            c = mfma16(a, h[st], c);
-            cx = mfma16(a, hx[st], cx);
-            cy = mfma16(a, hy[st], cy);
+            const f16x8 a = Wh((fB2 - fB1) / kSet + 2 * b + st);
+            c = mfma16h(a, __builtin_bit_cast(f16x8, h[st]), c);
+            cx = mfma16h(a, __builtin_bit_cast(f16x8, hx[st]), cx);
+            cy = mfma16h(a, __builtin_bit_cast(f16x8, hy[st]), cy);
         }
-        for (int i = 0; i < 4; i++) {
-            // z' = -L z: sigmoid(z) = 1 / (1 + 2^z'); h' = -L h; the derivative
-            // s + h (1 - s) = s + h' (1 - s) (-1/L)
-            const float zp = c[i], sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(zp)), hv = zp * sg;
-            const float d = __builtin_fmaf(hv, __builtin_fmaf(sg, kLn2, -kLn2), sg);
-            const int st = b >> 1, e = 4 * (b & 1) + i;
-            h2[st][e] = (__bf16)hv;
-            h2x[st][e] = (__bf16)(d * cx[i]);
-            h2y[st][e] = (__bf16)(d * cy[i]);
+        for (int i = 0; i < 4; i += 2) {
+            f16x2 H, D;
+            silu_pair(c[i], c[i + 1], H, D);
+            const int st = b >> 1, q = 2 * (b & 1) + i / 2;
+            h2[st][q] = __builtin_bit_cast(uint32_t, H);
+            h2x[st][q] = __builtin_bit_cast(uint32_t, tangent_pair(D, cx[i], cx[i + 1]));
+            h2y[st][q] = __builtin_bit_cast(uint32_t, tangent_pair(D, cy[i], cy[i + 1]));
         }
     }
     f32x4 c = {}, cx = {}, cy = {};
     c[0] = g == 0 ? b30 : 0.f;
     c[1] = g == 0 ? b31 : 0.f;
     for (int st = 0; st < 2; st++) {
-        const bf16x8 a = W((fB3 - fB1) / kSet + st);
-        c = mfma16(a, h2[st], c);
-        cx = mfma16(a, h2x[st], cx);
-        cy = mfma16(a, h2y[st], cy);
+        const f16x8 a = Wh((fB3 - fB1) / kSet + st);
+        c = mfma16h(a, __builtin_bit_cast(f16x8, h2[st]), c);
+        cx = mfma16h(a, __builtin_bit_cast(f16x8, h2x[st]), cx);
+        cy = mfma16h(a, __builtin_bit_cast(f16x8, h2y[st]), cy);
     }
     y[0] = c[0];
     y[1] = c[1];
@@ -571,32 +592,40 @@ __device__ inline void mlp_infer_jvp16(const bf16x8* sw, const float* sb1, const
     yy[1] = cy[1];
 }
 
-// The value chain alone (no tangents), same tiles and fragments.
-__device__ inline void mlp_infer_val16(const bf16x8* sw, const float* sb1, const float* sb2, float b30, float b31,
+// The value chain alone (no tangents), same tiles, fragments and rounding.
+__device__ inline void mlp_infer_val16(const uint4* sw, const float* sb1, const float* sb2, float b30, float b31,
                                        bf16x8 x, float (&y)[2]) {
     const int o = lds_offset_zero();
     const int ln = lane_id() + o, g = lane_id() >> 4;
-    auto W = [&](int set) { return sw[set * kWave + ln]; };
-    bf16x8 h[2];
+    auto Wb = [&](int set) { return __builtin_bit_cast(bf16x8, sw[set * kWave + ln]); };
+    auto Wh = [&](int set) { return __builtin_bit_cast(f16x8, sw[set * kWave + ln]); };
+    u32x4 h[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c;
         for (int i = 0; i < 4; i++) c[i] = sb1[16 * b + 4 * g + i + o];
-        c = mfma16(W((fB1 - fB1) / kSet + b), x, c);
-        for (int i = 0; i < 4; i++)
-            h[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(c[i])));
+        c = mfma16(Wb((fB1 - fB1) / kSet + b), x, c);
+        for (int i = 0; i < 4; i += 2) {
+            f16x2 H, D;
+            silu_pair(c[i], c[i + 1], H, D);
+            h[b >> 1][2 * (b & 1) + i / 2] = __builtin_bit_cast(uint32_t, H);
+        }
     }
-    bf16x8 h2[2];
+    u32x4 h2[2];
     for (int b = 0; b < 4; b++) {
         f32x4 c;
         for (int i = 0; i < 4; i++) c[i] = sb2[16 * b + 4 * g + i + o];
-        for (int st = 0; st < 2; st++) c = mfma16(W((fB2 - fB1) / kSet + 2 * b + st), h[st], c);
-        for (int i = 0; i < 4; i++)
-            h2[b >> 1][4 * (b & 1) + i] = (__bf16)(c[i] * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(c[i])));
+        for (int st = 0; st < 2; st++)
+            c = mfma16h(Wh((fB2 - fB1) / kSet + 2 * b + st), __builtin_bit_cast(f16x8, h[st]), c);
+        for (int i = 0; i < 4; i += 2) {
+            f16x2 H, D;
+            silu_pair(c[i], c[i + 1], H, D);
+            h2[b >> 1][2 * (b & 1) + i / 2] = __builtin_bit_cast(uint32_t, H);
+        }
     }
     f32x4 c = {};
     c[0] = g == 0 ? b30 : 0.f;
     c[1] = g == 0 ? b31 : 0.f;
-    for (int st = 0; st < 2; st++) c = mfma16(W((fB3 - fB1) / kSet + st), h2[st], c);
+    for (int st = 0; st < 2; st++) c = mfma16h(Wh((fB3 - fB1) / kSet + st), __builtin_bit_cast(f16x8, h2[st]), c);
     y[0] = c[0];
     y[1] = c[1];
 }
@@ -655,7 +684,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (lane < 6) out[lane] = 0.;
         return;
     }
-    const bf16x8* swf = reinterpret_cast<const bf16x8*>(sw);
     const float b30 = P[oB3], b31 = P[oB3 + 1];
     const float ysd0 = P[oYsd], ysd1 = P[oYsd + 1];
     const double px = T.nx[L], py = T.ny[L];
@@ -755,9 +783,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         float y[2], yx[2] = {0.f, 0.f}, yy[2] = {0.f, 0.f};
         if constexpr (kJvp)
-            mlp_infer_jvp16(swf, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
+            mlp_infer_jvp16(sw, sb1, sb2, b30, b31, xin, txin, tyin, y, yx, yy);
         else
-            mlp_infer_val16(swf, sb1, sb2, b30, b31, xin, y);
+            mlp_infer_val16(sw, sb1, sb2, b30, b31, xin, y);
         // lane group tc % 4 keeps this tile's outputs (from lanes 0..15) and
         // geometry; every fourth tile all 64 lanes expand four tiles at once
         const int k = tc & 3;
