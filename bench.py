@@ -21,9 +21,10 @@ convective velocity against the FP64 direct sum on --error-sample targets.
 roofline: the kernel with the largest device time in the learned step --
 the near-field pair kernel (11 FP32 flops per near pair,
 MConvectiveFast.cpp:116-120, against the FP32 vector peak 157.3 TFLOP/s) or
-the learned far-field network kernel (9984 bf16 matrix flops per network
-pass, three passes -- value and two forward-mode tangents -- per (leaf, far
-node) term, against the dense bf16 peak 2.5 PFLOP/s) -- measured with HIP events on
+the learned far-field network kernel (9984 matrix flops per network pass,
+layer 1 in bf16 and layers 2-3 in fp16, three passes -- value and two
+forward-mode tangents -- per (leaf, far node) term, against the dense
+bf16/fp16 peak 2.5 PFLOP/s) -- measured with HIP events on
 the library's stream, in an untimed pass with the step's kernels run one at a
 time (the timed step overlaps the near field, the far field and the
 diffusive on separate streams; phases_ms then overlap too).
@@ -585,9 +586,9 @@ def main():
     rls = [rl_near]
     if learned:
         far_tf = acc["net_passes"] * FLOPS_PER_NET_PASS / (far_ms * 1e-3) / 1e12
-        rls.append(roofline("k_far_learned (learned far-field network, bf16 MFMA)", "mfma", far_tf, BF16_PEAK_TFLOPS,
+        rls.append(roofline("k_far_learned (learned far-field network, bf16/fp16 MFMA)", "mfma", far_tf, BF16_PEAK_TFLOPS,
                             "TFLOP/s", traffic_bytes("k_far_learned"),
-                            {"compute_unit": "bf16 matrix cores, dense peak",
+                            {"compute_unit": "bf16/fp16 matrix cores, dense peak (the same for both)",
                              "algorithmic_flops": acc["net_passes"] * FLOPS_PER_NET_PASS,
                              "network_passes": acc["net_passes"], "far_pairs": acc["far_pairs"],
                              "kernel_ms": far_ms, "kernel_ms_in_timed_step": float(ph[7]),
@@ -681,7 +682,7 @@ def main():
             "workload": f"karman-street wake, {n} vortices, one step: tree + epsilon (merge) + convective + "
                         f"diffusive, {'learned far field' if learned else 'plain tree path'} at far criterion {fc}",
             "n": n, "far_criteria": fc, "re": 600.0, "accumulation": "kahan", "pair_dtype": "fp32",
-            "tree_epsilon_dtype": "fp64", "far_field": "learned (bf16 MLP)" if learned else "plain",
+            "tree_epsilon_dtype": "fp64", "far_field": "learned (bf16/fp16 MLP)" if learned else "plain",
             "far_gradient": a.far_gradient if learned else None,
         },
         "phases_ms": {"tree": float(ph[0]), "epsilon": float(ph[1]), "convective": float(ph[2]),

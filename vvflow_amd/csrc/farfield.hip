@@ -12,9 +12,10 @@
 //     :20-34, :139-151);
 //   * plus a correction predicted by a small MLP from the node's moments and
 //     the target offset: 12 inputs, two hidden layers of 64 SiLU units, 2
-//     outputs, bf16 weights and activations with FP32 accumulation on the
-//     matrix cores (inference on v_mfma_f32_16x16x32_bf16, training on
-//     v_mfma_f32_32x32x16_bf16).  The network runs per
+//     outputs, with FP32 accumulation on the matrix cores (training: bf16
+//     weights and activations on v_mfma_f32_32x32x16_bf16; inference: bf16
+//     features and layer-1 weights on v_mfma_f32_16x16x32_bf16, fp16 hidden
+//     values and layer-2/3 weights on v_mfma_f32_16x16x32_f16).  The network runs per
 //     (leaf, far node) at the leaf centre with forward-mode tangents, and
 //     each target takes the first-order expansion of the correction about
 //     the centre (k_far_learned, k_finish_learned).
