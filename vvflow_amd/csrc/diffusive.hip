@@ -6,7 +6,7 @@
 // |r_i - r_j| * (1/eps_i) <= 8 (the reference's exp-argument cutoff), so near
 // leaves whose box lies farther than 8 eps from the whole chunk are skipped
 // without loss; the sources of the others are staged in LDS and shared by
-// all lanes.  Vortex sums (S1, S2) are FP32 with Kahan compensation in
+// all lanes.  Vortex sums (S1, S2) are FP32, block-compensated (BlockSum), in
 // leaf-local coordinates; body-segment terms (S0, S3) are FP64.  The clamp
 // and the velocity update are FP64.
 //
@@ -40,15 +40,18 @@ constexpr double kExpArgLimit = -8.;   // MDiffusiveFast.cpp:6
 
 __device__ inline double sgn(double x) { return (x > 0) ? 1 : ((x < 0) ? -1 : 0); }
 
-struct Kahan {
-    float s = 0.f, c = 0.f;
-    __device__ inline void add(float v) {
-        const float yv = v - c;
-        const float t = s + yv;
-        c = (t - s) - yv;
+// Block-compensated sum: terms go by FMA (or an add) into the low-order
+// part q, which a compensated (Fast2Sum) step folds into the running sum s
+// every few terms; q keeps that step's rounding error, so every rounding is
+// relative to a few terms, never to the running sum.
+struct BlockSum {
+    float s = 0.f, q = 0.f;
+    __device__ inline void fold() {
+        const float t = s + q;
+        q = q - (t - s);
         s = t;
     }
-    __device__ inline double value() const { return (double)s - (double)c; }
+    __device__ inline double value() const { return (double)s + (double)q; }
 };
 
 // Sum over the slots of target t (lanes t, t + ntc, ..., in slot order).
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
         double R = gi != 0 ? (-kExpArgLimit) / iei : 0.0;
         R = wave_max(R);
         const double R2 = R * R * (1.0 + 1e-6);
-        Kahan s2x, s2y, s1;
+        BlockSum s2x, s2y, s1;
         // staged sources: positive ones from the front of the buffer, negative
         // ones from the back, each in staging order; a target runs over its
         // own sign's part only
@@ -120,7 +123,14 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (active) {
                 const int k0 = pos ? 0 : kStage - nneg, k1 = pos ? npos : kStage;
+                int nk = 0;
                 for (int k = k0 + slot; k < k1; k += nslot) {
+                    // the partial sums fold every eight sources of this lane
+                    if ((++nk & 7) == 0) {
+                        s2x.fold();
+                        s2y.fold();
+                        s1.fold();
+                    }
                     const float gj = sg[k];
                     const float dx = tx - sx[k];
                     const float dy = ty - sy[k];
@@ -131,9 +141,9 @@ __global__ __launch_bounds__(kBlock) void k_diffusive(TreeDev T, const double* _
                     if (exparg < (float)kExpArgLimit) continue;
                     const float i1 = gj * __expf(exparg);
                     const float f = i1 * rinv;
-                    s2x.add(dx * f);
-                    s2y.add(dy * f);
-                    s1.add(i1);
+                    s2x.q = __builtin_fmaf(dx, f, s2x.q);
+                    s2y.q = __builtin_fmaf(dy, f, s2y.q);
+                    s1.q += i1;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
