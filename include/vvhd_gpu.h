@@ -208,8 +208,9 @@ int vvhd_tree_nodes(vvhd_ctx* ctx, int32_t* nnodes, double* geom, int32_t* ints,
 int vvhd_epsilon(vvhd_ctx* ctx, int merge, int32_t* nn1, int32_t* nn2, int32_t* merged);
 
 /* ---- convective: MConvectiveFast::process_all_lists (:36-114) ---------
- * v += U_inf + near-field Lamb-kernel sum (FP32 pair sums, Kahan-compensated
- * when kahan != 0, in leaf-local coordinates) + far-field Taylor terms
+ * v += U_inf + near-field Lamb-kernel sum (FP32 pair sums, compensated when
+ * kahan != 0: terms into low-order parts folded into the running sums every
+ * eight terms, in leaf-local coordinates) + far-field Taylor terms
  * (FP64).  inf_x/inf_y: Space::inf_speed() evaluated by the caller. */
 int vvhd_convective(vvhd_ctx* ctx, double inf_x, double inf_y, int kahan);
 /* Allocation-free variant of vvhd_convective (no counterpart in the
