@@ -836,7 +836,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
 // Monopole part of the learned far field of every vortex: the exact field at
 // the vortex of the far nodes' per-sign monopoles (FP32 terms from FP64
-// differences, Kahan sums, lane partials combined in FP64); the leaf's network
+// differences, block-compensated sums, lane partials combined in FP64); the leaf's network
 // correction is added by k_finish_learned.  One wave per leaf; targets in chunks of up to 16 padded to a power of two, the far
 // list split over 64 / npad lanes per target; the monopoles of 64 far nodes
 // at a time are staged in LDS.
@@ -868,7 +868,11 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
         const int k = lane % npad, fo = lane / npad;
         const bool tvalid = k < ntc;
         const double tx = tvalid ? x[base + k] : 0., ty = tvalid ? y[base + k] : 0.;
-        float su = 0.f, cu = 0.f, sv = 0.f, cv = 0.f;
+        // block-compensated sums: the terms go by FMA into the low-order
+        // parts qu, qv, folded into su, sv (Fast2Sum) every eight far nodes
+        // of a lane, so every rounding is relative to a few terms
+        float su = 0.f, qu = 0.f, sv = 0.f, qv = 0.f;
+        int nterm = 0;
         for (int64_t eb = fb; eb < fe; eb += kWave) {
             if (eb + lane < fe) {
                 const FarNode& nd = nodes[T.far_list[eb + lane]];
@@ -885,32 +889,32 @@ __global__ __launch_bounds__(kBlock) void k_far_mono(TreeDev T, const FarNode* _
             const int nb = (int)min<int64_t>(kWave, fe - eb);
             if (tvalid)
                 for (int q = fo; q < nb; q += nper) {
+                    if ((++nterm & 7) == 0) {
+                        float t = su + qu;
+                        qu = qu - (t - su);
+                        su = t;
+                        t = sv + qv;
+                        qv = qv - (t - sv);
+                        sv = t;
+                    }
                     const float dx = (float)(tx - scx[q]), dy = (float)(ty - scy[q]);
                     const float gp = sm[0][q], gm = sm[1][q];
-                    float u = 0.f, v = 0.f;
                     if (gp != 0.f) {
                         const float ex = dx - sm[2][q], ey = dy - sm[3][q];
                         const float fac = gp * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
-                        u -= ey * fac;
-                        v += ex * fac;
+                        qu = __builtin_fmaf(-ey, fac, qu);
+                        qv = __builtin_fmaf(ex, fac, qv);
                     }
                     if (gm != 0.f) {
                         const float ex = dx - sm[4][q], ey = dy - sm[5][q];
                         const float fac = gm * __builtin_amdgcn_rcpf(ex * ex + ey * ey);
-                        u -= ey * fac;
-                        v += ex * fac;
+                        qu = __builtin_fmaf(-ey, fac, qu);
+                        qv = __builtin_fmaf(ex, fac, qv);
                     }
-                    float yk = u - cu, tk = su + yk;
-                    cu = (tk - su) - yk;
-                    su = tk;
-                    yk = v - cv;
-                    tk = sv + yk;
-                    cv = (tk - sv) - yk;
-                    sv = tk;
                 }
             wave_sync();
         }
-        double du = (double)su - (double)cu, dv = (double)sv - (double)cv;
+        double du = (double)su + (double)qu, dv = (double)sv + (double)qv;
         for (int o = npad; o < kWave; o <<= 1) {
             du += __shfl_xor(du, o, kWave);
             dv += __shfl_xor(dv, o, kWave);
